@@ -1,0 +1,201 @@
+/* trax_hip.h -- C ABI of the MI355X (gfx950) triplet-seeding library.
+ *
+ * This is the device boundary that replaces the `clever` OpenCL layer the
+ * reference's algorithm classes sit on (src/datastructures/KernelWrapper.hpp:
+ * 12-35 and the clever::context / clever::vector / KERNEL_CLASS call sites,
+ * e.g. src/algorithms/GridBuilder.cpp:15-161).  Plain C, opaque handles, int
+ * status codes (0 = ok, negative = error; trax_last_error() gives the text),
+ * no exceptions across the ABI.  Device memory is passed as plain pointers.
+ *
+ * Three levels:
+ *   1. context, streams, device memory (clever::context, clever::vector,
+ *      transfer::download / downloadScalar);
+ *   2. one launcher per kernel group of the reference (KERNEL_CLASS objects of
+ *      GridBuilder, PrefixSum, PairGeneratorBeamspot, TripletThetaPhiPredictor,
+ *      TripletThetaPhiFilter, TripletConnectivityTight, TrackletCircleFitter,
+ *      CellularAutomaton), taking device pointers;
+ *   3. whole-path entry points: the per-event-group call sequence of
+ *      buildTriplets (src/triplet_builder/TripletBuilder.cpp:297-510) and the
+ *      batched graph-captured pipeline, with results as named arrays.
+ */
+#ifndef TRAX_HIP_H
+#define TRAX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRAX_OK 0
+#define TRAX_ERR_ARG (-1)      /* invalid argument */
+#define TRAX_ERR_HIP (-2)      /* HIP runtime error */
+#define TRAX_ERR_INPUT (-3)    /* input violates a precondition (e.g. a hit outside the grid) */
+#define TRAX_ERR_CAPACITY (-4) /* an output exceeded the preallocated capacity */
+#define TRAX_ERR_STATE (-5)    /* call out of sequence */
+
+typedef struct trax_ctx trax_ctx;       /* clever::context: device, default queue, scratch */
+typedef struct trax_stream trax_stream; /* an in-order device queue */
+typedef struct trax_hits trax_hits;     /* HitCollection + EventSupplement + LayerSupplement (host) */
+typedef struct trax_config trax_config; /* GridConfig + TripletConfigurations + GeometrySupplement */
+typedef struct trax_result trax_result; /* named output arrays of one run */
+typedef struct trax_batch trax_batch;   /* batched pipeline: capacities, workspace, graph */
+
+/* text of the last error of the calling thread (static storage) */
+const char* trax_last_error(void);
+
+/* ---- 1. context, streams, memory --------------------------------------- */
+
+typedef struct {
+    char name[64];
+    uint32_t cu_count;
+    uint32_t lds_bytes_per_block; /* clever::context::getLocalMemSize */
+    uint32_t max_block;
+    uint32_t wave_size;
+    uint64_t global_mem_bytes;
+} trax_device_info;
+
+/* TripletBuilder.cpp:66 createContext */
+int trax_ctx_create(int device, trax_ctx** out);
+void trax_ctx_destroy(trax_ctx* ctx);
+int trax_device_info_get(trax_ctx* ctx, trax_device_info* out);
+
+int trax_stream_create(trax_ctx* ctx, trax_stream** out);
+int trax_stream_destroy(trax_stream* s);
+/* NULL = the context's default stream (clever::context::finish_default_queue) */
+int trax_stream_sync(trax_ctx* ctx, trax_stream* s);
+
+/* clever::vector<T,1>(count, ctx) / release_buffer */
+int trax_buf_alloc(trax_ctx* ctx, uint64_t bytes, void** dptr);
+int trax_buf_free(trax_ctx* ctx, void* dptr);
+/* fill with a 32-bit value by a kernel (safe inside captured graphs) */
+int trax_buf_fill_u32(trax_ctx* ctx, void* dptr, uint32_t value, uint64_t count, trax_stream* s);
+/* clever transfer::toDevice / download, asynchronous on the stream */
+int trax_h2d(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s);
+int trax_d2h(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s);
+
+/* ---- 2. kernel launchers ------------------------------------------------ */
+
+/* PrefixSum::run (src/algorithms/PrefixSum.h:43-44, PrefixSum.cpp:8-36):
+ * exclusive in-place scan of n u32 values; the sum of all n values is written
+ * to *d_total when d_total is not NULL (the element the reference downloads
+ * with downloadScalar after scanning count+1 entries). */
+int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint32_t n, uint32_t* d_total, trax_stream* s);
+
+/* GridBuilder::run (src/algorithms/GridBuilder.h:42): gridCount + PrefixSum +
+ * gridStore.  Hits in the seven SoA columns are scattered into the z-phi grid
+ * (stable within a cell).  d_grid receives E*L*(nZ+1)*(nPhi+1) cells; the
+ * *_out columns receive the reordered hits.  The supplements must be the
+ * running sums HitCollection::addEvent produces. */
+typedef struct {
+    const float* x; const float* y; const float* z;
+    const uint32_t* layer; const uint32_t* det_id; const uint32_t* hit_id; const uint32_t* event;
+} trax_hit_cols;
+typedef struct {
+    float* x; float* y; float* z;
+    uint32_t* layer; uint32_t* det_id; uint32_t* hit_id; uint32_t* event;
+} trax_hit_cols_out;
+int trax_grid_build(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_t n_hits, uint32_t n_z,
+                    uint32_t n_phi, const uint32_t* d_event_offsets, const uint32_t* d_layer_nhits,
+                    const uint32_t* d_layer_offsets, trax_hit_cols in, trax_hit_cols_out out, uint32_t* d_grid,
+                    trax_stream* s);
+
+/* ---- 3. whole-path entry points ---------------------------------------- */
+
+/* HitCollection / EventSupplement / LayerSupplement (src/datastructures/
+ * HitCollection.h:28, EventSupplement.h:14, LayerSupplement.h:13) */
+int trax_hits_create(uint32_t max_layer, trax_hits** out);
+void trax_hits_destroy(trax_hits* h);
+/* HitCollection::addEvent (src/datastructures/HitCollection.cpp:21-180):
+ * one event given as its raw hit list (PEvent hits: position, layer,
+ * detector id, simulated track id, simulated track pT).  Uses the layers of
+ * the configuration's layer triplets and its minPt; max_tracks = -1 for all. */
+int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_t n, const float* x, const float* y,
+                        const float* z, const uint32_t* layer, const uint32_t* det_id,
+                        const uint32_t* track_id, const float* track_pt, int max_tracks, int only_tracks);
+uint32_t trax_hits_count(const trax_hits* h);
+uint32_t trax_hits_events(const trax_hits* h);
+/* column access: "x","y","z","layer","det_id","hit_id","event",
+ * "event_nhits","event_offset","layer_nhits","layer_offset" */
+int trax_hits_column(const trax_hits* h, const char* name, const void** data, uint64_t* count);
+
+/* configuration: grid, layer triplets (TripletConfiguration.h:33 column
+ * order), per-layer radius brackets (GeometrySupplement.h:21), minPt */
+int trax_config_create(uint32_t n_z, uint32_t n_phi, float min_pt, trax_config** out);
+void trax_config_destroy(trax_config* c);
+int trax_config_add_triplet(trax_config* c, uint32_t layer1, uint32_t layer2, uint32_t layer3, float d_theta_cut,
+                            float sigma_z, float d_phi_cut, float sigma_phi, float tip_cut, uint32_t pair_spread_z,
+                            float z0, uint32_t pair_spread_phi, uint32_t n_candidates);
+int trax_config_set_geometry(trax_config* c, uint32_t n, const uint32_t* layer, const float* min_radius,
+                             const float* max_radius);
+/* work-group size of the reference (fixes the pair order) and dEtaCut */
+int trax_config_set_exec(trax_config* c, uint32_t n_threads, float d_eta_cut);
+float trax_config_min_radius_curvature(const trax_config* c);
+
+/* The plain per-event-group path: buildTriplets' call sequence over the host
+ * algorithm classes, one blocking count download per stage, exact-size
+ * allocations, CA convergence checked on the host every iteration
+ * (TripletBuilder.cpp:297-510).  stop_after: 0 = all stages, else the last
+ * stage to run (1 grid, 2 pairs, 3 predictor, 4 filter, 5 connectivity+CA). */
+int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_hits* hits, int stop_after,
+                   trax_result** out);
+
+/* Batched pipeline: every stage of many events in one launch sequence with
+ * device-resident sizes, preallocated capacities, captured in a hipGraph and
+ * replayed per batch.  Capacities are upper bounds per event group. */
+typedef struct {
+    uint32_t max_events;
+    uint32_t max_hits;
+    uint32_t max_pairs;
+    uint32_t max_cands;
+    uint32_t max_tracklets;
+    uint32_t max_conn;
+    uint32_t max_track_entries;
+    uint32_t use_graph;    /* capture the sequence once, replay per batch */
+    uint32_t ca_iterations; /* CA iterations in the sequence (0 = layer triplets + 2) */
+} trax_batch_caps;
+typedef struct {
+    uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
+    uint32_t blocks_per_cu;
+} trax_tuning;
+int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_caps* caps,
+                      const trax_tuning* tuning, trax_stream* s, trax_batch** out);
+void trax_batch_destroy(trax_batch* b);
+/* upload a HitCollection and enqueue the whole pipeline; asynchronous */
+int trax_batch_submit(trax_batch* b, const trax_hits* hits);
+/* wait for the submitted run; TRAX_ERR_CAPACITY if an output overflowed */
+int trax_batch_wait(trax_batch* b);
+/* results of the last completed run (downloads them) */
+int trax_batch_result(trax_batch* b, trax_result** out);
+/* counts of the last completed run: pairs, candidates, tracklets,
+ * connections, track entries, CA iterations */
+int trax_batch_counts(trax_batch* b, uint32_t out[8]);
+/* device bytes held by the batch workspace */
+uint64_t trax_batch_footprint(const trax_batch* b);
+
+/* per-kernel timing of the last batched run, from HIP events around each
+ * stage on the batch's stream: stage names "grid", "pairs", "predict",
+ * "filter", "connectivity", "ca"; milliseconds */
+int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms);
+
+/* named result arrays: "grid", "hits.x", ..., "pairs", "pair_offsets",
+ * "cands", "cand_offsets", "tracklets.h1/h2/h3", "tracklet_offsets", "eta",
+ * "conn.basis", "conn.followers", "conn.event", "conn.connectable",
+ * "conn.nonconnectable", "pt", "ca.states", "ca.best_basis", "tracks",
+ * "track_offsets", "ca.iterations", "diag" */
+int trax_result_get(const trax_result* r, const char* name, const void** data, uint64_t* count, int* is_float);
+double trax_result_timing_ms(const trax_result* r, const char* stage);
+void trax_result_destroy(trax_result* r);
+
+/* host and device evaluation of the shared float built-ins (atan2, acos,
+ * atan, asinh), for accuracy and host/device agreement tests.
+ * fn: 0 atan2(y, x), 1 acos(x), 2 atan(x), 3 asinh(x) */
+int trax_math_eval_host(int fn, const float* x, const float* y, float* out, uint64_t n);
+int trax_math_eval_device(trax_ctx* ctx, int fn, const float* x, const float* y, float* out, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRAX_HIP_H */

@@ -1,0 +1,135 @@
+"""ORACLE -- test infrastructure only (CPU restatement of the reference's
+kernels, trax_oracle.cpp).  Loaded by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker; the product never imports it."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "trax_oracle.cpp")
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+P = C.c_void_p
+U32 = C.c_uint32
+
+
+def build() -> str:
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
+           SRC, "-o", tmp, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+class Input(C.Structure):
+    _fields_ = [
+        ("x", P), ("y", P), ("z", P), ("layer", P), ("det_id", P), ("hit_id", P), ("event", P), ("n_hits", U32),
+        ("n_events", U32), ("n_layers", U32), ("evt_nhits", P), ("evt_offset", P), ("lay_nhits", P),
+        ("lay_offset", P), ("n_z", U32), ("n_phi", U32), ("min_r", P), ("max_r", P), ("n_geom", U32),
+        ("n_trip", U32), ("l1", P), ("l2", P), ("l3", P), ("dtheta", P), ("sigma_z", P), ("dphi", P),
+        ("sigma_phi", P), ("tip", P), ("z0", P), ("pair_spread_phi", P), ("min_radius_curvature", C.c_float),
+        ("n_threads", U32), ("d_eta_cut", C.c_float), ("sparse_pairs", C.c_int), ("stop_after", C.c_int),
+        ("n_workers", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.oracle_run.restype = P
+        L.oracle_run.argtypes = [C.POINTER(Input)]
+        L.oracle_error.restype = C.c_char_p
+        L.oracle_error.argtypes = [P]
+        L.oracle_get.restype = C.c_longlong
+        L.oracle_get.argtypes = [P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_int)]
+        L.oracle_timing.restype = C.c_double
+        L.oracle_timing.argtypes = [P, C.c_char_p]
+        L.oracle_free.argtypes = [P]
+        L.oracle_prefix_sum.argtypes = [P, U32, U32]
+        _lib = L
+    return _lib
+
+
+NAMES = ["grid", "hits.x", "hits.y", "hits.z", "hits.layer", "hits.det_id", "hits.hit_id", "hits.event", "pairs",
+         "pair_offsets", "cands", "cand_offsets", "tracklets.h1", "tracklets.h2", "tracklets.h3",
+         "tracklet_offsets", "eta", "conn.count_prefix", "conn.basis", "conn.followers", "conn.event",
+         "conn.connectable", "conn.nonconnectable", "pt", "ca.states", "ca.iterations", "ca.best_basis", "tracks",
+         "track_offsets", "diag"]
+
+
+def run(cols, wl, min_radius_curvature, stop_after=0, sparse_pairs=True, workers=None):
+    """Run the oracle on HitCollection columns (a dict as produced by
+    trax_amd.api.HitSet.columns()) for a Workload.  Returns named arrays."""
+    from trax_amd.config import geometry_arrays, triplet_arrays
+    keep = []
+
+    def a(v, dt):
+        v = np.ascontiguousarray(v, dtype=dt)
+        keep.append(v)
+        return C.c_void_p(v.ctypes.data)
+
+    layer, rmin, rmax = geometry_arrays()
+    t = triplet_arrays(wl.triplets)
+    inp = Input()
+    inp.x, inp.y, inp.z = a(cols["x"], np.float32), a(cols["y"], np.float32), a(cols["z"], np.float32)
+    inp.layer, inp.det_id = a(cols["layer"], np.uint32), a(cols["det_id"], np.uint32)
+    inp.hit_id, inp.event = a(cols["hit_id"], np.uint32), a(cols["event"], np.uint32)
+    inp.n_hits = cols["x"].size
+    inp.n_events = cols["event_nhits"].size
+    inp.n_layers = max(tt.layer3 for tt in wl.triplets)
+    inp.evt_nhits, inp.evt_offset = a(cols["event_nhits"], np.uint32), a(cols["event_offset"], np.uint32)
+    inp.lay_nhits, inp.lay_offset = a(cols["layer_nhits"], np.uint32), a(cols["layer_offset"], np.uint32)
+    inp.n_z, inp.n_phi = wl.n_z, wl.n_phi
+    inp.min_r, inp.max_r, inp.n_geom = a(rmin, np.float32), a(rmax, np.float32), rmin.size
+    inp.n_trip = len(wl.triplets)
+    inp.l1, inp.l2, inp.l3 = a(t["layer1"], np.uint32), a(t["layer2"], np.uint32), a(t["layer3"], np.uint32)
+    inp.dtheta, inp.sigma_z = a(t["dThetaCut"], np.float32), a(t["sigmaZ"], np.float32)
+    inp.dphi, inp.sigma_phi = a(t["dPhiCut"], np.float32), a(t["sigmaPhi"], np.float32)
+    inp.tip, inp.z0 = a(t["tipCut"], np.float32), a(t["z0"], np.float32)
+    inp.pair_spread_phi = a(t["pairSpreadPhi"], np.uint32)
+    inp.min_radius_curvature = min_radius_curvature
+    inp.n_threads = wl.n_threads
+    inp.d_eta_cut = wl.d_eta_cut
+    inp.sparse_pairs = int(sparse_pairs)
+    inp.stop_after = stop_after
+    inp.n_workers = workers if workers is not None else min(8, os.cpu_count() or 1)
+    L = lib()
+    h = L.oracle_run(C.byref(inp))
+    try:
+        err = L.oracle_error(h)
+        if err:
+            raise RuntimeError("oracle: " + err.decode())
+        out = {}
+        for k in NAMES:
+            d, f = P(), C.c_int()
+            n = L.oracle_get(h, k.encode(), C.byref(d), C.byref(f))
+            if n < 0:
+                continue
+            dt = np.float32 if f.value else np.uint32
+            out[k] = (np.zeros(0, dt) if n == 0 else
+                      np.ctypeslib.as_array(C.cast(d, C.POINTER(C.c_uint32)), shape=(n,)).view(dt).copy())
+        for st in ("grid", "pairs", "predict", "filter", "connectivity_ca"):
+            v = L.oracle_timing(h, st.encode())
+            if v >= 0:
+                out["time." + st] = v
+        return out
+    finally:
+        L.oracle_free(h)
+
+
+def prefix_sum(values, wg):
+    v = np.ascontiguousarray(values, dtype=np.uint32).copy()
+    lib().oracle_prefix_sum(C.c_void_p(v.ctypes.data), v.size, wg)
+    return v

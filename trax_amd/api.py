@@ -1,0 +1,369 @@
+"""Python host API over the C ABI of libtrax_hip.so (include/trax_hip.h).
+
+The product path is the HIP library: this module only marshals numpy arrays
+through ctypes.  It raises if the library has not been built.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .config import Workload, geometry_arrays, triplet_arrays
+from .generator import Event
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtrax_hip.so")
+
+P = C.c_void_p
+U32 = C.c_uint32
+_lib = None
+
+
+class TraxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"trax error {code}: {msg}")
+        self.code = code
+
+
+TRAX_ERR_CAPACITY = -4
+TRAX_ERR_INPUT = -3
+
+
+class HitCols(C.Structure):
+    _fields_ = [(n, P) for n in ("x", "y", "z", "layer", "det_id", "hit_id", "event")]
+
+
+class BatchCaps(C.Structure):
+    _fields_ = [(n, U32) for n in ("max_events", "max_hits", "max_pairs", "max_cands", "max_tracklets",
+                                   "max_conn", "max_track_entries", "use_graph", "ca_iterations")]
+
+
+class Tuning(C.Structure):
+    _fields_ = [(n, U32) for n in ("grid_count_block", "grid_store_block", "pair_block", "predict_block",
+                                   "filter_block", "link_block", "blocks_per_cu")]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("cu_count", U32), ("lds_bytes_per_block", U32), ("max_block", U32),
+                ("wave_size", U32), ("global_mem_bytes", C.c_uint64)]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with trax_amd.build.build() "
+                           "(there is no fallback path)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "trax_last_error": (C.c_char_p, []),
+        "trax_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+        "trax_ctx_destroy": (None, [P]),
+        "trax_device_info_get": (C.c_int, [P, C.POINTER(DeviceInfo)]),
+        "trax_stream_create": (C.c_int, [P, C.POINTER(P)]),
+        "trax_stream_destroy": (C.c_int, [P]),
+        "trax_stream_sync": (C.c_int, [P, P]),
+        "trax_buf_alloc": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
+        "trax_buf_free": (C.c_int, [P, P]),
+        "trax_buf_fill_u32": (C.c_int, [P, P, U32, C.c_uint64, P]),
+        "trax_h2d": (C.c_int, [P, P, P, C.c_uint64, P]),
+        "trax_d2h": (C.c_int, [P, P, P, C.c_uint64, P]),
+        "trax_prefix_sum": (C.c_int, [P, P, U32, P, P]),
+        "trax_grid_build": (C.c_int, [P, U32, U32, U32, U32, U32, P, P, P, HitCols, HitCols, P, P]),
+        "trax_hits_create": (C.c_int, [U32, C.POINTER(P)]),
+        "trax_hits_destroy": (None, [P]),
+        "trax_hits_add_event": (C.c_int, [P, P, U32, P, P, P, P, P, P, P, C.c_int, C.c_int]),
+        "trax_hits_count": (U32, [P]),
+        "trax_hits_events": (U32, [P]),
+        "trax_hits_column": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64)]),
+        "trax_config_create": (C.c_int, [U32, U32, C.c_float, C.POINTER(P)]),
+        "trax_config_destroy": (None, [P]),
+        "trax_config_add_triplet": (C.c_int, [P, U32, U32, U32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                              C.c_float, U32, C.c_float, U32, U32]),
+        "trax_config_set_geometry": (C.c_int, [P, U32, P, P, P]),
+        "trax_config_set_exec": (C.c_int, [P, U32, C.c_float]),
+        "trax_config_min_radius_curvature": (C.c_float, [P]),
+        "trax_run_plain": (C.c_int, [P, P, P, C.c_int, C.POINTER(P)]),
+        "trax_batch_create": (C.c_int, [P, P, C.POINTER(BatchCaps), C.POINTER(Tuning), P, C.POINTER(P)]),
+        "trax_batch_destroy": (None, [P]),
+        "trax_batch_submit": (C.c_int, [P, P]),
+        "trax_batch_wait": (C.c_int, [P]),
+        "trax_batch_result": (C.c_int, [P, C.POINTER(P)]),
+        "trax_batch_counts": (C.c_int, [P, C.POINTER(U32)]),
+        "trax_batch_footprint": (C.c_uint64, [P]),
+        "trax_batch_stage_ms": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_float)]),
+        "trax_result_get": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+        "trax_result_timing_ms": (C.c_double, [P, C.c_char_p]),
+        "trax_result_destroy": (None, [P]),
+        "trax_math_eval_host": (C.c_int, [C.c_int, P, P, P, C.c_uint64]),
+        "trax_math_eval_device": (C.c_int, [P, C.c_int, P, P, P, C.c_uint64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise TraxError(rc, lib().trax_last_error().decode())
+
+
+def ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------------------
+class Config:
+    """GridConfig + TripletConfigurations + GeometrySupplement of a workload."""
+
+    def __init__(self, wl: Workload):
+        L = lib()
+        h = P()
+        check(L.trax_config_create(wl.n_z, wl.n_phi, wl.min_pt, C.byref(h)))
+        self.h = h
+        self.wl = wl
+        for t in wl.triplets:
+            check(L.trax_config_add_triplet(h, t.layer1, t.layer2, t.layer3, t.dThetaCut, t.sigmaZ, t.dPhiCut,
+                                            t.sigmaPhi, t.tipCut, t.pairSpreadZ, t.z0, t.pairSpreadPhi,
+                                            t.nCandidates))
+        layer, rmin, rmax = geometry_arrays()
+        self._geom = (layer, rmin, rmax)
+        check(L.trax_config_set_geometry(h, layer.size, ptr(layer), ptr(rmin), ptr(rmax)))
+        check(L.trax_config_set_exec(h, wl.n_threads, wl.d_eta_cut))
+        self.max_layer = max(t.layer3 for t in wl.triplets)
+
+    @property
+    def min_radius_curvature(self) -> float:
+        return float(lib().trax_config_min_radius_curvature(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_config_destroy(self.h)
+            self.h = None
+
+
+class HitSet:
+    """HitCollection + EventSupplement + LayerSupplement built by addEvent."""
+
+    def __init__(self, cfg: Config, events: List[Event] = ()):
+        h = P()
+        check(lib().trax_hits_create(cfg.max_layer, C.byref(h)))
+        self.h = h
+        self.cfg = cfg
+        for ev in events:
+            self.add(ev)
+
+    def add(self, ev: Event, max_tracks: int = -1, only_tracks: bool = False):
+        a = [np.ascontiguousarray(v) for v in (ev.x, ev.y, ev.z, ev.layer, ev.det_id, ev.track_id, ev.track_pt)]
+        check(lib().trax_hits_add_event(self.h, self.cfg.h, a[0].size, *[ptr(v) for v in a], max_tracks,
+                                        int(only_tracks)))
+
+    @property
+    def n_hits(self) -> int:
+        return int(lib().trax_hits_count(self.h))
+
+    @property
+    def n_events(self) -> int:
+        return int(lib().trax_hits_events(self.h))
+
+    def column(self, name: str) -> np.ndarray:
+        d, n = P(), C.c_uint64()
+        check(lib().trax_hits_column(self.h, name.encode(), C.byref(d), C.byref(n)))
+        dt = np.float32 if name in ("x", "y", "z") else np.uint32
+        if n.value == 0:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(C.cast(d, C.POINTER(C.c_uint32)), shape=(n.value,)).view(dt).copy()
+
+    def columns(self) -> Dict[str, np.ndarray]:
+        return {k: self.column(k) for k in ("x", "y", "z", "layer", "det_id", "hit_id", "event", "event_nhits",
+                                            "event_offset", "layer_nhits", "layer_offset")}
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_hits_destroy(self.h)
+            self.h = None
+
+
+def _result_dict(rh) -> Dict[str, np.ndarray]:
+    L = lib()
+    out = {}
+    names = ["grid", "hits.x", "hits.y", "hits.z", "hits.layer", "hits.det_id", "hits.hit_id", "hits.event",
+             "pairs", "pair_offsets", "cands", "cand_offsets", "tracklets.h1", "tracklets.h2", "tracklets.h3",
+             "tracklet_offsets", "eta", "conn.basis", "conn.followers", "conn.event", "conn.connectable",
+             "conn.nonconnectable", "pt", "ca.states", "ca.best_basis", "tracks", "track_offsets",
+             "ca.iterations", "diag"]
+    for k in names:
+        d, n, f = P(), C.c_uint64(), C.c_int()
+        if L.trax_result_get(rh, k.encode(), C.byref(d), C.byref(n), C.byref(f)) != 0:
+            continue
+        dt = np.float32 if f.value else np.uint32
+        if n.value == 0:
+            out[k] = np.zeros(0, dt)
+        else:
+            out[k] = np.ctypeslib.as_array(C.cast(d, C.POINTER(C.c_uint32)), shape=(n.value,)).view(dt).copy()
+    for st in ("grid", "pairs", "predict", "filter", "connectivity", "circle", "ca"):
+        t = L.trax_result_timing_ms(rh, st.encode())
+        if t >= 0:
+            out["time." + st] = t
+    L.trax_result_destroy(rh)
+    return out
+
+
+class Device:
+    """A device context (clever::context)."""
+
+    def __init__(self, device: int = 0):
+        h = P()
+        check(lib().trax_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def info(self) -> DeviceInfo:
+        d = DeviceInfo()
+        check(lib().trax_device_info_get(self.h, C.byref(d)))
+        return d
+
+    def run_plain(self, cfg: Config, hits: HitSet, stop_after: int = 0) -> Dict[str, np.ndarray]:
+        rh = P()
+        check(lib().trax_run_plain(self.h, cfg.h, hits.h, stop_after, C.byref(rh)))
+        return _result_dict(rh)
+
+    def math_device(self, fn: int, x: np.ndarray, y: Optional[np.ndarray] = None) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        y = None if y is None else np.ascontiguousarray(y, np.float32)
+        out = np.empty_like(x)
+        check(lib().trax_math_eval_device(self.h, fn, ptr(x), None if y is None else ptr(y), ptr(out), x.size))
+        return out
+
+    def stream(self) -> "Stream":
+        return Stream(self)
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Stream:
+    def __init__(self, dev: Device):
+        h = P()
+        check(lib().trax_stream_create(dev.h, C.byref(h)))
+        self.h = h
+        self.dev = dev
+
+    def sync(self):
+        check(lib().trax_stream_sync(self.dev.h, self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_stream_destroy(self.h)
+            self.h = None
+
+
+def math_host(fn: int, x: np.ndarray, y: Optional[np.ndarray] = None) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = None if y is None else np.ascontiguousarray(y, np.float32)
+    out = np.empty_like(x)
+    check(lib().trax_math_eval_host(fn, ptr(x), None if y is None else ptr(y), ptr(out), x.size))
+    return out
+
+
+@dataclass
+class Caps:
+    max_events: int
+    max_hits: int
+    max_pairs: int
+    max_cands: int
+    max_tracklets: int
+    max_conn: int
+    max_track_entries: int
+    use_graph: bool = True
+    ca_iterations: int = 0
+
+    @staticmethod
+    def for_hits(max_events: int, max_hits: int, pairs_per_hit: float = 8.0, cands_per_pair: float = 4.0,
+                 trk_per_hit: float = 1.0, use_graph: bool = True) -> "Caps":
+        p = int(max_hits * pairs_per_hit) + 1024
+        q = int(p * cands_per_pair) + 1024
+        r = int(max_hits * trk_per_hit) + 1024
+        return Caps(max_events, max_hits, p, q, r, 4 * r + 1024, 2 * r + 1024, use_graph)
+
+    def grown(self, factor: float = 2.0) -> "Caps":
+        g = lambda v: int(v * factor) + 1024
+        return Caps(self.max_events, self.max_hits, g(self.max_pairs), g(self.max_cands), g(self.max_tracklets),
+                    g(self.max_conn), g(self.max_track_entries), self.use_graph, self.ca_iterations)
+
+
+class Batch:
+    """Batched pipeline: preallocated workspace, stage graphs replayed per batch."""
+
+    def __init__(self, dev: Device, cfg: Config, caps: Caps, tuning: Optional[Dict[str, int]] = None,
+                 stream: Optional[Stream] = None):
+        c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
+                      caps.max_conn, caps.max_track_entries, int(caps.use_graph), caps.ca_iterations)
+        t = Tuning(**(tuning or {}))
+        h = P()
+        check(lib().trax_batch_create(dev.h, cfg.h, C.byref(c), C.byref(t), None if stream is None else stream.h,
+                                      C.byref(h)))
+        self.h = h
+        self.caps = caps
+        self.dev = dev
+        self.cfg = cfg
+        self.stream = stream
+
+    def submit(self, hits: HitSet):
+        check(lib().trax_batch_submit(self.h, hits.h))
+
+    def wait(self) -> int:
+        rc = lib().trax_batch_wait(self.h)
+        if rc not in (0, TRAX_ERR_CAPACITY):
+            check(rc)
+        return rc
+
+    def counts(self) -> Dict[str, int]:
+        a = (U32 * 8)()
+        check(lib().trax_batch_counts(self.h, a))
+        keys = ("pairs", "cands", "tracklets", "conn", "track_entries", "ca_iterations", "connectable", "err")
+        return dict(zip(keys, list(a)))
+
+    def result(self) -> Dict[str, np.ndarray]:
+        rh = P()
+        check(lib().trax_batch_result(self.h, C.byref(rh)))
+        return _result_dict(rh)
+
+    def stage_ms(self, stage: str) -> float:
+        v = C.c_float()
+        check(lib().trax_batch_stage_ms(self.h, stage.encode(), C.byref(v)))
+        return float(v.value)
+
+    @property
+    def footprint(self) -> int:
+        return int(lib().trax_batch_footprint(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def run_batch_fit(dev: Device, cfg: Config, hits: HitSet, caps: Caps, tuning=None, tries: int = 6):
+    """Run one batch, growing the capacities on overflow; returns (batch, result)."""
+    for _ in range(tries):
+        b = Batch(dev, cfg, caps, tuning)
+        b.submit(hits)
+        if b.wait() == 0:
+            return b, b.result()
+        b.close()
+        caps = caps.grown()
+    raise TraxError(TRAX_ERR_CAPACITY, "capacities kept overflowing")

@@ -1,0 +1,914 @@
+// C ABI of the gfx950 triplet-seeding library (declared in include/trax_hip.h).
+#include "../../include/trax_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host/trax_host.h"
+#include "kernels/kernels.h"
+
+using namespace trax;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(TRAX_ERR_HIP, e.what());
+    } catch (const InputError& e) {
+        return fail(TRAX_ERR_INPUT, e.what());
+    } catch (const CapacityError& e) {
+        return fail(TRAX_ERR_CAPACITY, e.what());
+    } catch (const std::exception& e) {
+        return fail(TRAX_ERR_ARG, e.what());
+    }
+}
+
+struct Arr {
+    std::vector<uint32_t> u;
+    std::vector<float> f;
+    bool is_float = false;
+};
+
+}  // namespace
+
+struct trax_result {
+    std::map<std::string, Arr> arrays;
+    std::map<std::string, double> timings;
+    void put(const std::string& k, std::vector<uint32_t> v) {
+        Arr a; a.u = std::move(v); arrays[k] = std::move(a);
+    }
+    void putf(const std::string& k, std::vector<float> v) {
+        Arr a; a.f = std::move(v); a.is_float = true; arrays[k] = std::move(a);
+    }
+};
+
+struct trax_ctx {
+    std::unique_ptr<Context> c;
+};
+struct trax_stream {
+    hipStream_t s = nullptr;
+};
+
+struct trax_config {
+    GridConfig grid;
+    TripletConfigurations trip{1.0f};
+    GeometrySupplement geom;
+    uint32_t n_threads = 256;
+    float d_eta_cut = 0.0256f;
+};
+
+struct trax_hits {
+    uint32_t max_layer = 0;
+    uint32_t n_events = 0;
+    HitCollection hits;
+    std::vector<uint32_t> evt_nhits, evt_offset, lay_nhits, lay_offset;
+};
+
+extern "C" const char* trax_last_error(void) { return g_last_error.c_str(); }
+
+// ---- context, streams, memory ---------------------------------------------
+
+extern "C" int trax_ctx_create(int device, trax_ctx** out) {
+    return guarded([&] {
+        if (!out) return fail(TRAX_ERR_ARG, "null output");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+            return fail(TRAX_ERR_HIP, "no such HIP device");
+        trax_ctx* c = new trax_ctx;
+        c->c.reset(new Context(device));
+        *out = c;
+        return TRAX_OK;
+    });
+}
+
+extern "C" void trax_ctx_destroy(trax_ctx* ctx) { delete ctx; }
+
+extern "C" int trax_device_info_get(trax_ctx* ctx, trax_device_info* out) {
+    return guarded([&] {
+        hipDeviceProp_t p;
+        hip_check(hipGetDeviceProperties(&p, ctx->c->device()), "hipGetDeviceProperties");
+        std::memset(out, 0, sizeof(*out));
+        std::strncpy(out->name, p.gcnArchName, sizeof(out->name) - 1);
+        out->cu_count = (uint32_t)p.multiProcessorCount;
+        out->lds_bytes_per_block = (uint32_t)p.sharedMemPerBlock;
+        out->max_block = (uint32_t)p.maxThreadsPerBlock;
+        out->wave_size = (uint32_t)p.warpSize;
+        out->global_mem_bytes = (uint64_t)p.totalGlobalMem;
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_stream_create(trax_ctx* ctx, trax_stream** out) {
+    return guarded([&] {
+        hip_check(hipSetDevice(ctx->c->device()), "hipSetDevice");
+        trax_stream* s = new trax_stream;
+        hip_check(hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking), "hipStreamCreate");
+        *out = s;
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_stream_destroy(trax_stream* s) {
+    return guarded([&] {
+        if (s) {
+            hip_check(hipStreamDestroy(s->s), "hipStreamDestroy");
+            delete s;
+        }
+        return TRAX_OK;
+    });
+}
+
+static hipStream_t pick(trax_ctx* ctx, trax_stream* s) { return s ? s->s : ctx->c->stream(); }
+
+extern "C" int trax_stream_sync(trax_ctx* ctx, trax_stream* s) {
+    return guarded([&] {
+        hip_check(hipStreamSynchronize(pick(ctx, s)), "hipStreamSynchronize");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_buf_alloc(trax_ctx* ctx, uint64_t bytes, void** dptr) {
+    return guarded([&] {
+        hip_check(hipSetDevice(ctx->c->device()), "hipSetDevice");
+        hip_check(hipMalloc(dptr, bytes ? bytes : 4), "hipMalloc");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_buf_free(trax_ctx* ctx, void* dptr) {
+    return guarded([&] {
+        (void)ctx;
+        hip_check(hipFree(dptr), "hipFree");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_buf_fill_u32(trax_ctx* ctx, void* dptr, uint32_t value, uint64_t count, trax_stream* s) {
+    return guarded([&] {
+        hip_check(launch_fill(static_cast<uint32_t*>(dptr), value, count, pick(ctx, s)), "fill");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_h2d(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s) {
+    return guarded([&] {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick(ctx, s)), "h2d");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_d2h(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s) {
+    return guarded([&] {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, pick(ctx, s)), "d2h");
+        return TRAX_OK;
+    });
+}
+
+// ---- kernel launchers -------------------------------------------------------
+
+extern "C" int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint32_t n, uint32_t* d_total, trax_stream* s) {
+    return guarded([&] {
+        hip_check(launch_scan(d_data, nullptr, n, n, d_total, ctx->c->scan_scratch(n), pick(ctx, s)), "scan");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_grid_build(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_t n_hits, uint32_t n_z,
+                               uint32_t n_phi, const uint32_t* d_event_offsets, const uint32_t* d_layer_nhits,
+                               const uint32_t* d_layer_offsets, trax_hit_cols in, trax_hit_cols_out out,
+                               uint32_t* d_grid, trax_stream* s) {
+    return guarded([&] {
+        Context& c = *ctx->c;
+        hipStream_t prev = c.stream();
+        c.set_stream(pick(ctx, s));
+        GridConfig gc;
+        gc.nEvents = n_events; gc.nLayers = n_layers; gc.nSectorsZ = n_z; gc.nSectorsPhi = n_phi;
+        c.reset_counts();
+        c.set_batch_shape(n_events, n_hits);
+        HitColumns hin, hout;
+        hin.x = const_cast<float*>(in.x); hin.y = const_cast<float*>(in.y); hin.z = const_cast<float*>(in.z);
+        hin.layer = const_cast<uint32_t*>(in.layer); hin.det = const_cast<uint32_t*>(in.det_id);
+        hin.id = const_cast<uint32_t*>(in.hit_id); hin.evt = const_cast<uint32_t*>(in.event);
+        hout.x = out.x; hout.y = out.y; hout.z = out.z; hout.layer = out.layer; hout.det = out.det_id;
+        hout.id = out.hit_id; hout.evt = out.event;
+        const GridGeom g = gc.geom();
+        hip_check(launch_grid_count(g, n_events * n_layers, d_event_offsets, d_layer_nhits, d_layer_offsets, in.x, in.y,
+                                    in.z, d_grid, c.counts(), c.launch(), c.stream()),
+                  "grid count");
+        hip_check(launch_grid_store(g, n_events * n_layers, d_event_offsets, d_layer_nhits, d_layer_offsets, d_grid, hin,
+                                    hout, n_hits, c.counts(), c.launch(), c.stream()),
+                  "grid store");
+        c.finish_default_queue();
+        c.check_errors();
+        c.set_stream(prev);
+        return TRAX_OK;
+    });
+}
+
+// ---- hits and configuration -----------------------------------------------------
+
+extern "C" int trax_hits_create(uint32_t max_layer, trax_hits** out) {
+    return guarded([&] {
+        if (max_layer == 0) return fail(TRAX_ERR_ARG, "max_layer must be positive");
+        trax_hits* h = new trax_hits;
+        h->max_layer = max_layer;
+        *out = h;
+        return TRAX_OK;
+    });
+}
+
+extern "C" void trax_hits_destroy(trax_hits* h) { delete h; }
+
+extern "C" int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_t n, const float* x, const float* y,
+                                   const float* z, const uint32_t* layer, const uint32_t* det_id,
+                                   const uint32_t* track_id, const float* track_pt, int max_tracks, int only_tracks) {
+    return guarded([&] {
+        if (cfg->trip.maxLayer() != h->max_layer)
+            return fail(TRAX_ERR_ARG, "configuration's highest layer differs from the collection's");
+        std::vector<PHit> ev(n);
+        for (uint32_t i = 0; i < n; ++i) ev[i] = PHit{x[i], y[i], z[i], layer[i], det_id[i], track_id[i], track_pt[i]};
+        const uint32_t e = h->n_events;
+        EventSupplement es(e + 1);
+        LayerSupplement ls(h->max_layer, e + 1);
+        std::copy(h->evt_nhits.begin(), h->evt_nhits.end(), es.nHits.begin());
+        std::copy(h->evt_offset.begin(), h->evt_offset.end(), es.offset.begin());
+        std::copy(h->lay_nhits.begin(), h->lay_nhits.end(), ls.nHits.begin());
+        std::copy(h->lay_offset.begin(), h->lay_offset.end(), ls.offset.begin());
+        h->hits.addEvent(ev, es, e, ls, cfg->trip, max_tracks, only_tracks != 0);
+        h->evt_nhits = es.nHits; h->evt_offset = es.offset;
+        h->lay_nhits = ls.nHits; h->lay_offset = ls.offset;
+        h->n_events = e + 1;
+        return TRAX_OK;
+    });
+}
+
+extern "C" uint32_t trax_hits_count(const trax_hits* h) { return h->hits.size(); }
+extern "C" uint32_t trax_hits_events(const trax_hits* h) { return h->n_events; }
+
+extern "C" int trax_hits_column(const trax_hits* h, const char* name, const void** data, uint64_t* count) {
+    const std::string k(name);
+    const HitCollection& c = h->hits;
+    auto ret = [&](const void* p, size_t n) { *data = p; *count = n; return TRAX_OK; };
+    if (k == "x") return ret(c.x.data(), c.x.size());
+    if (k == "y") return ret(c.y.data(), c.y.size());
+    if (k == "z") return ret(c.z.data(), c.z.size());
+    if (k == "layer") return ret(c.layer.data(), c.layer.size());
+    if (k == "det_id") return ret(c.detId.data(), c.detId.size());
+    if (k == "hit_id") return ret(c.hitId.data(), c.hitId.size());
+    if (k == "event") return ret(c.event.data(), c.event.size());
+    if (k == "event_nhits") return ret(h->evt_nhits.data(), h->evt_nhits.size());
+    if (k == "event_offset") return ret(h->evt_offset.data(), h->evt_offset.size());
+    if (k == "layer_nhits") return ret(h->lay_nhits.data(), h->lay_nhits.size());
+    if (k == "layer_offset") return ret(h->lay_offset.data(), h->lay_offset.size());
+    return fail(TRAX_ERR_ARG, "unknown column " + k);
+}
+
+extern "C" int trax_config_create(uint32_t n_z, uint32_t n_phi, float min_pt, trax_config** out) {
+    return guarded([&] {
+        if (!n_z || !n_phi) return fail(TRAX_ERR_ARG, "grid needs at least one cell");
+        trax_config* c = new trax_config;
+        c->grid.nSectorsZ = n_z;
+        c->grid.nSectorsPhi = n_phi;
+        c->trip.minPt_ = min_pt;
+        *out = c;
+        return TRAX_OK;
+    });
+}
+
+extern "C" void trax_config_destroy(trax_config* c) { delete c; }
+
+extern "C" int trax_config_add_triplet(trax_config* c, uint32_t layer1, uint32_t layer2, uint32_t layer3,
+                                       float d_theta_cut, float sigma_z, float d_phi_cut, float sigma_phi,
+                                       float tip_cut, uint32_t pair_spread_z, float z0, uint32_t pair_spread_phi,
+                                       uint32_t n_candidates) {
+    if (!layer1 || !layer2 || !layer3) return fail(TRAX_ERR_ARG, "layers are numbered from 1");
+    c->trip.addWithValue(layer1, layer2, layer3, d_theta_cut, sigma_z, d_phi_cut, sigma_phi, tip_cut, pair_spread_z,
+                         z0, pair_spread_phi, n_candidates);
+    return TRAX_OK;
+}
+
+extern "C" int trax_config_set_geometry(trax_config* c, uint32_t n, const uint32_t* layer, const float* min_radius,
+                                        const float* max_radius) {
+    c->geom = GeometrySupplement();
+    for (uint32_t i = 0; i < n; ++i) c->geom.addWithValue(layer[i], min_radius[i], max_radius[i]);
+    return TRAX_OK;
+}
+
+extern "C" int trax_config_set_exec(trax_config* c, uint32_t n_threads, float d_eta_cut) {
+    if (n_threads == 0) return fail(TRAX_ERR_ARG, "n_threads must be positive");
+    c->n_threads = n_threads;
+    c->d_eta_cut = d_eta_cut;
+    return TRAX_OK;
+}
+
+extern "C" float trax_config_min_radius_curvature(const trax_config* c) { return c->trip.minRadiusCurvature(); }
+
+static void validate_config(const trax_config* cfg) {
+    const uint32_t L = cfg->trip.maxLayer();
+    if (cfg->trip.size() == 0) throw InputError("no layer triplet configured");
+    if (cfg->geom.size() < L) throw InputError("geometry supplement has fewer layers than the triplets use");
+    for (uint32_t i = 0; i < cfg->trip.size(); ++i)
+        if (cfg->trip.layer1[i] > L || cfg->trip.layer2[i] > L) throw InputError("bad layer triplet");
+}
+
+// ---- plain per-event-group path ---------------------------------------------------
+
+extern "C" int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_hits* h, int stop_after,
+                              trax_result** out) {
+    return guarded([&] {
+        validate_config(cfg);
+        Context& c = *ctx->c;
+        hip_check(hipSetDevice(c.device()), "hipSetDevice");
+        auto* r = new trax_result;
+        std::unique_ptr<trax_result> guard(r);
+        const uint32_t E = h->n_events, L = h->max_layer, T = cfg->trip.size(), N = h->hits.size();
+        const uint32_t nt = cfg->n_threads;
+        auto t0 = std::chrono::steady_clock::now();
+        auto lap = [&](const char* k) {
+            auto t1 = std::chrono::steady_clock::now();
+            r->timings[k] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            t0 = t1;
+        };
+        auto want = [&](int s) { return stop_after == 0 || stop_after >= s; };
+        c.reset_counts();
+        GridConfig gc = cfg->grid;
+        gc.nEvents = E;
+        gc.nLayers = L;
+        Grid grid(c, gc);
+        HitCollection hits = h->hits;
+        EventSupplement es(E);
+        es.nHits = h->evt_nhits; es.offset = h->evt_offset;
+        LayerSupplement ls(L, E);
+        ls.nHits = h->lay_nhits; ls.offset = h->lay_offset;
+        HitMirror dh(c, N);
+        dh.upload(c, hits);  // hits.transfer.toDevice (TripletBuilder.cpp:297-298)
+        GridBuilder(c).run(hits, dh, nt, es, ls, grid, true);
+        lap("grid");
+        r->put("grid", grid.host);
+        r->putf("hits.x", hits.x); r->putf("hits.y", hits.y); r->putf("hits.z", hits.z);
+        r->put("hits.layer", hits.layer); r->put("hits.det_id", hits.detId);
+        r->put("hits.hit_id", hits.hitId); r->put("hits.event", hits.event);
+        auto diag = [&] {
+            std::vector<uint32_t> d(4);
+            d[0] = c.download_count(&c.counts()->diag_pair_short);
+            d[1] = c.download_count(&c.counts()->diag_pred_short);
+            d[2] = c.download_count(&c.counts()->diag_pred_trunc);
+            d[3] = c.download_count(&c.counts()->diag_wrap_overlap);
+            r->put("diag", d);
+        };
+        if (!want(2)) { diag(); *out = guard.release(); return TRAX_OK; }
+        ConfigMirror cm(c, cfg->trip, cfg->geom, nt, cfg->d_eta_cut);
+        std::unique_ptr<Pairing> pairs(PairGeneratorBeamspot(c).run(dh, cm, nt, cfg->trip, grid));
+        r->put("pairs", pairs->pairing.download(c));
+        r->put("pair_offsets", pairs->pairingOffsets.download(c));
+        lap("pairs");
+        if (!want(3)) { diag(); *out = guard.release(); return TRAX_OK; }
+        std::unique_ptr<Pairing> cands(TripletThetaPhiPredictor(c).run(dh, DetectorGeometry(), cm, Dictionary(), (int)nt,
+                                                                      cfg->trip, grid, *pairs));
+        r->put("cands", cands->pairing.download(c));
+        r->put("cand_offsets", cands->pairingOffsets.download(c));
+        lap("predict");
+        if (!want(4)) { diag(); *out = guard.release(); return TRAX_OK; }
+        std::unique_ptr<TrackletCollection> tr(
+            TripletThetaPhiFilter(c).run(dh, grid, *pairs, *cands, (int)nt, cm, cfg->trip, true));
+        pairs.reset();  // TripletBuilder.cpp:357-358
+        cands.reset();
+        r->put("tracklets.h1", tr->h1); r->put("tracklets.h2", tr->h2); r->put("tracklets.h3", tr->h3);
+        r->put("tracklet_offsets", tr->trackletOffsets.download(c));
+        lap("filter");
+        diag();
+        if (!want(5)) { *out = guard.release(); return TRAX_OK; }
+        ConnectivityOut co = TripletConnectivityTight(c).run(dh, *tr, cm, cfg->trip, cfg->d_eta_cut, nt);
+        if (co.eta) r->putf("eta", co.eta->download(c));
+        else r->putf("eta", {});
+        auto dl = [&](const std::unique_ptr<UVec>& v, uint32_t n) {
+            std::vector<uint32_t> a = v ? v->download(c) : std::vector<uint32_t>();
+            a.resize(std::min<size_t>(a.size(), n));
+            return a;
+        };
+        r->put("conn.basis", dl(co.basis, co.nConn));
+        r->put("conn.followers", dl(co.followers, co.nConn));
+        r->put("conn.event", dl(co.event, co.nConn));
+        r->put("conn.connectable", dl(co.connectable, co.nConnectable));
+        r->put("conn.nonconnectable", dl(co.nonconnectable, co.nNonConnectable));
+        lap("connectivity");
+        if (co.nConn == 0) { *out = guard.release(); return TRAX_OK; }  // TripletBuilder.cpp:414-422
+        std::unique_ptr<FVec> pt(TrackletCircleFitter(c).run(dh, *tr, *co.connectable, co.nConnectable, nt));
+        lap("circle");
+        CAOut ca = CellularAutomaton(c).run(*co.basis, *co.followers, co.nConn, *pt, tr->size(), nt);
+        r->putf("pt", pt->download(c));
+        r->put("ca.states", ca.states->download(c));
+        r->put("ca.best_basis", ca.bestBasis->download(c));
+        r->put("tracks", dl(ca.trackCollection, ca.nTrackEntries));
+        r->put("track_offsets", ca.trackOffsets->download(c));
+        r->put("ca.iterations", {ca.iterations});
+        lap("ca");
+        *out = guard.release();
+        return TRAX_OK;
+    });
+}
+
+// ---- batched pipeline -------------------------------------------------------------
+
+namespace {
+
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t used = 0, cap = 0;
+    template <class T>
+    T* take(uint64_t n) {
+        size_t bytes = (size_t)std::max<uint64_t>(n, 1) * sizeof(T);
+        bytes = (bytes + 255) & ~(size_t)255;
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        if (used > cap) throw CapacityError("arena overflow");
+        return p;
+    }
+};
+
+const char* kStages[] = {"grid", "pairs", "predict", "filter", "connectivity", "ca"};
+constexpr int kNumStages = 6;
+
+}  // namespace
+
+struct trax_batch {
+    trax_ctx* owner = nullptr;
+    hipStream_t st = nullptr;
+    trax_config cfg;
+    trax_batch_caps caps{};
+    LaunchCfg lc;
+    uint32_t E = 0, L = 0, T = 0, C = 0, slot_cap = 0, scan_cap = 0, ca_iter = 0;
+    std::unique_ptr<ConfigMirror> cm;
+    DevCounts* cnt = nullptr;
+    ScanScratch scan;
+    Arena arena;
+    uint8_t* arena_mem = nullptr;
+    uint32_t* staging = nullptr;  // pinned: 7 hit columns, supplements, shape
+    size_t staging_words = 0;
+    DevCounts* h_counts = nullptr;  // pinned copy after a run
+    // device buffers
+    HitColumns hin, hout;
+    uint32_t *sup = nullptr, *grid = nullptr, *seg_base = nullptr, *slot_counts = nullptr, *pairs = nullptr,
+             *pair_off = nullptr, *pred_counts = nullptr, *cands = nullptr, *cand_off = nullptr,
+             *word_counts = nullptr, *th1 = nullptr, *th2 = nullptr, *th3 = nullptr, *t_off = nullptr;
+    uint64_t* bitmap = nullptr;
+    LinkBufs lb;
+    CABufs cb;
+    hipGraph_t graphs[kNumStages] = {};
+    hipGraphExec_t execs[kNumStages] = {};
+    bool captured = false;
+    hipEvent_t ev[kNumStages + 1] = {};
+    uint32_t n_events = 0, n_hits = 0;
+    bool submitted = false, have_counts = false;
+
+    void enqueue_stage(int s);
+    void enqueue_ca_tail();
+};
+
+void trax_batch::enqueue_stage(int s) {
+    const GridGeom g = cfg.grid.geom();
+    const TripletCols tc = cm->cols();
+    const uint32_t* eo = sup;
+    const uint32_t* ln = sup + E;
+    const uint32_t* lo = sup + E + (size_t)E * L;
+    const uint32_t nseg = E * T;
+    const uint32_t P = caps.max_pairs, Q = caps.max_cands, R = caps.max_tracklets, K = caps.max_conn,
+                   N = caps.max_hits;
+    auto ok = [](hipError_t e, const char* w) { hip_check(e, w); };
+    switch (s) {
+        case 0:
+            ok(launch_fill(reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords, 0u,
+                           sizeof(DevCounts) / 4 - kCountsShapeWords, st), "reset");
+            ok(launch_grid_count(g, E * L, eo, ln, lo, hin.x, hin.y, hin.z, grid, cnt, lc, st), "grid count");
+            ok(launch_grid_store(g, E * L, eo, ln, lo, grid, hin, hout, N, cnt, lc, st), "grid store");
+            break;
+        case 1:
+            ok(launch_pair_plan(g, tc, nseg, grid, seg_base, cnt, st), "pair plan");
+            ok(launch_scan(seg_base, &cnt->n_segments, 1u, nseg + 1ull, &cnt->n_pair_slots, scan, st), "scan");
+            ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, seg_base, slot_counts, cnt, lc, st),
+               "pair count");
+            ok(launch_scan(slot_counts, &cnt->n_pair_slots, 1u, slot_cap + 1ull, &cnt->n_pairs, scan, st), "scan");
+            ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, seg_base, slot_counts, pairs,
+                                 pair_off, cnt, lc, st),
+               "pair store");
+            break;
+        case 2:
+            ok(launch_predict_count(g, tc, P, grid, hout.x, hout.y, hout.z, hout.layer, hout.evt, pairs, pred_counts,
+                                    cnt, lc, st),
+               "predict count");
+            ok(launch_scan(pred_counts, &cnt->n_pairs, 1u, P + 1ull, &cnt->n_cands, scan, st), "scan");
+            ok(launch_fill(cand_off, 0u, nseg + 1ull, st), "fill");
+            ok(launch_predict_store(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.layer, hout.evt, pairs,
+                                    pred_counts, cands, cand_off, cnt, lc, st),
+               "predict store");
+            break;
+        case 3:
+            ok(launch_filter_count(tc, Q, hout.x, hout.y, hout.z, hout.layer, pairs, cands, bitmap, word_counts, cnt,
+                                   lc, st),
+               "filter count");
+            ok(launch_scan(word_counts, &cnt->n_filter_words, 1u, Q / 64 + 2ull, &cnt->n_tracklets, scan, st), "scan");
+            ok(launch_fill(t_off, 0u, nseg + 1ull, st), "fill");
+            ok(launch_filter_store(tc, Q, R, hout.layer, hout.evt, pairs, cands, bitmap, word_counts, th1, th2, th3,
+                                   t_off, cnt, lc, st),
+               "filter store");
+            break;
+        case 4:
+            ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
+            ok(launch_link_count(tc, R, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link count");
+            ok(launch_scan(lb.counts, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_conn, scan, st), "scan");
+            ok(launch_link_store(tc, R, K, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link store");
+            ok(launch_link_flags(R, lb, cnt, lc, st), "link flags");
+            ok(launch_scan(lb.flag_a, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_connectable, scan, st), "scan");
+            ok(launch_scan(lb.flag_b, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_nonconnectable, scan, st), "scan");
+            ok(launch_link_compact(R, lb, cnt, lc, st), "link compact");
+            ok(launch_circle_fit(R, hout.x, hout.y, th1, th2, th3, lb, cnt, lc, st), "circle");
+            break;
+        case 5:
+            ok(launch_ca_init(R, cb, cnt, st), "ca init");
+            ok(launch_ca_iterations(K, lb.basis, lb.foll, cb, cnt, 0, ca_iter, lc, st), "ca iterations");
+            enqueue_ca_tail();
+            break;
+    }
+}
+
+void trax_batch::enqueue_ca_tail() {
+    const uint32_t R = caps.max_tracklets, K = caps.max_conn;
+    hip_check(launch_ca_basis_count(R, K, lb.basis, lb.foll, cb, cnt, lc, st), "ca basis count");
+    hip_check(launch_scan(cb.fbc, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_basis, scan, st), "scan");
+    hip_check(launch_ca_basis_store(R, K, K, lb.basis, lb.foll, lb.pt, cb, cnt, lc, st), "ca basis store");
+    hip_check(launch_scan(cb.handler, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_track_entries, scan, st), "scan");
+    hip_check(launch_ca_tracks(R, caps.max_track_entries, cb, cnt, scan, lc, st, 0), "ca tracks");
+}
+
+extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_caps* caps,
+                                 const trax_tuning* tuning, trax_stream* s, trax_batch** out) {
+    return guarded([&] {
+        validate_config(cfg);
+        hip_check(hipSetDevice(ctx->c->device()), "hipSetDevice");
+        std::unique_ptr<trax_batch> b(new trax_batch);
+        b->owner = ctx;
+        b->st = pick(ctx, s);
+        b->cfg = *cfg;
+        b->caps = *caps;
+        if (tuning) {
+            b->lc.grid_count_block = tuning->grid_count_block;
+            b->lc.grid_store_block = tuning->grid_store_block;
+            b->lc.pair_block = tuning->pair_block;
+            b->lc.predict_block = tuning->predict_block;
+            b->lc.filter_block = tuning->filter_block;
+            b->lc.link_block = tuning->link_block;
+            b->lc.blocks_per_cu = tuning->blocks_per_cu;
+        }
+        auto& B = *b;
+        B.E = caps->max_events;
+        B.L = cfg->trip.maxLayer();
+        B.T = cfg->trip.size();
+        B.cfg.grid.nEvents = B.E;
+        B.cfg.grid.nLayers = B.L;
+        B.C = B.cfg.grid.cells();
+        B.ca_iter = caps->ca_iterations ? caps->ca_iterations : B.T + 2;
+        const uint64_t E = B.E, L = B.L, T = B.T, C = B.C, N = caps->max_hits, P = caps->max_pairs,
+                       Q = caps->max_cands, R = caps->max_tracklets, K = caps->max_conn, TE = caps->max_track_entries;
+        const uint64_t slot_cap = N + E * T * cfg->n_threads;
+        if (slot_cap >= 0xffffffffull || E * L * C >= 0xffffffffull) throw InputError("capacities too large");
+        B.slot_cap = (uint32_t)slot_cap;
+        B.cm.reset(new ConfigMirror(*ctx->c, cfg->trip, cfg->geom, cfg->n_threads, cfg->d_eta_cut));
+        // one arena for every buffer of the pipeline
+        const uint64_t words = 14 * N + E + 2 * E * L + E * L * C + (E * T + 1) * 4 + slot_cap + 1 + 2 * P + P + 1 +
+                               2 * Q + 2 * (Q / 64 + 2) + 3 * R + R + 1 + N + 1 + N + 9 * (R + 1) + 4 * K + 2 * K + 4 * R +
+                               TE + 3 + 64;
+        B.arena.cap = words * 4 + 256 * 96;
+        hip_check(hipMalloc(&B.arena_mem, B.arena.cap), "hipMalloc(arena)");
+        B.arena.base = B.arena_mem;
+        Arena& a = B.arena;
+        auto cols = [&](HitColumns& c) {
+            c.x = a.take<float>(N); c.y = a.take<float>(N); c.z = a.take<float>(N);
+            c.layer = a.take<uint32_t>(N); c.det = a.take<uint32_t>(N); c.id = a.take<uint32_t>(N);
+            c.evt = a.take<uint32_t>(N);
+        };
+        cols(B.hin);
+        cols(B.hout);
+        B.sup = a.take<uint32_t>(E + 2 * E * L);
+        B.grid = a.take<uint32_t>(E * L * C);
+        B.seg_base = a.take<uint32_t>(E * T + 1);
+        B.slot_counts = a.take<uint32_t>(slot_cap + 1);
+        B.pairs = a.take<uint32_t>(2 * P);
+        B.pair_off = a.take<uint32_t>(E * T + 1);
+        B.pred_counts = a.take<uint32_t>(P + 1);
+        B.cands = a.take<uint32_t>(2 * Q);
+        B.cand_off = a.take<uint32_t>(E * T + 1);
+        B.bitmap = a.take<uint64_t>(Q / 64 + 2);
+        B.word_counts = a.take<uint32_t>(Q / 64 + 2);
+        B.th1 = a.take<uint32_t>(R); B.th2 = a.take<uint32_t>(R); B.th3 = a.take<uint32_t>(R);
+        B.t_off = a.take<uint32_t>(E * T + 1);
+        LinkBufs& lb = B.lb;
+        lb.eta = a.take<float>(R);
+        lb.hit_count = a.take<uint32_t>(N + 1);
+        lb.hit_cursor = a.take<uint32_t>(N);
+        lb.csr = a.take<uint32_t>(R);
+        lb.conn = a.take<uint32_t>(R + 1);
+        lb.counts = a.take<uint32_t>(R + 1);
+        lb.basis = a.take<uint32_t>(K); lb.foll = a.take<uint32_t>(K); lb.pev = a.take<uint32_t>(K);
+        lb.flag_a = a.take<uint32_t>(R + 1); lb.flag_b = a.take<uint32_t>(R + 1);
+        lb.connectable = a.take<uint32_t>(R); lb.nonconnectable = a.take<uint32_t>(R);
+        lb.pt = a.take<float>(R);
+        CABufs& cb = B.cb;
+        cb.st_a = a.take<uint32_t>(R); cb.st_b = a.take<uint32_t>(R);
+        cb.fbc = a.take<uint32_t>(R + 1); cb.cursor = a.take<uint32_t>(R);
+        cb.bidx = a.take<uint32_t>(K); cb.bdiff = a.take<float>(K);
+        cb.best = a.take<uint32_t>(R); cb.isbest = a.take<uint32_t>(R);
+        cb.handler = a.take<uint32_t>(R + 1);
+        cb.tracks = a.take<uint32_t>(TE);
+        cb.flags = a.take<uint32_t>(3);
+        B.scan_cap = (uint32_t)std::max<uint64_t>({N + 1, slot_cap + 1, P + 1, Q / 64 + 2, R + 1, E * T + 1});
+        const uint32_t tiles = scan_tiles(B.scan_cap);
+        hip_check(hipMalloc(&B.scan.status, (size_t)(tiles + 1) * sizeof(uint64_t)), "hipMalloc(scan)");
+        B.scan.max_tiles = tiles;
+        hip_check(hipMalloc(&B.cnt, sizeof(DevCounts)), "hipMalloc(counts)");
+        DevCounts init{};
+        init.cap_slots = (uint32_t)slot_cap; init.cap_pairs = (uint32_t)P; init.cap_cands = (uint32_t)Q;
+        init.cap_tracklets = (uint32_t)R; init.cap_conn = (uint32_t)K; init.cap_basis = (uint32_t)K;
+        init.cap_tracks = (uint32_t)TE;
+        hip_check(hipMemcpy(B.cnt, &init, sizeof(DevCounts), hipMemcpyHostToDevice), "h2d counts");
+        B.staging_words = 7 * N + E + 2 * E * L + 2;
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&B.staging), B.staging_words * 4, hipHostMallocDefault),
+                  "hipHostMalloc");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&B.h_counts), sizeof(DevCounts), hipHostMallocDefault),
+                  "hipHostMalloc");
+        for (auto& e : B.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        (void)stride_blocks(1, 256, B.lc);  // resolve the CU count before any capture
+        *out = b.release();
+        return TRAX_OK;
+    });
+}
+
+extern "C" void trax_batch_destroy(trax_batch* b) {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->st);
+    for (int i = 0; i < kNumStages; ++i) {
+        if (b->execs[i]) (void)hipGraphExecDestroy(b->execs[i]);
+        if (b->graphs[i]) (void)hipGraphDestroy(b->graphs[i]);
+    }
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->arena_mem) (void)hipFree(b->arena_mem);
+    if (b->scan.status) (void)hipFree(b->scan.status);
+    if (b->cnt) (void)hipFree(b->cnt);
+    if (b->staging) (void)hipHostFree(b->staging);
+    if (b->h_counts) (void)hipHostFree(b->h_counts);
+    delete b;
+}
+
+extern "C" uint64_t trax_batch_footprint(const trax_batch* b) {
+    return b->arena.cap + (uint64_t)(b->scan.max_tiles + 1) * 8 + sizeof(DevCounts);
+}
+
+extern "C" int trax_batch_submit(trax_batch* b, const trax_hits* h) {
+    return guarded([&] {
+        const uint32_t E = h->n_events, N = h->hits.size(), L = b->L;
+        if (E == 0) throw InputError("empty hit collection");
+        if (E > b->E || N > b->caps.max_hits) throw CapacityError("batch exceeds the event or hit capacity");
+        if (h->max_layer != L) throw InputError("hit collection layers differ from the configuration");
+        uint32_t running = 0;
+        for (uint32_t e = 0; e < E; ++e) {
+            if (h->evt_offset[e] != running) throw InputError("event offsets are not running sums");
+            running += h->evt_nhits[e];
+        }
+        hip_check(hipSetDevice(b->owner->c->device()), "hipSetDevice");
+        hip_check(hipStreamSynchronize(b->st), "hipStreamSynchronize");  // the staging buffer is reused
+        const uint32_t Ecap = b->E, Ncap = b->caps.max_hits;
+        uint32_t* s = b->staging;
+        const HitCollection& c = h->hits;
+        std::memcpy(s + 0 * (size_t)Ncap, c.x.data(), N * 4ull);
+        std::memcpy(s + 1 * (size_t)Ncap, c.y.data(), N * 4ull);
+        std::memcpy(s + 2 * (size_t)Ncap, c.z.data(), N * 4ull);
+        std::memcpy(s + 3 * (size_t)Ncap, c.layer.data(), N * 4ull);
+        std::memcpy(s + 4 * (size_t)Ncap, c.detId.data(), N * 4ull);
+        std::memcpy(s + 5 * (size_t)Ncap, c.hitId.data(), N * 4ull);
+        std::memcpy(s + 6 * (size_t)Ncap, c.event.data(), N * 4ull);
+        uint32_t* sp = s + 7 * (size_t)Ncap;
+        std::memcpy(sp, h->evt_offset.data(), E * 4ull);
+        std::memcpy(sp + Ecap, h->lay_nhits.data(), (size_t)E * L * 4);
+        std::memcpy(sp + Ecap + (size_t)Ecap * L, h->lay_offset.data(), (size_t)E * L * 4);
+        uint32_t* shape = sp + Ecap + 2 * (size_t)Ecap * L;
+        shape[0] = E;
+        shape[1] = N;
+        hipStream_t st = b->st;
+        float* dcols[3] = {b->hin.x, b->hin.y, b->hin.z};
+        uint32_t* ucols[4] = {b->hin.layer, b->hin.det, b->hin.id, b->hin.evt};
+        for (int k = 0; k < 3; ++k)
+            hip_check(hipMemcpyAsync(dcols[k], s + k * (size_t)Ncap, N * 4ull, hipMemcpyHostToDevice, st), "h2d");
+        for (int k = 0; k < 4; ++k)
+            hip_check(hipMemcpyAsync(ucols[k], s + (3 + k) * (size_t)Ncap, N * 4ull, hipMemcpyHostToDevice, st), "h2d");
+        hip_check(hipMemcpyAsync(b->sup, sp, (Ecap + 2 * (size_t)Ecap * L) * 4, hipMemcpyHostToDevice, st), "h2d");
+        hip_check(hipMemcpyAsync(b->cnt, shape, 8, hipMemcpyHostToDevice, st), "h2d");
+        if (b->caps.use_graph && !b->captured) {
+            for (int k = 0; k < kNumStages; ++k) {
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+                b->enqueue_stage(k);
+                hip_check(hipStreamEndCapture(st, &b->graphs[k]), "end capture");
+                hip_check(hipGraphInstantiate(&b->execs[k], b->graphs[k], nullptr, nullptr, 0), "instantiate");
+            }
+            b->captured = true;
+        }
+        for (int k = 0; k < kNumStages; ++k) {
+            hip_check(hipEventRecord(b->ev[k], st), "event");
+            if (b->caps.use_graph) hip_check(hipGraphLaunch(b->execs[k], st), "graph launch");
+            else b->enqueue_stage(k);
+        }
+        hip_check(hipEventRecord(b->ev[kNumStages], st), "event");
+        hip_check(hipMemcpyAsync(b->h_counts, b->cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
+        b->n_events = E;
+        b->n_hits = N;
+        b->submitted = true;
+        b->have_counts = false;
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_batch_wait(trax_batch* b) {
+    return guarded([&] {
+        if (!b->submitted) return fail(TRAX_ERR_STATE, "nothing submitted");
+        hip_check(hipStreamSynchronize(b->st), "hipStreamSynchronize");
+        // the CA forward phase ran a fixed number of iterations; continue on the
+        // host if the last one still found living cells
+        uint32_t flags[3];
+        hip_check(hipMemcpy(flags, b->cb.flags, sizeof(flags), hipMemcpyDeviceToHost), "d2h");
+        uint32_t k = b->ca_iter;
+        if (b->h_counts->n_conn && flags[(k - 1) % 3]) {
+            while (true) {
+                hip_check(launch_ca_iterations(b->caps.max_conn, b->lb.basis, b->lb.foll, b->cb, b->cnt, k, 1, b->lc,
+                                               b->st),
+                          "ca iteration");
+                hip_check(hipMemcpyAsync(flags, b->cb.flags, sizeof(flags), hipMemcpyDeviceToHost, b->st), "d2h");
+                hip_check(hipStreamSynchronize(b->st), "sync");
+                if (!flags[k % 3]) break;
+                ++k;
+            }
+            b->enqueue_ca_tail();
+            hip_check(hipMemcpyAsync(b->h_counts, b->cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, b->st), "d2h");
+            hip_check(hipStreamSynchronize(b->st), "sync");
+        }
+        b->have_counts = true;
+        const uint32_t err = b->h_counts->err;
+        if (err & kErrCapacity) return fail(TRAX_ERR_CAPACITY, "an output exceeded its capacity");
+        if (err & kErrCellRange) return fail(TRAX_ERR_INPUT, "a hit lies outside the z-phi grid");
+        if (err) return fail(TRAX_ERR_INPUT, "device reported an input error");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_batch_counts(trax_batch* b, uint32_t out[8]) {
+    if (!b->have_counts) return fail(TRAX_ERR_STATE, "no completed run");
+    const DevCounts& c = *b->h_counts;
+    out[0] = c.n_pairs; out[1] = c.n_cands; out[2] = c.n_tracklets; out[3] = c.n_conn;
+    out[4] = c.n_track_entries; out[5] = c.ca_iterations; out[6] = c.n_connectable; out[7] = c.err;
+    return TRAX_OK;
+}
+
+extern "C" int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms) {
+    return guarded([&] {
+        for (int k = 0; k < kNumStages; ++k)
+            if (std::strcmp(stage, kStages[k]) == 0) {
+                hip_check(hipEventElapsedTime(ms, b->ev[k], b->ev[k + 1]), "hipEventElapsedTime");
+                return TRAX_OK;
+            }
+        if (std::strcmp(stage, "total") == 0) {
+            hip_check(hipEventElapsedTime(ms, b->ev[0], b->ev[kNumStages]), "hipEventElapsedTime");
+            return TRAX_OK;
+        }
+        return fail(TRAX_ERR_ARG, "unknown stage");
+    });
+}
+
+extern "C" int trax_batch_result(trax_batch* b, trax_result** out) {
+    return guarded([&] {
+        if (!b->have_counts) return fail(TRAX_ERR_STATE, "no completed run");
+        const DevCounts& c = *b->h_counts;
+        auto* r = new trax_result;
+        std::unique_ptr<trax_result> guard(r);
+        const uint32_t E = b->n_events, N = b->n_hits, T = b->T;
+        auto dl = [&](const void* src, uint64_t n) {
+            std::vector<uint32_t> v(n);
+            if (n) hip_check(hipMemcpy(v.data(), src, n * 4, hipMemcpyDeviceToHost), "d2h");
+            return v;
+        };
+        auto dlf = [&](const void* src, uint64_t n) {
+            std::vector<float> v(n);
+            if (n) hip_check(hipMemcpy(v.data(), src, n * 4, hipMemcpyDeviceToHost), "d2h");
+            return v;
+        };
+        const uint32_t P = std::min(c.n_pairs, b->caps.max_pairs), Q = std::min(c.n_cands, b->caps.max_cands),
+                       R = std::min(c.n_tracklets, b->caps.max_tracklets), K = std::min(c.n_conn, b->caps.max_conn);
+        r->put("grid", dl(b->grid, (uint64_t)E * b->L * b->C));
+        r->putf("hits.x", dlf(b->hout.x, N)); r->putf("hits.y", dlf(b->hout.y, N)); r->putf("hits.z", dlf(b->hout.z, N));
+        r->put("hits.layer", dl(b->hout.layer, N)); r->put("hits.det_id", dl(b->hout.det, N));
+        r->put("hits.hit_id", dl(b->hout.id, N)); r->put("hits.event", dl(b->hout.evt, N));
+        r->put("pairs", dl(b->pairs, 2ull * P));
+        r->put("pair_offsets", dl(b->pair_off, (uint64_t)E * T + 1));
+        r->put("cands", dl(b->cands, 2ull * Q));
+        r->put("cand_offsets", dl(b->cand_off, (uint64_t)E * T + 1));
+        r->put("tracklets.h1", dl(b->th1, R)); r->put("tracklets.h2", dl(b->th2, R)); r->put("tracklets.h3", dl(b->th3, R));
+        r->put("tracklet_offsets", dl(b->t_off, (uint64_t)E * T + 1));
+        r->putf("eta", dlf(b->lb.eta, R));
+        r->put("conn.basis", dl(b->lb.basis, K));
+        r->put("conn.followers", dl(b->lb.foll, K));
+        r->put("conn.event", dl(b->lb.pev, K));
+        r->put("conn.connectable", dl(b->lb.connectable, K ? std::min(c.n_connectable, R) : 0));
+        r->put("conn.nonconnectable", dl(b->lb.nonconnectable, K ? std::min(c.n_nonconnectable, R) : 0));
+        if (K) {
+            r->putf("pt", dlf(b->lb.pt, R));
+            r->put("ca.states", dl(b->cb.st_a, R));
+            r->put("ca.best_basis", dl(b->cb.best, R));
+            r->put("tracks", dl(b->cb.tracks, std::min(c.n_track_entries, b->caps.max_track_entries)));
+            r->put("track_offsets", dl(b->cb.handler, R + 1ull));
+            r->put("ca.iterations", {c.ca_iterations});
+        }
+        r->put("diag", {c.diag_pair_short, c.diag_pred_short, c.diag_pred_trunc, c.diag_wrap_overlap});
+        *out = guard.release();
+        return TRAX_OK;
+    });
+}
+
+// ---- results --------------------------------------------------------------------
+
+extern "C" int trax_result_get(const trax_result* r, const char* name, const void** data, uint64_t* count,
+                               int* is_float) {
+    auto it = r->arrays.find(name);
+    if (it == r->arrays.end()) return fail(TRAX_ERR_ARG, std::string("no result array ") + name);
+    *is_float = it->second.is_float ? 1 : 0;
+    if (it->second.is_float) {
+        *data = it->second.f.data();
+        *count = it->second.f.size();
+    } else {
+        *data = it->second.u.data();
+        *count = it->second.u.size();
+    }
+    return TRAX_OK;
+}
+
+extern "C" double trax_result_timing_ms(const trax_result* r, const char* stage) {
+    auto it = r->timings.find(stage);
+    return it == r->timings.end() ? -1.0 : it->second;
+}
+
+extern "C" void trax_result_destroy(trax_result* r) { delete r; }
+
+// ---- math probes ----------------------------------------------------------------
+
+namespace {
+__host__ __device__ inline float math_fn(int fn, float x, float y) {
+    switch (fn) {
+        case 0: return tx_atan2f(y, x);
+        case 1: return tx_acosf(x);
+        case 2: return tx_atanf(x);
+        default: return tx_asinhf(x);
+    }
+}
+__global__ void k_math(int fn, const float* x, const float* y, float* out, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = math_fn(fn, x[i], y ? y[i] : 0.0f);
+}
+}  // namespace
+
+extern "C" int trax_math_eval_host(int fn, const float* x, const float* y, float* out, uint64_t n) {
+    if (fn < 0 || fn > 3) return fail(TRAX_ERR_ARG, "unknown function");
+    for (uint64_t i = 0; i < n; ++i) out[i] = math_fn(fn, x[i], y ? y[i] : 0.0f);
+    return TRAX_OK;
+}
+
+extern "C" int trax_math_eval_device(trax_ctx* ctx, int fn, const float* x, const float* y, float* out, uint64_t n) {
+    return guarded([&] {
+        if (fn < 0 || fn > 3) return fail(TRAX_ERR_ARG, "unknown function");
+        Context& c = *ctx->c;
+        FVec dx(c, n), dy(c, n), dout(c, n);
+        hip_check(hipMemcpy(dx.get_mem(), x, n * 4, hipMemcpyHostToDevice), "h2d");
+        if (y) hip_check(hipMemcpy(dy.get_mem(), y, n * 4, hipMemcpyHostToDevice), "h2d");
+        uint64_t blocks = (n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        if (blocks == 0) blocks = 1;
+        hipLaunchKernelGGL(k_math, dim3((uint32_t)blocks), dim3(256), 0, c.stream(), fn, dx.get_mem(),
+                           y ? dy.get_mem() : nullptr, dout.get_mem(), n);
+        hip_check(hipGetLastError(), "launch");
+        c.finish_default_queue();
+        hip_check(hipMemcpy(out, dout.get_mem(), n * 4, hipMemcpyDeviceToHost), "d2h");
+        return TRAX_OK;
+    });
+}

@@ -1,0 +1,503 @@
+// Host classes of the triplet-seeding path (see trax_host.h).
+#include "trax_host.h"
+
+#include <algorithm>
+#include <cstring>
+#include <set>
+
+namespace trax {
+
+// ---------------------------------------------------------------------------
+Context::Context(int device) : device_(device) {
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    stream_ = own_stream_;
+    hip_check(hipMalloc(&d_cnt_, sizeof(DevCounts)), "hipMalloc(counts)");
+    DevCounts init{};
+    // the plain path sizes every buffer exactly from the device counts
+    init.cap_slots = init.cap_pairs = init.cap_cands = init.cap_tracklets = init.cap_conn = init.cap_basis =
+        init.cap_tracks = 0xffffffffu;
+    hip_check(hipMemcpy(d_cnt_, &init, sizeof(DevCounts), hipMemcpyHostToDevice), "h2d counts");
+    hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 256, hipHostMallocDefault), "hipHostMalloc");
+    int v = 0;
+    hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device), "attr");
+    lds_bytes_ = (uint32_t)v;
+    hip_check(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device), "attr");
+    cu_count_ = (uint32_t)v;
+}
+
+Context::~Context() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (scan_.status) (void)hipFree(scan_.status);
+    if (d_cnt_) (void)hipFree(d_cnt_);
+    if (h_pinned_) (void)hipHostFree(h_pinned_);
+    if (own_stream_) (void)hipStreamDestroy(own_stream_);
+}
+
+const ScanScratch& Context::scan_scratch(uint64_t n_cap) {
+    const uint32_t tiles = scan_tiles(n_cap);
+    if (tiles > scan_.max_tiles) {
+        if (scan_.status) {
+            finish_default_queue();
+            hip_check(hipFree(scan_.status), "hipFree(scan)");
+        }
+        const uint32_t t = std::max<uint32_t>(tiles, 1024);
+        hip_check(hipMalloc(&scan_.status, (size_t)(t + 1) * sizeof(uint64_t)), "hipMalloc(scan)");
+        scan_.max_tiles = t;
+    }
+    return scan_;
+}
+
+uint32_t Context::download_count(const uint32_t* field) {
+    hip_check(hipMemcpyAsync(h_pinned_, field, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "d2h count");
+    finish_default_queue();
+    return h_pinned_[0];
+}
+
+void Context::set_batch_shape(uint32_t n_events, uint32_t n_hits) {
+    h_pinned_[8] = n_events;
+    h_pinned_[9] = n_hits;
+    hip_check(hipMemcpyAsync(d_cnt_, h_pinned_ + 8, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, stream_),
+              "h2d shape");
+    finish_default_queue();
+}
+
+void Context::reset_counts() {
+    hip_check(launch_fill(reinterpret_cast<uint32_t*>(d_cnt_) + kCountsShapeWords, 0u,
+                          sizeof(DevCounts) / 4 - kCountsShapeWords, stream_),
+              "reset counts");
+}
+
+void Context::set_count(uint32_t* field, uint32_t v) {
+    h_pinned_[16] = v;
+    hip_check(hipMemcpyAsync(field, h_pinned_ + 16, sizeof(uint32_t), hipMemcpyHostToDevice, stream_), "h2d count");
+    finish_default_queue();
+}
+
+void Context::check_errors() {
+    const uint32_t e = download_count(&d_cnt_->err);
+    if (e & kErrCellRange) throw InputError("a hit lies outside the z-phi grid");
+    if (e & kErrCapacity) throw CapacityError("an output exceeded its capacity");
+    if (e & kErrIndexRange) throw InputError("an index is out of range");
+}
+
+// ---------------------------------------------------------------------------
+void TripletConfigurations::addWithValue(uint l1, uint l2, uint l3, float dTheta, float sZ, float dPhi,
+                                         float sPhi, float tip, uint spreadZ, float z0_, uint spreadPhi,
+                                         uint nCand) {
+    layer1.push_back(l1); layer2.push_back(l2); layer3.push_back(l3);
+    dThetaCut.push_back(dTheta); sigmaZ.push_back(sZ); dPhiCut.push_back(dPhi); sigmaPhi.push_back(sPhi);
+    tipCut.push_back(tip); pairSpreadZ.push_back(spreadZ); z0.push_back(z0_);
+    pairSpreadPhi.push_back(spreadPhi); nCandidates.push_back(nCand);
+}
+
+float TripletConfigurations::minRadiusCurvature() const {
+    const double Q = 1.602177E-19;
+    const double GEV_C = 5.344286E-19;
+    const float Bz = (float)IDEAL_BZ;  // the float parameter of minRadiusCurvature(float, float)
+    return (float)(minPt_ * GEV_C / (Bz * Q) * 1E2);
+}
+
+uint TripletConfigurations::maxLayer() const {
+    uint m = 0;
+    for (uint l : layer3) m = std::max(m, l);
+    return m;
+}
+
+HitCollection::tTrackList HitCollection::addEvent(const std::vector<PHit>& ev, EventSupplement& eventSupplement,
+                                                  uint evtInGroup, LayerSupplement& layerSupplement,
+                                                  const TripletConfigurations& layerTriplets, int numTracks,
+                                                  bool onlyTracks) {
+    const float minPt = layerTriplets.minPt_;
+    std::set<uint> usedLayers;
+    uint maxLayer = 0;
+    for (uint i = 0; i < layerTriplets.size(); ++i) {
+        usedLayers.insert(layerTriplets.layer1[i]);
+        usedLayers.insert(layerTriplets.layer2[i]);
+        usedLayers.insert(layerTriplets.layer3[i]);
+        maxLayer = std::max(maxLayer, layerTriplets.layer3[i]);
+    }
+    tTrackList allTracks;  // associate hits to tracks (HitCollection.cpp:41-54)
+    for (const PHit& h : ev) {
+        if (onlyTracks && h.simTrackId == 0) continue;
+        if (h.simTrackPt < minPt) continue;
+        allTracks[h.simTrackId].push_back(h);
+    }
+    // sort hits of each track by layer; std::sort leaves equal layers in any
+    // order, the stable sort keeps the event order (one of those outcomes)
+    for (auto& t : allTracks)
+        std::stable_sort(t.second.begin(), t.second.end(),
+                         [](const PHit& a, const PHit& b) { return a.layer < b.layer; });
+    tTrackList layers, findableTracks;
+    int foundTracks = 0;
+    for (auto it = allTracks.begin(); it != allTracks.end() && foundTracks != numTracks; ++it) {
+        if (onlyTracks && it->first == 0) continue;
+        std::vector<bool> covLayers(maxLayer);
+        for (const PHit& h : it->second)
+            if (h.layer >= 1 && h.layer <= maxLayer) covLayers[h.layer - 1] = true;
+        bool missingHits = false;
+        for (uint l : usedLayers)
+            if (!covLayers[l - 1]) missingHits = true;
+        if (it->second[0].simTrackPt < minPt) continue;
+        for (const PHit& h : it->second) {
+            if (!missingHits) findableTracks[it->first].push_back(h);
+            else if (!onlyTracks) layers[h.layer].push_back(h);
+        }
+        if (!missingHits) ++foundTracks;
+    }
+    for (auto& t : findableTracks)
+        for (const PHit& h : t.second)
+            if (h.layer <= maxLayer) layers[h.layer].push_back(h);
+    uint offset = 0;
+    for (uint i = 1; i <= maxLayer; ++i) {
+        layerSupplement.nHits[evtInGroup * maxLayer + i - 1] = (uint)layers[i].size();
+        layerSupplement.offset[evtInGroup * maxLayer + i - 1] = offset;
+        offset += (uint)layers[i].size();
+    }
+    eventSupplement.nHits[evtInGroup] = offset;
+    eventSupplement.offset[evtInGroup] =
+        evtInGroup > 0 ? eventSupplement.offset[evtInGroup - 1] + eventSupplement.nHits[evtInGroup - 1] : 0;
+    for (uint i = 1; i <= maxLayer; ++i)
+        for (const PHit& h : layers[i]) {
+            x.push_back(h.x); y.push_back(h.y); z.push_back(h.z);
+            layer.push_back(h.layer); detId.push_back(h.detectorId);
+            hitId.push_back(h.simTrackId); event.push_back(evtInGroup);
+        }
+    return findableTracks;
+}
+
+GridGeom GridConfig::geom() const {
+    GridGeom g;
+    g.min_z = MIN_Z;
+    g.sector_z = sectorSizeZ();
+    g.n_z = nSectorsZ;
+    g.min_phi = MIN_PHI;
+    g.sector_phi = sectorSizePhi();
+    g.n_phi = nSectorsPhi;
+    g.n_layers = nLayers;
+    g.cells = cells();
+    return g;
+}
+
+ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const GeometrySupplement& g,
+                           uint nThreads, float dEtaCut) {
+    const uint T = t.size(), G = g.size();
+    const size_t words = 10 * (size_t)T + 2 * (size_t)G;
+    std::vector<uint32_t> host(words);
+    auto put_u = [&](size_t at, const std::vector<uint>& v) { std::memcpy(&host[at], v.data(), v.size() * 4); };
+    auto put_f = [&](size_t at, const std::vector<float>& v) { std::memcpy(&host[at], v.data(), v.size() * 4); };
+    put_u(0 * T, t.layer1); put_u(1 * T, t.layer2); put_u(2 * T, t.layer3);
+    put_f(3 * T, t.dThetaCut); put_f(4 * T, t.sigmaZ); put_f(5 * T, t.dPhiCut); put_f(6 * T, t.sigmaPhi);
+    put_f(7 * T, t.tipCut); put_f(8 * T, t.z0); put_u(9 * T, t.pairSpreadPhi);
+    put_f(10 * T, g.minRadius); put_f(10 * T + G, g.maxRadius);
+    blob_.reset(new DeviceVector<uint8_t>(ctx, words * 4 + 4));
+    hip_check(hipMemcpyAsync(blob_->get_mem(), host.data(), words * 4, hipMemcpyHostToDevice, ctx.stream()), "h2d cfg");
+    ctx.finish_default_queue();
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(blob_->get_mem());
+    const float* f = reinterpret_cast<const float*>(d);
+    cols_.l1 = d; cols_.l2 = d + T; cols_.l3 = d + 2 * T;
+    cols_.dtheta = f + 3 * T; cols_.sigma_z = f + 4 * T; cols_.dphi = f + 5 * T; cols_.sigma_phi = f + 6 * T;
+    cols_.tip = f + 7 * T; cols_.z0 = f + 8 * T; cols_.pair_spread_phi = d + 9 * T;
+    cols_.min_r = f + 10 * T; cols_.max_r = f + 10 * T + G;
+    cols_.n_trip = T;
+    cols_.n_geom = G;
+    cols_.min_radius_curvature = t.minRadiusCurvature();
+    cols_.n_threads = nThreads;
+    cols_.d_eta_cut = dEtaCut;
+}
+
+HitMirror::HitMirror(Context& ctx, uint n) : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 7ull * (n ? n : 1))) {}
+
+HitColumns HitMirror::cols() const {
+    HitColumns c;
+    uint32_t* b = buf_->get_mem();
+    const size_t n = n_ ? n_ : 1;
+    c.x = reinterpret_cast<float*>(b); c.y = reinterpret_cast<float*>(b + n); c.z = reinterpret_cast<float*>(b + 2 * n);
+    c.layer = b + 3 * n; c.det = b + 4 * n; c.id = b + 5 * n; c.evt = b + 6 * n;
+    return c;
+}
+
+void HitMirror::upload(Context& ctx, const HitCollection& h) {
+    const HitColumns c = cols();
+    const size_t bytes = (size_t)n_ * 4;
+    if (!n_) return;
+    auto cp = [&](void* dst, const void* src) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx.stream()), "h2d hits");
+    };
+    cp(c.x, h.x.data()); cp(c.y, h.y.data()); cp(c.z, h.z.data());
+    cp(c.layer, h.layer.data()); cp(c.det, h.detId.data()); cp(c.id, h.hitId.data()); cp(c.evt, h.event.data());
+}
+
+void HitMirror::download(Context& ctx, HitCollection& h) const {
+    const HitColumns c = cols();
+    const size_t bytes = (size_t)n_ * 4;
+    h.x.resize(n_); h.y.resize(n_); h.z.resize(n_);
+    h.layer.resize(n_); h.detId.resize(n_); h.hitId.resize(n_); h.event.resize(n_);
+    if (!n_) return;
+    auto cp = [&](void* dst, const void* src) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx.stream()), "d2h hits");
+    };
+    cp(h.x.data(), c.x); cp(h.y.data(), c.y); cp(h.z.data(), c.z);
+    cp(h.layer.data(), c.layer); cp(h.detId.data(), c.det); cp(h.hitId.data(), c.id); cp(h.event.data(), c.evt);
+    ctx.finish_default_queue();
+}
+
+// ---------------------------------------------------------------------------
+void PrefixSum::run(uint32_t* input, uint size, uint nThreads, uint32_t* d_total) {
+    (void)nThreads;  // the single-pass scan does not depend on the work-group size
+    hip_check(launch_scan(input, nullptr, size, size, d_total, ctx.scan_scratch(size), ctx.stream()), "scan");
+}
+
+uint64_t GridBuilder::run(HitCollection& hits, HitMirror& dhits, uint nThreads, const EventSupplement& es,
+                          const LayerSupplement& ls, Grid& grid, bool download) {
+    (void)nThreads;
+    const GridConfig& gc = grid.config;
+    const uint E = gc.nEvents, L = gc.nLayers, N = hits.size();
+    if (es.nEvents != E || ls.nLayers != L) throw InputError("supplement shape does not match the grid");
+    // the grid values are global hit indices: the supplements must be running sums
+    uint running = 0;
+    for (uint e = 0; e < E; ++e) {
+        if (es.offset[e] != running) throw InputError("event offsets are not running sums");
+        uint lr = 0;
+        for (uint l = 0; l < L; ++l) {
+            if (ls.offset[e * L + l] != lr) throw InputError("layer offsets are not running sums");
+            lr += ls.nHits[e * L + l];
+        }
+        if (lr != es.nHits[e]) throw InputError("layer hit counts do not add up to the event's");
+        running += es.nHits[e];
+    }
+    if (running != N) throw InputError("event hit counts do not add up to the collection");
+    std::vector<uint32_t> sup(E + 2 * (size_t)E * L);
+    std::copy(es.offset.begin(), es.offset.end(), sup.begin());
+    std::copy(ls.nHits.begin(), ls.nHits.end(), sup.begin() + E);
+    std::copy(ls.offset.begin(), ls.offset.end(), sup.begin() + E + (size_t)E * L);
+    UVec dsup(ctx, sup.size());
+    dsup.upload(ctx, sup);
+    ctx.set_batch_shape(E, N);
+    const GridGeom g = gc.geom();
+    const uint32_t* d_eo = dsup.get_mem();
+    const uint32_t* d_ln = d_eo + E;
+    const uint32_t* d_lo = d_ln + (size_t)E * L;
+    hip_check(launch_grid_count(g, E * L, d_eo, d_ln, d_lo, dhits.cols().x, dhits.cols().y, dhits.cols().z,
+                                grid.boundary.get_mem(), ctx.counts(), ctx.launch(), ctx.stream()),
+              "grid count");
+    ctx.finish_default_queue();  // GridBuilder.cpp:52
+    HitMirror out(ctx, N);       // hits.transfer.initBuffers (GridBuilder.cpp:70)
+    hip_check(launch_grid_store(g, E * L, d_eo, d_ln, d_lo, grid.boundary.get_mem(), dhits.cols(), out.cols(), N,
+                                ctx.counts(), ctx.launch(), ctx.stream()),
+              "grid store");
+    ctx.finish_default_queue();  // GridBuilder.cpp:146
+    ctx.check_errors();
+    dhits.swap(out);  // the old buffers are released with `out`
+    if (download) {
+        grid.host = grid.boundary.download(ctx);  // GridBuilder.cpp:64
+        dhits.download(ctx, hits);                // GridBuilder.cpp:151
+    }
+    return 0;
+}
+
+Pairing* PairGeneratorBeamspot::run(const HitMirror& hits, const ConfigMirror& cfg, uint nThreads,
+                                    const TripletConfigurations& layerTriplets, const Grid& grid) {
+    (void)nThreads;
+    const uint E = grid.config.nEvents, T = layerTriplets.size();
+    const GridGeom g = grid.config.geom();
+    const TripletCols tc = cfg.cols();
+    DevCounts* cnt = ctx.counts();
+    const uint nseg = E * T;
+    UVec seg_base(ctx, nseg + 1ull);
+    hip_check(launch_pair_plan(g, tc, nseg, grid.boundary.get_mem(), seg_base.get_mem(), cnt, ctx.stream()), "pair plan");
+    PrefixSum(ctx).run(seg_base.get_mem(), nseg + 1, nThreads, &cnt->n_pair_slots);
+    const uint nslot = ctx.download_count(&cnt->n_pair_slots);
+    UVec slot_counts(ctx, nslot + 1ull);
+    const HitColumns hc = hits.cols();
+    hip_check(launch_pair_count(g, tc, nseg, nslot, grid.boundary.get_mem(), hc.x, hc.y, hc.z, seg_base.get_mem(),
+                                slot_counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
+              "pair count");
+    PrefixSum(ctx).run(slot_counts.get_mem(), nslot + 1, nThreads, &cnt->n_pairs);
+    const uint nPairs = ctx.download_count(&cnt->n_pairs);  // PairGeneratorBeamspot.cpp:124
+    Pairing* p = new Pairing(ctx, nPairs, E, T);
+    hip_check(launch_pair_store(g, tc, nseg, nslot, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z,
+                                seg_base.get_mem(), slot_counts.get_mem(), p->pairing.get_mem(),
+                                p->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
+              "pair store");
+    ctx.finish_default_queue();  // the temporaries are released on return
+    return p;
+}
+
+Pairing* TripletThetaPhiPredictor::run(const HitMirror& hits, const DetectorGeometry&, const ConfigMirror& cfg,
+                                       const Dictionary&, int nThreads, const TripletConfigurations& layerTriplets,
+                                       const Grid& grid, const Pairing& pairs) {
+    const uint E = grid.config.nEvents, T = layerTriplets.size();
+    const GridGeom g = grid.config.geom();
+    const TripletCols tc = cfg.cols();
+    DevCounts* cnt = ctx.counts();
+    const uint nPairs = pairs.n;
+    ctx.set_count(&cnt->n_pairs, pairs.n);
+    UVec counts(ctx, nPairs + 1ull);
+    const HitColumns hc = hits.cols();
+    hip_check(launch_predict_count(g, tc, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.layer, hc.evt,
+                                   pairs.pairing.get_mem(), counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
+              "predict count");
+    PrefixSum(ctx).run(counts.get_mem(), nPairs + 1, (uint)nThreads, &cnt->n_cands);
+    const uint nCand = ctx.download_count(&cnt->n_cands);  // TripletThetaPhiPredictor.cpp:68
+    Pairing* c = new Pairing(ctx, nCand, E, T);
+    hip_check(launch_fill(c->pairingOffsets.get_mem(), 0u, c->pairingOffsets.get_count(), ctx.stream()), "fill");
+    hip_check(launch_predict_store(g, tc, nPairs, nCand, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.layer,
+                                   hc.evt, pairs.pairing.get_mem(), counts.get_mem(), c->pairing.get_mem(),
+                                   c->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
+              "predict store");
+    ctx.finish_default_queue();
+    return c;
+}
+
+TrackletCollection* TripletThetaPhiFilter::run(const HitMirror& hits, const Grid& grid, const Pairing& pairs,
+                                               const Pairing& cands, int nThreads, const ConfigMirror& cfg,
+                                               const TripletConfigurations& layerTriplets, bool download) {
+    const uint E = grid.config.nEvents, T = layerTriplets.size();
+    const TripletCols tc = cfg.cols();
+    DevCounts* cnt = ctx.counts();
+    const uint nCand = cands.n;
+    const uint nWords = (nCand + 63) / 64;
+    ctx.set_count(&cnt->n_cands, cands.n);
+    DeviceVector<uint64_t> bitmap(ctx, nWords ? nWords : 1);
+    UVec word_counts(ctx, nWords + 1ull);
+    const HitColumns hc = hits.cols();
+    hip_check(launch_filter_count(tc, nCand, hc.x, hc.y, hc.z, hc.layer, pairs.pairing.get_mem(),
+                                  cands.pairing.get_mem(), bitmap.get_mem(), word_counts.get_mem(), cnt,
+                                  ctx.launch(), ctx.stream()),
+              "filter count");
+    PrefixSum(ctx).run(word_counts.get_mem(), nWords + 1, (uint)nThreads, &cnt->n_tracklets);
+    const uint R = ctx.download_count(&cnt->n_tracklets);  // TripletThetaPhiFilter.cpp:85
+    TrackletCollection* t = new TrackletCollection(ctx, R, E, T);
+    hip_check(launch_fill(t->trackletOffsets.get_mem(), 0u, t->trackletOffsets.get_count(), ctx.stream()), "fill");
+    hip_check(launch_filter_store(tc, nCand, R, hc.layer, hc.evt, pairs.pairing.get_mem(), cands.pairing.get_mem(),
+                                  bitmap.get_mem(), word_counts.get_mem(), t->hit1.get_mem(), t->hit2.get_mem(),
+                                  t->hit3.get_mem(), t->trackletOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
+              "filter store");
+    ctx.finish_default_queue();
+    if (download) {  // TripletThetaPhiFilter.cpp:184
+        t->h1 = t->hit1.download(ctx);
+        t->h2 = t->hit2.download(ctx);
+        t->h3 = t->hit3.download(ctx);
+    }
+    return t;
+}
+
+ConnectivityOut TripletConnectivityTight::run(const HitMirror& hits, TrackletCollection& tr, const ConfigMirror& cfg,
+                                              const TripletConfigurations& layerTriplets, float dEtaCut,
+                                              uint nThreads) {
+    (void)layerTriplets;
+    ConnectivityOut out;
+    TripletCols tc = cfg.cols();
+    tc.d_eta_cut = dEtaCut;
+    DevCounts* cnt = ctx.counts();
+    const uint R = tr.size();
+    const uint N = hits.n_;
+    ctx.set_count(&cnt->n_tracklets, tr.n);
+    if (R == 0) return out;
+    out.eta.reset(new FVec(ctx, R));
+    UVec hit_count(ctx, N + 1ull), hit_cursor(ctx, N ? N : 1), csr(ctx, R), conn(ctx, R + 1ull),
+        counts(ctx, R + 1ull), flag_a(ctx, R + 1ull), flag_b(ctx, R + 1ull);
+    LinkBufs b;
+    b.eta = out.eta->get_mem();
+    b.hit_count = hit_count.get_mem(); b.hit_cursor = hit_cursor.get_mem(); b.csr = csr.get_mem();
+    b.conn = conn.get_mem(); b.counts = counts.get_mem(); b.flag_a = flag_a.get_mem(); b.flag_b = flag_b.get_mem();
+    const HitColumns hc = hits.cols();
+    hip_check(launch_link_prepare(tc, N, R, hc.x, hc.y, hc.z, tr.hit1.get_mem(), tr.hit3.get_mem(), b, cnt,
+                                  ctx.scan_scratch(N + 1ull), ctx.launch(), ctx.stream()),
+              "link prepare");
+    hip_check(launch_link_count(tc, R, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(),
+                                tr.trackletOffsets.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
+              "link count");
+    PrefixSum(ctx).run(counts.get_mem(), R + 1, nThreads, &cnt->n_conn);
+    out.nConn = ctx.download_count(&cnt->n_conn);  // TripletConnectivityTight.cpp:89
+    if (out.nConn == 0) return out;                // .cpp:93-95: null tuple
+    out.basis.reset(new UVec(ctx, out.nConn));
+    out.followers.reset(new UVec(ctx, out.nConn));
+    out.event.reset(new UVec(ctx, out.nConn));
+    b.basis = out.basis->get_mem(); b.foll = out.followers->get_mem(); b.pev = out.event->get_mem();
+    hip_check(launch_link_store(tc, R, out.nConn, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(),
+                                tr.hit3.get_mem(), tr.trackletOffsets.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
+              "link store");
+    hip_check(launch_link_flags(R, b, cnt, ctx.launch(), ctx.stream()), "link flags");
+    PrefixSum(ctx).run(flag_a.get_mem(), R + 1, nThreads, &cnt->n_connectable);
+    out.nConnectable = ctx.download_count(&cnt->n_connectable);  // .cpp:200
+    PrefixSum(ctx).run(flag_b.get_mem(), R + 1, nThreads, &cnt->n_nonconnectable);
+    out.nNonConnectable = ctx.download_count(&cnt->n_nonconnectable);  // .cpp:211
+    out.connectable.reset(new UVec(ctx, std::max<uint>(out.nConnectable, 1)));
+    out.nonconnectable.reset(new UVec(ctx, std::max<uint>(out.nNonConnectable, 1)));
+    b.connectable = out.connectable->get_mem();
+    b.nonconnectable = out.nonconnectable->get_mem();
+    hip_check(launch_link_compact(R, b, cnt, ctx.launch(), ctx.stream()), "link compact");
+    ctx.finish_default_queue();
+    ctx.check_errors();
+    return out;
+}
+
+FVec* TrackletCircleFitter::run(const HitMirror& hits, const TrackletCollection& tr, const UVec& valid, uint nValid,
+                                uint nThreads) {
+    (void)nThreads;
+    DevCounts* cnt = ctx.counts();
+    FVec* pt = new FVec(ctx, tr.size() ? tr.size() : 1);
+    ctx.set_count(&cnt->n_connectable, nValid);
+    LinkBufs b;
+    b.connectable = valid.get_mem();
+    b.pt = pt->get_mem();
+    const HitColumns hc = hits.cols();
+    hip_check(launch_circle_fit(tr.size(), hc.x, hc.y, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(), b,
+                                cnt, ctx.launch(), ctx.stream()),
+              "circle fit");
+    ctx.finish_default_queue();
+    return pt;
+}
+
+CAOut CellularAutomaton::run(const UVec& basis, const UVec& followers, uint nConn, const FVec& pt, uint R,
+                             uint nThreads) {
+    CAOut out;
+    DevCounts* cnt = ctx.counts();
+    ctx.set_count(&cnt->n_tracklets, R);
+    ctx.set_count(&cnt->n_conn, nConn);
+    out.states.reset(new UVec(ctx, R));
+    UVec st_b(ctx, R), flags(ctx, 3), fbc(ctx, R + 1ull), cursor(ctx, R), isbest(ctx, R);
+    out.trackOffsets.reset(new UVec(ctx, R + 1ull));
+    out.bestBasis.reset(new UVec(ctx, R));
+    CABufs b;
+    b.st_a = out.states->get_mem(); b.st_b = st_b.get_mem(); b.flags = flags.get_mem();
+    b.fbc = fbc.get_mem(); b.cursor = cursor.get_mem(); b.best = out.bestBasis->get_mem();
+    b.isbest = isbest.get_mem(); b.handler = out.trackOffsets->get_mem();
+    hip_check(launch_ca_init(R, b, cnt, ctx.stream()), "ca init");
+    // forward phase: one iteration per launch, living cells read back each time
+    // (CellularAutomaton.cpp:40-76)
+    uint k = 0;
+    while (true) {
+        hip_check(launch_ca_iterations(nConn, basis.get_mem(), followers.get_mem(), b, cnt, k, 1, ctx.launch(),
+                                       ctx.stream()),
+                  "ca iteration");
+        const uint alive = ctx.download_count(flags.get_mem() + (k % 3));
+        ++k;
+        if (!alive) break;
+    }
+    out.iterations = ctx.download_count(&cnt->ca_iterations);
+    hip_check(launch_ca_basis_count(R, nConn, basis.get_mem(), followers.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
+              "ca basis count");
+    PrefixSum(ctx).run(fbc.get_mem(), R + 1, nThreads, &cnt->n_basis);
+    const uint nb = ctx.download_count(&cnt->n_basis);  // CellularAutomaton.cpp:138
+    UVec bidx(ctx, nb ? nb : 1);
+    FVec bdiff(ctx, nb ? nb : 1);
+    b.bidx = bidx.get_mem();
+    b.bdiff = bdiff.get_mem();
+    hip_check(launch_ca_basis_store(R, nConn, nb, basis.get_mem(), followers.get_mem(), pt.get_mem(), b, cnt,
+                                    ctx.launch(), ctx.stream()),
+              "ca basis store");
+    PrefixSum(ctx).run(b.handler, R + 1, nThreads, &cnt->n_track_entries);
+    out.nTrackEntries = ctx.download_count(&cnt->n_track_entries);  // CellularAutomaton.cpp:284
+    out.trackCollection.reset(new UVec(ctx, out.nTrackEntries ? out.nTrackEntries : 1));
+    b.tracks = out.trackCollection->get_mem();
+    hip_check(launch_ca_tracks(R, out.nTrackEntries, b, cnt, ctx.scan_scratch(R + 1ull), ctx.launch(), ctx.stream(), 0),
+              "ca tracks");
+    ctx.finish_default_queue();
+    ctx.check_errors();
+    return out;
+}
+
+}  // namespace trax

@@ -1,0 +1,330 @@
+// C++ host classes of the triplet-seeding path, named and shaped after the
+// reference's containers (src/datastructures) and algorithm classes
+// (src/algorithms), running on the gfx950 launchers instead of clever/OpenCL.
+//
+// Ownership follows the reference: algorithm run() calls return heap objects
+// owned by the caller (Pairing*, TrackletCollection*, device vectors); the
+// GridBuilder rewrites the hit columns of the HitCollection in place.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../kernels/kernels.h"
+
+namespace trax {
+
+using uint = uint32_t;
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string& where)
+        : std::runtime_error(where + ": " + hipGetErrorString(c)), code(c) {}
+};
+struct InputError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct CapacityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* where) {
+    if (e != hipSuccess) throw HipError(e, where);
+}
+
+// clever::context: one device, one in-order queue, scan scratch, the
+// device-resident size record and the launch tuning
+class Context {
+public:
+    explicit Context(int device);
+    ~Context();
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+    void set_stream(hipStream_t s) { stream_ = s ? s : own_stream_; }
+    DevCounts* counts() const { return d_cnt_; }
+    const ScanScratch& scan_scratch(uint64_t n_cap);  // grows on demand
+    LaunchCfg& launch() { return lc_; }
+    uint32_t local_mem_size() const { return lds_bytes_; }  // getLocalMemSize
+    uint32_t cu_count() const { return cu_count_; }
+    void finish_default_queue() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+    // transfer::downloadScalar of one DevCounts field (blocking)
+    uint32_t download_count(const uint32_t* field);
+    // host -> device copy of the batch shape into the size record
+    void set_batch_shape(uint32_t n_events, uint32_t n_hits);
+    void reset_counts();  // zero every size / flag of the record except the shape
+    void set_count(uint32_t* field, uint32_t v);  // blocking host -> device of one field
+    void check_errors();  // raise on device-side error bits
+
+private:
+    int device_;
+    hipStream_t own_stream_ = nullptr, stream_ = nullptr;
+    DevCounts* d_cnt_ = nullptr;
+    uint32_t* h_pinned_ = nullptr;
+    ScanScratch scan_;
+    uint32_t lds_bytes_ = 0, cu_count_ = 0;
+    LaunchCfg lc_;
+};
+
+// clever::vector<T, 1>
+template <class T>
+class DeviceVector {
+public:
+    DeviceVector(Context& ctx, uint64_t count) : count_(count) {
+        hip_check(hipMalloc(&ptr_, bytes() ? bytes() : sizeof(T)), "hipMalloc");
+        (void)ctx;
+    }
+    ~DeviceVector() {
+        if (ptr_) (void)hipFree(ptr_);
+    }
+    DeviceVector(const DeviceVector&) = delete;
+    DeviceVector& operator=(const DeviceVector&) = delete;
+    T* get_mem() const { return ptr_; }
+    uint64_t get_count() const { return count_; }
+    uint64_t bytes() const { return count_ * sizeof(T); }
+    std::vector<T> download(Context& ctx) const {
+        std::vector<T> v(count_);
+        if (count_) {
+            hip_check(hipMemcpyAsync(v.data(), ptr_, bytes(), hipMemcpyDeviceToHost, ctx.stream()), "d2h");
+            ctx.finish_default_queue();
+        }
+        return v;
+    }
+    void upload(Context& ctx, const std::vector<T>& v) {
+        if (v.size() && count_)
+            hip_check(hipMemcpyAsync(ptr_, v.data(), std::min<uint64_t>(v.size(), count_) * sizeof(T),
+                                     hipMemcpyHostToDevice, ctx.stream()),
+                      "h2d");
+    }
+
+private:
+    T* ptr_ = nullptr;
+    uint64_t count_;
+};
+
+// -- containers -------------------------------------------------------------
+
+struct PHit {  // PB_Event::PHit (src/datastructures/serialize/Event.proto)
+    float x, y, z;
+    uint32_t layer, detectorId, simTrackId;
+    float simTrackPt;
+};
+
+// TripletConfigurations (src/datastructures/TripletConfiguration.h:39-72)
+class TripletConfigurations {
+public:
+    explicit TripletConfigurations(float minPt) : minPt_(minPt) {}
+    uint size() const { return (uint)layer1.size(); }
+    void addWithValue(uint l1, uint l2, uint l3, float dTheta, float sZ, float dPhi, float sPhi, float tip,
+                      uint spreadZ, float z0_, uint spreadPhi, uint nCand);
+    float minRadiusCurvature() const;  // TripletConfiguration.cpp:36-43
+    uint maxLayer() const;
+    std::vector<uint> layer1, layer2, layer3;
+    std::vector<float> dThetaCut, sigmaZ, dPhiCut, sigmaPhi, tipCut;
+    std::vector<uint> pairSpreadZ;
+    std::vector<float> z0;
+    std::vector<uint> pairSpreadPhi, nCandidates;
+    float minPt_;
+    static constexpr double IDEAL_BZ = 3.8112;
+};
+
+// GeometrySupplement (src/datastructures/GeometrySupplement.h:21-81)
+class GeometrySupplement {
+public:
+    void addWithValue(uint layer, float minR, float maxR) {
+        detectorLayer.push_back(layer); minRadius.push_back(minR); maxRadius.push_back(maxR);
+    }
+    uint size() const { return (uint)minRadius.size(); }
+    std::vector<uint> detectorLayer;
+    std::vector<float> minRadius, maxRadius;
+};
+
+// inert arguments of TripletThetaPhiPredictor::run (never read by its kernels,
+// src/algorithms/TripletThetaPhiPredictor.h:54)
+struct DetectorGeometry {};
+struct Dictionary {};
+
+// EventSupplement / LayerSupplement (EventSupplement.h:14, LayerSupplement.h:13)
+struct EventSupplement {
+    explicit EventSupplement(uint n = 1) : nEvents(n), nHits(n, 0), offset(n, 0) {}
+    uint nEvents;
+    std::vector<uint> nHits, offset;
+};
+struct LayerSupplement {
+    LayerSupplement(uint layers, uint events = 1)
+        : nLayers(layers), nHits((size_t)layers * events, 0), offset((size_t)layers * events, 0) {}
+    uint nLayers;
+    std::vector<uint> nHits, offset;
+};
+
+// HitCollection (src/datastructures/HitCollection.h:28-62)
+class HitCollection {
+public:
+    typedef std::map<uint, std::vector<PHit>> tTrackList;
+    uint size() const { return (uint)x.size(); }
+    // HitCollection.cpp:21-180
+    tTrackList addEvent(const std::vector<PHit>& event, EventSupplement& eventSupplement, uint evtInGroup,
+                        LayerSupplement& layerSupplement, const TripletConfigurations& layerTriplets,
+                        int numTracks = -1, bool onlyTracks = false);
+    std::vector<float> x, y, z;
+    std::vector<uint> layer, detId, hitId, event;
+};
+
+// GridConfig / Grid (src/datastructures/Grid.h:5-131)
+struct GridConfig {
+    uint nEvents = 1, nLayers = 1, nSectorsZ = 1, nSectorsPhi = 1;
+    static constexpr float MIN_Z = -300;
+    static constexpr float MAX_Z = 300;
+    static constexpr float MIN_PHI = -(float)M_PI;
+    static constexpr float MAX_PHI = (float)M_PI;
+    float sectorSizeZ() const { return (MAX_Z - MIN_Z) / nSectorsZ; }
+    float sectorSizePhi() const { return (MAX_PHI - MIN_PHI) / nSectorsPhi; }
+    uint cells() const { return (nSectorsZ + 1) * (nSectorsPhi + 1); }
+    GridGeom geom() const;
+};
+
+// device mirrors of the configuration columns
+struct ConfigMirror {
+    ConfigMirror(Context& ctx, const TripletConfigurations& t, const GeometrySupplement& g, uint nThreads,
+                 float dEtaCut);
+    TripletCols cols() const { return cols_; }
+
+private:
+    std::unique_ptr<DeviceVector<uint8_t>> blob_;
+    TripletCols cols_;
+};
+
+// hit columns on the device (clever::OpenCLTransfer of the HitCollection)
+struct HitMirror {
+    HitMirror(Context& ctx, uint n);
+    void upload(Context& ctx, const HitCollection& h);
+    void download(Context& ctx, HitCollection& h) const;
+    HitColumns cols() const;
+    void swap(HitMirror& o) { buf_.swap(o.buf_); n_ = o.n_; }
+    uint n_;
+    std::unique_ptr<DeviceVector<uint32_t>> buf_;  // 7 columns of n entries
+};
+
+struct Grid {
+    Grid(Context& ctx, const GridConfig& cfg)
+        : config(cfg), boundary(ctx, (uint64_t)cfg.nEvents * cfg.nLayers * cfg.cells()) {}
+    GridConfig config;
+    DeviceVector<uint32_t> boundary;
+    std::vector<uint> host;  // downloaded after the build (GridBuilder.cpp:64)
+};
+
+// Pairing (src/datastructures/Pairings.h:9-59): uint2 pairs + E*T+1 offsets
+struct Pairing {
+    Pairing(Context& ctx, uint nTotalPairings, uint nEvents, uint nLayerTriplets)
+        : pairing(ctx, 2ull * nTotalPairings), pairingOffsets(ctx, (uint64_t)nEvents * nLayerTriplets + 1),
+          n(nTotalPairings) {}
+    DeviceVector<uint32_t> pairing;
+    DeviceVector<uint32_t> pairingOffsets;
+    uint n;
+};
+
+// TrackletCollection (src/datastructures/TrackletCollection.h:36-78)
+struct TrackletCollection {
+    TrackletCollection(Context& ctx, uint items, uint nEvents, uint nLayerTriplets)
+        : hit1(ctx, items), hit2(ctx, items), hit3(ctx, items),
+          trackletOffsets(ctx, (uint64_t)nEvents * nLayerTriplets + 1), n(items) {}
+    uint size() const { return n; }
+    DeviceVector<uint32_t> hit1, hit2, hit3;
+    DeviceVector<uint32_t> trackletOffsets;
+    std::vector<uint> h1, h2, h3;  // host copies (TripletThetaPhiFilter.cpp:184)
+    uint n;
+};
+
+using UVec = DeviceVector<uint32_t>;
+using FVec = DeviceVector<float>;
+
+// -- algorithms -----------------------------------------------------------
+
+class PrefixSum {  // src/algorithms/PrefixSum.h:16-56
+public:
+    explicit PrefixSum(Context& c) : ctx(c) {}
+    // exclusive in-place scan of `size` entries; the sum goes to *d_total
+    void run(uint32_t* input, uint size, uint nThreads, uint32_t* d_total = nullptr);
+    Context& ctx;
+};
+
+class GridBuilder {  // src/algorithms/GridBuilder.h:25-46
+public:
+    explicit GridBuilder(Context& c) : ctx(c) {}
+    uint64_t run(HitCollection& hits, HitMirror& dhits, uint nThreads, const EventSupplement& eventSupplement,
+                 const LayerSupplement& layerSupplement, Grid& grid, bool download = true);
+    Context& ctx;
+};
+
+class PairGeneratorBeamspot {  // src/algorithms/PairGeneratorBeamspot.h:24-41
+public:
+    explicit PairGeneratorBeamspot(Context& c) : ctx(c) {}
+    Pairing* run(const HitMirror& hits, const ConfigMirror& cfg, uint nThreads, const TripletConfigurations& layerTriplets,
+                 const Grid& grid);
+    Context& ctx;
+};
+
+class TripletThetaPhiPredictor {  // src/algorithms/TripletThetaPhiPredictor.h:44-47
+public:
+    explicit TripletThetaPhiPredictor(Context& c) : ctx(c) {}
+    Pairing* run(const HitMirror& hits, const DetectorGeometry& geom, const ConfigMirror& cfg, const Dictionary& dict,
+                 int nThreads, const TripletConfigurations& layerTriplets, const Grid& grid, const Pairing& pairs);
+    Context& ctx;
+};
+
+class TripletThetaPhiFilter {  // src/algorithms/TripletThetaPhiFilter.h:46-48
+public:
+    explicit TripletThetaPhiFilter(Context& c) : ctx(c) {}
+    TrackletCollection* run(const HitMirror& hits, const Grid& grid, const Pairing& pairs,
+                            const Pairing& tripletCandidates, int nThreads, const ConfigMirror& cfg,
+                            const TripletConfigurations& layerTriplets, bool download = true);
+    Context& ctx;
+};
+
+struct ConnectivityOut {
+    std::unique_ptr<UVec> basis, followers, event, connectable, nonconnectable;
+    std::unique_ptr<FVec> eta;
+    uint nConn = 0, nConnectable = 0, nNonConnectable = 0;
+};
+
+class TripletConnectivityTight {  // src/algorithms/TripletConnectivityTight.h:42-44
+public:
+    explicit TripletConnectivityTight(Context& c) : ctx(c) {}
+    // empty basis (null) when no connection exists, as the reference's nullptr tuple
+    ConnectivityOut run(const HitMirror& hits, TrackletCollection& tracklets, const ConfigMirror& cfg,
+                        const TripletConfigurations& layerTriplets, float dEtaCut, uint nThreads);
+    Context& ctx;
+};
+
+class TrackletCircleFitter {  // src/algorithms/TrackletCircleFitter.h:25-29
+public:
+    explicit TrackletCircleFitter(Context& c) : ctx(c) {}
+    FVec* run(const HitMirror& hits, const TrackletCollection& tracklets, const UVec& validTrackletsIndices,
+              uint nValid, uint nThreads);
+    Context& ctx;
+};
+
+struct CAOut {
+    std::unique_ptr<UVec> trackCollection, trackOffsets, states, bestBasis;
+    uint nTrackEntries = 0, iterations = 0;
+};
+
+class CellularAutomaton {  // src/algorithms/CellularAutomaton.h:43-49
+public:
+    explicit CellularAutomaton(Context& c) : ctx(c) {}
+    CAOut run(const UVec& tripletsBasis, const UVec& tripletsFollowers, uint nConn, const FVec& tripletsPt,
+              uint nTriplets, uint nThreads);
+    Context& ctx;
+};
+
+}  // namespace trax

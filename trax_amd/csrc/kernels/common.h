@@ -1,0 +1,167 @@
+// Device-side helpers shared by the gfx950 kernels (wave64 primitives, LDS
+// block scans) and the device-resident size record every stage reads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../trax_math.h"
+
+namespace trax {
+
+constexpr uint32_t kSentinel = 0xffffffffu;
+constexpr int kWave = 64;
+
+// error bits set on the device (DevCounts::err)
+constexpr uint32_t kErrCellRange = 1u;     // a hit maps outside the z-phi grid
+constexpr uint32_t kErrCapacity = 2u;      // an output exceeded its preallocated capacity
+constexpr uint32_t kErrIndexRange = 4u;    // an index read from an input is out of range
+
+// Device-resident sizes of one pipeline run.  Stage kernels read their work
+// size from here, so that one launch sequence (and one captured graph) serves
+// any batch within the preallocated capacities.
+struct DevCounts {
+    uint32_t n_events;       // batch shape, written by the host before a run
+    uint32_t n_hits;
+    // capacities of the output buffers (0xffffffff: sized exactly by the host)
+    uint32_t cap_slots, cap_pairs, cap_cands, cap_tracklets, cap_conn, cap_basis, cap_tracks;
+    // sizes and flags written by the kernels, cleared before every run
+    uint32_t n_segments;     // events * layer triplets
+    uint32_t n_filter_words; // ceil(candidates / 64)
+    uint32_t n_pair_slots;   // pair count slots (nThreads-major layout)
+    uint32_t n_pairs;        // P
+    uint32_t n_cands;        // Q
+    uint32_t n_tracklets;    // R
+    uint32_t n_conn;         // K, (basis, follower) connections
+    uint32_t n_connectable;
+    uint32_t n_nonconnectable;
+    uint32_t n_basis;        // CA backward: basis entries
+    uint32_t n_track_entries;
+    uint32_t alive;          // CA living cells of the last iteration
+    uint32_t ca_iterations;
+    uint32_t err;
+    uint32_t diag_pair_short;   // pair slots counted but not written
+    uint32_t diag_pred_short;   // candidate slots counted but not written
+    uint32_t diag_pred_trunc;   // pairs whose store found more than counted
+    uint32_t diag_wrap_overlap; // outer hits whose wrapped phi window overlaps itself
+    uint32_t pad[5];
+};
+constexpr uint32_t kCountsShapeWords = 9;  // n_events, n_hits and the seven capacities
+
+// a device-resident size bounded by its buffer's capacity; an excess is
+// reported through the error bits and the work is cut to the capacity
+__device__ inline uint32_t bounded(uint32_t v, uint32_t cap, DevCounts* cnt) {
+    if (v > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->err, kErrCapacity);
+        return cap;
+    }
+    return v;
+}
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+__device__ inline uint64_t lanemask_lt() {
+    const uint32_t l = lane_id();
+    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
+}
+
+// inclusive wave scan of u32
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, kWave);
+        if (l >= (uint32_t)d) v += o;
+    }
+    return v;
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Exclusive scan of one u32 per thread over the whole block.  `wsum` is LDS
+// scratch of blockDim/64 + 1 entries.  Returns the exclusive prefix; *total
+// receives the block sum.  Contains __syncthreads().
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const uint32_t tid = threadIdx.x, w = tid / kWave, nw = blockDim.x / kWave;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane_id() == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < nw; ++i) {
+            const uint32_t t = wsum[i];
+            wsum[i] = run;
+            run += t;
+        }
+        wsum[nw] = run;
+    }
+    __syncthreads();
+    const uint32_t ex = wsum[w] + inc - v;
+    *total = wsum[nw];
+    __syncthreads();
+    return ex;
+}
+
+// Exclusive scan of n entries held in LDS, in place, by the whole block.
+// Returns the total.  Contains __syncthreads().
+__device__ inline uint32_t lds_excl_scan(uint32_t* a, uint32_t n, uint32_t* wsum) {
+    const uint32_t tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t per = (n + bd - 1) / bd;
+    const uint32_t lo = tid * per < n ? tid * per : n;
+    const uint32_t hi = lo + per < n ? lo + per : n;
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += a[i];
+    uint32_t total;
+    uint32_t run = block_excl_scan(s, wsum, &total);
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t t = a[i];
+        a[i] = run;
+        run += t;
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ inline uint64_t atomic_load_u64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void atomic_store_u64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// upper_bound - 1 over a sorted u32 array: last index s with a[s] <= key
+__device__ inline uint32_t find_segment(const uint32_t* a, uint32_t n_seg, uint32_t key) {
+    uint32_t lo = 0, hi = n_seg;  // search in [0, n_seg)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= key) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// grid geometry (GridConfig, src/datastructures/Grid.h:5-39)
+struct GridGeom {
+    float min_z, sector_z;
+    uint32_t n_z;
+    float min_phi, sector_phi;
+    uint32_t n_phi;
+    uint32_t n_layers;
+    uint32_t cells;  // (n_z + 1) * (n_phi + 1)
+};
+
+// cell of a hit, gridCount (src/algorithms/GridBuilder.h:80-84); kSentinel when
+// the z or phi sector falls outside the (n_z + 1) x (n_phi + 1) grid, where the
+// reference would index its local histogram out of bounds
+__device__ inline uint32_t hit_cell(const GridGeom& g, float x, float y, float z) {
+    const uint32_t zs = tx_f2u_sat(floorf((z - g.min_z) / g.sector_z));
+    const float phi = tx_atan2f(y, x);
+    const uint32_t ps = tx_f2u_sat(floorf((phi - g.min_phi) / g.sector_phi));
+    if (zs > g.n_z || ps > g.n_phi) return kSentinel;
+    return zs * (g.n_phi + 1) + ps;
+}
+
+}  // namespace trax

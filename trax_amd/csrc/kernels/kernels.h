@@ -1,0 +1,171 @@
+// Host-side launchers of the gfx950 kernels (internal C++ interface used by
+// the C ABI and the host algorithm classes).  Every launcher enqueues on the
+// given stream and returns the HIP status of the launch.  Work sizes that
+// depend on data (pairs, candidates, tracklets, connections) are read by the
+// kernels from the device-resident DevCounts record; the *_cap arguments are
+// host-side upper bounds used to size grids and to detect overflow.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace trax {
+
+struct ScanScratch {
+    uint64_t* status = nullptr;  // max_tiles + 1 words; the last one holds the tile counter
+    uint32_t max_tiles = 0;
+};
+
+struct HitColumns {
+    float* x = nullptr; float* y = nullptr; float* z = nullptr;
+    uint32_t* layer = nullptr; uint32_t* det = nullptr; uint32_t* id = nullptr; uint32_t* evt = nullptr;
+};
+
+// per-kernel block sizes (0 = default); chosen by the autotuner
+struct LaunchCfg {
+    uint32_t grid_count_block = 0;
+    uint32_t grid_store_block = 0;
+    uint32_t pair_block = 0;
+    uint32_t predict_block = 0;
+    uint32_t filter_block = 0;
+    uint32_t link_block = 0;
+    uint32_t blocks_per_cu = 0;  // grid-stride kernels: blocks per CU (0 = 8)
+};
+
+// layer-triplet configuration columns on the device (TripletConfigurations,
+// src/datastructures/TripletConfiguration.h:33) and GeometrySupplement radii
+struct TripletCols {
+    const uint32_t* l1; const uint32_t* l2; const uint32_t* l3;
+    const float* dtheta; const float* sigma_z; const float* dphi; const float* sigma_phi;
+    const float* tip; const float* z0; const uint32_t* pair_spread_phi;
+    const float* min_r; const float* max_r;  // index layer - 1
+    uint32_t n_trip;
+    uint32_t n_geom;
+    float min_radius_curvature;
+    uint32_t n_threads;  // the reference's work-group size; fixes the pair order
+    float d_eta_cut;
+};
+
+uint32_t scan_tiles(uint64_t n_cap);
+uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc);
+hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
+// fill *d_n + n_add entries (at most n_cap)
+hipError_t launch_fill_dn(uint32_t* p, uint32_t v, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
+                          hipStream_t st);
+// exclusive scan in place of (*d_n + n_add) elements (d_n may be null), at most
+// n_cap; writes the sum of all elements to *d_total when d_total is non-null
+hipError_t launch_scan(uint32_t* data, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
+                       uint32_t* d_total, const ScanScratch& s, hipStream_t st);
+// forward fill of zero entries of an offsets array (E*T + 1 entries, E on device)
+hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_trip, hipStream_t st);
+
+// GridBuilder
+hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
+                             const uint32_t* lay_nhits, const uint32_t* lay_offset, const float* hx,
+                             const float* hy, const float* hz, uint32_t* grid, DevCounts* cnt,
+                             const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_grid_store(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
+                             const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid,
+                             const HitColumns& in, const HitColumns& out, uint32_t n_hits_cap,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+
+// PairGeneratorBeamspot: plan -> scan(seg_base) -> count -> scan(slot_counts) -> store
+hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
+                            const uint32_t* grid, uint32_t* seg_base, DevCounts* cnt, hipStream_t st);
+hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
+                             uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
+                             const float* hz, const uint32_t* seg_base, uint32_t* slot_counts,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
+                             uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
+                             const float* hx, const float* hy, const float* hz, const uint32_t* seg_base,
+                             const uint32_t* slot_counts, uint32_t* pairs, uint32_t* pair_offsets,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+
+// TripletThetaPhiPredictor: count -> scan(counts) -> store (+ offsets, forward fill)
+hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
+                                const uint32_t* grid, const float* hx, const float* hy, const float* hz,
+                                const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* pairs,
+                                uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
+                                uint32_t n_cand_cap, const uint32_t* grid, const float* hx, const float* hy,
+                                const float* hz, const uint32_t* hlayer, const uint32_t* hevt,
+                                const uint32_t* pairs, const uint32_t* counts, uint32_t* cands,
+                                uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc,
+                                hipStream_t st);
+
+// TripletThetaPhiFilter: count (bitmap + per-word counts) -> scan -> store (+ offsets)
+hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const float* hx,
+                               const float* hy, const float* hz, const uint32_t* hlayer,
+                               const uint32_t* pairs, const uint32_t* cands, uint64_t* bitmap,
+                               uint32_t* word_counts, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_filter_store(const TripletCols& tc, uint32_t n_cand_cap, uint32_t n_trk_cap,
+                               const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* pairs,
+                               const uint32_t* cands, const uint64_t* bitmap, const uint32_t* word_counts,
+                               uint32_t* th1, uint32_t* th2, uint32_t* th3, uint32_t* t_off,
+                               DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+
+// TripletConnectivityTight
+struct LinkBufs {
+    float* eta = nullptr;            // R
+    uint32_t* hit_count = nullptr;   // n_hits + 1: tracklets per inner hit, scanned to CSR starts
+    uint32_t* hit_cursor = nullptr;  // n_hits
+    uint32_t* csr = nullptr;         // R
+    uint32_t* conn = nullptr;        // R: connectivity predicate
+    uint32_t* counts = nullptr;      // R + 1: followers per base, scanned
+    uint32_t* basis = nullptr; uint32_t* foll = nullptr; uint32_t* pev = nullptr;  // K
+    uint32_t* flag_a = nullptr; uint32_t* flag_b = nullptr;  // R + 1
+    uint32_t* connectable = nullptr; uint32_t* nonconnectable = nullptr;  // R
+    float* pt = nullptr;             // R
+};
+hipError_t launch_link_prepare(const TripletCols& tc, uint32_t n_hits_cap, uint32_t n_trk_cap,
+                               const float* hx, const float* hy, const float* hz, const uint32_t* th1,
+                               const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
+                               const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
+                             const uint32_t* hevt, const uint32_t* th1, const uint32_t* th2,
+                             const uint32_t* th3, const uint32_t* t_off, const LinkBufs& b, DevCounts* cnt,
+                             const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_link_store(const TripletCols& tc, uint32_t n_trk_cap, uint32_t n_conn_cap,
+                             const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* th1,
+                             const uint32_t* th2, const uint32_t* th3, const uint32_t* t_off,
+                             const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_link_flags(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                             hipStream_t st);
+hipError_t launch_link_compact(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                               hipStream_t st);
+hipError_t launch_circle_fit(uint32_t n_trk_cap, const float* hx, const float* hy, const uint32_t* th1,
+                             const uint32_t* th2, const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
+                             const LaunchCfg& lc, hipStream_t st);
+
+// CellularAutomaton
+struct CABufs {
+    uint32_t* st_a = nullptr; uint32_t* st_b = nullptr;  // R
+    uint32_t* fbc = nullptr;       // R + 1
+    uint32_t* cursor = nullptr;    // R
+    uint32_t* bidx = nullptr;      // basis capacity
+    float* bdiff = nullptr;        // basis capacity
+    uint32_t* best = nullptr;      // R
+    uint32_t* isbest = nullptr;    // R
+    uint32_t* handler = nullptr;   // R + 1
+    uint32_t* tracks = nullptr;    // track-entry capacity
+    uint32_t* flags = nullptr;     // 3 rotating "alive" flags
+};
+hipError_t launch_ca_init(uint32_t n_trk_cap, const CABufs& b, DevCounts* cnt, hipStream_t st);
+// iterations first_iter .. first_iter + n_iter - 1; an iteration after the
+// one that found no living cell returns at once
+hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, const uint32_t* foll,
+                                const CABufs& b, DevCounts* cnt, uint32_t first_iter, uint32_t n_iter,
+                                const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_ca_basis_count(uint32_t n_trk_cap, uint32_t n_conn_cap, const uint32_t* basis,
+                                 const uint32_t* foll, const CABufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                                 hipStream_t st);
+hipError_t launch_ca_basis_store(uint32_t n_trk_cap, uint32_t n_conn_cap, uint32_t n_basis_cap,
+                                 const uint32_t* basis, const uint32_t* foll, const float* pt,
+                                 const CABufs& b, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+hipError_t launch_ca_tracks(uint32_t n_trk_cap, uint32_t n_track_cap, const CABufs& b, DevCounts* cnt,
+                            const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st, int part);
+
+}  // namespace trax

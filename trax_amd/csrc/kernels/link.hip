@@ -1,0 +1,548 @@
+// TripletThetaPhiFilter, TripletConnectivityTight, TrackletCircleFitter and
+// CellularAutomaton kernels (src/algorithms/TripletThetaPhiFilter.h:50-280,
+// TripletConnectivityTight.h:47-346, TrackletCircleFitter.h:32-101,
+// CellularAutomaton.h:52-327 and their .cpp drivers).
+//
+// Filter: one lane per candidate; the cut results of a wave become one 64-bit
+//   word of the validity bitmap (ballot) and its popcount, the popcounts are
+//   scanned and each wave writes its tracklets in candidate order.
+// Connectivity: the reference compares every base triplet with every triplet
+//   of the next layer triplet (O(R_l * R_l+1)).  A follower must start with the
+//   base's middle hit, so here the tracklets are bucketed by their first hit
+//   (a CSR built with one atomic count, one scan and one fill) and each base
+//   tests only the tracklets of its middle hit, restricted to the same
+//   [begin, end) offsets window the reference scans; matches are emitted in
+//   ascending follower index, the reference's order.
+// CA: the forward iterations run as a fixed number of launches with a device
+//   "alive" flag (an iteration after the one that found no living cell returns
+//   at once), so no host read-back is needed per iteration.
+#include "kernels.h"
+
+namespace trax {
+namespace {
+
+__device__ inline uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline uint32_t gstride() { return gridDim.x * blockDim.x; }
+
+// TripletThetaPhiFilter.h:81-152
+__device__ inline bool filter_valid(const TripletCols& tc, const float* hx, const float* hy, const float* hz,
+                                    uint32_t h1, uint32_t h2, uint32_t h3, uint32_t lt) {
+    const float dThetaCut = tc.dtheta[lt], dPhiCut = tc.dphi[lt], tipCut = tc.tip[lt];
+    const float minRadius = tc.min_radius_curvature;
+    const float x1 = hx[h1], y1 = hy[h1], z1 = hz[h1];
+    const float x2 = hx[h2], y2 = hy[h2], z2 = hz[h2];
+    const float x3 = hx[h3], y3 = hy[h3], z3 = hz[h3];
+    bool valid = true;
+    float angle1 = tx_atan2f(sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
+    float angle2 = tx_atan2f(sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
+    float delta = fabsf(angle2 / angle1);
+    valid = valid && (1 - dThetaCut <= delta && delta <= 1 + dThetaCut);
+    angle1 = tx_atan2f((y2 - y1), (x2 - x1));
+    angle2 = tx_atan2f((y3 - y2), (x3 - x2));
+    delta = angle2 - angle1;
+    const float PI = TX_PI_F;
+    delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
+    valid = valid && (fabsf(delta) <= dPhiCut);
+    const float p1x = x1, p1y = y1, p1z = x1 * x1 + y1 * y1;
+    const float p2x = x2, p2y = y2, p2z = x2 * x2 + y2 * y2;
+    const float p3x = x3, p3y = y3, p3z = x3 * x3 + y3 * y3;
+    const float ax = p2x - p1x, ay = p2y - p1y, az = p2z - p1z;
+    const float bx = p3x - p1x, by = p3y - p1y, bz = p3z - p1z;
+    float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+    {
+        const float d = nx * nx + ny * ny + nz * nz;
+        if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
+    }
+    const float cox = (-nx) / (2 * nz), coy = (-ny) / (2 * nz);
+    const float cc = -(nx * p1x + ny * p1y + nz * p1z);
+    const float cR = sqrtf((1 - nz * nz - 4 * cc * nz) / (4 * nz * nz));
+    float vx = -cox, vy = -coy;
+    {
+        const float d = vx * vx + vy * vy;
+        if (d != 0.0f) { const float l = sqrtf(d); vx = vx / l; vy = vy / l; }
+    }
+    const float pcax = cox + cR * vx, pcay = coy + cR * vy;
+    const float tip_squared = pcax * pcax + pcay * pcay;
+    valid = valid && (tip_squared <= (tipCut * tipCut));
+    valid = valid && (cR >= minRadius);
+    return valid;
+}
+
+__global__ void k_filter_count(TripletCols tc, const float* __restrict__ hx, const float* __restrict__ hy,
+                               const float* __restrict__ hz, const uint32_t* __restrict__ hlayer,
+                               const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ cands,
+                               uint64_t* __restrict__ bitmap, uint32_t* __restrict__ word_counts,
+                               DevCounts* __restrict__ cnt) {
+    const uint32_t n = bounded(cnt->n_cands, cnt->cap_cands, cnt);
+    const uint32_t nwords = (n + 63) / 64;
+    if (gtid() == 0) cnt->n_filter_words = nwords;
+    const uint32_t lane = lane_id();
+    for (uint32_t word = gtid() / kWave; word <= nwords; word += gstride() / kWave) {
+        if (word == nwords) {
+            if (lane == 0) word_counts[nwords] = 0;
+            continue;
+        }
+        const uint32_t gid = word * 64 + lane;
+        bool valid = false;
+        if (gid < n) {
+            const uint32_t pi = cands[2 * (size_t)gid], h3 = cands[2 * (size_t)gid + 1];
+            if (pi != kSentinel && h3 != kSentinel) {
+                const uint32_t h1 = pairs[2 * (size_t)pi], h2 = pairs[2 * (size_t)pi + 1];
+                if (h1 != kSentinel && h2 != kSentinel) {
+                    const uint32_t lt = hlayer[h1] - 1;
+                    if (lt < tc.n_trip) valid = filter_valid(tc, hx, hy, hz, h1, h2, h3, lt);
+                }
+            }
+        }
+        const uint64_t m = __ballot(valid);
+        if (lane == 0) {
+            bitmap[word] = m;
+            word_counts[word] = (uint32_t)__popcll(m);
+        }
+    }
+}
+
+__global__ void k_filter_store(const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ cands,
+                               const uint64_t* __restrict__ bitmap, const uint32_t* __restrict__ word_counts,
+                               uint32_t* __restrict__ th1, uint32_t* __restrict__ th2, uint32_t* __restrict__ th3,
+                               uint32_t n_trk_cap, DevCounts* __restrict__ cnt) {
+    const uint32_t nwords = cnt->n_filter_words;
+    const uint32_t lane = lane_id();
+    bool over = false;
+    for (uint32_t word = gtid() / kWave; word < nwords; word += gstride() / kWave) {
+        const uint64_t m = bitmap[word];
+        if ((m >> lane) & 1ull) {
+            const uint32_t pos = word_counts[word] + (uint32_t)__popcll(m & lanemask_lt());
+            const uint32_t gid = word * 64 + lane;
+            const uint32_t pi = cands[2 * (size_t)gid];
+            if (pos < n_trk_cap) {
+                th1[pos] = pairs[2 * (size_t)pi];
+                th2[pos] = pairs[2 * (size_t)pi + 1];
+                th3[pos] = cands[2 * (size_t)gid + 1];
+            } else {
+                over = true;
+            }
+        }
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// filterOffsetStore (TripletThetaPhiFilter.h:221-261), literal: the entry after
+// the last tracklet's group receives its index R - 1
+__global__ void k_filter_offsets(TripletCols tc, const uint32_t* __restrict__ hlayer,
+                                 const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ th1,
+                                 uint32_t* __restrict__ t_off, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    const uint64_t n_off = (uint64_t)cnt->n_events * tc.n_trip + 1;
+    for (uint32_t gid = gtid(); gid < R; gid += gstride()) {
+        const uint32_t hitId = th1[gid];
+        const uint32_t layerTriplet = hlayer[hitId];
+        const uint32_t event = hevt[hitId];
+        const uint64_t base = (uint64_t)event * tc.n_trip + layerTriplet;
+        if (gid > 0) {
+            const uint32_t prev = th1[gid - 1];
+            if (layerTriplet != hlayer[prev] || event != hevt[prev])
+                if (base - 1 < n_off) t_off[base - 1] = gid;
+            if (gid + 1 == R && base < n_off) t_off[base] = gid;
+        } else if (base - 1 < n_off) {
+            t_off[base - 1] = gid;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_eta(const float* __restrict__ hx, const float* __restrict__ hy, const float* __restrict__ hz,
+                      const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th3,
+                      float* __restrict__ eta, uint32_t* __restrict__ hit_count,
+                      DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t < R; t += gstride()) {  // tripletEtaCalculatorStore (.h:60-75)
+        const uint32_t a = th1[t], b = th3[t];
+        const float px = hx[b] - hx[a], py = hy[b] - hy[a], pz = hz[b] - hz[a];
+        const float tt = pz / sqrtf(px * px + py * py);
+        eta[t] = tx_asinhf(tt);
+        atomicAdd(&hit_count[a], 1u);
+    }
+}
+
+__global__ void k_csr_fill(const uint32_t* __restrict__ th1, const uint32_t* __restrict__ hit_start,
+                           uint32_t* __restrict__ hit_cursor, uint32_t* __restrict__ csr,
+                           DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t < R; t += gstride()) {
+        const uint32_t h = th1[t];
+        csr[hit_start[h] + atomicAdd(&hit_cursor[h], 1u)] = t;
+    }
+}
+
+struct LinkRange {
+    uint32_t begin, end, ev;
+};
+
+// guards of tripletConnectivityTightCount/Store (.h:109-132)
+__device__ inline bool link_range(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
+                                  const uint32_t* th1, const uint32_t* th2, const uint32_t* t_off,
+                                  uint32_t n_off, uint32_t R, uint32_t t, LinkRange& r) {
+    const uint32_t limit = n_off >= 2 ? t_off[n_off - 2] : 0u;
+    if (t >= limit) return false;
+    const uint32_t inner = th2[t];
+    r.ev = hevt[inner];
+    const uint64_t ib = (uint64_t)r.ev * tc.n_trip + (hlayer[inner] - 1);
+    if (ib + 1 >= n_off) return false;
+    r.begin = t_off[ib];
+    r.end = t_off[ib + 1];
+    if (r.begin >= R) return false;
+    return r.ev == hevt[th1[r.begin]];
+}
+
+__global__ void k_link_count(TripletCols tc, const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
+                             const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
+                             const uint32_t* __restrict__ th3, const uint32_t* __restrict__ t_off, LinkBufs b,
+                             DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    const uint32_t n_off = cnt->n_events * tc.n_trip + 1;
+    for (uint32_t t = gtid(); t <= R; t += gstride()) {
+        uint32_t n = 0;
+        LinkRange r;
+        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, r)) {
+            const uint32_t h = th2[t], h3 = th3[t];
+            const float et = b.eta[t];
+            for (uint32_t q = b.hit_count[h]; q < b.hit_count[h + 1]; ++q) {
+                const uint32_t i = b.csr[q];
+                if (i >= r.begin && i < r.end && th2[i] == h3 && fabsf(et - b.eta[i]) < tc.d_eta_cut) {
+                    n++;
+                    b.conn[i] = 1u;
+                }
+            }
+            if (n) b.conn[t] = 1u;
+        }
+        b.counts[t] = n;
+    }
+}
+
+__global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
+                             const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
+                             const uint32_t* __restrict__ th3, const uint32_t* __restrict__ t_off, LinkBufs b,
+                             uint32_t n_conn_cap, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    const uint32_t n_off = cnt->n_events * tc.n_trip + 1;
+    bool over = false;
+    for (uint32_t t = gtid(); t < R; t += gstride()) {
+        uint32_t so = b.counts[t];
+        const uint32_t sn = b.counts[t + 1];
+        if (so == sn) continue;
+        if (sn > n_conn_cap) { over = true; continue; }
+        LinkRange r;
+        if (!link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, r)) continue;
+        const uint32_t h = th2[t], h3 = th3[t];
+        const float et = b.eta[t];
+        const uint32_t q0 = b.hit_count[h], q1 = b.hit_count[h + 1];
+        uint32_t last = 0;
+        bool first = true;
+        for (; so < sn; ++so) {  // emit the followers in ascending index
+            uint32_t best = kSentinel;
+            for (uint32_t q = q0; q < q1; ++q) {
+                const uint32_t i = b.csr[q];
+                if ((first || i > last) && i < best && i >= r.begin && i < r.end && th2[i] == h3 &&
+                    fabsf(et - b.eta[i]) < tc.d_eta_cut)
+                    best = i;
+            }
+            if (best == kSentinel) break;
+            b.basis[so] = t;
+            b.foll[so] = best;
+            b.pev[so] = r.ev;
+            last = best;
+            first = false;
+        }
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// predicates over the first R - 1 tracklets (TripletConnectivityTight.cpp:184-256);
+// nothing when no connection exists (the reference returns null lists, .cpp:93-95)
+__global__ void k_link_flags(LinkBufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    for (uint32_t t = gtid(); t <= R; t += gstride()) {
+        const bool in = K > 0 && t + 1 < R;
+        const uint32_t c = (t < R) ? b.conn[t] : 0u;
+        b.flag_a[t] = in ? (c != 0u) : 0u;
+        b.flag_b[t] = in ? (c == 0u) : 0u;
+    }
+}
+
+__global__ void k_link_compact(LinkBufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t + 1 < R; t += gstride()) {
+        if (b.flag_a[t] != b.flag_a[t + 1]) b.connectable[b.flag_a[t]] = t;
+        if (b.flag_b[t] != b.flag_b[t + 1]) b.nonconnectable[b.flag_b[t]] = t;
+    }
+}
+
+// trackletCircleFitterStore (TrackletCircleFitter.h:47-97)
+__global__ void k_circle(const float* __restrict__ hx, const float* __restrict__ hy,
+                         const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
+                         const uint32_t* __restrict__ th3, LinkBufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t n = bounded(cnt->n_connectable, cnt->cap_tracklets, cnt);
+    for (uint32_t j = gtid(); j < n; j += gstride()) {
+        const uint32_t t = b.connectable[j];
+        const uint32_t h0 = th1[t], h1 = th2[t], h2 = th3[t];
+        const float x0 = hx[h0], y0 = hy[h0], x1 = hx[h1], y1 = hy[h1], x2 = hx[h2], y2 = hy[h2];
+        const float p0x = x0, p0y = y0, p0z = x0 * x0 + y0 * y0;
+        const float p1x = x1, p1y = y1, p1z = x1 * x1 + y1 * y1;
+        const float p2x = x2, p2y = y2, p2z = x2 * x2 + y2 * y2;
+        const float ax = p1x - p0x, ay = p1y - p0y, az = p1z - p0z;
+        const float bx = p2x - p0x, by = p2y - p0y, bz = p2z - p0z;
+        float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+        const float d = nx * nx + ny * ny + nz * nz;
+        if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
+        const float cc = -(nx * p0x + ny * p0y + nz * p0z);
+        const float nz2 = nz * nz;
+        b.pt[t] = sqrtf((1 - nz2 - 4 * cc * nz) / (4 * nz2));
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_ca_init(CABufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t < R; t += gstride()) { b.st_a[t] = 1u; b.st_b[t] = 1u; }
+    if (gtid() == 0) {
+        b.flags[0] = 0u; b.flags[1] = 0u; b.flags[2] = 1u;
+        cnt->ca_iterations = 0;
+    }
+}
+
+// CellularAutomaton iteration (.h:52-100) number k of the forward phase
+__global__ void k_ca_iter(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll, CABufs b,
+                          uint32_t k, DevCounts* __restrict__ cnt) {
+    const uint32_t prev = b.flags[(k + 2) % 3];
+    if (gtid() == 0) b.flags[(k + 1) % 3] = 0u;
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    if (!prev || K == 0) return;
+    if (gtid() == 0) atomicAdd(&cnt->ca_iterations, 1u);
+    const uint32_t* cur = (k & 1u) ? b.st_b : b.st_a;
+    uint32_t* nxt = (k & 1u) ? b.st_a : b.st_b;
+    bool alive = false;
+    for (uint32_t p = gtid(); p < K; p += gstride()) {
+        const uint32_t f = foll[p];
+        const uint32_t fs = cur[f];
+        const uint32_t same = cur[basis[p]] == fs;
+        atomicMax(&nxt[f], fs + same);
+        alive |= same != 0u;
+    }
+    if (__ballot(alive) != 0ull && lane_id() == 0) b.flags[k % 3] = 1u;
+}
+
+__global__ void k_ca_basis_count(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll, CABufs b,
+                                 DevCounts* __restrict__ cnt) {
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    for (uint32_t p = gtid(); p < K; p += gstride()) {  // followerBasisCount (.h:152-168)
+        const uint32_t f = foll[p];
+        if (b.st_a[basis[p]] + 1u == b.st_a[f]) atomicAdd(&b.fbc[f], 1u);
+    }
+}
+
+__global__ void k_ca_basis_store(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll,
+                                 const float* __restrict__ pt, CABufs b, uint32_t n_basis_cap,
+                                 DevCounts* __restrict__ cnt) {
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    bool over = false;
+    for (uint32_t p = gtid(); p < K; p += gstride()) {  // followerBasisStore (.h:188-205)
+        const uint32_t bs = basis[p], f = foll[p];
+        if (b.st_a[bs] + 1u != b.st_a[f]) continue;
+        const uint32_t so = b.fbc[f] + atomicAdd(&b.cursor[f], 1u);
+        if (so >= n_basis_cap) { over = true; continue; }
+        b.bidx[so] = bs;
+        b.bdiff[so] = fabsf(pt[bs] - pt[f]);
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// followerBestBasisStore (.h:224-263) over a basis list put back into
+// connection-pair (= ascending basis) order
+__global__ void k_ca_best(CABufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t < R; t += gstride()) {
+        const uint32_t begin = b.fbc[t], end = b.fbc[t + 1];
+        if (begin == end) continue;
+        for (uint32_t i = begin + 1; i < end; ++i) {
+            const uint32_t ki = b.bidx[i];
+            const float kd = b.bdiff[i];
+            uint32_t j = i;
+            while (j > begin && b.bidx[j - 1] > ki) {
+                b.bidx[j] = b.bidx[j - 1];
+                b.bdiff[j] = b.bdiff[j - 1];
+                --j;
+            }
+            b.bidx[j] = ki;
+            b.bdiff[j] = kd;
+        }
+        float minPt = b.bdiff[begin];
+        uint32_t minIdx = b.bidx[begin];
+        for (uint32_t i = begin + 1; i < end; ++i) {
+            const float d = b.bdiff[i];
+            const uint32_t test = d < minPt;
+            minPt = minPt * (float)(!test) + d * (float)test;
+            minIdx = minIdx * (uint32_t)(!test) + b.bidx[i] * test;
+        }
+        b.best[t] = minIdx;
+        if (minIdx < R) b.isbest[minIdx] = 1u;
+    }
+}
+
+__global__ void k_ca_handler(CABufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    for (uint32_t t = gtid(); t <= R; t += gstride())  // handlersStateStore (.h:280-288)
+        b.handler[t] = (t < R && K > 0) ? (b.isbest[t] == 0u) * b.st_a[t] : 0u;
+}
+
+__global__ void k_ca_tracks(CABufs b, uint32_t n_track_cap, DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    bool over = false;
+    for (uint32_t t = gtid(); t < R; t += gstride()) {  // trackCollectionStore (.h:305-323)
+        const uint32_t so = b.handler[t];
+        const uint32_t len = b.handler[t + 1] - so;
+        if (len == 0) continue;
+        if (so + len > n_track_cap) { over = true; continue; }
+        uint32_t bi = t;
+        for (uint32_t i = 0; i < len; ++i) {
+            b.tracks[so + i] = bi;
+            if (bi >= R) break;
+            bi = b.best[bi];
+        }
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+}  // namespace
+
+#define TRAX_LAUNCH(kern, work, blk, ...)                                                          \
+    do {                                                                                           \
+        hipLaunchKernelGGL(kern, dim3(stride_blocks((work), (blk), lc)), dim3(blk), 0, st, __VA_ARGS__); \
+        hipError_t e_ = hipGetLastError();                                                         \
+        if (e_ != hipSuccess) return e_;                                                           \
+    } while (0)
+
+hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const float* hx, const float* hy,
+                               const float* hz, const uint32_t* hlayer, const uint32_t* pairs,
+                               const uint32_t* cands, uint64_t* bitmap, uint32_t* word_counts, DevCounts* cnt,
+                               const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t b = lc.filter_block ? lc.filter_block : 256;
+    TRAX_LAUNCH(k_filter_count, (uint64_t)n_cand_cap + 64, b, tc, hx, hy, hz, hlayer, pairs, cands, bitmap,
+                word_counts, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_filter_store(const TripletCols& tc, uint32_t n_cand_cap, uint32_t n_trk_cap,
+                               const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* pairs,
+                               const uint32_t* cands, const uint64_t* bitmap, const uint32_t* word_counts,
+                               uint32_t* th1, uint32_t* th2, uint32_t* th3, uint32_t* t_off, DevCounts* cnt,
+                               const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t b = lc.filter_block ? lc.filter_block : 256;
+    TRAX_LAUNCH(k_filter_store, (uint64_t)n_cand_cap + 64, b, pairs, cands, bitmap, word_counts, th1, th2, th3,
+                n_trk_cap, cnt);
+    TRAX_LAUNCH(k_filter_offsets, n_trk_cap, 256u, tc, hlayer, hevt, th1, t_off, cnt);
+    return launch_forward_fill(t_off, cnt, tc.n_trip, st);
+}
+
+hipError_t launch_link_prepare(const TripletCols& tc, uint32_t n_hits_cap, uint32_t n_trk_cap,
+                               const float* hx, const float* hy, const float* hz, const uint32_t* th1,
+                               const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
+                               const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
+    (void)tc;
+    const uint32_t blk = lc.link_block ? lc.link_block : 256;
+    hipError_t e = launch_fill(b.hit_count, 0u, (uint64_t)n_hits_cap + 1, st);
+    if (e != hipSuccess) return e;
+    e = launch_fill(b.hit_cursor, 0u, n_hits_cap, st);
+    if (e != hipSuccess) return e;
+    TRAX_LAUNCH(k_eta, n_trk_cap, blk, hx, hy, hz, th1, th3, b.eta, b.hit_count, cnt);
+    e = launch_scan(b.hit_count, &cnt->n_hits, 1u, (uint64_t)n_hits_cap + 1, nullptr, ss, st);
+    if (e != hipSuccess) return e;
+    TRAX_LAUNCH(k_csr_fill, n_trk_cap, blk, th1, b.hit_count, b.hit_cursor, b.csr, cnt);
+    return launch_fill(b.conn, 0u, (uint64_t)n_trk_cap + 1, st);
+}
+
+hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
+                             const uint32_t* hevt, const uint32_t* th1, const uint32_t* th2, const uint32_t* th3,
+                             const uint32_t* t_off, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                             hipStream_t st) {
+    const uint32_t blk = lc.link_block ? lc.link_block : 256;
+    TRAX_LAUNCH(k_link_count, (uint64_t)n_trk_cap + 1, blk, tc, hlayer, hevt, th1, th2, th3, t_off, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_link_store(const TripletCols& tc, uint32_t n_trk_cap, uint32_t n_conn_cap,
+                             const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* th1,
+                             const uint32_t* th2, const uint32_t* th3, const uint32_t* t_off, const LinkBufs& b,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t blk = lc.link_block ? lc.link_block : 256;
+    TRAX_LAUNCH(k_link_store, n_trk_cap, blk, tc, hlayer, hevt, th1, th2, th3, t_off, b, n_conn_cap, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_link_flags(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                             hipStream_t st) {
+    TRAX_LAUNCH(k_link_flags, (uint64_t)n_trk_cap + 1, 256u, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_link_compact(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                               hipStream_t st) {
+    TRAX_LAUNCH(k_link_compact, n_trk_cap, 256u, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_circle_fit(uint32_t n_trk_cap, const float* hx, const float* hy, const uint32_t* th1,
+                             const uint32_t* th2, const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
+                             const LaunchCfg& lc, hipStream_t st) {
+    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(b.pt), 0u, n_trk_cap, st);
+    if (e != hipSuccess) return e;
+    TRAX_LAUNCH(k_circle, n_trk_cap, 256u, hx, hy, th1, th2, th3, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_ca_init(uint32_t n_trk_cap, const CABufs& b, DevCounts* cnt, hipStream_t st) {
+    LaunchCfg lc;
+    TRAX_LAUNCH(k_ca_init, n_trk_cap, 256u, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, const uint32_t* foll,
+                                const CABufs& b, DevCounts* cnt, uint32_t first_iter, uint32_t n_iter,
+                                const LaunchCfg& lc, hipStream_t st) {
+    for (uint32_t k = first_iter; k < first_iter + n_iter; ++k) TRAX_LAUNCH(k_ca_iter, n_conn_cap, 256u, basis, foll, b, k, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_ca_basis_count(uint32_t n_trk_cap, uint32_t n_conn_cap, const uint32_t* basis,
+                                 const uint32_t* foll, const CABufs& b, DevCounts* cnt, const LaunchCfg& lc,
+                                 hipStream_t st) {
+    hipError_t e = launch_fill(b.fbc, 0u, (uint64_t)n_trk_cap + 1, st);
+    if (e != hipSuccess) return e;
+    e = launch_fill(b.cursor, 0u, n_trk_cap, st);
+    if (e != hipSuccess) return e;
+    TRAX_LAUNCH(k_ca_basis_count, n_conn_cap, 256u, basis, foll, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_ca_basis_store(uint32_t n_trk_cap, uint32_t n_conn_cap, uint32_t n_basis_cap,
+                                 const uint32_t* basis, const uint32_t* foll, const float* pt, const CABufs& b,
+                                 DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    TRAX_LAUNCH(k_ca_basis_store, n_conn_cap, 256u, basis, foll, pt, b, n_basis_cap, cnt);
+    hipError_t e = launch_fill(b.best, kSentinel, n_trk_cap, st);
+    if (e != hipSuccess) return e;
+    e = launch_fill(b.isbest, 0u, n_trk_cap, st);
+    if (e != hipSuccess) return e;
+    TRAX_LAUNCH(k_ca_best, n_trk_cap, 256u, b, cnt);
+    TRAX_LAUNCH(k_ca_handler, (uint64_t)n_trk_cap + 1, 256u, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_ca_tracks(uint32_t n_trk_cap, uint32_t n_track_cap, const CABufs& b, DevCounts* cnt,
+                            const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st, int part) {
+    (void)ss;
+    (void)part;
+    TRAX_LAUNCH(k_ca_tracks, n_trk_cap, 256u, b, n_track_cap, cnt);
+    return hipSuccess;
+}
+
+}  // namespace trax

@@ -1,0 +1,454 @@
+// PairGeneratorBeamspot and TripletThetaPhiPredictor kernels
+// (src/algorithms/PairGeneratorBeamspot.h:43-525, .cpp:5-183;
+//  src/algorithms/TripletThetaPhiPredictor.h:49-447, .cpp:10-170).
+//
+// Pair order.  The reference gives each (event, layer pair) one work-group of
+// nThreads work-items; item t handles outer hits t, t + nThreads, ... and
+// writes its pairs contiguously, inner hits ascending (pairStore scans a dense
+// N1 x N2 bitmap).  Here every outer hit is its own work item, and the count
+// array is laid out "thread-major": segment s owns nThreads * K slots
+// (K = ceil(N2 / nThreads)), slot t*K + k holding the count of outer hit
+// k*nThreads + t.  One exclusive scan over the slots then gives every outer
+// hit its output position in exactly the reference's order, and the store
+// kernel enumerates the inner window in ascending index order (the order in
+// which the reference reads its bitmap) -- without the O(N1*N2) bitmap.
+//
+// Predictor order.  One work item per pair, candidates in the reference's
+// z-sector / wrap-around walk order, the store bounded by the count, both as
+// in the reference (including the two different rOrigin expressions of the
+// count and store kernels).
+#include "kernels.h"
+
+namespace trax {
+namespace {
+
+struct Window {
+    float zLow, zHigh, phiLow, phiHigh;
+    uint32_t zLo, zHi, pLo, pHi;
+    bool wrap;
+};
+
+// outer hit i of layer pair p: window in the inner layer (.h:108-163)
+__device__ inline Window pair_window(const GridGeom& g, const TripletCols& tc, uint32_t p, uint32_t lay1,
+                                     float X, float Y, float Z) {
+    Window w;
+    const float z0 = tc.z0[p];
+    const float tip = tc.tip[p];
+    const float sigmaZ = tc.sigma_z[p];
+    const float sigmaPhi = tx_bits2f(tc.pair_spread_phi[p]);  // .cpp:59 passes pairSpreadPhi
+    const float minR1 = tc.min_r[lay1], maxR1 = tc.max_r[lay1];
+    const float mrc = tc.min_radius_curvature;
+    const float signRadius = (Y > 0.0f) ? 1.0f : ((Y < 0.0f) ? -1.0f : (tx_isnan(Y) ? 0.0f : Y));
+    const float r = signRadius * sqrtf(X * X + Y * Y);
+    const float cotThetaHigh = (Z - z0) / r;
+    const float cotThetaLow = (Z + z0) / r;
+    float tmp = signRadius * maxR1 * cotThetaHigh + z0;
+    float zHigh = signRadius * minR1 * cotThetaHigh + z0;
+    zHigh = (float)(tmp < zHigh) * zHigh + (float)(tmp > zHigh) * tmp;
+    tmp = signRadius * maxR1 * cotThetaLow - z0;
+    float zLow = signRadius * minR1 * cotThetaLow - z0;
+    zLow = (float)(tmp < zLow) * tmp + (float)(tmp > zLow) * zLow;
+    zLow -= sigmaZ;
+    zHigh += sigmaZ;
+    const int zl = tx_f2i_sat(floorf((zLow - g.min_z) / g.sector_z));
+    w.zLo = (uint32_t)(zl > 0 ? zl : 0);
+    const uint32_t zh = tx_f2u_sat(floorf((zHigh - g.min_z) / g.sector_z)) + 1u;
+    w.zHi = zh < g.n_z ? zh : g.n_z;
+    const float phi = tx_atan2f(Y, X);
+    tmp = fabsf(tx_acosf(r / (2 * mrc)) - tx_acosf(maxR1 / (2 * mrc)));
+    float dPhi = fabsf(tx_acosf(r / (2 * mrc)) - tx_acosf(minR1 / (2 * mrc)));
+    dPhi = (float)(tmp < dPhi) * dPhi + (float)(tmp > dPhi) * tmp;
+    tmp = tx_atanf(tip * (r - minR1)) / (r * minR1);
+    float phiHigh = tx_atanf(tip * (r - maxR1)) / (r * maxR1);
+    dPhi += tmp > phiHigh ? tmp : phiHigh;
+    float phiLow = phi - dPhi - sigmaPhi;
+    phiHigh = phi + dPhi + sigmaPhi;
+    const float PI = TX_PI_F;
+    w.wrap = phiLow < -PI || phiHigh > PI || phiLow > PI || phiHigh < -PI;
+    phiLow -= (float)((phiLow > PI) * 2) * PI;
+    phiLow += (float)((phiLow < -PI) * 2) * PI;
+    phiHigh -= (float)((phiHigh > PI) * 2) * PI;
+    phiHigh += (float)((phiHigh < -PI) * 2) * PI;
+    const uint32_t lim = g.n_phi + 1;  // keeps grid reads inside the layer (only NaN windows exceed it)
+    const uint32_t pl = tx_f2u_sat(floorf((phiLow - g.min_phi) / g.sector_phi));
+    const uint32_t ph = tx_f2u_sat(floorf((phiHigh - g.min_phi) / g.sector_phi) + 1.0f);
+    w.pLo = pl < lim ? pl : lim;
+    w.pHi = ph < lim ? ph : lim;
+    w.zLow = zLow; w.zHigh = zHigh; w.phiLow = phiLow; w.phiHigh = phiHigh;
+    return w;
+}
+
+__device__ inline bool in_window(const Window& w, float hz, float actPhi) {
+    const float PI = TX_PI_F;
+    bool valid = w.zLow <= hz && hz <= w.zHigh;
+    return valid && ((!w.wrap && (w.phiLow <= actPhi && actPhi <= w.phiHigh)) ||
+                     (w.wrap && ((w.phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= w.phiHigh))));
+}
+
+// one pair segment (event, layer pair) -> number of count slots
+__global__ void k_pair_plan(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
+                            uint32_t* __restrict__ seg_base, DevCounts* __restrict__ cnt) {
+    const uint32_t nseg = cnt->n_events * tc.n_trip;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += gridDim.x * blockDim.x) {
+        if (s == 0) cnt->n_segments = nseg;
+        if (s == nseg) { seg_base[s] = 0; continue; }
+        const uint32_t e = s / tc.n_trip, p = s % tc.n_trip;
+        const uint32_t* g2 = grid + ((size_t)e * g.n_layers + tc.l2[p] - 1) * g.cells;
+        const uint32_t n2 = g2[g.cells - 1] - g2[0];
+        const uint32_t K = (n2 + tc.n_threads - 1) / tc.n_threads;
+        seg_base[s] = K * tc.n_threads;
+    }
+}
+
+struct SlotInfo {
+    uint32_t seg, e, p, t, k;
+    bool live;
+};
+
+__device__ inline SlotInfo slot_info(const TripletCols& tc, const uint32_t* seg_base, uint32_t nseg,
+                                     uint32_t slot) {
+    SlotInfo si;
+    si.seg = find_segment(seg_base, nseg, slot);
+    const uint32_t o = slot - seg_base[si.seg];
+    const uint32_t span = seg_base[si.seg + 1] - seg_base[si.seg];
+    const uint32_t K = span / tc.n_threads;
+    si.t = o / K;
+    si.k = o - si.t * K;
+    si.e = si.seg / tc.n_trip;
+    si.p = si.seg - si.e * tc.n_trip;
+    si.live = true;
+    return si;
+}
+
+__global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
+                             const float* __restrict__ hx, const float* __restrict__ hy,
+                             const float* __restrict__ hz, const uint32_t* __restrict__ seg_base,
+                             uint32_t* __restrict__ slot_counts, DevCounts* __restrict__ cnt) {
+    const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
+    const uint32_t np1 = g.n_phi + 1;
+    uint32_t overlap = 0;
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot <= nslot; slot += gridDim.x * blockDim.x) {
+        if (slot == nslot) { slot_counts[slot] = 0; continue; }
+        const SlotInfo si = slot_info(tc, seg_base, nseg, slot);
+        const uint32_t lay2 = tc.l2[si.p] - 1, lay1 = tc.l1[si.p] - 1;
+        const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
+        const uint32_t layer2Offset = g2[0], end = g2[g.cells - 1];
+        const uint32_t il = si.k * tc.n_threads + si.t;
+        uint32_t nFound = 0;
+        if (il < end - layer2Offset) {
+            const uint32_t i = layer2Offset + il;
+            const uint32_t* lg1 = grid + ((size_t)si.e * g.n_layers + lay1) * g.cells;
+            const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i]);
+            if (w.wrap && w.pHi > w.pLo) overlap++;
+            for (uint32_t zs = w.zLo; zs < w.zHi; ++zs) {  // (.h:172-230)
+                const uint32_t zStart = lg1[zs * np1];
+                const uint32_t zEnd = lg1[(zs + 1) * np1 - 1];
+                const uint32_t zLen = zEnd - zStart;
+                const uint32_t end2 = (uint32_t)w.wrap * zEnd + lg1[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+                for (uint32_t j = lg1[zs * np1 + w.pLo]; j < end2; ++j) {
+                    const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
+                    nFound += in_window(w, hz[index], tx_atan2f(hy[index], hx[index]));
+                }
+            }
+        }
+        slot_counts[slot] = nFound;
+    }
+    if (overlap) atomicAdd(&cnt->diag_wrap_overlap, overlap);
+}
+
+__global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
+                             const float* __restrict__ hx, const float* __restrict__ hy,
+                             const float* __restrict__ hz, const uint32_t* __restrict__ seg_base,
+                             const uint32_t* __restrict__ slot_counts, uint32_t* __restrict__ pairs,
+                             uint32_t n_pair_cap, DevCounts* __restrict__ cnt) {
+    const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
+    const uint32_t np1 = g.n_phi + 1;
+    uint32_t short_slots = 0;
+    bool over = false;
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < nslot; slot += gridDim.x * blockDim.x) {
+        uint32_t pos = slot_counts[slot];
+        const uint32_t next = slot_counts[slot + 1];
+        if (pos == next) continue;
+        if (next > n_pair_cap) { over = true; continue; }
+        const SlotInfo si = slot_info(tc, seg_base, nseg, slot);
+        const uint32_t lay2 = tc.l2[si.p] - 1, lay1 = tc.l1[si.p] - 1;
+        const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
+        const uint32_t i = g2[0] + si.k * tc.n_threads + si.t;
+        const uint32_t* lg1 = grid + ((size_t)si.e * g.n_layers + lay1) * g.cells;
+        const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i]);
+        // every z sector visits the index intervals [a1, b1) (j below the sector
+        // end) and [a2, b2) (j past the end, wrapped back); the bitmap read of
+        // the reference emits their union in ascending order
+        for (uint32_t zs = w.zLo; zs < w.zHi && pos < next; ++zs) {
+            const uint32_t zStart = lg1[zs * np1];
+            const uint32_t zEnd = lg1[(zs + 1) * np1 - 1];
+            const uint32_t zLen = zEnd - zStart;
+            const uint32_t jStart = lg1[zs * np1 + w.pLo];
+            const uint32_t end2 = (uint32_t)w.wrap * zEnd + lg1[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+            uint32_t a1 = jStart, b1 = end2 < zEnd ? end2 : zEnd;
+            if (b1 < a1) b1 = a1;
+            const uint32_t j2 = jStart > zEnd ? jStart : zEnd;
+            uint32_t a2 = j2 - zLen, b2 = end2 > j2 ? end2 - zLen : a2;
+            if (a2 < a1) {  // order the two intervals by start
+                uint32_t ta = a1, tb = b1; a1 = a2; b1 = b2; a2 = ta; b2 = tb;
+            }
+            // [a1, b1) starts first; merge overlap
+            uint32_t lo[2] = {a1, b1 > a2 ? b1 : a2};
+            uint32_t hi[2] = {b1, b2};
+            for (int r = 0; r < 2; ++r)
+                for (uint32_t index = lo[r]; index < hi[r] && pos < next; ++index) {
+                    if (in_window(w, hz[index], tx_atan2f(hy[index], hx[index]))) {
+                        pairs[2 * (size_t)pos] = index;
+                        pairs[2 * (size_t)pos + 1] = i;
+                        pos++;
+                    }
+                }
+        }
+        for (uint32_t q = pos; q < next; ++q) {
+            pairs[2 * (size_t)q] = kSentinel;
+            pairs[2 * (size_t)q + 1] = kSentinel;
+        }
+        short_slots += next - pos;
+    }
+    if (short_slots) atomicAdd(&cnt->diag_pair_short, short_slots);
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+__global__ void k_pair_offsets(const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
+                               uint32_t* __restrict__ pair_offsets, DevCounts* __restrict__ cnt) {
+    const uint32_t nseg = cnt->n_segments;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += gridDim.x * blockDim.x)
+        pair_offsets[s] = (s == 0) ? 0u : slot_counts[seg_base[s]];
+}
+
+// ---------------------------------------------------------------------------
+// predictor window (.h:77-188 count, .h:261-355 store)
+struct PWin {
+    Window w;
+    const uint32_t* lg3;
+    uint32_t ev, lt;
+};
+
+__device__ inline bool pred_window(const GridGeom& g, const TripletCols& tc, const uint32_t* grid,
+                                   const float* hx, const float* hy, const float* hz, const uint32_t* hlayer,
+                                   const uint32_t* hevt, uint32_t n_events, uint32_t firstHit,
+                                   uint32_t secondHit, bool store_variant, PWin& pw) {
+    const uint32_t ev = hevt[firstHit];
+    const uint32_t lt = hlayer[firstHit] - 1;  // (.h:88): the triplet index is the inner layer - 1
+    if (ev >= n_events || lt >= tc.n_trip) return false;
+    const uint32_t layer = tc.l3[lt] - 1;
+    if (layer >= g.n_layers || layer >= tc.n_geom) return false;
+    pw.ev = ev;
+    pw.lt = lt;
+    pw.lg3 = grid + ((size_t)ev * g.n_layers + layer) * g.cells;
+    Window& w = pw.w;
+    const float sigmaZ = tc.sigma_z[lt], sigmaPhi = tc.sigma_phi[lt];
+    const float minR = tc.min_r[layer], maxR = tc.max_r[layer];
+    const float mrc = tc.min_radius_curvature;
+    const float p1x = hx[firstHit], p1y = hy[firstHit], p1z = hz[firstHit];
+    const float p2x = hx[secondHit], p2y = hy[secondHit], p2z = hz[secondHit];
+    float tip2 = p1y * (p2x - p1x) - p1x * (p2y - p1y);
+    tip2 *= tip2;
+    tip2 /= ((p2x - p1x) * (p2x - p1x) + (p2y - p1y) * (p2y - p1y));
+    float rOrigin;
+    if (!store_variant) {
+        rOrigin = p1x * p1x + p1y * p1y;
+        rOrigin = sqrtf(rOrigin - tip2);
+    } else {
+        rOrigin = sqrtf(p1x * p1x + p1y * p1y);
+        rOrigin = sqrtf(rOrigin * rOrigin - tip2);
+    }
+    const float cotTheta = (p2z - p1z) / (sqrtf(p2x * p2x + p2y * p2y - tip2) - rOrigin);
+    float tmp = p1z + (sqrtf(maxR * maxR - tip2) - rOrigin) * cotTheta;
+    float zLow = p1z + (sqrtf(minR * minR - tip2) - rOrigin) * cotTheta;
+    float zHigh = (float)(tmp > zLow) * tmp + (float)(tmp < zLow) * zLow;
+    zLow = (float)(tmp > zLow) * zLow + (float)(tmp < zLow) * tmp;
+    zLow -= sigmaZ;
+    zHigh += sigmaZ;
+    const int zl = tx_f2i_sat(floorf((zLow - g.min_z) / g.sector_z));
+    w.zLo = (uint32_t)(zl > 0 ? zl : 0);
+    const uint32_t zh = tx_f2u_sat(floorf((zHigh - g.min_z) / g.sector_z)) + 1u;
+    w.zHi = zh < g.n_z ? zh : g.n_z;
+    const float phi = tx_atan2f(p2y, p2x);
+    tmp = tx_atan2f(p1y, p1x);
+    const float PI = TX_PI_F;
+    float dPhi = phi - tmp;
+    dPhi += (dPhi > PI) ? -2 * PI : (dPhi < -PI) ? 2 * PI : 0;
+    dPhi = fabsf(dPhi);
+    const float dHits = sqrtf((p2x - p1x) * (p2x - p1x) + (p2y - p1y) * (p2y - p1y));
+    tmp = fabsf(tx_acosf(dHits / (2 * mrc)) - tx_acosf(maxR / (2 * mrc)));
+    float phiLow = fabsf(tx_acosf(dHits / (2 * mrc)) - tx_acosf(minR / (2 * mrc)));
+    const float m2 = (tmp < phiLow) ? phiLow : tmp;  // OpenCL max(x, y): y if x < y, else x
+    dPhi = (dPhi < m2) ? m2 : dPhi;
+    float phiHigh = phi + dPhi + sigmaPhi;
+    phiLow = phi - dPhi - sigmaPhi;
+    w.wrap = phiLow < -PI || phiHigh > PI || phiLow > PI || phiHigh < -PI;
+    phiLow -= (float)((phiLow > PI) * 2) * PI;
+    phiLow += (float)((phiLow < -PI) * 2) * PI;
+    phiHigh -= (float)((phiHigh > PI) * 2) * PI;
+    phiHigh += (float)((phiHigh < -PI) * 2) * PI;
+    const uint32_t pl = tx_f2u_sat(floorf((phiLow - g.min_phi) / g.sector_phi));
+    const uint32_t ph = tx_f2u_sat(floorf((phiHigh - g.min_phi) / g.sector_phi)) + 1u;
+    w.pLo = pl < g.n_phi + 1 ? pl : g.n_phi + 1;  // in range for every finite window
+    w.pHi = ph < g.n_phi ? ph : g.n_phi;
+    w.zLow = zLow; w.zHigh = zHigh; w.phiLow = phiLow; w.phiHigh = phiHigh;
+    return true;
+}
+
+__global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
+                                const float* __restrict__ hx, const float* __restrict__ hy,
+                                const float* __restrict__ hz, const uint32_t* __restrict__ hlayer,
+                                const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
+                                uint32_t* __restrict__ counts, DevCounts* __restrict__ cnt) {
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    const uint32_t np1 = g.n_phi + 1;
+    for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid <= n; gid += gridDim.x * blockDim.x) {
+        uint32_t nFound = 0;
+        if (gid < n) {
+            const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
+            PWin pw;
+            if (h1 != kSentinel && h2 != kSentinel &&
+                pred_window(g, tc, grid, hx, hy, hz, hlayer, hevt, n_events, h1, h2, false, pw)) {
+                const Window& w = pw.w;
+                for (uint32_t zs = w.zLo; zs < w.zHi; ++zs) {  // (.h:190-217)
+                    const uint32_t zStart = pw.lg3[zs * np1];
+                    const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
+                    const uint32_t zLen = zEnd - zStart;
+                    const uint32_t end2 = (uint32_t)w.wrap * zEnd + pw.lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+                    for (uint32_t j = pw.lg3[zs * np1 + w.pLo]; j < end2; ++j) {
+                        const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
+                        nFound += in_window(w, hz[index], tx_atan2f(hy[index], hx[index]));
+                    }
+                }
+            }
+        }
+        counts[gid] = nFound;
+    }
+}
+
+__global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
+                                const float* __restrict__ hx, const float* __restrict__ hy,
+                                const float* __restrict__ hz, const uint32_t* __restrict__ hlayer,
+                                const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
+                                const uint32_t* __restrict__ counts, uint32_t* __restrict__ cands,
+                                uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
+                                DevCounts* __restrict__ cnt) {
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    const uint32_t np1 = g.n_phi + 1;
+    uint32_t short_slots = 0, trunc = 0;
+    bool over = false;
+    for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += gridDim.x * blockDim.x) {
+        uint32_t pos = counts[gid];
+        const uint32_t next = counts[gid + 1];
+        if (next > n_cand_cap) { over = true; continue; }
+        const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
+        PWin pw;
+        const bool ok = h1 != kSentinel && h2 != kSentinel &&
+                        pred_window(g, tc, grid, hx, hy, hz, hlayer, hevt, n_events, h1, h2, true, pw);
+        if (ok && pos < next) {
+            const Window& w = pw.w;
+            bool more = false;
+            for (uint32_t zs = w.zLo; zs < w.zHi && !more; ++zs) {  // (.h:360-402)
+                const uint32_t zStart = pw.lg3[zs * np1];
+                const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
+                const uint32_t zLen = zEnd - zStart;
+                const uint32_t end2 = (uint32_t)w.wrap * zEnd + pw.lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+                for (uint32_t j = pw.lg3[zs * np1 + w.pLo]; j < end2; ++j) {
+                    const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
+                    const bool v = in_window(w, hz[index], tx_atan2f(hy[index], hx[index]));
+                    if (!(pos < next)) {
+                        if (v) { more = true; break; }
+                        continue;
+                    }
+                    if (v) {
+                        cands[2 * (size_t)pos] = gid;
+                        cands[2 * (size_t)pos + 1] = index;
+                        pos++;
+                    }
+                }
+            }
+            trunc += more;
+        }
+        for (uint32_t q = pos; q < next; ++q) {
+            cands[2 * (size_t)q] = kSentinel;
+            cands[2 * (size_t)q + 1] = kSentinel;
+        }
+        short_slots += next - pos;
+        // triplet offsets at the end of each (event, layer triplet) group (.h:404-418)
+        if (h1 != kSentinel) {
+            const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
+            if (ev < n_events && lt < tc.n_trip) {
+                bool last = true;
+                if (gid + 1 < n) {
+                    const uint32_t nh = pairs[2 * (size_t)(gid + 1)];
+                    if (nh != kSentinel) last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
+                }
+                if (last) cand_offsets[ev * tc.n_trip + lt + 1] = next;
+            }
+        }
+    }
+    if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
+    if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+}  // namespace
+
+hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
+                            const uint32_t* grid, uint32_t* seg_base, DevCounts* cnt, hipStream_t st) {
+    const uint32_t blocks = (n_seg_cap + 1 + 255) / 256;
+    hipLaunchKernelGGL(k_pair_plan, dim3(blocks), dim3(256), 0, st, g, tc, grid, seg_base, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
+                             uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
+                             const float* hz, const uint32_t* seg_base, uint32_t* slot_counts,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    (void)n_seg_cap;
+    const uint32_t b = lc.pair_block ? lc.pair_block : 256;
+    hipLaunchKernelGGL(k_pair_count, dim3(stride_blocks((uint64_t)n_slot_cap + 1, b, lc)), dim3(b), 0, st, g,
+                       tc, grid, hx, hy, hz, seg_base, slot_counts, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
+                             uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
+                             const float* hx, const float* hy, const float* hz, const uint32_t* seg_base,
+                             const uint32_t* slot_counts, uint32_t* pairs, uint32_t* pair_offsets,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t b = lc.pair_block ? lc.pair_block : 256;
+    hipLaunchKernelGGL(k_pair_store, dim3(stride_blocks(n_slot_cap, b, lc)), dim3(b), 0, st, g, tc, grid, hx,
+                       hy, hz, seg_base, slot_counts, pairs, n_pair_cap, cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pair_offsets, dim3((n_seg_cap + 1 + 255) / 256), dim3(256), 0, st, seg_base,
+                       slot_counts, pair_offsets, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
+                                const uint32_t* grid, const float* hx, const float* hy, const float* hz,
+                                const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* pairs,
+                                uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t b = lc.predict_block ? lc.predict_block : 256;
+    hipLaunchKernelGGL(k_predict_count, dim3(stride_blocks((uint64_t)n_pair_cap + 1, b, lc)), dim3(b), 0, st,
+                       g, tc, grid, hx, hy, hz, hlayer, hevt, pairs, counts, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
+                                uint32_t n_cand_cap, const uint32_t* grid, const float* hx, const float* hy,
+                                const float* hz, const uint32_t* hlayer, const uint32_t* hevt,
+                                const uint32_t* pairs, const uint32_t* counts, uint32_t* cands,
+                                uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc,
+                                hipStream_t st) {
+    const uint32_t b = lc.predict_block ? lc.predict_block : 256;
+    hipLaunchKernelGGL(k_predict_store, dim3(stride_blocks(n_pair_cap, b, lc)), dim3(b), 0, st, g, tc, grid,
+                       hx, hy, hz, hlayer, hevt, pairs, counts, cands, cand_offsets, n_cand_cap, cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
+}
+
+}  // namespace trax

@@ -1,0 +1,203 @@
+// Generic primitives: exclusive u32 scan (the PrefixSum of the reference,
+// src/algorithms/PrefixSum.h:58-248) and u32 fill.
+//
+// The reference scans with a recursive Blelloch tree (2*wg elements per
+// work-group, one launch per level plus a uniform-add pass per level).  Here
+// the scan is a single-pass decoupled look-back scan: each 256-thread block
+// owns a tile of 2048 elements, publishes its aggregate, and wave 0 looks back
+// over up to 64 predecessor tiles at a time.  Every element is read once and
+// written once (8 bytes per element of HBM traffic).  The result is the same
+// exclusive running sum modulo 2^32.
+#include "kernels.h"
+
+namespace trax {
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ inline uint32_t pad_idx(uint32_t i) { return i + (i >> 5); }
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_excl(uint32_t* __restrict__ data,
+                                                        const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                        uint32_t* __restrict__ d_total,
+                                                        uint64_t* __restrict__ status,
+                                                        uint32_t* __restrict__ counter, uint32_t n_cap) {
+    __shared__ uint32_t s_items[kScanTile + kScanTile / 32];
+    __shared__ uint32_t s_wsum[kScanBlock / kWave + 1];
+    __shared__ uint32_t s_tile, s_prefix;
+    const uint32_t tid = threadIdx.x;
+    uint32_t n = (d_n ? *d_n : 0u) + n_add;
+    if (n > n_cap) n = n_cap;  // a size past its buffer is reported by the stage kernels
+    if (tid == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * kScanTile;
+    if (base >= n) {
+        if (tile == 0 && tid == 0 && d_total) *d_total = 0;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t li = k * kScanBlock + tid;
+        const uint64_t i = base + li;
+        s_items[pad_idx(li)] = i < n ? data[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = s_items[pad_idx(tid * kScanItems + k)];
+        sum += v[k];
+    }
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(sum, s_wsum, &total);
+    if (tid < kWave) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            if (tid == 0) atomic_store_u64(&status[0], (2ull << 32) | total);
+        } else {
+            if (tid == 0) atomic_store_u64(&status[tile], (1ull << 32) | total);
+            int64_t j = (int64_t)tile - 1 - (int64_t)tid;
+            while (true) {
+                uint64_t s;
+                uint32_t flag;
+                while (true) {
+                    s = (j >= 0) ? atomic_load_u64(&status[j]) : (2ull << 32);
+                    flag = (uint32_t)(s >> 32);
+                    if (__ballot(flag == 0u) == 0ull) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                const uint64_t incl = __ballot(flag == 2u);
+                const uint32_t stop = incl ? (uint32_t)(__ffsll((unsigned long long)incl) - 1) : 63u;
+                excl += wave_sum(tid <= stop ? (uint32_t)s : 0u);
+                if (incl) break;
+                j -= kWave;
+            }
+            if (tid == 0) atomic_store_u64(&status[tile], (2ull << 32) | (uint32_t)(excl + total));
+        }
+        if (tid == 0) s_prefix = excl;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + ex;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        s_items[pad_idx(tid * kScanItems + k)] = run;
+        run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t li = k * kScanBlock + tid;
+        const uint64_t i = base + li;
+        if (i < n) data[i] = s_items[pad_idx(li)];
+    }
+    if (d_total && tid == 0 && base + kScanTile >= n) *d_total = s_prefix + total;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u32_dn(uint32_t* __restrict__ p, uint32_t v, const uint32_t* __restrict__ d_n,
+                              uint32_t n_add) {
+    const uint32_t n = *d_n + n_add;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Sequential forward fill of zero entries ("monotonize", TripletThetaPhiPredictor.h:
+// 432-447, TripletThetaPhiFilter.h:267-280, read in program order): one block
+// scans with the operator (a, b) -> b != 0 ? b : a.
+__global__ __launch_bounds__(256) void k_forward_fill(uint32_t* __restrict__ off,
+                                                     const DevCounts* __restrict__ cnt, uint32_t n_trip) {
+    __shared__ uint32_t s_last[256 / kWave];
+    __shared__ uint32_t s_carry;
+    const uint32_t n = cnt->n_events * n_trip + 1;
+    const uint32_t tid = threadIdx.x, w = tid / kWave, l = lane_id();
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += blockDim.x) {
+        const uint32_t i = base + tid;
+        uint32_t v = i < n ? off[i] : 0u;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t o = __shfl_up(v, d, kWave);
+            if (l >= (uint32_t)d && v == 0u) v = o;
+        }
+        if (l == kWave - 1) s_last[w] = v;
+        __syncthreads();
+        uint32_t carry = s_carry;
+        for (uint32_t k = 0; k < w; ++k)
+            if (s_last[k] != 0u) carry = s_last[k];
+        if (v == 0u) v = carry;
+        if (i < n) off[i] = v;
+        __syncthreads();
+        if (tid == blockDim.x - 1) s_carry = v;
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc) {
+    static uint32_t n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n_cu = (uint32_t)v;
+        else
+            n_cu = 256;
+    }
+    const uint64_t per_cu = lc.blocks_per_cu ? lc.blocks_per_cu : 8;
+    uint64_t want = (work_cap + block - 1) / block;
+    const uint64_t mx = per_cu * n_cu;
+    if (want > mx) want = mx;
+    if (want < 1) want = 1;
+    return (uint32_t)want;
+}
+
+hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_trip, hipStream_t st) {
+    hipLaunchKernelGGL(k_forward_fill, dim3(1), dim3(256), 0, st, off, cnt, n_trip);
+    return hipGetLastError();
+}
+
+uint32_t scan_tiles(uint64_t n_cap) { return (uint32_t)((n_cap + kScanTile - 1) / kScanTile) + 1; }
+
+hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_dn(uint32_t* p, uint32_t v, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
+                          hipStream_t st) {
+    uint64_t blocks = (n_cap + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32_dn, dim3((uint32_t)blocks), dim3(256), 0, st, p, v, d_n, n_add);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(uint32_t* data, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
+                       uint32_t* d_total, const ScanScratch& s, hipStream_t st) {
+    const uint32_t tiles = scan_tiles(n_cap);
+    if (tiles > s.max_tiles) return hipErrorInvalidValue;
+    // status words and the tile counter are cleared by a kernel, not a
+    // memset node, so that the sequence can be captured into a graph
+    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(s.status), 0u, 2ull * tiles + 2, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kScanBlock), 0, st, data, d_n, n_add, d_total,
+                       s.status, reinterpret_cast<uint32_t*>(s.status + tiles),
+                       (uint32_t)(n_cap < 0xffffffffull ? n_cap : 0xffffffffull));
+    return hipGetLastError();
+}
+
+}  // namespace trax
