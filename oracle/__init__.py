@@ -129,6 +129,46 @@ def run(cols, wl, min_radius_curvature, stop_after=0, sparse_pairs=True, workers
         L.oracle_free(h)
 
 
+def run_link(hits, wl, min_radius_curvature, h1, h2, h3, n_events=1):
+    """Connectivity, circle fit and CA of the oracle on given tracklets.  `hits`
+    holds grid-ordered columns x, y, z, layer, event (a hand-built fixture)."""
+    keep = []
+
+    def a(v, dt):
+        v = np.ascontiguousarray(v, dtype=dt)
+        keep.append(v)
+        return C.c_void_p(v.ctypes.data)
+
+    inp = Input()
+    inp.x, inp.y, inp.z = a(hits["x"], np.float32), a(hits["y"], np.float32), a(hits["z"], np.float32)
+    inp.layer, inp.event = a(hits["layer"], np.uint32), a(hits["event"], np.uint32)
+    inp.n_hits = len(hits["x"])
+    inp.n_events = n_events
+    inp.n_layers = max(t.layer3 for t in wl.triplets)
+    inp.n_trip = len(wl.triplets)
+    inp.n_threads = wl.n_threads
+    inp.d_eta_cut = wl.d_eta_cut
+    inp.min_radius_curvature = min_radius_curvature
+    inp.n_workers = 1
+    L = lib()
+    L.oracle_run_link.restype = P
+    L.oracle_run_link.argtypes = [C.POINTER(Input), P, P, P, U32]
+    h = L.oracle_run_link(C.byref(inp), a(h1, np.uint32), a(h2, np.uint32), a(h3, np.uint32), len(h1))
+    try:
+        out = {}
+        for k in NAMES + ["tracklet_offsets"]:
+            d, f = P(), C.c_int()
+            n = L.oracle_get(h, k.encode(), C.byref(d), C.byref(f))
+            if n < 0:
+                continue
+            dt = np.float32 if f.value else np.uint32
+            out[k] = (np.zeros(0, dt) if n == 0 else
+                      np.ctypeslib.as_array(C.cast(d, C.POINTER(C.c_uint32)), shape=(n,)).view(dt).copy())
+        return out
+    finally:
+        L.oracle_free(h)
+
+
 def prefix_sum(values, wg):
     v = np.ascontiguousarray(values, dtype=np.uint32).copy()
     lib().oracle_prefix_sum(C.c_void_p(v.ctypes.data), v.size, wg)

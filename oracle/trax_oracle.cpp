@@ -572,6 +572,9 @@ bool predict(Ctx& c) {
             u32 pos = prefix[gid];
             const u32 next = prefix[gid + 1];
             if (h1 == SENT || h2 == SENT) continue;
+            // a pair whose count is zero writes nothing (.h:376-377); the
+            // truncation diagnostic covers the pairs with a nonzero count
+            if (pos == next) continue;
             PredWindow w; u32 ev;
             if (!pred_window(c, h1, h2, true, w, ev)) { short_by[gid] = next - pos; continue; }
             const u32* lGrid3 = &c.grid[((size_t)ev * c.L + w.lay3) * c.C];
@@ -657,6 +660,8 @@ bool filter_valid(const Ctx& c, u32 h1, u32 h2, u32 h3, u32 lt) {
     return valid;
 }
 
+void tracklet_offsets(Ctx& c);
+
 bool filter(Ctx& c) {
     const Input& in = c.in;
     const u32 nCand = (u32)(c.cands.size() / 2);
@@ -694,9 +699,16 @@ bool filter(Ctx& c) {
                 pos++;
             }
     }
+    tracklet_offsets(c);
+    return true;
+}
+
+// filterOffsetStore (.h:233-256) and filterOffsetMonotonizeStore (.h:267-280)
+void tracklet_offsets(Ctx& c) {
+    const u32 R = (u32)c.th1.size();
     c.t_off.assign(c.E * c.T + 1, 0);
     const size_t n_off = c.t_off.size();
-    for (u32 gid = 0; gid < R; ++gid) {  // filterOffsetStore (.h:233-256)
+    for (u32 gid = 0; gid < R; ++gid) {
         const u32 hitId = c.th1[gid];
         const u32 layerTriplet = c.hlayer[hitId];
         const u32 event = c.hevt[hitId];
@@ -711,7 +723,6 @@ bool filter(Ctx& c) {
         }
     }
     forward_fill(c.t_off);
-    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -946,6 +957,26 @@ double oracle_timing(oracle_result* h, const char* stage) {
 }
 
 void oracle_free(oracle_result* h) { delete reinterpret_cast<Result*>(h); }
+
+// connectivity + circle fit + CA on given tracklets; the Input's hit columns
+// are taken as already grid-ordered (hand-built fixtures)
+oracle_result* oracle_run_link(const Input* in, const uint32_t* h1, const uint32_t* h2, const uint32_t* h3,
+                               uint32_t n) {
+    Result* r = new Result();
+    Ctx c(*in, *r);
+    c.hx.assign(in->x, in->x + in->n_hits);
+    c.hy.assign(in->y, in->y + in->n_hits);
+    c.hz.assign(in->z, in->z + in->n_hits);
+    c.hlayer.assign(in->layer, in->layer + in->n_hits);
+    c.hevt.assign(in->event, in->event + in->n_hits);
+    c.th1.assign(h1, h1 + n);
+    c.th2.assign(h2, h2 + n);
+    c.th3.assign(h3, h3 + n);
+    tracklet_offsets(c);
+    r->put("tracklet_offsets", c.t_off);
+    connectivity_ca(c);
+    return reinterpret_cast<oracle_result*>(r);
+}
 
 // the reference's PrefixSum on a host array (KAT and brute-force tests)
 void oracle_prefix_sum(uint32_t* data, uint32_t n, uint32_t wg) {

@@ -1,0 +1,123 @@
+"""Host-side checks that need no GPU: the float built-ins against float64,
+HitCollection::addEvent ordering against a plain restatement, the configuration
+constants, and the C ABI exports."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import small_workload
+from trax_amd import make_events, workload
+from trax_amd.api import Config, HitSet, LIB_PATH, math_host
+from trax_amd.config import min_radius_curvature
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _ulp_err(got, ref64):
+    r32 = ref64.astype(np.float32)
+    ulp = np.spacing(np.abs(r32)).astype(np.float64)
+    return np.abs(got.astype(np.float64) - ref64) / ulp
+
+
+def test_atan2_accuracy():
+    rng = np.random.default_rng(0)
+    y = rng.uniform(-100, 100, 2_000_000).astype(np.float32)
+    x = rng.uniform(-100, 100, 2_000_000).astype(np.float32)
+    e = _ulp_err(math_host(0, x, y), np.arctan2(y.astype(np.float64), x.astype(np.float64)))
+    assert e.max() <= 2.0
+
+
+def test_atan2_special_values():
+    sy = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.inf, 1.0], np.float32)
+    sx = np.array([0.0, 0.0, -0.0, -0.0, np.inf, -np.inf, np.inf, -np.inf, 1.0, 0.0], np.float32)
+    got = math_host(0, sx, sy)
+    ref = np.arctan2(sy, sx)
+    assert np.array_equal(got.view(np.uint32), ref.astype(np.float32).view(np.uint32))
+    assert np.isnan(math_host(0, np.array([np.nan], np.float32), np.array([1.0], np.float32)))[0]
+
+
+def test_acos_atan_asinh_accuracy():
+    a = np.linspace(-1, 1, 2_000_001).astype(np.float32)
+    assert _ulp_err(math_host(1, a), np.arccos(a.astype(np.float64))).max() <= 2.0
+    assert np.isnan(math_host(1, np.array([1.0000001, -1.5, np.nan], np.float32))).all()
+    w = np.concatenate([np.linspace(-60, 60, 1_000_001), np.geomspace(1e-30, 1e30, 200_000),
+                        -np.geomspace(1e-30, 1e30, 200_000)]).astype(np.float32)
+    assert _ulp_err(math_host(2, w), np.arctan(w.astype(np.float64))).max() <= 2.0
+    assert _ulp_err(math_host(3, w), np.arcsinh(w.astype(np.float64))).max() <= 4.0
+
+
+def test_min_radius_curvature():
+    # TripletConfiguration.cpp:36-43 at minPt = 1 GeV: 87.52 cm
+    cfg = Config(workload(2))
+    assert cfg.min_radius_curvature == pytest.approx(87.522, abs=1e-3)
+    assert cfg.min_radius_curvature == min_radius_curvature(1.0)
+
+
+def _add_event_restated(ev, triplets, min_pt, evt_in_group):
+    """HitCollection::addEvent (src/datastructures/HitCollection.cpp:21-180),
+    in plain Python loops."""
+    used = set()
+    max_layer = 0
+    for t in triplets:
+        used |= {t.layer1, t.layer2, t.layer3}
+        max_layer = max(max_layer, t.layer3)
+    tracks = {}
+    for i in range(ev.n_hits):
+        if ev.track_pt[i] < np.float32(min_pt):
+            continue
+        tracks.setdefault(int(ev.track_id[i]), []).append(i)
+    layers, findable = {}, {}
+    for tid in sorted(tracks):
+        hits = sorted(tracks[tid], key=lambda i: int(ev.layer[i]))  # stable
+        cov = {int(ev.layer[i]) for i in hits}
+        missing = any(l not in cov for l in used)
+        for i in hits:
+            if not missing:
+                findable.setdefault(tid, []).append(i)
+            else:
+                layers.setdefault(int(ev.layer[i]), []).append(i)
+    for tid in sorted(findable):
+        for i in findable[tid]:
+            if ev.layer[i] <= max_layer:
+                layers.setdefault(int(ev.layer[i]), []).append(i)
+    order = [i for l in range(1, max_layer + 1) for i in layers.get(l, [])]
+    counts = [len(layers.get(l, [])) for l in range(1, max_layer + 1)]
+    return np.array(order, np.int64), counts
+
+
+def test_add_event_matches_restatement():
+    wl = small_workload(3, events=3, tracks_min=10, tracks_max=60)
+    evs = make_events(wl, seed=9)
+    # make some hits fall below minPt and drop some layers of a few tracks
+    for ev in evs:
+        ev.track_pt[::17] = 0.5
+        keep = np.ones(ev.n_hits, bool)
+        keep[(ev.track_id % 7 == 3) & (ev.layer == 4)] = False
+        for k in ("x", "y", "z", "layer", "det_id", "track_id", "track_pt"):
+            setattr(ev, k, getattr(ev, k)[keep])
+    cfg = Config(wl)
+    hs = HitSet(cfg, evs)
+    cols = hs.columns()
+    start = 0
+    for e, ev in enumerate(evs):
+        order, counts = _add_event_restated(ev, wl.triplets, wl.min_pt, e)
+        n = order.size
+        assert cols["event_nhits"][e] == n and cols["event_offset"][e] == start
+        assert list(cols["layer_nhits"][e * wl.n_layers:(e + 1) * wl.n_layers]) == counts
+        assert np.array_equal(cols["x"][start:start + n], ev.x[order])
+        assert np.array_equal(cols["hit_id"][start:start + n], ev.track_id[order])
+        assert np.all(cols["event"][start:start + n] == e)
+        start += n
+    assert start == hs.n_hits
+
+
+def test_abi_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "trax_hip.h")).read()
+    names = set(re.findall(r"\b(trax_[a-z0-9_]+)\s*\(", header))
+    assert len(names) > 30
+    lib = C.CDLL(LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
