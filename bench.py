@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the triplet-seeding path on one MI355X.
+
+Workload: 256 events with log-uniform track counts in [50, 5000] and Poisson
+noise (mean 5 hits per track), 10 barrel layers, 8 consecutive layer triplets,
+32 z x 128 phi grid.  A step is one pass of the whole path (grid build, pairs,
+prediction, filter, connectivity, circle fit, cellular automaton) over the
+256-event batch through the batched, graph-captured pipeline, with the hits
+already resident in device memory.  The plain per-event path (the reference's
+call sequence: one event per launch sequence, blocking count read-backs,
+exact allocations) is timed on a sample of the same events for comparison,
+and the oracle (the CPU restatement of the reference's kernels) on a smaller
+sample as the CPU baseline.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_EVENTS = 256
+SEED = 2024
+
+# HBM peak of MI355X (bytes/s)
+HBM_PEAK = 8.0e12
+
+
+def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads):
+    """Algorithmic (compulsory) HBM bytes of each phase: every input stream read
+    once, every output written once; the per-unit figures of the measurement
+    plan (grid 68 B/hit + 16 B/cell, scan 8 B/element, ...)."""
+    P, Q, R, K = counts["pairs"], counts["cands"], counts["tracklets"], counts["conn"]
+    slots = counts.get("slots", 0)
+    words = (Q + 63) // 64
+    return {
+        "grid_count": 12 * n_hits + 12 * cells,       # count (12 B/hit, 4 B/cell) + scan (8 B/cell)
+        "grid_store": 56 * n_hits + 4 * cells,        # 28 B/hit read + 28 B/hit write + cells
+        "pair_count": 12 * seg_hits + 4 * slots,
+        "pair_scan": 8 * slots,
+        "pair_store": 12 * seg_hits + 4 * slots + 8 * P,
+        "predict_count": 8 * P + 12 * trip_hits + 4 * P,
+        "predict_scan": 8 * P,
+        "predict_store": 8 * P + 12 * trip_hits + 8 * Q,
+        "filter_count": 8 * Q + 8 * Q + 12 * words,
+        "filter_scan": 8 * words,
+        "filter_store": 12 * words + 8 * R + 12 * R,
+        "link_count": 12 * R + 4 * R + 8 * R,
+        "link_store": 12 * R + 4 * R + 12 * K,
+    }
+
+
+def cpu_baseline(wl, events, cfg_cls, hitset_cls, budget_s=20.0):
+    import oracle
+    cores = min(16, os.cpu_count() or 1)
+    n, t = 0, 0.0
+    for k in range(4, len(events) + 1, 4):
+        sample = events[:k]
+        cfg = cfg_cls(wl)
+        hs = hitset_cls(cfg, sample)
+        t0 = time.perf_counter()
+        oracle.run(hs.columns(), wl, cfg.min_radius_curvature, workers=cores)
+        t = time.perf_counter() - t0
+        n = k
+        if t > budget_s / 2 or k >= 32:
+            break
+    return {"value": n / t, "unit": "events/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} of the {N_EVENTS} batch events as one event group, "
+                      f"oracle (C++ restatement of the reference kernels) on {cores} host threads, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-events", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (traffic-counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: events shard across GPUs with no communication, "
+              f"so --gpus {args.gpus} is not measured here", file=sys.stderr)
+        sys.exit(2)
+
+    from trax_amd import make_events, workload
+    from trax_amd.api import Batch, Caps, Config, Device, HitSet, run_batch_fit
+
+    wl = workload(3)
+    events = make_events(wl, N_EVENTS, seed=SEED)
+    cfg = Config(wl)
+    hs = HitSet(cfg, events)
+    dev = Device(0)
+
+    # capacity plan from a first run (grown on overflow), then fixed with headroom
+    probe, _ = run_batch_fit(dev, cfg, hs, Caps.for_hits(N_EVENTS, hs.n_hits, use_graph=False))
+    c = probe.counts()
+    probe.close()
+    head = lambda v: int(v * 1.15) + 4096
+    caps = Caps(N_EVENTS, hs.n_hits, head(c["pairs"]), head(c["cands"]), head(c["tracklets"]), head(c["conn"]),
+                head(c["track_entries"]), use_graph=not args.no_graph)
+    b = Batch(dev, cfg, caps)
+    b.submit(hs)  # upload once; later steps run on the resident hits
+    if b.wait() != 0:
+        raise RuntimeError("capacity overflow after planning")
+    for _ in range(args.warmup):
+        b.rerun()
+        b.wait()
+
+    phases = ["grid_count", "grid_store", "pair_plan", "pair_count", "pair_scan", "pair_store", "predict_count",
+              "predict_scan", "predict_store", "filter_count", "filter_scan", "filter_store", "link_prepare",
+              "link_count", "link_store", "circle_fit", "ca_forward", "ca_backward"]
+    acc = dict.fromkeys(phases, 0.0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.rerun()
+        if b.wait() != 0:
+            raise RuntimeError("capacity overflow")
+        for p in phases:
+            acc[p] += b.stage_ms(p)
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    counts = b.counts()
+    ev_s = N_EVENTS / (ms_step / 1e3)
+    trip_s = counts["tracklets"] / (ms_step / 1e3)
+    phase_ms = {p: acc[p] / args.steps for p in phases}
+
+    # PCIe-inclusive rate (hits uploaded every step)
+    t2 = time.perf_counter()
+    for _ in range(5):
+        b.submit(hs)
+        b.wait()
+    pcie_ev_s = 5 * N_EVENTS / (time.perf_counter() - t2)
+
+    # plain per-event path on a sample of the same events
+    n_plain = min(args.plain_events, N_EVENTS)
+    plain_ev_s, batch_ev_s_same_mix = None, None
+    if n_plain > 0:
+        singles = [HitSet(cfg, [ev]) for ev in events[:n_plain]]
+        dev.run_plain_bench(cfg, singles[0])  # warm-up
+        plain_ms = sum(dev.run_plain_bench(cfg, s) for s in singles)
+        sample_hits = sum(s.n_hits for s in singles)
+        plain_ev_s = n_plain / (plain_ms / 1e3)
+        # the batch rate on the same mix of event sizes
+        batch_ev_s_same_mix = ev_s * (hs.n_hits / N_EVENTS) / (sample_hits / n_plain)
+
+    if args.copy_calibration:
+        # a plain copy of known size: HBM traffic counters are scaled by what
+        # they read for it (4 B read + 4 B written per element)
+        import ctypes as C
+        from trax_amd.api import check, lib
+        n = 1 << 26
+        a, bb = C.c_void_p(), C.c_void_p()
+        check(lib().trax_buf_alloc(dev.h, 4 * n, C.byref(a)))
+        check(lib().trax_buf_alloc(dev.h, 4 * n, C.byref(bb)))
+        check(lib().trax_buf_fill_u32(dev.h, a, 7, n, None))
+        for _ in range(3):
+            check(lib().trax_buf_copy_u32(dev.h, bb, a, n, None))
+        check(lib().trax_stream_sync(dev.h, None))
+        lib().trax_buf_free(dev.h, a)
+        lib().trax_buf_free(dev.h, bb)
+
+    # roofline of the dominant phase, HBM-bound
+    cols = hs.columns()
+    lay = cols["layer_nhits"].reshape(N_EVENTS, wl.n_layers).astype(int)
+    seg_hits = sum(int(lay[:, t.layer1 - 1].sum() + lay[:, t.layer2 - 1].sum()) for t in wl.triplets)
+    trip_hits = sum(int(lay[:, t.layer1 - 1].sum() + lay[:, t.layer2 - 1].sum() + lay[:, t.layer3 - 1].sum())
+                    for t in wl.triplets)
+    cells = N_EVENTS * wl.n_layers * (wl.n_z + 1) * (wl.n_phi + 1)
+    counts["slots"] = hs.n_hits + N_EVENTS * len(wl.triplets) * wl.n_threads
+    pb = phase_bytes(hs.n_hits, seg_hits, trip_hits, counts, cells, wl.n_threads)
+    gbs = {p: pb[p] / (phase_ms[p] * 1e-3) / 1e9 for p in pb if phase_ms[p] > 0}
+    dom = max(pb, key=lambda p: phase_ms[p])
+    achieved = pb[dom] / (phase_ms[dom] * 1e-3)
+    grid_ms = phase_ms["grid_count"] + phase_ms["grid_store"]
+    grid_gbs = (pb["grid_count"] + pb["grid_store"]) / (grid_ms * 1e-3) / 1e9
+
+    line = {
+        "metric": "events/s & triplets/s, batched vs plain per-event HIP; p50/p99 event latency",
+        "value": round(ev_s, 1),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "events_per_step": N_EVENTS, "hits_per_step": hs.n_hits,
+                   "layers": wl.n_layers, "layer_triplets": len(wl.triplets), "grid": [wl.n_z, wl.n_phi],
+                   "seed": SEED, "graph": not args.no_graph},
+        "triplets_per_s": round(trip_s, 1),
+        "triplets_per_step": counts["tracklets"],
+        "pairs_per_step": counts["pairs"],
+        "candidates_per_step": counts["cands"],
+        "plain_events_per_s": round(plain_ev_s, 1) if plain_ev_s else None,
+        "plain_sample_events": n_plain,
+        "batched_vs_plain_same_mix": round(batch_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s else None,
+        "pcie_inclusive_events_per_s": round(pcie_ev_s, 1),
+        "grid_build_hbm_gbs": round(grid_gbs, 1),
+        "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+        "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": dom, "algorithmic_bytes": int(pb[dom])},
+        "latency_ms": {"p50": None, "p99": None},
+    }
+    b.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(wl, events, Config, HitSet)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

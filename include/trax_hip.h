@@ -71,6 +71,10 @@ int trax_buf_alloc(trax_ctx* ctx, uint64_t bytes, void** dptr);
 int trax_buf_free(trax_ctx* ctx, void* dptr);
 /* fill with a 32-bit value by a kernel (safe inside captured graphs) */
 int trax_buf_fill_u32(trax_ctx* ctx, void* dptr, uint32_t value, uint64_t count, trax_stream* s);
+/* device-to-device copy of count u32 values by a kernel (the CA's memcpy
+ * kernel, src/algorithms/CellularAutomaton.h:120-139); also the known-size
+ * copy used to calibrate HBM traffic counters */
+int trax_buf_copy_u32(trax_ctx* ctx, void* dst, const void* src, uint64_t count, trax_stream* s);
 /* clever transfer::toDevice / download, asynchronous on the stream */
 int trax_h2d(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s);
 int trax_d2h(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s);
@@ -140,6 +144,10 @@ float trax_config_min_radius_curvature(const trax_config* c);
  * stage to run (1 grid, 2 pairs, 3 predictor, 4 filter, 5 connectivity+CA). */
 int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_hits* hits, int stop_after,
                    trax_result** out);
+/* The same sequence with only the downloads the reference driver makes
+ * (grid and hits after GridBuilder, tracklets after the filter, CA outputs);
+ * *ms receives the host wall time from upload to the last download. */
+int trax_run_plain_bench(trax_ctx* ctx, const trax_config* cfg, const trax_hits* hits, double* ms);
 
 /* Batched pipeline: every stage of many events in one launch sequence with
  * device-resident sizes, preallocated capacities, captured in a hipGraph and
@@ -164,6 +172,8 @@ int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_ca
 void trax_batch_destroy(trax_batch* b);
 /* upload a HitCollection and enqueue the whole pipeline; asynchronous */
 int trax_batch_submit(trax_batch* b, const trax_hits* hits);
+/* enqueue the pipeline again on the inputs already resident on the device */
+int trax_batch_rerun(trax_batch* b);
 /* wait for the submitted run; TRAX_ERR_CAPACITY if an output overflowed */
 int trax_batch_wait(trax_batch* b);
 /* results of the last completed run (downloads them) */
@@ -174,9 +184,13 @@ int trax_batch_counts(trax_batch* b, uint32_t out[8]);
 /* device bytes held by the batch workspace */
 uint64_t trax_batch_footprint(const trax_batch* b);
 
-/* per-kernel timing of the last batched run, from HIP events around each
- * stage on the batch's stream: stage names "grid", "pairs", "predict",
- * "filter", "connectivity", "ca"; milliseconds */
+/* timing of the last batched run from HIP events recorded between the phases
+ * on the batch's stream, in milliseconds.  Phases: "grid_count", "grid_store",
+ * "pair_plan", "pair_count", "pair_scan", "pair_store", "predict_count",
+ * "predict_scan", "predict_store", "filter_count", "filter_scan",
+ * "filter_store", "link_prepare", "link_count", "link_store", "circle_fit",
+ * "ca_forward", "ca_backward"; stages (the reference's algorithm classes):
+ * "grid", "pairs", "predict", "filter", "connectivity", "ca"; and "total". */
 int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms);
 
 /* named result arrays: "grid", "hits.x", ..., "pairs", "pair_offsets",

@@ -55,7 +55,8 @@ def test_scan_matches_closed_form(dev, n):
     rng = np.random.default_rng(n)
     v = rng.integers(0, 1 << 12, n, dtype=np.uint32)
     out, total = _scan_device(dev, v)
-    ref = np.concatenate([[0], np.cumsum(v.astype(np.uint64))[:-1]]).astype(np.uint32)
+    run = np.concatenate([np.zeros(1, np.uint64), np.cumsum(v.astype(np.uint64))[:-1]])
+    ref = (run & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     assert np.array_equal(out, ref)
     assert total == int(v.astype(np.uint64).sum() & 0xFFFFFFFF)
 

@@ -91,6 +91,8 @@ def lib():
         "trax_batch_destroy": (None, [P]),
         "trax_batch_submit": (C.c_int, [P, P]),
         "trax_batch_wait": (C.c_int, [P]),
+        "trax_batch_rerun": (C.c_int, [P]),
+        "trax_run_plain_bench": (C.c_int, [P, P, P, C.POINTER(C.c_double)]),
         "trax_batch_result": (C.c_int, [P, C.POINTER(P)]),
         "trax_batch_counts": (C.c_int, [P, C.POINTER(U32)]),
         "trax_batch_footprint": (C.c_uint64, [P]),
@@ -233,6 +235,12 @@ class Device:
         check(lib().trax_run_plain(self.h, cfg.h, hits.h, stop_after, C.byref(rh)))
         return _result_dict(rh)
 
+    def run_plain_bench(self, cfg: Config, hits: HitSet) -> float:
+        """Plain path with only the reference driver's own downloads; host ms."""
+        ms = C.c_double()
+        check(lib().trax_run_plain_bench(self.h, cfg.h, hits.h, C.byref(ms)))
+        return float(ms.value)
+
     def math_device(self, fn: int, x: np.ndarray, y: Optional[np.ndarray] = None) -> np.ndarray:
         x = np.ascontiguousarray(x, np.float32)
         y = None if y is None else np.ascontiguousarray(y, np.float32)
@@ -321,6 +329,10 @@ class Batch:
 
     def submit(self, hits: HitSet):
         check(lib().trax_batch_submit(self.h, hits.h))
+
+    def rerun(self):
+        """Run the pipeline again on the inputs already resident on the device."""
+        check(lib().trax_batch_rerun(self.h))
 
     def wait(self) -> int:
         rc = lib().trax_batch_wait(self.h)

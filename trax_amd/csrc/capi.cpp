@@ -167,6 +167,14 @@ extern "C" int trax_buf_fill_u32(trax_ctx* ctx, void* dptr, uint32_t value, uint
     });
 }
 
+extern "C" int trax_buf_copy_u32(trax_ctx* ctx, void* dst, const void* src, uint64_t count, trax_stream* s) {
+    return guarded([&] {
+        hip_check(launch_copy(static_cast<uint32_t*>(dst), static_cast<const uint32_t*>(src), count, pick(ctx, s)),
+                  "copy");
+        return TRAX_OK;
+    });
+}
+
 extern "C" int trax_h2d(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_stream* s) {
     return guarded([&] {
         hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick(ctx, s)), "h2d");
@@ -330,14 +338,17 @@ static void validate_config(const trax_config* cfg) {
 
 // ---- plain per-event-group path ---------------------------------------------------
 
-extern "C" int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_hits* h, int stop_after,
-                              trax_result** out) {
+// collect = false keeps only the downloads the reference driver itself makes
+// (grid and hits after the build, tracklets after the filter, the CA outputs)
+static int run_plain_impl(trax_ctx* ctx, const trax_config* cfg, const trax_hits* h, int stop_after, bool collect,
+                          trax_result** out) {
     return guarded([&] {
         validate_config(cfg);
         Context& c = *ctx->c;
         hip_check(hipSetDevice(c.device()), "hipSetDevice");
         auto* r = new trax_result;
         std::unique_ptr<trax_result> guard(r);
+        auto D = [&](const UVec& v) { return collect ? v.download(c) : std::vector<uint32_t>(); };
         const uint32_t E = h->n_events, L = h->max_layer, T = cfg->trip.size(), N = h->hits.size();
         const uint32_t nt = cfg->n_threads;
         auto t0 = std::chrono::steady_clock::now();
@@ -376,14 +387,14 @@ extern "C" int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_
         if (!want(2)) { diag(); *out = guard.release(); return TRAX_OK; }
         ConfigMirror cm(c, cfg->trip, cfg->geom, nt, cfg->d_eta_cut);
         std::unique_ptr<Pairing> pairs(PairGeneratorBeamspot(c).run(dh, cm, nt, cfg->trip, grid));
-        r->put("pairs", pairs->pairing.download(c));
-        r->put("pair_offsets", pairs->pairingOffsets.download(c));
+        r->put("pairs", D(pairs->pairing));
+        r->put("pair_offsets", D(pairs->pairingOffsets));
         lap("pairs");
         if (!want(3)) { diag(); *out = guard.release(); return TRAX_OK; }
         std::unique_ptr<Pairing> cands(TripletThetaPhiPredictor(c).run(dh, DetectorGeometry(), cm, Dictionary(), (int)nt,
                                                                       cfg->trip, grid, *pairs));
-        r->put("cands", cands->pairing.download(c));
-        r->put("cand_offsets", cands->pairingOffsets.download(c));
+        r->put("cands", D(cands->pairing));
+        r->put("cand_offsets", D(cands->pairingOffsets));
         lap("predict");
         if (!want(4)) { diag(); *out = guard.release(); return TRAX_OK; }
         std::unique_ptr<TrackletCollection> tr(
@@ -391,15 +402,15 @@ extern "C" int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_
         pairs.reset();  // TripletBuilder.cpp:357-358
         cands.reset();
         r->put("tracklets.h1", tr->h1); r->put("tracklets.h2", tr->h2); r->put("tracklets.h3", tr->h3);
-        r->put("tracklet_offsets", tr->trackletOffsets.download(c));
+        r->put("tracklet_offsets", D(tr->trackletOffsets));
         lap("filter");
-        diag();
+        if (collect) diag();
         if (!want(5)) { *out = guard.release(); return TRAX_OK; }
         ConnectivityOut co = TripletConnectivityTight(c).run(dh, *tr, cm, cfg->trip, cfg->d_eta_cut, nt);
-        if (co.eta) r->putf("eta", co.eta->download(c));
+        if (co.eta && collect) r->putf("eta", co.eta->download(c));
         else r->putf("eta", {});
-        auto dl = [&](const std::unique_ptr<UVec>& v, uint32_t n) {
-            std::vector<uint32_t> a = v ? v->download(c) : std::vector<uint32_t>();
+        auto dl = [&](const std::unique_ptr<UVec>& v, uint32_t n, bool always = false) {
+            std::vector<uint32_t> a = (v && (collect || always)) ? v->download(c) : std::vector<uint32_t>();
             a.resize(std::min<size_t>(a.size(), n));
             return a;
         };
@@ -413,16 +424,31 @@ extern "C" int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_
         std::unique_ptr<FVec> pt(TrackletCircleFitter(c).run(dh, *tr, *co.connectable, co.nConnectable, nt));
         lap("circle");
         CAOut ca = CellularAutomaton(c).run(*co.basis, *co.followers, co.nConn, *pt, tr->size(), nt);
-        r->putf("pt", pt->download(c));
-        r->put("ca.states", ca.states->download(c));
-        r->put("ca.best_basis", ca.bestBasis->download(c));
-        r->put("tracks", dl(ca.trackCollection, ca.nTrackEntries));
+        r->putf("pt", pt->download(c));  // TripletBuilder.cpp:474-481
+        if (collect) r->put("ca.states", ca.states->download(c));
+        if (collect) r->put("ca.best_basis", ca.bestBasis->download(c));
+        r->put("tracks", dl(ca.trackCollection, ca.nTrackEntries, true));
         r->put("track_offsets", ca.trackOffsets->download(c));
         r->put("ca.iterations", {ca.iterations});
         lap("ca");
         *out = guard.release();
         return TRAX_OK;
     });
+}
+
+extern "C" int trax_run_plain(trax_ctx* ctx, const trax_config* cfg, const trax_hits* h, int stop_after,
+                              trax_result** out) {
+    return run_plain_impl(ctx, cfg, h, stop_after, true, out);
+}
+
+extern "C" int trax_run_plain_bench(trax_ctx* ctx, const trax_config* cfg, const trax_hits* h, double* ms) {
+    trax_result* r = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = run_plain_impl(ctx, cfg, h, 0, false, &r);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    delete r;
+    return rc;
 }
 
 // ---- batched pipeline -------------------------------------------------------------
@@ -443,8 +469,20 @@ struct Arena {
     }
 };
 
-const char* kStages[] = {"grid", "pairs", "predict", "filter", "connectivity", "ca"};
-constexpr int kNumStages = 6;
+// phases of the batched launch sequence: one graph each, HIP events between
+// them time every phase on the batch's stream
+const char* kPhases[] = {"grid_count",   "grid_store",    "pair_plan",     "pair_count",   "pair_scan",
+                         "pair_store",   "predict_count", "predict_scan",  "predict_store", "filter_count",
+                         "filter_scan",  "filter_store",  "link_prepare",  "link_count",   "link_store",
+                         "circle_fit",   "ca_forward",    "ca_backward"};
+constexpr int kNumStages = 18;
+// the reference's algorithm classes as ranges of phases
+struct StageRange {
+    const char* name;
+    int first, last;
+};
+const StageRange kStageRanges[] = {{"grid", 0, 1},    {"pairs", 2, 5},         {"predict", 6, 8},
+                                   {"filter", 9, 11}, {"connectivity", 12, 15}, {"ca", 16, 17}};
 
 }  // namespace
 
@@ -480,6 +518,28 @@ struct trax_batch {
 
     void enqueue_stage(int s);
     void enqueue_ca_tail();
+
+    // enqueue the whole sequence on the resident inputs (graph replay or eager)
+    void launch() {
+        if (caps.use_graph && !captured) {
+            for (int k = 0; k < kNumStages; ++k) {
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+                enqueue_stage(k);
+                hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
+                hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
+            }
+            captured = true;
+        }
+        for (int k = 0; k < kNumStages; ++k) {
+            hip_check(hipEventRecord(ev[k], st), "event");
+            if (caps.use_graph) hip_check(hipGraphLaunch(execs[k], st), "graph launch");
+            else enqueue_stage(k);
+        }
+        hip_check(hipEventRecord(ev[kNumStages], st), "event");
+        hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
+        submitted = true;
+        have_counts = false;
+    }
 };
 
 void trax_batch::enqueue_stage(int s) {
@@ -493,56 +553,80 @@ void trax_batch::enqueue_stage(int s) {
                    N = caps.max_hits;
     auto ok = [](hipError_t e, const char* w) { hip_check(e, w); };
     switch (s) {
-        case 0:
+        case 0:  // grid_count
             ok(launch_fill(reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords, 0u,
                            sizeof(DevCounts) / 4 - kCountsShapeWords, st), "reset");
             ok(launch_grid_count(g, E * L, eo, ln, lo, hin.x, hin.y, hin.z, grid, cnt, lc, st), "grid count");
+            break;
+        case 1:  // grid_store
             ok(launch_grid_store(g, E * L, eo, ln, lo, grid, hin, hout, N, cnt, lc, st), "grid store");
             break;
-        case 1:
+        case 2:  // pair_plan
             ok(launch_pair_plan(g, tc, nseg, grid, seg_base, cnt, st), "pair plan");
             ok(launch_scan(seg_base, &cnt->n_segments, 1u, nseg + 1ull, &cnt->n_pair_slots, scan, st), "scan");
+            break;
+        case 3:  // pair_count
             ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, seg_base, slot_counts, cnt, lc, st),
                "pair count");
+            break;
+        case 4:  // pair_scan
             ok(launch_scan(slot_counts, &cnt->n_pair_slots, 1u, slot_cap + 1ull, &cnt->n_pairs, scan, st), "scan");
+            break;
+        case 5:  // pair_store
             ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, seg_base, slot_counts, pairs,
                                  pair_off, cnt, lc, st),
                "pair store");
             break;
-        case 2:
+        case 6:  // predict_count
             ok(launch_predict_count(g, tc, P, grid, hout.x, hout.y, hout.z, hout.layer, hout.evt, pairs, pred_counts,
                                     cnt, lc, st),
                "predict count");
+            break;
+        case 7:  // predict_scan
             ok(launch_scan(pred_counts, &cnt->n_pairs, 1u, P + 1ull, &cnt->n_cands, scan, st), "scan");
             ok(launch_fill(cand_off, 0u, nseg + 1ull, st), "fill");
+            break;
+        case 8:  // predict_store
             ok(launch_predict_store(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.layer, hout.evt, pairs,
                                     pred_counts, cands, cand_off, cnt, lc, st),
                "predict store");
             break;
-        case 3:
+        case 9:  // filter_count
             ok(launch_filter_count(tc, Q, hout.x, hout.y, hout.z, hout.layer, pairs, cands, bitmap, word_counts, cnt,
                                    lc, st),
                "filter count");
+            break;
+        case 10:  // filter_scan
             ok(launch_scan(word_counts, &cnt->n_filter_words, 1u, Q / 64 + 2ull, &cnt->n_tracklets, scan, st), "scan");
             ok(launch_fill(t_off, 0u, nseg + 1ull, st), "fill");
+            break;
+        case 11:  // filter_store
             ok(launch_filter_store(tc, Q, R, hout.layer, hout.evt, pairs, cands, bitmap, word_counts, th1, th2, th3,
                                    t_off, cnt, lc, st),
                "filter store");
             break;
-        case 4:
+        case 12:  // link_prepare
             ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
+            break;
+        case 13:  // link_count
             ok(launch_link_count(tc, R, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link count");
             ok(launch_scan(lb.counts, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_conn, scan, st), "scan");
+            break;
+        case 14:  // link_store
             ok(launch_link_store(tc, R, K, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link store");
             ok(launch_link_flags(R, lb, cnt, lc, st), "link flags");
             ok(launch_scan(lb.flag_a, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_connectable, scan, st), "scan");
             ok(launch_scan(lb.flag_b, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_nonconnectable, scan, st), "scan");
             ok(launch_link_compact(R, lb, cnt, lc, st), "link compact");
+            break;
+        case 15:  // circle_fit
             ok(launch_circle_fit(R, hout.x, hout.y, th1, th2, th3, lb, cnt, lc, st), "circle");
             break;
-        case 5:
+        case 16:  // ca_forward
             ok(launch_ca_init(R, cb, cnt, st), "ca init");
             ok(launch_ca_iterations(K, lb.basis, lb.foll, cb, cnt, 0, ca_iter, lc, st), "ca iterations");
+            break;
+        case 17:  // ca_backward
             enqueue_ca_tail();
             break;
     }
@@ -591,13 +675,8 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         B.slot_cap = (uint32_t)slot_cap;
         B.cm.reset(new ConfigMirror(*ctx->c, cfg->trip, cfg->geom, cfg->n_threads, cfg->d_eta_cut));
         // one arena for every buffer of the pipeline
-        const uint64_t words = 14 * N + E + 2 * E * L + E * L * C + (E * T + 1) * 4 + slot_cap + 1 + 2 * P + P + 1 +
-                               2 * Q + 2 * (Q / 64 + 2) + 3 * R + R + 1 + N + 1 + N + 9 * (R + 1) + 4 * K + 2 * K + 4 * R +
-                               TE + 3 + 64;
-        B.arena.cap = words * 4 + 256 * 96;
-        hip_check(hipMalloc(&B.arena_mem, B.arena.cap), "hipMalloc(arena)");
-        B.arena.base = B.arena_mem;
-        Arena& a = B.arena;
+        // the layout is run twice: once to measure the arena, once to carve it
+        auto layout = [&](Arena& a) {
         auto cols = [&](HitColumns& c) {
             c.x = a.take<float>(N); c.y = a.take<float>(N); c.z = a.take<float>(N);
             c.layer = a.take<uint32_t>(N); c.det = a.take<uint32_t>(N); c.id = a.take<uint32_t>(N);
@@ -637,6 +716,14 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         cb.handler = a.take<uint32_t>(R + 1);
         cb.tracks = a.take<uint32_t>(TE);
         cb.flags = a.take<uint32_t>(3);
+        };
+        Arena measure;
+        measure.cap = ~(size_t)0;
+        layout(measure);
+        B.arena.cap = measure.used;
+        hip_check(hipMalloc(&B.arena_mem, B.arena.cap), "hipMalloc(arena)");
+        B.arena.base = B.arena_mem;
+        layout(B.arena);
         B.scan_cap = (uint32_t)std::max<uint64_t>({N + 1, slot_cap + 1, P + 1, Q / 64 + 2, R + 1, E * T + 1});
         const uint32_t tiles = scan_tiles(B.scan_cap);
         hip_check(hipMalloc(&B.scan.status, (size_t)(tiles + 1) * sizeof(uint64_t)), "hipMalloc(scan)");
@@ -719,26 +806,19 @@ extern "C" int trax_batch_submit(trax_batch* b, const trax_hits* h) {
             hip_check(hipMemcpyAsync(ucols[k], s + (3 + k) * (size_t)Ncap, N * 4ull, hipMemcpyHostToDevice, st), "h2d");
         hip_check(hipMemcpyAsync(b->sup, sp, (Ecap + 2 * (size_t)Ecap * L) * 4, hipMemcpyHostToDevice, st), "h2d");
         hip_check(hipMemcpyAsync(b->cnt, shape, 8, hipMemcpyHostToDevice, st), "h2d");
-        if (b->caps.use_graph && !b->captured) {
-            for (int k = 0; k < kNumStages; ++k) {
-                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-                b->enqueue_stage(k);
-                hip_check(hipStreamEndCapture(st, &b->graphs[k]), "end capture");
-                hip_check(hipGraphInstantiate(&b->execs[k], b->graphs[k], nullptr, nullptr, 0), "instantiate");
-            }
-            b->captured = true;
-        }
-        for (int k = 0; k < kNumStages; ++k) {
-            hip_check(hipEventRecord(b->ev[k], st), "event");
-            if (b->caps.use_graph) hip_check(hipGraphLaunch(b->execs[k], st), "graph launch");
-            else b->enqueue_stage(k);
-        }
-        hip_check(hipEventRecord(b->ev[kNumStages], st), "event");
-        hip_check(hipMemcpyAsync(b->h_counts, b->cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
         b->n_events = E;
         b->n_hits = N;
-        b->submitted = true;
+        b->launch();
         b->have_counts = false;
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_batch_rerun(trax_batch* b) {
+    return guarded([&] {
+        if (!b->n_events) return fail(TRAX_ERR_STATE, "no batch uploaded");
+        hip_check(hipSetDevice(b->owner->c->device()), "hipSetDevice");
+        b->launch();
         return TRAX_OK;
     });
 }
@@ -786,8 +866,13 @@ extern "C" int trax_batch_counts(trax_batch* b, uint32_t out[8]) {
 extern "C" int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms) {
     return guarded([&] {
         for (int k = 0; k < kNumStages; ++k)
-            if (std::strcmp(stage, kStages[k]) == 0) {
+            if (std::strcmp(stage, kPhases[k]) == 0) {
                 hip_check(hipEventElapsedTime(ms, b->ev[k], b->ev[k + 1]), "hipEventElapsedTime");
+                return TRAX_OK;
+            }
+        for (const StageRange& r : kStageRanges)
+            if (std::strcmp(stage, r.name) == 0) {
+                hip_check(hipEventElapsedTime(ms, b->ev[r.first], b->ev[r.last + 1]), "hipEventElapsedTime");
                 return TRAX_OK;
             }
         if (std::strcmp(stage, "total") == 0) {

@@ -51,6 +51,7 @@ struct TripletCols {
 uint32_t scan_tiles(uint64_t n_cap);
 uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
+hipError_t launch_copy(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t st);
 // fill *d_n + n_add entries (at most n_cap)
 hipError_t launch_fill_dn(uint32_t* p, uint32_t v, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
                           hipStream_t st);

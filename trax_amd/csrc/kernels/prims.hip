@@ -103,6 +103,12 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
         p[i] = v;
 }
 
+__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_fill_u32_dn(uint32_t* __restrict__ p, uint32_t v, const uint32_t* __restrict__ d_n,
                               uint32_t n_add) {
     const uint32_t n = *d_n + n_add;
@@ -174,6 +180,14 @@ hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_copy_u32, dim3((uint32_t)blocks), dim3(256), 0, st, dst, src, n);
     return hipGetLastError();
 }
 
