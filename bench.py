@@ -43,9 +43,7 @@ def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads):
         "pair_count": 12 * seg_hits + 4 * slots,
         "pair_scan": 8 * slots,
         "pair_store": 12 * seg_hits + 4 * slots + 8 * P,
-        "predict_count": 8 * P + 12 * trip_hits + 4 * P,
-        "predict_scan": 8 * P,
-        "predict_store": 8 * P + 12 * trip_hits + 8 * Q,
+        "predict": 8 * P + 16 * trip_hits + 4 * P + 8 * Q,   # pairs, hit x/y/z/phi, offsets, candidates
         "filter_count": 8 * Q + 8 * Q + 12 * words,
         "filter_scan": 8 * words,
         "filter_store": 12 * words + 8 * R + 12 * R,
@@ -113,9 +111,9 @@ def main():
         b.rerun()
         b.wait()
 
-    phases = ["grid_count", "grid_store", "pair_plan", "pair_count", "pair_scan", "pair_store", "predict_count",
-              "predict_scan", "predict_store", "filter_count", "filter_scan", "filter_store", "link_prepare",
-              "link_count", "link_store", "circle_fit", "ca_forward", "ca_backward"]
+    phases = ["grid_count", "grid_store", "pair_plan", "pair_count", "pair_scan", "pair_store", "predict",
+              "filter_count", "filter_scan", "filter_store", "link_prepare", "link_count", "link_store",
+              "circle_fit", "ca_forward", "ca_backward"]
     acc = dict.fromkeys(phases, 0.0)
     t0 = time.perf_counter()
     for _ in range(args.steps):

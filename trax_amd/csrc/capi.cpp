@@ -471,18 +471,17 @@ struct Arena {
 
 // phases of the batched launch sequence: one graph each, HIP events between
 // them time every phase on the batch's stream
-const char* kPhases[] = {"grid_count",   "grid_store",    "pair_plan",     "pair_count",   "pair_scan",
-                         "pair_store",   "predict_count", "predict_scan",  "predict_store", "filter_count",
-                         "filter_scan",  "filter_store",  "link_prepare",  "link_count",   "link_store",
-                         "circle_fit",   "ca_forward",    "ca_backward"};
-constexpr int kNumStages = 18;
+const char* kPhases[] = {"grid_count",   "grid_store",   "pair_plan",  "pair_count", "pair_scan",   "pair_store",
+                         "predict",      "filter_count", "filter_scan", "filter_store", "link_prepare", "link_count",
+                         "link_store",   "circle_fit",   "ca_forward", "ca_backward"};
+constexpr int kNumStages = 16;
 // the reference's algorithm classes as ranges of phases
 struct StageRange {
     const char* name;
     int first, last;
 };
-const StageRange kStageRanges[] = {{"grid", 0, 1},    {"pairs", 2, 5},         {"predict", 6, 8},
-                                   {"filter", 9, 11}, {"connectivity", 12, 15}, {"ca", 16, 17}};
+const StageRange kStageRanges[] = {{"grid", 0, 1},   {"pairs", 2, 5},          {"predict", 6, 6},
+                                   {"filter", 7, 9}, {"connectivity", 10, 13}, {"ca", 14, 15}};
 
 }  // namespace
 
@@ -566,67 +565,60 @@ void trax_batch::enqueue_stage(int s) {
             ok(launch_scan(seg_base, &cnt->n_segments, 1u, nseg + 1ull, &cnt->n_pair_slots, scan, st), "scan");
             break;
         case 3:  // pair_count
-            ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, seg_base, slot_counts, cnt, lc, st),
+            ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, hout.phi, seg_base, slot_counts,
+                                 cnt, lc, st),
                "pair count");
             break;
         case 4:  // pair_scan
             ok(launch_scan(slot_counts, &cnt->n_pair_slots, 1u, slot_cap + 1ull, &cnt->n_pairs, scan, st), "scan");
             break;
         case 5:  // pair_store
-            ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, seg_base, slot_counts, pairs,
-                                 pair_off, cnt, lc, st),
+            ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, hout.phi, seg_base,
+                                 slot_counts, pairs, pair_off, cnt, lc, st),
                "pair store");
             break;
-        case 6:  // predict_count
-            ok(launch_predict_count(g, tc, P, grid, hout.x, hout.y, hout.z, hout.layer, hout.evt, pairs, pred_counts,
-                                    cnt, lc, st),
-               "predict count");
-            break;
-        case 7:  // predict_scan
-            ok(launch_scan(pred_counts, &cnt->n_pairs, 1u, P + 1ull, &cnt->n_cands, scan, st), "scan");
+        case 6:  // predict: count, scan and store in one pass
             ok(launch_fill(cand_off, 0u, nseg + 1ull, st), "fill");
+            ok(launch_predict_fused(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.phi, hout.layer, hout.evt, pairs,
+                                    cands, cand_off, cnt, scan, lc, st),
+               "predict");
             break;
-        case 8:  // predict_store
-            ok(launch_predict_store(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.layer, hout.evt, pairs,
-                                    pred_counts, cands, cand_off, cnt, lc, st),
-               "predict store");
-            break;
-        case 9:  // filter_count
+        case 7:  // filter_count
             ok(launch_filter_count(tc, Q, hout.x, hout.y, hout.z, hout.layer, pairs, cands, bitmap, word_counts, cnt,
                                    lc, st),
                "filter count");
             break;
-        case 10:  // filter_scan
+        case 8:  // filter_scan
             ok(launch_scan(word_counts, &cnt->n_filter_words, 1u, Q / 64 + 2ull, &cnt->n_tracklets, scan, st), "scan");
             ok(launch_fill(t_off, 0u, nseg + 1ull, st), "fill");
             break;
-        case 11:  // filter_store
+        case 9:  // filter_store
             ok(launch_filter_store(tc, Q, R, hout.layer, hout.evt, pairs, cands, bitmap, word_counts, th1, th2, th3,
                                    t_off, cnt, lc, st),
                "filter store");
             break;
-        case 12:  // link_prepare
+        case 10:  // link_prepare
             ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
             break;
-        case 13:  // link_count
+        case 11:  // link_count
             ok(launch_link_count(tc, R, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link count");
             ok(launch_scan(lb.counts, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_conn, scan, st), "scan");
             break;
-        case 14:  // link_store
+        case 12:  // link_store
             ok(launch_link_store(tc, R, K, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link store");
             ok(launch_link_flags(R, lb, cnt, lc, st), "link flags");
             ok(launch_scan(lb.flag_a, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_connectable, scan, st), "scan");
             ok(launch_scan(lb.flag_b, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_nonconnectable, scan, st), "scan");
             ok(launch_link_compact(R, lb, cnt, lc, st), "link compact");
             break;
-        case 15:  // circle_fit
+        case 13:  // circle_fit
             ok(launch_circle_fit(R, hout.x, hout.y, th1, th2, th3, lb, cnt, lc, st), "circle");
             break;
-        case 16:  // ca_forward
+        case 14:  // ca_forward
             ok(launch_ca_init(R, cb, cnt, st), "ca init");
             ok(launch_ca_iterations(K, lb.basis, lb.foll, cb, cnt, 0, ca_iter, lc, st), "ca iterations");
             break;
-        case 17:  // ca_backward
+        case 15:  // ca_backward
             enqueue_ca_tail();
             break;
     }
@@ -684,6 +676,7 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         };
         cols(B.hin);
         cols(B.hout);
+        B.hout.phi = a.take<float>(N);
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
         B.seg_base = a.take<uint32_t>(E * T + 1);
@@ -725,7 +718,7 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         B.arena.base = B.arena_mem;
         layout(B.arena);
         B.scan_cap = (uint32_t)std::max<uint64_t>({N + 1, slot_cap + 1, P + 1, Q / 64 + 2, R + 1, E * T + 1});
-        const uint32_t tiles = scan_tiles(B.scan_cap);
+        const uint32_t tiles = std::max(scan_tiles(B.scan_cap), predict_fused_tiles(P));
         hip_check(hipMalloc(&B.scan.status, (size_t)(tiles + 1) * sizeof(uint64_t)), "hipMalloc(scan)");
         B.scan.max_tiles = tiles;
         hip_check(hipMalloc(&B.cnt, sizeof(DevCounts)), "hipMalloc(counts)");

@@ -206,7 +206,7 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
     cols_.d_eta_cut = dEtaCut;
 }
 
-HitMirror::HitMirror(Context& ctx, uint n) : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 7ull * (n ? n : 1))) {}
+HitMirror::HitMirror(Context& ctx, uint n) : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 8ull * (n ? n : 1))) {}
 
 HitColumns HitMirror::cols() const {
     HitColumns c;
@@ -214,6 +214,7 @@ HitColumns HitMirror::cols() const {
     const size_t n = n_ ? n_ : 1;
     c.x = reinterpret_cast<float*>(b); c.y = reinterpret_cast<float*>(b + n); c.z = reinterpret_cast<float*>(b + 2 * n);
     c.layer = b + 3 * n; c.det = b + 4 * n; c.id = b + 5 * n; c.evt = b + 6 * n;
+    c.phi = reinterpret_cast<float*>(b + 7 * n);
     return c;
 }
 
@@ -310,13 +311,13 @@ Pairing* PairGeneratorBeamspot::run(const HitMirror& hits, const ConfigMirror& c
     const uint nslot = ctx.download_count(&cnt->n_pair_slots);
     UVec slot_counts(ctx, nslot + 1ull);
     const HitColumns hc = hits.cols();
-    hip_check(launch_pair_count(g, tc, nseg, nslot, grid.boundary.get_mem(), hc.x, hc.y, hc.z, seg_base.get_mem(),
+    hip_check(launch_pair_count(g, tc, nseg, nslot, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi, seg_base.get_mem(),
                                 slot_counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "pair count");
     PrefixSum(ctx).run(slot_counts.get_mem(), nslot + 1, nThreads, &cnt->n_pairs);
     const uint nPairs = ctx.download_count(&cnt->n_pairs);  // PairGeneratorBeamspot.cpp:124
     Pairing* p = new Pairing(ctx, nPairs, E, T);
-    hip_check(launch_pair_store(g, tc, nseg, nslot, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z,
+    hip_check(launch_pair_store(g, tc, nseg, nslot, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi,
                                 seg_base.get_mem(), slot_counts.get_mem(), p->pairing.get_mem(),
                                 p->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "pair store");
@@ -335,14 +336,14 @@ Pairing* TripletThetaPhiPredictor::run(const HitMirror& hits, const DetectorGeom
     ctx.set_count(&cnt->n_pairs, pairs.n);
     UVec counts(ctx, nPairs + 1ull);
     const HitColumns hc = hits.cols();
-    hip_check(launch_predict_count(g, tc, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.layer, hc.evt,
+    hip_check(launch_predict_count(g, tc, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi, hc.layer, hc.evt,
                                    pairs.pairing.get_mem(), counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "predict count");
     PrefixSum(ctx).run(counts.get_mem(), nPairs + 1, (uint)nThreads, &cnt->n_cands);
     const uint nCand = ctx.download_count(&cnt->n_cands);  // TripletThetaPhiPredictor.cpp:68
     Pairing* c = new Pairing(ctx, nCand, E, T);
     hip_check(launch_fill(c->pairingOffsets.get_mem(), 0u, c->pairingOffsets.get_count(), ctx.stream()), "fill");
-    hip_check(launch_predict_store(g, tc, nPairs, nCand, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.layer,
+    hip_check(launch_predict_store(g, tc, nPairs, nCand, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi, hc.layer,
                                    hc.evt, pairs.pairing.get_mem(), counts.get_mem(), c->pairing.get_mem(),
                                    c->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "predict store");

@@ -133,6 +133,40 @@ __device__ inline void atomic_store_u64(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Decoupled look-back over tiles processed in increasing order (tile ids come
+// from an atomic counter, so every predecessor of a running tile has been
+// started).  Called by all 64 lanes of one wave; publishes the tile's
+// aggregate, returns the exclusive prefix of the tile and publishes its
+// inclusive prefix.  status words: (flag << 32) | value, flag 1 = aggregate,
+// 2 = inclusive prefix.
+__device__ inline uint32_t tile_lookback(uint64_t* status, uint32_t tile, uint32_t total) {
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) atomic_store_u64(&status[0], (2ull << 32) | total);
+        return 0u;
+    }
+    if (lane == 0) atomic_store_u64(&status[tile], (1ull << 32) | total);
+    uint32_t excl = 0;
+    int64_t j = (int64_t)tile - 1 - (int64_t)lane;
+    while (true) {
+        uint64_t s;
+        uint32_t flag;
+        while (true) {
+            s = (j >= 0) ? atomic_load_u64(&status[j]) : (2ull << 32);
+            flag = (uint32_t)(s >> 32);
+            if (__ballot(flag == 0u) == 0ull) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const uint64_t incl = __ballot(flag == 2u);
+        const uint32_t stop = incl ? (uint32_t)(__ffsll((unsigned long long)incl) - 1) : 63u;
+        excl += wave_sum(lane <= stop ? (uint32_t)s : 0u);
+        if (incl) break;
+        j -= kWave;
+    }
+    if (lane == 0) atomic_store_u64(&status[tile], (2ull << 32) | (uint32_t)(excl + total));
+    return excl;
+}
+
 // upper_bound - 1 over a sorted u32 array: last index s with a[s] <= key
 __device__ inline uint32_t find_segment(const uint32_t* a, uint32_t n_seg, uint32_t key) {
     uint32_t lo = 0, hi = n_seg;  // search in [0, n_seg)
@@ -156,12 +190,14 @@ struct GridGeom {
 // cell of a hit, gridCount (src/algorithms/GridBuilder.h:80-84); kSentinel when
 // the z or phi sector falls outside the (n_z + 1) x (n_phi + 1) grid, where the
 // reference would index its local histogram out of bounds
-__device__ inline uint32_t hit_cell(const GridGeom& g, float x, float y, float z) {
+__device__ inline uint32_t hit_cell_phi(const GridGeom& g, float z, float phi) {
     const uint32_t zs = tx_f2u_sat(floorf((z - g.min_z) / g.sector_z));
-    const float phi = tx_atan2f(y, x);
     const uint32_t ps = tx_f2u_sat(floorf((phi - g.min_phi) / g.sector_phi));
     if (zs > g.n_z || ps > g.n_phi) return kSentinel;
     return zs * (g.n_phi + 1) + ps;
+}
+__device__ inline uint32_t hit_cell(const GridGeom& g, float x, float y, float z) {
+    return hit_cell_phi(g, z, tx_atan2f(y, x));
 }
 
 }  // namespace trax

@@ -63,6 +63,7 @@ struct HitCols {
 struct HitColsOut {
     float* x; float* y; float* z;
     uint32_t* layer; uint32_t* det; uint32_t* id; uint32_t* evt;
+    float* phi;
 };
 
 __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t* __restrict__ d_nevents,
@@ -85,11 +86,12 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
         for (uint32_t chunk = 0; chunk < n; chunk += blockDim.x) {
             const uint32_t i = chunk + threadIdx.x;
             bool valid = i < n;
-            float x = 0.f, y = 0.f, z = 0.f;
+            float x = 0.f, y = 0.f, z = 0.f, phi = 0.f;
             uint32_t c = kSentinel;
             if (valid) {
                 x = in.x[off + i]; y = in.y[off + i]; z = in.z[off + i];
-                c = hit_cell(g, x, y, z);
+                phi = tx_atan2f(y, x);
+                c = hit_cell_phi(g, z, phi);
                 valid = c < g.cells;
             }
             // rank among lanes of this wave with the same cell
@@ -124,6 +126,7 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
                 out.det[pos] = in.det[off + i];
                 out.id[pos] = in.id[off + i];
                 out.evt[pos] = in.evt[off + i];
+                if (out.phi) out.phi[pos] = phi;
             }
         }
         __syncthreads();
@@ -152,7 +155,7 @@ hipError_t launch_grid_store(const GridGeom& g, uint32_t n_items_cap, const uint
     const uint32_t blocks = n_items_cap ? n_items_cap : 1;
     const size_t lds = (size_t)g.cells * sizeof(uint32_t);
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
-    HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt};
+    HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.phi};
     hipLaunchKernelGGL(k_grid_store, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset,
                        lay_nhits, lay_offset, grid, ic, oc, n_hits_cap);
     return hipGetLastError();

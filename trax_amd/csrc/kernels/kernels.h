@@ -21,6 +21,10 @@ struct ScanScratch {
 struct HitColumns {
     float* x = nullptr; float* y = nullptr; float* z = nullptr;
     uint32_t* layer = nullptr; uint32_t* det = nullptr; uint32_t* id = nullptr; uint32_t* evt = nullptr;
+    // device-only column written by the grid scatter: phi = atan2(y, x) of
+    // every reordered hit, the value every window test of the reference
+    // recomputes (bit-identical: the same built-in on the same inputs)
+    float* phi = nullptr;
 };
 
 // per-kernel block sizes (0 = default); chosen by the autotuner
@@ -77,25 +81,33 @@ hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n
                             const uint32_t* grid, uint32_t* seg_base, DevCounts* cnt, hipStream_t st);
 hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
-                             const float* hz, const uint32_t* seg_base, uint32_t* slot_counts,
+                             const float* hz, const float* hphi, const uint32_t* seg_base, uint32_t* slot_counts,
                              DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
-                             const float* hx, const float* hy, const float* hz, const uint32_t* seg_base,
-                             const uint32_t* slot_counts, uint32_t* pairs, uint32_t* pair_offsets,
-                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+                             const float* hx, const float* hy, const float* hz, const float* hphi,
+                             const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
+                             uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 
 // TripletThetaPhiPredictor: count -> scan(counts) -> store (+ offsets, forward fill)
 hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* pairs,
-                                uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const uint32_t* pairs, uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc,
+                                hipStream_t st);
 hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 uint32_t n_cand_cap, const uint32_t* grid, const float* hx, const float* hy,
-                                const float* hz, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float* hz, const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, const uint32_t* counts, uint32_t* cands,
-                                uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc,
-                                hipStream_t st);
+                                uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+// count + scan + store in one pass (the offsets array must be zeroed before);
+// uses predict_fused_tiles(n_pair_cap) look-back words of the scan scratch
+uint32_t predict_fused_tiles(uint64_t n_pair_cap);
+hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
+                                const uint32_t* grid, const float* hx, const float* hy, const float* hz,
+                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
+                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
 
 // TripletThetaPhiFilter: count (bitmap + per-word counts) -> scan -> store (+ offsets)
 hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const float* hx,

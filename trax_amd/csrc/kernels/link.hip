@@ -32,17 +32,19 @@ __device__ inline bool filter_valid(const TripletCols& tc, const float* hx, cons
     const float x1 = hx[h1], y1 = hy[h1], z1 = hz[h1];
     const float x2 = hx[h2], y2 = hy[h2], z2 = hz[h2];
     const float x3 = hx[h3], y3 = hy[h3], z3 = hz[h3];
-    bool valid = true;
-    float angle1 = tx_atan2f(sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
-    float angle2 = tx_atan2f(sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
-    float delta = fabsf(angle2 / angle1);
-    valid = valid && (1 - dThetaCut <= delta && delta <= 1 + dThetaCut);
-    angle1 = tx_atan2f((y2 - y1), (x2 - x1));
-    angle2 = tx_atan2f((y3 - y2), (x3 - x2));
-    delta = angle2 - angle1;
+    // the four cuts are a conjunction of pure expressions: a candidate is
+    // rejected at the first cut it fails (the result is the reference's)
+    float angle1 = tx_atan2f((y2 - y1), (x2 - x1));
+    float angle2 = tx_atan2f((y3 - y2), (x3 - x2));
+    float delta = angle2 - angle1;
     const float PI = TX_PI_F;
     delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
-    valid = valid && (fabsf(delta) <= dPhiCut);
+    if (!(fabsf(delta) <= dPhiCut)) return false;
+    angle1 = tx_atan2f(sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
+    angle2 = tx_atan2f(sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
+    delta = fabsf(angle2 / angle1);
+    if (!(1 - dThetaCut <= delta && delta <= 1 + dThetaCut)) return false;
+    bool valid = true;
     const float p1x = x1, p1y = y1, p1z = x1 * x1 + y1 * y1;
     const float p2x = x2, p2y = y2, p2z = x2 * x2 + y2 * y2;
     const float p3x = x3, p3y = y3, p3z = x3 * x3 + y3 * y3;

@@ -16,7 +16,14 @@
 // Predictor order.  One work item per pair, candidates in the reference's
 // z-sector / wrap-around walk order, the store bounded by the count, both as
 // in the reference (including the two different rOrigin expressions of the
-// count and store kernels).
+// count and store kernels).  The batched path fuses count, scan and store into
+// one pass (k_predict_fused): each pair's window is walked once with both
+// expressions, the store-side matches are kept in LDS, tile offsets come from
+// a decoupled look-back, and each block writes its candidates as one
+// contiguous, coalesced run.
+//
+// Every window walk reads the hit's phi from the column the grid scatter
+// wrote (the same atan2 of the same coordinates) instead of recomputing it.
 #include "kernels.h"
 
 namespace trax {
@@ -30,7 +37,7 @@ struct Window {
 
 // outer hit i of layer pair p: window in the inner layer (.h:108-163)
 __device__ inline Window pair_window(const GridGeom& g, const TripletCols& tc, uint32_t p, uint32_t lay1,
-                                     float X, float Y, float Z) {
+                                     float X, float Y, float Z, float phi) {
     Window w;
     const float z0 = tc.z0[p];
     const float tip = tc.tip[p];
@@ -54,7 +61,6 @@ __device__ inline Window pair_window(const GridGeom& g, const TripletCols& tc, u
     w.zLo = (uint32_t)(zl > 0 ? zl : 0);
     const uint32_t zh = tx_f2u_sat(floorf((zHigh - g.min_z) / g.sector_z)) + 1u;
     w.zHi = zh < g.n_z ? zh : g.n_z;
-    const float phi = tx_atan2f(Y, X);
     tmp = fabsf(tx_acosf(r / (2 * mrc)) - tx_acosf(maxR1 / (2 * mrc)));
     float dPhi = fabsf(tx_acosf(r / (2 * mrc)) - tx_acosf(minR1 / (2 * mrc)));
     dPhi = (float)(tmp < dPhi) * dPhi + (float)(tmp > dPhi) * tmp;
@@ -102,7 +108,6 @@ __global__ void k_pair_plan(GridGeom g, TripletCols tc, const uint32_t* __restri
 
 struct SlotInfo {
     uint32_t seg, e, p, t, k;
-    bool live;
 };
 
 __device__ inline SlotInfo slot_info(const TripletCols& tc, const uint32_t* seg_base, uint32_t nseg,
@@ -116,14 +121,14 @@ __device__ inline SlotInfo slot_info(const TripletCols& tc, const uint32_t* seg_
     si.k = o - si.t * K;
     si.e = si.seg / tc.n_trip;
     si.p = si.seg - si.e * tc.n_trip;
-    si.live = true;
     return si;
 }
 
 __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                              const float* __restrict__ hx, const float* __restrict__ hy,
-                             const float* __restrict__ hz, const uint32_t* __restrict__ seg_base,
-                             uint32_t* __restrict__ slot_counts, DevCounts* __restrict__ cnt) {
+                             const float* __restrict__ hz, const float* __restrict__ hphi,
+                             const uint32_t* __restrict__ seg_base, uint32_t* __restrict__ slot_counts,
+                             DevCounts* __restrict__ cnt) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t overlap = 0;
@@ -138,7 +143,7 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
         if (il < end - layer2Offset) {
             const uint32_t i = layer2Offset + il;
             const uint32_t* lg1 = grid + ((size_t)si.e * g.n_layers + lay1) * g.cells;
-            const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i]);
+            const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i], hphi[i]);
             if (w.wrap && w.pHi > w.pLo) overlap++;
             for (uint32_t zs = w.zLo; zs < w.zHi; ++zs) {  // (.h:172-230)
                 const uint32_t zStart = lg1[zs * np1];
@@ -147,7 +152,7 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
                 const uint32_t end2 = (uint32_t)w.wrap * zEnd + lg1[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
                 for (uint32_t j = lg1[zs * np1 + w.pLo]; j < end2; ++j) {
                     const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
-                    nFound += in_window(w, hz[index], tx_atan2f(hy[index], hx[index]));
+                    nFound += in_window(w, hz[index], hphi[index]);
                 }
             }
         }
@@ -158,9 +163,9 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
 
 __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                              const float* __restrict__ hx, const float* __restrict__ hy,
-                             const float* __restrict__ hz, const uint32_t* __restrict__ seg_base,
-                             const uint32_t* __restrict__ slot_counts, uint32_t* __restrict__ pairs,
-                             uint32_t n_pair_cap, DevCounts* __restrict__ cnt) {
+                             const float* __restrict__ hz, const float* __restrict__ hphi,
+                             const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
+                             uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t short_slots = 0;
@@ -175,7 +180,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
         const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
         const uint32_t i = g2[0] + si.k * tc.n_threads + si.t;
         const uint32_t* lg1 = grid + ((size_t)si.e * g.n_layers + lay1) * g.cells;
-        const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i]);
+        const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i], hphi[i]);
         // every z sector visits the index intervals [a1, b1) (j below the sector
         // end) and [a2, b2) (j past the end, wrapped back); the bitmap read of
         // the reference emits their union in ascending order
@@ -192,12 +197,11 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
             if (a2 < a1) {  // order the two intervals by start
                 uint32_t ta = a1, tb = b1; a1 = a2; b1 = b2; a2 = ta; b2 = tb;
             }
-            // [a1, b1) starts first; merge overlap
-            uint32_t lo[2] = {a1, b1 > a2 ? b1 : a2};
-            uint32_t hi[2] = {b1, b2};
+            const uint32_t lo[2] = {a1, b1 > a2 ? b1 : a2};
+            const uint32_t hi[2] = {b1, b2};
             for (int r = 0; r < 2; ++r)
                 for (uint32_t index = lo[r]; index < hi[r] && pos < next; ++index) {
-                    if (in_window(w, hz[index], tx_atan2f(hy[index], hx[index]))) {
+                    if (in_window(w, hz[index], hphi[index])) {
                         pairs[2 * (size_t)pos] = index;
                         pairs[2 * (size_t)pos + 1] = i;
                         pos++;
@@ -215,7 +219,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
 }
 
 __global__ void k_pair_offsets(const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
-                               uint32_t* __restrict__ pair_offsets, DevCounts* __restrict__ cnt) {
+                               uint32_t* __restrict__ pair_offsets, const DevCounts* __restrict__ cnt) {
     const uint32_t nseg = cnt->n_segments;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += gridDim.x * blockDim.x)
         pair_offsets[s] = (s == 0) ? 0u : slot_counts[seg_base[s]];
@@ -230,9 +234,9 @@ struct PWin {
 };
 
 __device__ inline bool pred_window(const GridGeom& g, const TripletCols& tc, const uint32_t* grid,
-                                   const float* hx, const float* hy, const float* hz, const uint32_t* hlayer,
-                                   const uint32_t* hevt, uint32_t n_events, uint32_t firstHit,
-                                   uint32_t secondHit, bool store_variant, PWin& pw) {
+                                   const float* hx, const float* hy, const float* hz, const float* hphi,
+                                   const uint32_t* hlayer, const uint32_t* hevt, uint32_t n_events,
+                                   uint32_t firstHit, uint32_t secondHit, bool store_variant, PWin& pw) {
     const uint32_t ev = hevt[firstHit];
     const uint32_t lt = hlayer[firstHit] - 1;  // (.h:88): the triplet index is the inner layer - 1
     if (ev >= n_events || lt >= tc.n_trip) return false;
@@ -269,8 +273,8 @@ __device__ inline bool pred_window(const GridGeom& g, const TripletCols& tc, con
     w.zLo = (uint32_t)(zl > 0 ? zl : 0);
     const uint32_t zh = tx_f2u_sat(floorf((zHigh - g.min_z) / g.sector_z)) + 1u;
     w.zHi = zh < g.n_z ? zh : g.n_z;
-    const float phi = tx_atan2f(p2y, p2x);
-    tmp = tx_atan2f(p1y, p1x);
+    const float phi = hphi[secondHit];
+    tmp = hphi[firstHit];
     const float PI = TX_PI_F;
     float dPhi = phi - tmp;
     dPhi += (dPhi > PI) ? -2 * PI : (dPhi < -PI) ? 2 * PI : 0;
@@ -295,11 +299,31 @@ __device__ inline bool pred_window(const GridGeom& g, const TripletCols& tc, con
     return true;
 }
 
+// the reference's walk over the third-layer window (.h:190-217, .h:360-402):
+// visit(index) for every hit of the z sectors and phi cells, in j order
+template <class V>
+__device__ inline void pred_walk(const Window& w, const uint32_t* lg3, uint32_t np1, V visit) {
+    for (uint32_t zs = w.zLo; zs < w.zHi; ++zs) {
+        const uint32_t zStart = lg3[zs * np1];
+        const uint32_t zEnd = lg3[(zs + 1) * np1];
+        const uint32_t zLen = zEnd - zStart;
+        const uint32_t end2 = (uint32_t)w.wrap * zEnd + lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+        for (uint32_t j = lg3[zs * np1 + w.pLo]; j < end2; ++j) {
+            if (!visit(j - (uint32_t)(j >= zEnd) * zLen)) return;
+        }
+    }
+}
+
+__device__ inline bool same_cells(const Window& a, const Window& b) {
+    return a.zLo == b.zLo && a.zHi == b.zHi && a.pLo == b.pLo && a.pHi == b.pHi && a.wrap == b.wrap;
+}
+
 __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                                 const float* __restrict__ hx, const float* __restrict__ hy,
-                                const float* __restrict__ hz, const uint32_t* __restrict__ hlayer,
-                                const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
-                                uint32_t* __restrict__ counts, DevCounts* __restrict__ cnt) {
+                                const float* __restrict__ hz, const float* __restrict__ hphi,
+                                const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
+                                const uint32_t* __restrict__ pairs, uint32_t* __restrict__ counts,
+                                DevCounts* __restrict__ cnt) {
     const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
     const uint32_t np1 = g.n_phi + 1;
     for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid <= n; gid += gridDim.x * blockDim.x) {
@@ -308,31 +332,39 @@ __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __re
             const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
             PWin pw;
             if (h1 != kSentinel && h2 != kSentinel &&
-                pred_window(g, tc, grid, hx, hy, hz, hlayer, hevt, n_events, h1, h2, false, pw)) {
-                const Window& w = pw.w;
-                for (uint32_t zs = w.zLo; zs < w.zHi; ++zs) {  // (.h:190-217)
-                    const uint32_t zStart = pw.lg3[zs * np1];
-                    const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
-                    const uint32_t zLen = zEnd - zStart;
-                    const uint32_t end2 = (uint32_t)w.wrap * zEnd + pw.lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
-                    for (uint32_t j = pw.lg3[zs * np1 + w.pLo]; j < end2; ++j) {
-                        const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
-                        nFound += in_window(w, hz[index], tx_atan2f(hy[index], hx[index]));
-                    }
-                }
+                pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, false, pw)) {
+                pred_walk(pw.w, pw.lg3, np1, [&](uint32_t index) {
+                    nFound += in_window(pw.w, hz[index], hphi[index]);
+                    return true;
+                });
             }
         }
         counts[gid] = nFound;
     }
 }
 
+// triplet offsets at the end of each (event, layer triplet) group (.h:404-418)
+__device__ inline void pred_group_end(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
+                                      const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
+                                      uint32_t h1, uint32_t next, uint32_t* cand_offsets) {
+    if (h1 == kSentinel) return;
+    const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
+    if (ev >= n_events || lt >= tc.n_trip) return;
+    bool last = true;
+    if (gid + 1 < n) {
+        const uint32_t nh = pairs[2 * (size_t)(gid + 1)];
+        if (nh != kSentinel) last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
+    }
+    if (last) cand_offsets[ev * tc.n_trip + lt + 1] = next;
+}
+
 __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                                 const float* __restrict__ hx, const float* __restrict__ hy,
-                                const float* __restrict__ hz, const uint32_t* __restrict__ hlayer,
-                                const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
-                                const uint32_t* __restrict__ counts, uint32_t* __restrict__ cands,
-                                uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
-                                DevCounts* __restrict__ cnt) {
+                                const float* __restrict__ hz, const float* __restrict__ hphi,
+                                const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
+                                const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ counts,
+                                uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets,
+                                uint32_t n_cand_cap, DevCounts* __restrict__ cnt) {
     const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t short_slots = 0, trunc = 0;
@@ -344,29 +376,22 @@ __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __re
         const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
         PWin pw;
         const bool ok = h1 != kSentinel && h2 != kSentinel &&
-                        pred_window(g, tc, grid, hx, hy, hz, hlayer, hevt, n_events, h1, h2, true, pw);
+                        pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, true, pw);
         if (ok && pos < next) {
-            const Window& w = pw.w;
             bool more = false;
-            for (uint32_t zs = w.zLo; zs < w.zHi && !more; ++zs) {  // (.h:360-402)
-                const uint32_t zStart = pw.lg3[zs * np1];
-                const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
-                const uint32_t zLen = zEnd - zStart;
-                const uint32_t end2 = (uint32_t)w.wrap * zEnd + pw.lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
-                for (uint32_t j = pw.lg3[zs * np1 + w.pLo]; j < end2; ++j) {
-                    const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
-                    const bool v = in_window(w, hz[index], tx_atan2f(hy[index], hx[index]));
-                    if (!(pos < next)) {
-                        if (v) { more = true; break; }
-                        continue;
-                    }
-                    if (v) {
-                        cands[2 * (size_t)pos] = gid;
-                        cands[2 * (size_t)pos + 1] = index;
-                        pos++;
-                    }
+            pred_walk(pw.w, pw.lg3, np1, [&](uint32_t index) {
+                const bool v = in_window(pw.w, hz[index], hphi[index]);
+                if (!(pos < next)) {
+                    more = v;
+                    return !v;
                 }
-            }
+                if (v) {
+                    cands[2 * (size_t)pos] = gid;
+                    cands[2 * (size_t)pos + 1] = index;
+                    pos++;
+                }
+                return true;
+            });
             trunc += more;
         }
         for (uint32_t q = pos; q < next; ++q) {
@@ -374,18 +399,142 @@ __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __re
             cands[2 * (size_t)q + 1] = kSentinel;
         }
         short_slots += next - pos;
-        // triplet offsets at the end of each (event, layer triplet) group (.h:404-418)
-        if (h1 != kSentinel) {
-            const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
-            if (ev < n_events && lt < tc.n_trip) {
-                bool last = true;
-                if (gid + 1 < n) {
-                    const uint32_t nh = pairs[2 * (size_t)(gid + 1)];
-                    if (nh != kSentinel) last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
+        pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, next, cand_offsets);
+    }
+    if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
+    if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// Count, scan and store of the predictor in one pass over tiles of
+// kPredBlock pairs (batched path).  Both window expressions are evaluated on
+// one walk when they cover the same cells (otherwise each is walked); the
+// store-side matches of every pair are kept in LDS (kPredList per pair; a pair
+// with more re-walks its window for the rest), the tile's output offset comes
+// from a decoupled look-back, and the tile's candidates are written as one
+// contiguous run by all threads.
+constexpr int kPredBlock = 256;
+constexpr int kPredList = 24;
+
+__global__ __launch_bounds__(kPredBlock) void k_predict_fused(
+    GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
+    const float* __restrict__ hy, const float* __restrict__ hz, const float* __restrict__ hphi,
+    const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
+    uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
+    DevCounts* __restrict__ cnt, uint64_t* __restrict__ status, uint32_t* __restrict__ counter) {
+    __shared__ uint32_t s_list[kPredBlock * kPredList];
+    __shared__ uint32_t s_excl[kPredBlock + 1];
+    __shared__ uint32_t s_nb[kPredBlock];
+    __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
+    __shared__ uint32_t s_tile, s_base;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    const uint32_t n_tiles = (n + kPredBlock - 1) / kPredBlock;
+    const uint32_t np1 = g.n_phi + 1;
+    uint32_t short_slots = 0, trunc = 0;
+    bool over = false;
+    while (true) {
+        if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
+        const uint32_t gid = tile * kPredBlock + tid;
+        uint32_t countA = 0, nB = 0;
+        uint32_t h1 = kSentinel;
+        PWin pa, pb;
+        bool ok = false, same = true;
+        if (gid < n) {
+            h1 = pairs[2 * (size_t)gid];
+            const uint32_t h2 = pairs[2 * (size_t)gid + 1];
+            ok = h1 != kSentinel && h2 != kSentinel &&
+                 pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, false, pa);
+            if (ok) {
+                pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, true, pb);
+                same = same_cells(pa.w, pb.w);
+                uint32_t* my = s_list + tid * kPredList;
+                if (same) {
+                    pred_walk(pa.w, pa.lg3, np1, [&](uint32_t index) {
+                        const float zz = hz[index], ph = hphi[index];
+                        countA += in_window(pa.w, zz, ph);
+                        if (in_window(pb.w, zz, ph)) {
+                            if (nB < kPredList) my[nB] = index;
+                            nB++;
+                        }
+                        return true;
+                    });
+                } else {
+                    pred_walk(pa.w, pa.lg3, np1, [&](uint32_t index) {
+                        countA += in_window(pa.w, hz[index], hphi[index]);
+                        return true;
+                    });
+                    pred_walk(pb.w, pb.lg3, np1, [&](uint32_t index) {
+                        if (in_window(pb.w, hz[index], hphi[index])) {
+                            if (nB < kPredList) my[nB] = index;
+                            nB++;
+                        }
+                        return true;
+                    });
                 }
-                if (last) cand_offsets[ev * tc.n_trip + lt + 1] = next;
             }
         }
+        uint32_t total;
+        const uint32_t excl = block_excl_scan(countA, s_wsum, &total);
+        if (tid < kWave) {
+            const uint32_t b = tile_lookback(status, tile, total);
+            if (tid == 0) s_base = b;
+        }
+        s_excl[tid] = excl;
+        s_nb[tid] = nB;
+        if (tid == 0) s_excl[kPredBlock] = total;
+        __syncthreads();
+        const uint32_t base = s_base;
+        if (tid == 0 && (tile + 1 >= n_tiles)) cnt->n_cands = base + total;
+        const bool fits = (uint64_t)base + total <= n_cand_cap;
+        over |= !fits;
+        if (fits) {
+            // coalesced write of the tile's run: slot s belongs to the pair
+            // whose exclusive prefix is the last one <= s
+            for (uint32_t s = tid; s < total; s += kPredBlock) {
+                uint32_t lo = 0, hi = kPredBlock;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_excl[mid] <= s) lo = mid; else hi = mid;
+                }
+                const uint32_t k = s - s_excl[lo], nb = s_nb[lo];
+                const uint64_t out = 2 * ((uint64_t)base + s);
+                if (k < nb) {
+                    if (k < (uint32_t)kPredList) {
+                        cands[out] = tile * kPredBlock + lo;
+                        cands[out + 1] = s_list[lo * kPredList + k];
+                    }
+                } else {
+                    cands[out] = kSentinel;
+                    cands[out + 1] = kSentinel;
+                }
+            }
+            // pairs with more store-side matches than the LDS list holds
+            const uint32_t keep = countA < nB ? countA : nB;
+            if (keep > (uint32_t)kPredList) {
+                uint32_t k = 0;
+                const uint64_t o = (uint64_t)base + excl;
+                pred_walk(pb.w, pb.lg3, np1, [&](uint32_t index) {
+                    if (in_window(pb.w, hz[index], hphi[index])) {
+                        if (k >= (uint32_t)kPredList) {
+                            cands[2 * (o + k)] = gid;
+                            cands[2 * (o + k) + 1] = index;
+                        }
+                        k++;
+                    }
+                    return k < keep;
+                });
+            }
+        }
+        if (gid < n) {
+            short_slots += countA > nB ? countA - nB : 0u;
+            trunc += (countA > 0 && nB > countA);
+            pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, base + excl + countA, cand_offsets);
+        }
+        __syncthreads();  // the LDS lists are reused by the next tile
     }
     if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
     if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
@@ -403,23 +552,23 @@ hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n
 
 hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
-                             const float* hz, const uint32_t* seg_base, uint32_t* slot_counts,
+                             const float* hz, const float* hphi, const uint32_t* seg_base, uint32_t* slot_counts,
                              DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     (void)n_seg_cap;
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     hipLaunchKernelGGL(k_pair_count, dim3(stride_blocks((uint64_t)n_slot_cap + 1, b, lc)), dim3(b), 0, st, g,
-                       tc, grid, hx, hy, hz, seg_base, slot_counts, cnt);
+                       tc, grid, hx, hy, hz, hphi, seg_base, slot_counts, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
-                             const float* hx, const float* hy, const float* hz, const uint32_t* seg_base,
-                             const uint32_t* slot_counts, uint32_t* pairs, uint32_t* pair_offsets,
-                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+                             const float* hx, const float* hy, const float* hz, const float* hphi,
+                             const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
+                             uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     hipLaunchKernelGGL(k_pair_store, dim3(stride_blocks(n_slot_cap, b, lc)), dim3(b), 0, st, g, tc, grid, hx,
-                       hy, hz, seg_base, slot_counts, pairs, n_pair_cap, cnt);
+                       hy, hz, hphi, seg_base, slot_counts, pairs, n_pair_cap, cnt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pair_offsets, dim3((n_seg_cap + 1 + 255) / 256), dim3(256), 0, st, seg_base,
@@ -429,24 +578,43 @@ hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t 
 
 hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* pairs,
-                                uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const uint32_t* pairs, uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc,
+                                hipStream_t st) {
     const uint32_t b = lc.predict_block ? lc.predict_block : 256;
     hipLaunchKernelGGL(k_predict_count, dim3(stride_blocks((uint64_t)n_pair_cap + 1, b, lc)), dim3(b), 0, st,
-                       g, tc, grid, hx, hy, hz, hlayer, hevt, pairs, counts, cnt);
+                       g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, pairs, counts, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 uint32_t n_cand_cap, const uint32_t* grid, const float* hx, const float* hy,
-                                const float* hz, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float* hz, const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, const uint32_t* counts, uint32_t* cands,
-                                uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc,
-                                hipStream_t st) {
+                                uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t b = lc.predict_block ? lc.predict_block : 256;
     hipLaunchKernelGGL(k_predict_store, dim3(stride_blocks(n_pair_cap, b, lc)), dim3(b), 0, st, g, tc, grid,
-                       hx, hy, hz, hlayer, hevt, pairs, counts, cands, cand_offsets, n_cand_cap, cnt);
+                       hx, hy, hz, hphi, hlayer, hevt, pairs, counts, cands, cand_offsets, n_cand_cap, cnt);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
+}
+
+uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap / kPredBlock) + 2; }
+
+hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
+                                const uint32_t* grid, const float* hx, const float* hy, const float* hz,
+                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
+                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t tiles = predict_fused_tiles(n_pair_cap);
+    if (tiles > ss.max_tiles) return hipErrorInvalidValue;
+    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(ss.status), 0u, 2ull * tiles + 2, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_predict_fused, dim3(stride_blocks(n_pair_cap, kPredBlock, lc)), dim3(kPredBlock), 0, st, g,
+                       tc, grid, hx, hy, hz, hphi, hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt,
+                       ss.status, reinterpret_cast<uint32_t*>(ss.status + tiles));
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
 }
