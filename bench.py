@@ -52,6 +52,39 @@ def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads):
     }
 
 
+def serving_stream(wl, n_events, seed, workers):
+    """The 10,000-event stream of the serving measurement, generated on a
+    process pool (before the process touches the GPU) into one HitSet."""
+    from trax_amd.api import Config, HitSet
+    from trax_amd.generator import make_events_parallel
+    cfg = Config(wl)
+    hs = HitSet(cfg)
+    for chunk in make_events_parallel(wl, n_events, seed=seed, workers=workers):
+        for ev in chunk:
+            hs.add(ev)
+    return cfg, hs
+
+
+def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events):
+    """Replay the stream with Poisson arrivals at rate_hz through the
+    continuous-batching server; latency = arrival to results on the host."""
+    import numpy as np
+    from trax_amd.api import Server
+    rng = np.random.default_rng(seed)
+    arrival = np.cumsum(rng.exponential(1.0 / rate_hz, hs.n_events))
+    arrival -= arrival[0]
+    sv = Server(dev, cfg, bounds, n_slots=n_slots, max_batch_events=max_batch_events, device_budget=64 << 30)
+    caps = sv.caps()
+    _, st = sv.replay(hs, arrival)
+    sv.close()
+    return {"events": st["events"], "arrival": "poisson", "rate_hz": rate_hz, "seed": seed,
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+            "events_per_s": round(st["events_per_s"], 1), "tracklets_per_s": round(st["tracklets_per_s"], 1),
+            "batches": st["batches"], "mean_batch_events": round(st["mean_batch_events"], 2),
+            "max_batch_events": st["max_batch_events"], "resubmits": st["resubmits"], "failed": st["failed"],
+            "slots": n_slots, "slot_hit_capacity": caps["max_hits"], "device_bytes": st["device_bytes"]}
+
+
 def cpu_baseline(wl, events, cfg_cls, hitset_cls, budget_s=20.0):
     import oracle
     cores = min(16, os.cpu_count() or 1)
@@ -81,6 +114,12 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (traffic-counter calibration)")
+    ap.add_argument("--serve-events", type=int, default=10000, help="events of the serving stream (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=4000.0, help="Poisson arrival rate of the stream (events/s)")
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-max-batch", type=int, default=256)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="processes generating the serving stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: events shard across GPUs with no communication, "
@@ -88,21 +127,29 @@ def main():
         sys.exit(2)
 
     from trax_amd import make_events, workload
-    from trax_amd.api import Batch, Caps, Config, Device, HitSet, run_batch_fit
+    from trax_amd.api import Batch, Caps, Config, Device, HitSet, OutputBounds, run_batch_fit
 
     wl = workload(3)
+    t_gen = time.perf_counter()
+    stream = serving_stream(wl, args.serve_events, SEED + 1, args.workers) if args.serve_events else None
     events = make_events(wl, N_EVENTS, seed=SEED)
     cfg = Config(wl)
     hs = HitSet(cfg, events)
+    print(f"inputs generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
     dev = Device(0)
 
     # capacity plan from a first run (grown on overflow), then fixed with headroom
     probe, _ = run_batch_fit(dev, cfg, hs, Caps.for_hits(N_EVENTS, hs.n_hits, use_graph=False))
     c = probe.counts()
     probe.close()
+    # output bounds per hit for the server: the whole mix and its 16 densest events
+    dense = HitSet(cfg, sorted(events, key=lambda ev: ev.n_hits)[-16:])
+    probe, _ = run_batch_fit(dev, cfg, dense, Caps.for_hits(16, dense.n_hits, use_graph=False))
+    bounds = OutputBounds.measure([c, probe.counts()], [hs.n_hits, dense.n_hits], headroom=1.25)
+    probe.close()
     head = lambda v: int(v * 1.15) + 4096
     caps = Caps(N_EVENTS, hs.n_hits, head(c["pairs"]), head(c["cands"]), head(c["tracklets"]), head(c["conn"]),
-                head(c["track_entries"]), use_graph=not args.no_graph)
+                head(c["track_entries"]), use_graph=not args.no_graph, per_event=True)
     b = Batch(dev, cfg, caps)
     b.submit(hs)  # upload once; later steps run on the resident hits
     if b.wait() != 0:
@@ -128,6 +175,16 @@ def main():
     ev_s = N_EVENTS / (ms_step / 1e3)
     trip_s = counts["tracklets"] / (ms_step / 1e3)
     phase_ms = {p: acc[p] / args.steps for p in phases}
+
+    serving = None
+    if stream is not None:
+        b.close()  # the server holds its own workspaces
+        print("serving stream", file=sys.stderr, flush=True)
+        serving = serve(dev, stream[0], stream[1], bounds, args.serve_rate, SEED + 2, args.serve_slots,
+                        args.serve_max_batch)
+        b = Batch(dev, cfg, caps)
+        b.submit(hs)
+        b.wait()
 
     # PCIe-inclusive rate (hits uploaded every step)
     t2 = time.perf_counter()
@@ -209,7 +266,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "kernel": dom, "algorithmic_bytes": int(pb[dom])},
-        "latency_ms": {"p50": None, "p99": None},
+        "latency_ms": {"p50": serving["p50_ms"] if serving else None,
+                       "p99": serving["p99_ms"] if serving else None},
+        "serving": serving,
     }
     b.close()
     if not args.no_cpu_baseline:

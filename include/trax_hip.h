@@ -162,11 +162,22 @@ typedef struct {
     uint32_t max_track_entries;
     uint32_t use_graph;    /* capture the sequence once, replay per batch */
     uint32_t ca_iterations; /* CA iterations in the sequence (0 = layer triplets + 2) */
+    /* 0: the batch is one event group, as buildTriplets runs a group of events
+     *    (the connectivity stage leaves out the group's last tracklet,
+     *    TripletConnectivityTight.cpp:277 and TripletThetaPhiFilter.h:250-253);
+     * 1: every event is its own group, so each event's results equal a run of
+     *    that event alone, whatever batch it shares */
+    uint32_t per_event;
 } trax_batch_caps;
 typedef struct {
     uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
     uint32_t blocks_per_cu;
 } trax_tuning;
+/* device and pinned host bytes a batch of these capacities would hold, without
+ * allocating (the planning step of buildTriplets' per-group buffer sizing,
+ * TripletBuilder.cpp:297-510, done once for all groups) */
+int trax_batch_plan(const trax_config* cfg, const trax_batch_caps* caps, uint64_t* device_bytes,
+                    uint64_t* pinned_bytes);
 int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_caps* caps,
                       const trax_tuning* tuning, trax_stream* s, trax_batch** out);
 void trax_batch_destroy(trax_batch* b);
@@ -186,12 +197,54 @@ uint64_t trax_batch_footprint(const trax_batch* b);
 
 /* timing of the last batched run from HIP events recorded between the phases
  * on the batch's stream, in milliseconds.  Phases: "grid_count", "grid_store",
- * "pair_plan", "pair_count", "pair_scan", "pair_store", "predict_count",
- * "predict_scan", "predict_store", "filter_count", "filter_scan",
- * "filter_store", "link_prepare", "link_count", "link_store", "circle_fit",
- * "ca_forward", "ca_backward"; stages (the reference's algorithm classes):
- * "grid", "pairs", "predict", "filter", "connectivity", "ca"; and "total". */
+ * "pair_plan", "pair_count", "pair_scan", "pair_store", "predict" (count,
+ * scan and store of the predictor in one pass), "filter_count",
+ * "filter_scan", "filter_store", "link_prepare", "link_count", "link_store",
+ * "circle_fit", "ca_forward", "ca_backward"; stages (the reference's
+ * algorithm classes): "grid", "pairs", "predict", "filter", "connectivity",
+ * "ca"; and "total". */
 int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms);
+
+/* Serving loop: events arrive one at a time and are grouped into batches of
+ * whole events (continuous batching), run on n_slots batches in flight, each
+ * with its own stream and workspace; each event's tracklets and tracks are
+ * copied back as soon as its batch completes.  No counterpart in the
+ * reference, which runs fixed event groups on one queue
+ * (TripletBuilder.cpp:297-552); the per-event results are the same. */
+typedef struct trax_server trax_server;
+typedef struct {
+    uint32_t n_slots;          /* batches in flight (default 2) */
+    uint32_t max_batch_events; /* events per batch (default 256) */
+    uint32_t max_batch_hits;   /* hits per batch; 0 = the most the device budget holds */
+    uint64_t device_budget;    /* device bytes for all slots (0 = 32 GiB) */
+    /* output capacities per input hit of a batch: upper bounds measured on
+     * the workload; a batch that still overflows is re-run as smaller batches */
+    float pairs_per_hit, cands_per_hit, tracklets_per_hit, conn_per_hit, track_entries_per_hit;
+    uint32_t keep_results; /* keep each event's tracklets and tracks (trax_server_event_result) */
+    uint32_t use_graph;
+} trax_serve_opts;
+typedef struct {
+    uint32_t events, failed, batches, resubmits, max_batch_events, slot_hits;
+    uint64_t tracklets, track_entries, device_bytes;
+    double wall_s, events_per_s, tracklets_per_s, mean_batch_events;
+    double p50_ms, p99_ms, max_ms; /* latency: arrival to results on the host */
+} trax_serve_stats;
+int trax_server_create(trax_ctx* ctx, const trax_config* cfg, const trax_serve_opts* opts, trax_server** out);
+void trax_server_destroy(trax_server* s);
+/* capacities of each slot (hit capacity from the device budget) */
+int trax_server_caps(const trax_server* s, trax_batch_caps* caps);
+/* Replay a stream in real time: event i of `events` arrives arrival_s[i]
+ * seconds (non-decreasing) after the call starts; returns when every event's
+ * results are on the host.  latency_ms[i] (optional) receives completion minus
+ * arrival (-1 for an event that overflowed a capacity on its own). */
+int trax_server_replay(trax_server* s, const trax_hits* events, const double* arrival_s, double* latency_ms,
+                       trax_serve_stats* stats);
+/* per-event results of the last replay, hit indices relative to the event's
+ * grid-ordered hits and tracklet indices relative to the event's tracklets:
+ * "tracklets.h1", "tracklets.h2", "tracklets.h3", "track_offsets", "tracks"
+ * (keep_results only), "n_tracklets", "n_track_entries" */
+int trax_server_event_result(const trax_server* s, uint32_t event, const char* name, const uint32_t** data,
+                             uint64_t* count);
 
 /* named result arrays: "grid", "hits.x", ..., "pairs", "pair_offsets",
  * "cands", "cand_offsets", "tracklets.h1/h2/h3", "tracklet_offsets", "eta",

@@ -50,3 +50,20 @@ def compare(got, want, keys=PIPELINE_KEYS):
             i = d[0]
             bad.append(f"{k}: {d.size} of {g.size} differ, first at {i}: {g[i]} != {w[i]}")
     return bad
+
+
+def extract_tracks(res):
+    """extractTracks (TripletBuilder.cpp:955-988) over a result: CA chains of at
+    least three tracklets as hit lists, innermost tracklet first."""
+    if "track_offsets" not in res:
+        return []
+    off, trk = res["track_offsets"].astype(np.int64), res["tracks"].astype(np.int64)
+    h1, h2, h3 = res["tracklets.h1"], res["tracklets.h2"], res["tracklets.h3"]
+    out = []
+    for i in range(off.size - 1):
+        n = off[i + 1] - off[i]
+        if n < 3:
+            continue
+        chain = trk[off[i]:off[i + 1]][::-1]
+        out.append([int(h1[chain[0]]), int(h2[chain[0]])] + [int(h3[t]) for t in chain])
+    return out

@@ -107,3 +107,60 @@ def test_batch_equals_per_event_runs(dev):
         t = _per_event_triplets(single, 1)[0]
         assert np.array_equal(t, batch_trip[e]), f"event {e}"
     b.close()
+
+
+def _event_slice(res, e):
+    """Event e's arrays of a batch run with each event its own group, rebased to
+    the event's first hit and first tracklet (the form a run of the event alone
+    has)."""
+    hev = res["hits.event"]
+    h0 = int(np.searchsorted(hev, e))
+    h1a = res["tracklets.h1"].astype(np.int64)
+    tev = hev[h1a] if h1a.size else np.zeros(0, np.int64)
+    t0, t1 = int(np.searchsorted(tev, e)), int(np.searchsorted(tev, e + 1))
+    out = {k: res[k][t0:t1].astype(np.int64) - h0 for k in ("tracklets.h1", "tracklets.h2", "tracklets.h3")}
+    out["eta"] = res["eta"][t0:t1]
+    basis = res["conn.basis"].astype(np.int64)
+    sel = (basis >= t0) & (basis < t1)
+    out["conn.basis"] = basis[sel] - t0
+    out["conn.followers"] = res["conn.followers"].astype(np.int64)[sel] - t0
+    for k in ("conn.connectable", "conn.nonconnectable"):
+        v = res[k].astype(np.int64)
+        out[k] = v[(v >= t0) & (v < t1)] - t0
+    if out["conn.basis"].size:
+        out["pt"] = res["pt"][t0:t1]
+        out["ca.states"] = res["ca.states"][t0:t1]
+        bb = res["ca.best_basis"][t0:t1].astype(np.int64)
+        out["ca.best_basis"] = np.where(bb == 0xFFFFFFFF, bb, bb - t0)
+        to = res["track_offsets"].astype(np.int64)
+        out["track_offsets"] = to[t0:t1 + 1] - to[t0]
+        out["tracks"] = res["tracks"][to[t0]:to[t1]].astype(np.int64) - t0
+    return out
+
+
+def test_batch_with_per_event_groups_equals_per_event_runs(dev):
+    """Each event its own group: every per-event array of the batch (tracklets,
+    eta, connections, predicate lists, circle radii, CA states, best bases,
+    tracks) equals the run of that event alone, bit for bit."""
+    wl = small_workload(3, events=6, tracks_min=20, tracks_max=600)
+    from trax_amd import make_events
+    from trax_amd.api import Config
+    cfg = Config(wl)
+    evs = make_events(wl, seed=5)
+    hs = HitSet(cfg, evs)
+    print(f"per-event groups: {hs.n_events} events, {hs.n_hits} hits", flush=True)
+    b, res = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, per_event=True))
+    b.close()
+    n_conn = 0
+    for e, ev in enumerate(evs):
+        single = dev.run_plain(cfg, HitSet(cfg, [ev]))
+        got = _event_slice(res, e)
+        for k, v in got.items():
+            w = single[k].astype(np.int64) if single[k].dtype == np.uint32 else single[k]
+            g = v.view(np.uint32) if v.dtype == np.float32 else v
+            w = w.view(np.uint32) if w.dtype == np.float32 else w
+            assert np.array_equal(g, w), f"event {e} {k}"
+        if single["conn.basis"].size == 0:
+            assert "ca.states" not in got
+        n_conn += single["conn.basis"].size
+    assert n_conn > 0

@@ -37,12 +37,26 @@ class HitCols(C.Structure):
 
 class BatchCaps(C.Structure):
     _fields_ = [(n, U32) for n in ("max_events", "max_hits", "max_pairs", "max_cands", "max_tracklets",
-                                   "max_conn", "max_track_entries", "use_graph", "ca_iterations")]
+                                   "max_conn", "max_track_entries", "use_graph", "ca_iterations", "per_event")]
 
 
 class Tuning(C.Structure):
     _fields_ = [(n, U32) for n in ("grid_count_block", "grid_store_block", "pair_block", "predict_block",
                                    "filter_block", "link_block", "blocks_per_cu")]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("n_slots", U32), ("max_batch_events", U32), ("max_batch_hits", U32), ("device_budget", C.c_uint64),
+                ("pairs_per_hit", C.c_float), ("cands_per_hit", C.c_float), ("tracklets_per_hit", C.c_float),
+                ("conn_per_hit", C.c_float), ("track_entries_per_hit", C.c_float), ("keep_results", U32),
+                ("use_graph", U32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, U32) for n in ("events", "failed", "batches", "resubmits", "max_batch_events", "slot_hits")] + \
+               [(n, C.c_uint64) for n in ("tracklets", "track_entries", "device_bytes")] + \
+               [(n, C.c_double) for n in ("wall_s", "events_per_s", "tracklets_per_s", "mean_batch_events",
+                                          "p50_ms", "p99_ms", "max_ms")]
 
 
 class DeviceInfo(C.Structure):
@@ -87,6 +101,12 @@ def lib():
         "trax_config_set_exec": (C.c_int, [P, U32, C.c_float]),
         "trax_config_min_radius_curvature": (C.c_float, [P]),
         "trax_run_plain": (C.c_int, [P, P, P, C.c_int, C.POINTER(P)]),
+        "trax_batch_plan": (C.c_int, [P, C.POINTER(BatchCaps), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "trax_server_create": (C.c_int, [P, P, C.POINTER(ServeOpts), C.POINTER(P)]),
+        "trax_server_destroy": (None, [P]),
+        "trax_server_caps": (C.c_int, [P, C.POINTER(BatchCaps)]),
+        "trax_server_replay": (C.c_int, [P, P, P, P, C.POINTER(ServeStats)]),
+        "trax_server_event_result": (C.c_int, [P, U32, C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64)]),
         "trax_batch_create": (C.c_int, [P, P, C.POINTER(BatchCaps), C.POINTER(Tuning), P, C.POINTER(P)]),
         "trax_batch_destroy": (None, [P]),
         "trax_batch_submit": (C.c_int, [P, P]),
@@ -295,19 +315,21 @@ class Caps:
     max_track_entries: int
     use_graph: bool = True
     ca_iterations: int = 0
+    per_event: bool = False  # each event its own group (results equal per-event runs)
 
     @staticmethod
     def for_hits(max_events: int, max_hits: int, pairs_per_hit: float = 8.0, cands_per_pair: float = 4.0,
-                 trk_per_hit: float = 1.0, use_graph: bool = True) -> "Caps":
+                 trk_per_hit: float = 1.0, use_graph: bool = True, per_event: bool = False) -> "Caps":
         p = int(max_hits * pairs_per_hit) + 1024
         q = int(p * cands_per_pair) + 1024
         r = int(max_hits * trk_per_hit) + 1024
-        return Caps(max_events, max_hits, p, q, r, 4 * r + 1024, 2 * r + 1024, use_graph)
+        return Caps(max_events, max_hits, p, q, r, 4 * r + 1024, 2 * r + 1024, use_graph, 0, per_event)
 
     def grown(self, factor: float = 2.0) -> "Caps":
         g = lambda v: int(v * factor) + 1024
         return Caps(self.max_events, self.max_hits, g(self.max_pairs), g(self.max_cands), g(self.max_tracklets),
-                    g(self.max_conn), g(self.max_track_entries), self.use_graph, self.ca_iterations)
+                    g(self.max_conn), g(self.max_track_entries), self.use_graph, self.ca_iterations,
+                    self.per_event)
 
 
 class Batch:
@@ -316,7 +338,8 @@ class Batch:
     def __init__(self, dev: Device, cfg: Config, caps: Caps, tuning: Optional[Dict[str, int]] = None,
                  stream: Optional[Stream] = None):
         c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
-                      caps.max_conn, caps.max_track_entries, int(caps.use_graph), caps.ca_iterations)
+                      caps.max_conn, caps.max_track_entries, int(caps.use_graph), caps.ca_iterations,
+                      int(caps.per_event))
         t = Tuning(**(tuning or {}))
         h = P()
         check(lib().trax_batch_create(dev.h, cfg.h, C.byref(c), C.byref(t), None if stream is None else stream.h,
@@ -379,3 +402,80 @@ def run_batch_fit(dev: Device, cfg: Config, hits: HitSet, caps: Caps, tuning=Non
         b.close()
         caps = caps.grown()
     raise TraxError(TRAX_ERR_CAPACITY, "capacities kept overflowing")
+
+
+def plan_bytes(cfg: Config, caps: Caps):
+    """(device bytes, pinned host bytes) a batch of these capacities holds."""
+    c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
+                  caps.max_conn, caps.max_track_entries, int(caps.use_graph), caps.ca_iterations,
+                      int(caps.per_event))
+    d, h = C.c_uint64(), C.c_uint64()
+    check(lib().trax_batch_plan(cfg.h, C.byref(c), C.byref(d), C.byref(h)))
+    return int(d.value), int(h.value)
+
+
+@dataclass
+class OutputBounds:
+    """Output counts per input hit of a batch: upper bounds for the capacities."""
+    pairs: float
+    cands: float
+    tracklets: float
+    conn: float
+    track_entries: float
+
+    @staticmethod
+    def measure(counts: List[Dict[str, int]], hits: List[int], headroom: float = 1.25) -> "OutputBounds":
+        """The largest ratio seen over batches of a workload, with headroom."""
+        keys = ("pairs", "cands", "tracklets", "conn", "track_entries")
+        r = {k: max(c[k] / max(h, 1) for c, h in zip(counts, hits)) * headroom for k in keys}
+        f = {k: max(v, 1e-3) for k, v in r.items()}  # capacities must be positive
+        return OutputBounds(f["pairs"], f["cands"], f["tracklets"], f["conn"], f["track_entries"])
+
+
+class Server:
+    """Continuous-batching serving loop over the batched pipeline."""
+
+    def __init__(self, dev: Device, cfg: Config, bounds: OutputBounds, n_slots: int = 2, max_batch_events: int = 256,
+                 max_batch_hits: int = 0, device_budget: int = 0, keep_results: bool = False,
+                 use_graph: bool = True):
+        o = ServeOpts(n_slots, max_batch_events, max_batch_hits, device_budget, bounds.pairs, bounds.cands,
+                      bounds.tracklets, bounds.conn, bounds.track_entries, int(keep_results), int(use_graph))
+        h = P()
+        check(lib().trax_server_create(dev.h, cfg.h, C.byref(o), C.byref(h)))
+        self.h = h
+        self.n_events = 0
+
+    def caps(self) -> Dict[str, int]:
+        c = BatchCaps()
+        check(lib().trax_server_caps(self.h, C.byref(c)))
+        return {n: int(getattr(c, n)) for n, _ in BatchCaps._fields_}
+
+    def replay(self, hits: HitSet, arrival_s: np.ndarray):
+        """Replay the events of `hits` arriving at arrival_s (seconds from the
+        start); returns (per-event latency in ms, stats dict)."""
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        if arr.size != hits.n_events:
+            raise ValueError("one arrival time per event")
+        lat = np.zeros(arr.size, np.float64)
+        st = ServeStats()
+        rc = lib().trax_server_replay(self.h, hits.h, ptr(arr), ptr(lat), C.byref(st))
+        if rc not in (0, TRAX_ERR_CAPACITY):
+            check(rc)
+        self.n_events = arr.size
+        stats = {n: (float(getattr(st, n)) if t is C.c_double else int(getattr(st, n))) for n, t in st._fields_}
+        return lat, stats
+
+    def event_result(self, event: int, name: str) -> np.ndarray:
+        d, n = P(), C.c_uint64()
+        check(lib().trax_server_event_result(self.h, event, name.encode(), C.byref(d), C.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, np.uint32)
+        return np.ctypeslib.as_array(C.cast(d, C.POINTER(C.c_uint32)), shape=(n.value,)).copy()
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

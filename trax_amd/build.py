@@ -23,6 +23,7 @@ SOURCES = [
     "kernels/link.hip",
     "host/trax_host.cpp",
     "capi.cpp",
+    "serve.cpp",
 ]
 
 FLAGS = [

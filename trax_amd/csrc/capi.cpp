@@ -1,85 +1,12 @@
 // C ABI of the gfx950 triplet-seeding library (declared in include/trax_hip.h).
-#include "../../include/trax_hip.h"
-
-#include <hip/hip_runtime.h>
+#include "capi_internal.h"
 
 #include <algorithm>
 #include <chrono>
 #include <cstring>
-#include <map>
-#include <memory>
-#include <string>
-#include <vector>
-
-#include "host/trax_host.h"
-#include "kernels/kernels.h"
 
 using namespace trax;
-
-namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-
-template <class F>
-int guarded(F f) {
-    try {
-        return f();
-    } catch (const HipError& e) {
-        return fail(TRAX_ERR_HIP, e.what());
-    } catch (const InputError& e) {
-        return fail(TRAX_ERR_INPUT, e.what());
-    } catch (const CapacityError& e) {
-        return fail(TRAX_ERR_CAPACITY, e.what());
-    } catch (const std::exception& e) {
-        return fail(TRAX_ERR_ARG, e.what());
-    }
-}
-
-struct Arr {
-    std::vector<uint32_t> u;
-    std::vector<float> f;
-    bool is_float = false;
-};
-
-}  // namespace
-
-struct trax_result {
-    std::map<std::string, Arr> arrays;
-    std::map<std::string, double> timings;
-    void put(const std::string& k, std::vector<uint32_t> v) {
-        Arr a; a.u = std::move(v); arrays[k] = std::move(a);
-    }
-    void putf(const std::string& k, std::vector<float> v) {
-        Arr a; a.f = std::move(v); a.is_float = true; arrays[k] = std::move(a);
-    }
-};
-
-struct trax_ctx {
-    std::unique_ptr<Context> c;
-};
-struct trax_stream {
-    hipStream_t s = nullptr;
-};
-
-struct trax_config {
-    GridConfig grid;
-    TripletConfigurations trip{1.0f};
-    GeometrySupplement geom;
-    uint32_t n_threads = 256;
-    float d_eta_cut = 0.0256f;
-};
-
-struct trax_hits {
-    uint32_t max_layer = 0;
-    uint32_t n_events = 0;
-    HitCollection hits;
-    std::vector<uint32_t> evt_nhits, evt_offset, lay_nhits, lay_offset;
-};
+using namespace trax_capi;
 
 extern "C" const char* trax_last_error(void) { return g_last_error.c_str(); }
 
@@ -135,7 +62,7 @@ extern "C" int trax_stream_destroy(trax_stream* s) {
     });
 }
 
-static hipStream_t pick(trax_ctx* ctx, trax_stream* s) { return s ? s->s : ctx->c->stream(); }
+static hipStream_t pick(trax_ctx* ctx, trax_stream* s) { return pick_stream(ctx, s); }
 
 extern "C" int trax_stream_sync(trax_ctx* ctx, trax_stream* s) {
     return guarded([&] {
@@ -252,16 +179,30 @@ extern "C" int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_
             return fail(TRAX_ERR_ARG, "configuration's highest layer differs from the collection's");
         std::vector<PHit> ev(n);
         for (uint32_t i = 0; i < n; ++i) ev[i] = PHit{x[i], y[i], z[i], layer[i], det_id[i], track_id[i], track_pt[i]};
-        const uint32_t e = h->n_events;
-        EventSupplement es(e + 1);
-        LayerSupplement ls(h->max_layer, e + 1);
-        std::copy(h->evt_nhits.begin(), h->evt_nhits.end(), es.nHits.begin());
-        std::copy(h->evt_offset.begin(), h->evt_offset.end(), es.offset.begin());
-        std::copy(h->lay_nhits.begin(), h->lay_nhits.end(), ls.nHits.begin());
-        std::copy(h->lay_offset.begin(), h->lay_offset.end(), ls.offset.begin());
-        h->hits.addEvent(ev, es, e, ls, cfg->trip, max_tracks, only_tracks != 0);
-        h->evt_nhits = es.nHits; h->evt_offset = es.offset;
-        h->lay_nhits = ls.nHits; h->lay_offset = ls.offset;
+        const uint32_t e = h->n_events, L = h->max_layer;
+        // the supplements grow by one event in place (moved in and out, not copied)
+        EventSupplement es(0);
+        LayerSupplement ls(L, 0);
+        es.nEvents = e + 1;
+        es.nHits = std::move(h->evt_nhits);
+        es.offset = std::move(h->evt_offset);
+        ls.nHits = std::move(h->lay_nhits);
+        ls.offset = std::move(h->lay_offset);
+        es.nHits.resize(e + 1, 0);
+        es.offset.resize(e + 1, 0);
+        ls.nHits.resize((size_t)(e + 1) * L, 0);
+        ls.offset.resize((size_t)(e + 1) * L, 0);
+        try {
+            h->hits.addEvent(ev, es, e, ls, cfg->trip, max_tracks, only_tracks != 0);
+        } catch (...) {
+            es.nHits.resize(e); es.offset.resize(e);
+            ls.nHits.resize((size_t)e * L); ls.offset.resize((size_t)e * L);
+            h->evt_nhits = std::move(es.nHits); h->evt_offset = std::move(es.offset);
+            h->lay_nhits = std::move(ls.nHits); h->lay_offset = std::move(ls.offset);
+            throw;
+        }
+        h->evt_nhits = std::move(es.nHits); h->evt_offset = std::move(es.offset);
+        h->lay_nhits = std::move(ls.nHits); h->lay_offset = std::move(ls.offset);
         h->n_events = e + 1;
         return TRAX_OK;
     });
@@ -328,7 +269,7 @@ extern "C" int trax_config_set_exec(trax_config* c, uint32_t n_threads, float d_
 
 extern "C" float trax_config_min_radius_curvature(const trax_config* c) { return c->trip.minRadiusCurvature(); }
 
-static void validate_config(const trax_config* cfg) {
+void trax_capi::validate_config(const trax_config* cfg) {
     const uint32_t L = cfg->trip.maxLayer();
     if (cfg->trip.size() == 0) throw InputError("no layer triplet configured");
     if (cfg->geom.size() < L) throw InputError("geometry supplement has fewer layers than the triplets use");
@@ -453,93 +394,60 @@ extern "C" int trax_run_plain_bench(trax_ctx* ctx, const trax_config* cfg, const
 
 // ---- batched pipeline -------------------------------------------------------------
 
-namespace {
+const char* const trax_capi::kPhases[kNumStages] = {
+    "grid_count", "grid_store",   "pair_plan",   "pair_count",   "pair_scan",    "pair_store", "predict",    "filter_count",
+    "filter_scan", "filter_store", "link_prepare", "link_count", "link_store", "circle_fit", "ca_forward", "ca_backward"};
+const trax_capi::StageRange trax_capi::kStageRanges[6] = {{"grid", 0, 1},   {"pairs", 2, 5},
+                                                          {"predict", 6, 6}, {"filter", 7, 9},
+                                                          {"connectivity", 10, 13}, {"ca", 14, 15}};
 
-struct Arena {
-    uint8_t* base = nullptr;
-    size_t used = 0, cap = 0;
-    template <class T>
-    T* take(uint64_t n) {
-        size_t bytes = (size_t)std::max<uint64_t>(n, 1) * sizeof(T);
-        bytes = (bytes + 255) & ~(size_t)255;
-        T* p = reinterpret_cast<T*>(base + used);
-        used += bytes;
-        if (used > cap) throw CapacityError("arena overflow");
-        return p;
-    }
-};
-
-// phases of the batched launch sequence: one graph each, HIP events between
-// them time every phase on the batch's stream
-const char* kPhases[] = {"grid_count",   "grid_store",   "pair_plan",  "pair_count", "pair_scan",   "pair_store",
-                         "predict",      "filter_count", "filter_scan", "filter_store", "link_prepare", "link_count",
-                         "link_store",   "circle_fit",   "ca_forward", "ca_backward"};
-constexpr int kNumStages = 16;
-// the reference's algorithm classes as ranges of phases
-struct StageRange {
-    const char* name;
-    int first, last;
-};
-const StageRange kStageRanges[] = {{"grid", 0, 1},   {"pairs", 2, 5},          {"predict", 6, 6},
-                                   {"filter", 7, 9}, {"connectivity", 10, 13}, {"ca", 14, 15}};
-
-}  // namespace
-
-struct trax_batch {
-    trax_ctx* owner = nullptr;
-    hipStream_t st = nullptr;
-    trax_config cfg;
-    trax_batch_caps caps{};
-    LaunchCfg lc;
-    uint32_t E = 0, L = 0, T = 0, C = 0, slot_cap = 0, scan_cap = 0, ca_iter = 0;
-    std::unique_ptr<ConfigMirror> cm;
-    DevCounts* cnt = nullptr;
-    ScanScratch scan;
-    Arena arena;
-    uint8_t* arena_mem = nullptr;
-    uint32_t* staging = nullptr;  // pinned: 7 hit columns, supplements, shape
-    size_t staging_words = 0;
-    DevCounts* h_counts = nullptr;  // pinned copy after a run
-    // device buffers
-    HitColumns hin, hout;
-    uint32_t *sup = nullptr, *grid = nullptr, *seg_base = nullptr, *slot_counts = nullptr, *pairs = nullptr,
-             *pair_off = nullptr, *pred_counts = nullptr, *cands = nullptr, *cand_off = nullptr,
-             *word_counts = nullptr, *th1 = nullptr, *th2 = nullptr, *th3 = nullptr, *t_off = nullptr;
-    uint64_t* bitmap = nullptr;
-    LinkBufs lb;
-    CABufs cb;
-    hipGraph_t graphs[kNumStages] = {};
-    hipGraphExec_t execs[kNumStages] = {};
-    bool captured = false;
-    hipEvent_t ev[kNumStages + 1] = {};
-    uint32_t n_events = 0, n_hits = 0;
-    bool submitted = false, have_counts = false;
-
-    void enqueue_stage(int s);
-    void enqueue_ca_tail();
-
-    // enqueue the whole sequence on the resident inputs (graph replay or eager)
-    void launch() {
-        if (caps.use_graph && !captured) {
-            for (int k = 0; k < kNumStages; ++k) {
-                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-                enqueue_stage(k);
-                hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
-                hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
-            }
-            captured = true;
-        }
+void trax_batch::launch() {
+    if (caps.use_graph && !captured) {
         for (int k = 0; k < kNumStages; ++k) {
-            hip_check(hipEventRecord(ev[k], st), "event");
-            if (caps.use_graph) hip_check(hipGraphLaunch(execs[k], st), "graph launch");
-            else enqueue_stage(k);
+            hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+            enqueue_stage(k);
+            hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
+            hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
         }
-        hip_check(hipEventRecord(ev[kNumStages], st), "event");
-        hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
-        submitted = true;
-        have_counts = false;
+        captured = true;
     }
-};
+    for (int k = 0; k < kNumStages; ++k) {
+        hip_check(hipEventRecord(ev[k], st), "event");
+        if (caps.use_graph) hip_check(hipGraphLaunch(execs[k], st), "graph launch");
+        else enqueue_stage(k);
+    }
+    hip_check(hipEventRecord(ev[kNumStages], st), "event");
+    hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
+    submitted = true;
+    have_counts = false;
+}
+
+int trax_batch::finish() {
+    // the CA forward phase ran a fixed number of iterations; continue on the
+    // host if the last one still found living cells
+    uint32_t flags[3];
+    hip_check(hipMemcpyAsync(flags, cb.flags, sizeof(flags), hipMemcpyDeviceToHost, st), "d2h");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uint32_t k = ca_iter;
+    if (h_counts->n_conn && !(h_counts->err & kErrCapacity) && flags[(k - 1) % 3]) {
+        while (true) {
+            hip_check(launch_ca_iterations(caps.max_conn, lb.basis, lb.foll, cb, cnt, k, 1, lc, st), "ca iteration");
+            hip_check(hipMemcpyAsync(flags, cb.flags, sizeof(flags), hipMemcpyDeviceToHost, st), "d2h");
+            hip_check(hipStreamSynchronize(st), "sync");
+            if (!flags[k % 3]) break;
+            ++k;
+        }
+        enqueue_ca_tail();
+        hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
+        hip_check(hipStreamSynchronize(st), "sync");
+    }
+    have_counts = true;
+    const uint32_t err = h_counts->err;
+    if (err & kErrCapacity) return fail(TRAX_ERR_CAPACITY, "an output exceeded its capacity");
+    if (err & kErrCellRange) return fail(TRAX_ERR_INPUT, "a hit lies outside the z-phi grid");
+    if (err) return fail(TRAX_ERR_INPUT, "device reported an input error");
+    return TRAX_OK;
+}
 
 void trax_batch::enqueue_stage(int s) {
     const GridGeom g = cfg.grid.geom();
@@ -599,6 +507,7 @@ void trax_batch::enqueue_stage(int s) {
             break;
         case 10:  // link_prepare
             ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
+            ok(launch_link_events(tc, E, R, hout.layer, hout.evt, th1, t_off, lb, cnt, st), "link events");
             break;
         case 11:  // link_count
             ok(launch_link_count(tc, R, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link count");
@@ -606,7 +515,7 @@ void trax_batch::enqueue_stage(int s) {
             break;
         case 12:  // link_store
             ok(launch_link_store(tc, R, K, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link store");
-            ok(launch_link_flags(R, lb, cnt, lc, st), "link flags");
+            ok(launch_link_flags(R, hout.evt, th1, lb, cnt, lc, st), "link flags");
             ok(launch_scan(lb.flag_a, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_connectable, scan, st), "scan");
             ok(launch_scan(lb.flag_b, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_nonconnectable, scan, st), "scan");
             ok(launch_link_compact(R, lb, cnt, lc, st), "link compact");
@@ -633,42 +542,42 @@ void trax_batch::enqueue_ca_tail() {
     hip_check(launch_ca_tracks(R, caps.max_track_entries, cb, cnt, scan, lc, st, 0), "ca tracks");
 }
 
-extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_caps* caps,
-                                 const trax_tuning* tuning, trax_stream* s, trax_batch** out) {
-    return guarded([&] {
-        validate_config(cfg);
-        hip_check(hipSetDevice(ctx->c->device()), "hipSetDevice");
-        std::unique_ptr<trax_batch> b(new trax_batch);
-        b->owner = ctx;
-        b->st = pick(ctx, s);
-        b->cfg = *cfg;
-        b->caps = *caps;
-        if (tuning) {
-            b->lc.grid_count_block = tuning->grid_count_block;
-            b->lc.grid_store_block = tuning->grid_store_block;
-            b->lc.pair_block = tuning->pair_block;
-            b->lc.predict_block = tuning->predict_block;
-            b->lc.filter_block = tuning->filter_block;
-            b->lc.link_block = tuning->link_block;
-            b->lc.blocks_per_cu = tuning->blocks_per_cu;
-        }
-        auto& B = *b;
-        B.E = caps->max_events;
-        B.L = cfg->trip.maxLayer();
-        B.T = cfg->trip.size();
-        B.cfg.grid.nEvents = B.E;
-        B.cfg.grid.nLayers = B.L;
-        B.C = B.cfg.grid.cells();
-        B.ca_iter = caps->ca_iterations ? caps->ca_iterations : B.T + 2;
-        const uint64_t E = B.E, L = B.L, T = B.T, C = B.C, N = caps->max_hits, P = caps->max_pairs,
-                       Q = caps->max_cands, R = caps->max_tracklets, K = caps->max_conn, TE = caps->max_track_entries;
-        const uint64_t slot_cap = N + E * T * cfg->n_threads;
-        if (slot_cap >= 0xffffffffull || E * L * C >= 0xffffffffull) throw InputError("capacities too large");
-        B.slot_cap = (uint32_t)slot_cap;
-        B.cm.reset(new ConfigMirror(*ctx->c, cfg->trip, cfg->geom, cfg->n_threads, cfg->d_eta_cut));
-        // one arena for every buffer of the pipeline
-        // the layout is run twice: once to measure the arena, once to carve it
-        auto layout = [&](Arena& a) {
+namespace {
+
+// fix the batch's shape from the configuration and capacities
+void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* caps) {
+    validate_config(cfg);
+    if (!caps->max_events || !caps->max_hits) throw InputError("capacities need at least one event and hit");
+    B.cfg = *cfg;
+    B.caps = *caps;
+    B.E = caps->max_events;
+    B.L = cfg->trip.maxLayer();
+    B.T = cfg->trip.size();
+    B.cfg.grid.nEvents = B.E;
+    B.cfg.grid.nLayers = B.L;
+    B.C = B.cfg.grid.cells();
+    B.ca_iter = caps->ca_iterations ? caps->ca_iterations : B.T + 2;
+    const uint64_t slot_cap = (uint64_t)caps->max_hits + (uint64_t)B.E * B.T * cfg->n_threads;
+    if (slot_cap >= 0xffffffffull || (uint64_t)B.E * B.L * B.C >= 0xffffffffull)
+        throw InputError("capacities too large");
+    B.slot_cap = (uint32_t)slot_cap;
+    const uint64_t P = caps->max_pairs, Q = caps->max_cands, R = caps->max_tracklets;
+    B.scan_cap = (uint32_t)std::max<uint64_t>(
+        {caps->max_hits + 1ull, slot_cap + 1, P + 1, Q / 64 + 2, R + 1, (uint64_t)B.E * B.T + 1});
+}
+
+uint32_t batch_scan_tiles(const trax_batch& B) {
+    return std::max(scan_tiles(B.scan_cap), predict_fused_tiles(B.caps.max_pairs));
+}
+
+}  // namespace
+
+// one arena for every buffer of the pipeline; run once to measure the arena,
+// once to carve it
+void trax_batch::layout(Arena& a) {
+    const uint64_t E = this->E, L = this->L, T = this->T, C = this->C, N = caps.max_hits, P = caps.max_pairs,
+                   Q = caps.max_cands, R = caps.max_tracklets, K = caps.max_conn, TE = caps.max_track_entries;
+    trax_batch& B = *this;
         auto cols = [&](HitColumns& c) {
             c.x = a.take<float>(N); c.y = a.take<float>(N); c.z = a.take<float>(N);
             c.layer = a.take<uint32_t>(N); c.det = a.take<uint32_t>(N); c.id = a.take<uint32_t>(N);
@@ -701,6 +610,11 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         lb.flag_a = a.take<uint32_t>(R + 1); lb.flag_b = a.take<uint32_t>(R + 1);
         lb.connectable = a.take<uint32_t>(R); lb.nonconnectable = a.take<uint32_t>(R);
         lb.pt = a.take<float>(R);
+        lb.per_event = caps.per_event ? 1u : 0u;
+        if (lb.per_event) {
+            lb.ev_first = a.take<uint32_t>(E); lb.ev_last = a.take<uint32_t>(E); lb.ev_has = a.take<uint32_t>(E);
+            lb.ev_off = a.take<uint32_t>(E * (T + 1));
+        }
         CABufs& cb = B.cb;
         cb.st_a = a.take<uint32_t>(R); cb.st_b = a.take<uint32_t>(R);
         cb.fbc = a.take<uint32_t>(R + 1); cb.cursor = a.take<uint32_t>(R);
@@ -709,21 +623,58 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         cb.handler = a.take<uint32_t>(R + 1);
         cb.tracks = a.take<uint32_t>(TE);
         cb.flags = a.take<uint32_t>(3);
-        };
+}
+
+extern "C" int trax_batch_plan(const trax_config* cfg, const trax_batch_caps* caps, uint64_t* device_bytes,
+                               uint64_t* pinned_bytes) {
+    return guarded([&] {
+        trax_batch B;
+        batch_shape(B, cfg, caps);
         Arena measure;
         measure.cap = ~(size_t)0;
-        layout(measure);
+        B.layout(measure);
+        if (device_bytes)
+            *device_bytes = measure.used + (uint64_t)(batch_scan_tiles(B) + 1) * 8 + sizeof(DevCounts);
+        if (pinned_bytes)
+            *pinned_bytes = (7ull * caps->max_hits + B.E + 2ull * B.E * B.L + 2) * 4 + sizeof(DevCounts);
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_caps* caps,
+                                 const trax_tuning* tuning, trax_stream* s, trax_batch** out) {
+    return guarded([&] {
+        hip_check(hipSetDevice(ctx->c->device()), "hipSetDevice");
+        std::unique_ptr<trax_batch> b(new trax_batch);
+        b->owner = ctx;
+        b->st = pick(ctx, s);
+        if (tuning) {
+            b->lc.grid_count_block = tuning->grid_count_block;
+            b->lc.grid_store_block = tuning->grid_store_block;
+            b->lc.pair_block = tuning->pair_block;
+            b->lc.predict_block = tuning->predict_block;
+            b->lc.filter_block = tuning->filter_block;
+            b->lc.link_block = tuning->link_block;
+            b->lc.blocks_per_cu = tuning->blocks_per_cu;
+        }
+        auto& B = *b;
+        batch_shape(B, cfg, caps);
+        const uint64_t E = B.E, L = B.L, N = caps->max_hits, P = caps->max_pairs, Q = caps->max_cands,
+                       R = caps->max_tracklets, K = caps->max_conn, TE = caps->max_track_entries;
+        B.cm.reset(new ConfigMirror(*ctx->c, cfg->trip, cfg->geom, cfg->n_threads, cfg->d_eta_cut));
+        Arena measure;
+        measure.cap = ~(size_t)0;
+        B.layout(measure);
         B.arena.cap = measure.used;
         hip_check(hipMalloc(&B.arena_mem, B.arena.cap), "hipMalloc(arena)");
         B.arena.base = B.arena_mem;
-        layout(B.arena);
-        B.scan_cap = (uint32_t)std::max<uint64_t>({N + 1, slot_cap + 1, P + 1, Q / 64 + 2, R + 1, E * T + 1});
-        const uint32_t tiles = std::max(scan_tiles(B.scan_cap), predict_fused_tiles(P));
+        B.layout(B.arena);
+        const uint32_t tiles = batch_scan_tiles(B);
         hip_check(hipMalloc(&B.scan.status, (size_t)(tiles + 1) * sizeof(uint64_t)), "hipMalloc(scan)");
         B.scan.max_tiles = tiles;
         hip_check(hipMalloc(&B.cnt, sizeof(DevCounts)), "hipMalloc(counts)");
         DevCounts init{};
-        init.cap_slots = (uint32_t)slot_cap; init.cap_pairs = (uint32_t)P; init.cap_cands = (uint32_t)Q;
+        init.cap_slots = B.slot_cap; init.cap_pairs = (uint32_t)P; init.cap_cands = (uint32_t)Q;
         init.cap_tracklets = (uint32_t)R; init.cap_conn = (uint32_t)K; init.cap_basis = (uint32_t)K;
         init.cap_tracks = (uint32_t)TE;
         hip_check(hipMemcpy(B.cnt, &init, sizeof(DevCounts), hipMemcpyHostToDevice), "h2d counts");
@@ -819,32 +770,8 @@ extern "C" int trax_batch_rerun(trax_batch* b) {
 extern "C" int trax_batch_wait(trax_batch* b) {
     return guarded([&] {
         if (!b->submitted) return fail(TRAX_ERR_STATE, "nothing submitted");
-        hip_check(hipStreamSynchronize(b->st), "hipStreamSynchronize");
-        // the CA forward phase ran a fixed number of iterations; continue on the
-        // host if the last one still found living cells
-        uint32_t flags[3];
-        hip_check(hipMemcpy(flags, b->cb.flags, sizeof(flags), hipMemcpyDeviceToHost), "d2h");
-        uint32_t k = b->ca_iter;
-        if (b->h_counts->n_conn && flags[(k - 1) % 3]) {
-            while (true) {
-                hip_check(launch_ca_iterations(b->caps.max_conn, b->lb.basis, b->lb.foll, b->cb, b->cnt, k, 1, b->lc,
-                                               b->st),
-                          "ca iteration");
-                hip_check(hipMemcpyAsync(flags, b->cb.flags, sizeof(flags), hipMemcpyDeviceToHost, b->st), "d2h");
-                hip_check(hipStreamSynchronize(b->st), "sync");
-                if (!flags[k % 3]) break;
-                ++k;
-            }
-            b->enqueue_ca_tail();
-            hip_check(hipMemcpyAsync(b->h_counts, b->cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, b->st), "d2h");
-            hip_check(hipStreamSynchronize(b->st), "sync");
-        }
-        b->have_counts = true;
-        const uint32_t err = b->h_counts->err;
-        if (err & kErrCapacity) return fail(TRAX_ERR_CAPACITY, "an output exceeded its capacity");
-        if (err & kErrCellRange) return fail(TRAX_ERR_INPUT, "a hit lies outside the z-phi grid");
-        if (err) return fail(TRAX_ERR_INPUT, "device reported an input error");
-        return TRAX_OK;
+        hip_check(hipSetDevice(b->owner->c->device()), "hipSetDevice");
+        return b->finish();
     });
 }
 

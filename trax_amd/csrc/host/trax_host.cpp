@@ -420,7 +420,7 @@ ConnectivityOut TripletConnectivityTight::run(const HitMirror& hits, TrackletCol
     hip_check(launch_link_store(tc, R, out.nConn, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(),
                                 tr.hit3.get_mem(), tr.trackletOffsets.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
               "link store");
-    hip_check(launch_link_flags(R, b, cnt, ctx.launch(), ctx.stream()), "link flags");
+    hip_check(launch_link_flags(R, hc.evt, tr.hit1.get_mem(), b, cnt, ctx.launch(), ctx.stream()), "link flags");
     PrefixSum(ctx).run(flag_a.get_mem(), R + 1, nThreads, &cnt->n_connectable);
     out.nConnectable = ctx.download_count(&cnt->n_connectable);  // .cpp:200
     PrefixSum(ctx).run(flag_b.get_mem(), R + 1, nThreads, &cnt->n_nonconnectable);

@@ -56,6 +56,9 @@ uint32_t scan_tiles(uint64_t n_cap);
 uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_copy(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t st);
+// event column of a batch of whole events from its event offsets (sizes on device)
+hipError_t launch_event_ids(uint32_t* evt, const uint32_t* evt_offset, const DevCounts* cnt, uint32_t n_events_cap,
+                            hipStream_t st);
 // fill *d_n + n_add entries (at most n_cap)
 hipError_t launch_fill_dn(uint32_t* p, uint32_t v, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
                           hipStream_t st);
@@ -132,6 +135,14 @@ struct LinkBufs {
     uint32_t* flag_a = nullptr; uint32_t* flag_b = nullptr;  // R + 1
     uint32_t* connectable = nullptr; uint32_t* nonconnectable = nullptr;  // R
     float* pt = nullptr;             // R
+    // each event its own group (per_event != 0): the connectivity stage sees
+    // every event as a run of that event alone would (its own tracklet
+    // offsets, its own last tracklet left out)
+    uint32_t per_event = 0;
+    uint32_t* ev_first = nullptr;  // E: first tracklet of each event (sentinel: none)
+    uint32_t* ev_last = nullptr;   // E: last tracklet of each event
+    uint32_t* ev_has = nullptr;    // E: the event has a connection
+    uint32_t* ev_off = nullptr;    // E * (T + 1): the event's own tracklet offsets
 };
 hipError_t launch_link_prepare(const TripletCols& tc, uint32_t n_hits_cap, uint32_t n_trk_cap,
                                const float* hx, const float* hy, const float* hz, const uint32_t* th1,
@@ -145,8 +156,13 @@ hipError_t launch_link_store(const TripletCols& tc, uint32_t n_trk_cap, uint32_t
                              const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* th1,
                              const uint32_t* th2, const uint32_t* th3, const uint32_t* t_off,
                              const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
-hipError_t launch_link_flags(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
-                             hipStream_t st);
+// each event its own group: first/last tracklet and own tracklet offsets of
+// every event (only when b.per_event)
+hipError_t launch_link_events(const TripletCols& tc, uint32_t n_events_cap, uint32_t n_trk_cap,
+                              const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* th1,
+                              const uint32_t* t_off, const LinkBufs& b, DevCounts* cnt, hipStream_t st);
+hipError_t launch_link_flags(uint32_t n_trk_cap, const uint32_t* hevt, const uint32_t* th1, const LinkBufs& b,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_link_compact(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
                                hipStream_t st);
 hipError_t launch_circle_fit(uint32_t n_trk_cap, const float* hx, const float* hy, const uint32_t* th1,

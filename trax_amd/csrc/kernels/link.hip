@@ -181,20 +181,62 @@ struct LinkRange {
     uint32_t begin, end, ev;
 };
 
-// guards of tripletConnectivityTightCount/Store (.h:109-132)
+// guards of tripletConnectivityTightCount/Store (.h:109-132).  The offsets are
+// the group's E*T + 1 tracklet offsets, or with each event its own group the
+// event's T + 1 offsets as a run of that event alone has them.
 __device__ inline bool link_range(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
                                   const uint32_t* th1, const uint32_t* th2, const uint32_t* t_off,
-                                  uint32_t n_off, uint32_t R, uint32_t t, LinkRange& r) {
-    const uint32_t limit = n_off >= 2 ? t_off[n_off - 2] : 0u;
-    if (t >= limit) return false;
+                                  uint32_t n_off, uint32_t R, uint32_t t, const LinkBufs& b, LinkRange& r) {
     const uint32_t inner = th2[t];
     r.ev = hevt[inner];
-    const uint64_t ib = (uint64_t)r.ev * tc.n_trip + (hlayer[inner] - 1);
+    const uint32_t* off = t_off;
+    uint64_t ib = (uint64_t)r.ev * tc.n_trip + (hlayer[inner] - 1);
+    if (b.per_event) {
+        off = b.ev_off + (uint64_t)r.ev * (tc.n_trip + 1);
+        n_off = tc.n_trip + 1;
+        ib = hlayer[inner] - 1;
+    }
+    const uint32_t limit = n_off >= 2 ? off[n_off - 2] : 0u;
+    if (t >= limit) return false;
     if (ib + 1 >= n_off) return false;
-    r.begin = t_off[ib];
-    r.end = t_off[ib + 1];
+    r.begin = off[ib];
+    r.end = off[ib + 1];
     if (r.begin >= R) return false;
     return r.ev == hevt[th1[r.begin]];
+}
+
+// first and last tracklet of each event (tracklets are ordered by event)
+__global__ void k_event_bounds(const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ th1, LinkBufs b,
+                               DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t < R; t += gstride()) {
+        const uint32_t e = hevt[th1[t]];
+        if (t == 0 || hevt[th1[t - 1]] != e) b.ev_first[e] = t;
+        if (t + 1 == R || hevt[th1[t + 1]] != e) b.ev_last[e] = t;
+    }
+}
+
+// The tracklet offsets a run of event e alone produces (filterOffsetStore and
+// its monotonize, TripletThetaPhiFilter.h:221-280), shifted by the event's
+// first tracklet f: groups before the first non-empty one hold f, the
+// non-empty range holds the group starts (empty groups inside it the
+// previous start), and from one past the last non-empty group on the value
+// is the event's last tracklet l (the run's R - 1).
+__global__ void k_event_offsets(TripletCols tc, const uint32_t* __restrict__ hlayer,
+                                const uint32_t* __restrict__ th1, const uint32_t* __restrict__ t_off, LinkBufs b,
+                                DevCounts* __restrict__ cnt) {
+    const uint32_t T = tc.n_trip, E = cnt->n_events;
+    const uint64_t n = (uint64_t)E * (T + 1);
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        const uint32_t e = (uint32_t)(i / (T + 1)), k = (uint32_t)(i % (T + 1));
+        const uint32_t f = b.ev_first[e], l = b.ev_last[e];
+        if (f == kSentinel) {
+            b.ev_off[i] = kSentinel;
+            continue;
+        }
+        const uint32_t lf = hlayer[th1[f]] - 1, ll = hlayer[th1[l]] - 1;
+        b.ev_off[i] = k < lf ? f : (k <= ll ? t_off[(uint64_t)e * T + k] : l);
+    }
 }
 
 __global__ void k_link_count(TripletCols tc, const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
@@ -206,7 +248,7 @@ __global__ void k_link_count(TripletCols tc, const uint32_t* __restrict__ hlayer
     for (uint32_t t = gtid(); t <= R; t += gstride()) {
         uint32_t n = 0;
         LinkRange r;
-        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, r)) {
+        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
             const uint32_t h = th2[t], h3 = th3[t];
             const float et = b.eta[t];
             for (uint32_t q = b.hit_count[h]; q < b.hit_count[h + 1]; ++q) {
@@ -216,7 +258,10 @@ __global__ void k_link_count(TripletCols tc, const uint32_t* __restrict__ hlayer
                     b.conn[i] = 1u;
                 }
             }
-            if (n) b.conn[t] = 1u;
+            if (n) {
+                b.conn[t] = 1u;
+                if (b.per_event) b.ev_has[r.ev] = 1u;
+            }
         }
         b.counts[t] = n;
     }
@@ -235,7 +280,7 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
         if (so == sn) continue;
         if (sn > n_conn_cap) { over = true; continue; }
         LinkRange r;
-        if (!link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, r)) continue;
+        if (!link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) continue;
         const uint32_t h = th2[t], h3 = th3[t];
         const float et = b.eta[t];
         const uint32_t q0 = b.hit_count[h], q1 = b.hit_count[h + 1];
@@ -261,11 +306,18 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
 }
 
 // predicates over the first R - 1 tracklets (TripletConnectivityTight.cpp:184-256);
-// nothing when no connection exists (the reference returns null lists, .cpp:93-95)
-__global__ void k_link_flags(LinkBufs b, DevCounts* __restrict__ cnt) {
+// nothing when no connection exists (the reference returns null lists, .cpp:93-95).
+// With each event its own group: over every event's tracklets but its last,
+// for the events that have a connection.
+__global__ void k_link_flags(const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ th1, LinkBufs b,
+                             DevCounts* __restrict__ cnt) {
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
     for (uint32_t t = gtid(); t <= R; t += gstride()) {
-        const bool in = K > 0 && t + 1 < R;
+        bool in = K > 0 && t + 1 < R;
+        if (b.per_event && t < R) {
+            const uint32_t e = hevt[th1[t]];
+            in = b.ev_has[e] != 0u && t != b.ev_last[e];
+        }
         const uint32_t c = (t < R) ? b.conn[t] : 0u;
         b.flag_a[t] = in ? (c != 0u) : 0u;
         b.flag_b[t] = in ? (c == 0u) : 0u;
@@ -481,9 +533,25 @@ hipError_t launch_link_store(const TripletCols& tc, uint32_t n_trk_cap, uint32_t
     return hipSuccess;
 }
 
-hipError_t launch_link_flags(uint32_t n_trk_cap, const LinkBufs& b, DevCounts* cnt, const LaunchCfg& lc,
-                             hipStream_t st) {
-    TRAX_LAUNCH(k_link_flags, (uint64_t)n_trk_cap + 1, 256u, b, cnt);
+hipError_t launch_link_events(const TripletCols& tc, uint32_t n_events_cap, uint32_t n_trk_cap,
+                              const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* th1,
+                              const uint32_t* t_off, const LinkBufs& b, DevCounts* cnt, hipStream_t st) {
+    if (!b.per_event) return hipSuccess;
+    LaunchCfg lc;
+    hipError_t e = launch_fill(b.ev_first, kSentinel, n_events_cap, st);
+    if (e != hipSuccess) return e;
+    e = launch_fill(b.ev_last, kSentinel, n_events_cap, st);
+    if (e != hipSuccess) return e;
+    e = launch_fill(b.ev_has, 0u, n_events_cap, st);
+    if (e != hipSuccess) return e;
+    TRAX_LAUNCH(k_event_bounds, n_trk_cap, 256u, hevt, th1, b, cnt);
+    TRAX_LAUNCH(k_event_offsets, (uint64_t)n_events_cap * (tc.n_trip + 1), 256u, tc, hlayer, th1, t_off, b, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_link_flags(uint32_t n_trk_cap, const uint32_t* hevt, const uint32_t* th1, const LinkBufs& b,
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    TRAX_LAUNCH(k_link_flags, (uint64_t)n_trk_cap + 1, 256u, hevt, th1, b, cnt);
     return hipSuccess;
 }
 

@@ -109,6 +109,18 @@ __global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restric
         dst[i] = src[i];
 }
 
+// event column of a batch of whole events: hit i of event e gets e, for the
+// n_events (device) events whose hits start at evt_offset[e] (running sums)
+__global__ void k_event_ids(uint32_t* __restrict__ evt, const uint32_t* __restrict__ evt_offset,
+                            const DevCounts* __restrict__ cnt) {
+    const uint32_t E = cnt->n_events, N = cnt->n_hits;
+    for (uint32_t e = blockIdx.x; e < E; e += gridDim.x) {
+        const uint32_t lo = evt_offset[e];
+        const uint32_t hi = e + 1 < E ? evt_offset[e + 1] : N;
+        for (uint32_t i = lo + threadIdx.x; i < hi && i < N; i += blockDim.x) evt[i] = e;
+    }
+}
+
 __global__ void k_fill_u32_dn(uint32_t* __restrict__ p, uint32_t v, const uint32_t* __restrict__ d_n,
                               uint32_t n_add) {
     const uint32_t n = *d_n + n_add;
@@ -188,6 +200,13 @@ hipError_t launch_copy(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_copy_u32, dim3((uint32_t)blocks), dim3(256), 0, st, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_event_ids(uint32_t* evt, const uint32_t* evt_offset, const DevCounts* cnt, uint32_t n_events_cap,
+                            hipStream_t st) {
+    const uint32_t blocks = n_events_cap < 1024 ? (n_events_cap ? n_events_cap : 1) : 1024;
+    hipLaunchKernelGGL(k_event_ids, dim3(blocks), dim3(256), 0, st, evt, evt_offset, cnt);
     return hipGetLastError();
 }
 

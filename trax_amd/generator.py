@@ -116,3 +116,19 @@ def make_events(wl: Workload, n_events: int = None, seed: int = 2024, first: int
     n_events = wl.extra.get("events", 1) if n_events is None else n_events
     sizes = event_sizes(wl, first + n_events, seed)[first:]
     return [make_event(wl, t, n, seed, first + i) for i, (t, n) in enumerate(sizes)]
+
+
+def _make_chunk(args):
+    wl, sizes, seed, first = args
+    return [make_event(wl, t, n, seed, first + i) for i, (t, n) in enumerate(sizes)]
+
+
+def make_events_parallel(wl: Workload, n_events: int, seed: int = 2024, workers: int = 8, chunk: int = 64):
+    """make_events over a process pool, chunk by chunk (yields lists of events in
+    order).  The events are identical to make_events' (each has its own seed);
+    start it before the process touches the GPU."""
+    from concurrent.futures import ProcessPoolExecutor
+    sizes = event_sizes(wl, n_events, seed)
+    jobs = [(wl, sizes[i:i + chunk], seed, i) for i in range(0, n_events, chunk)]
+    with ProcessPoolExecutor(max_workers=max(1, workers)) as ex:
+        yield from ex.map(_make_chunk, jobs)
