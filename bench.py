@@ -65,7 +65,7 @@ def serving_stream(wl, n_events, seed, workers):
     return cfg, hs
 
 
-def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events):
+def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_hits):
     """Replay the stream with Poisson arrivals at rate_hz through the
     continuous-batching server; latency = arrival to results on the host."""
     import numpy as np
@@ -73,7 +73,8 @@ def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events):
     rng = np.random.default_rng(seed)
     arrival = np.cumsum(rng.exponential(1.0 / rate_hz, hs.n_events))
     arrival -= arrival[0]
-    sv = Server(dev, cfg, bounds, n_slots=n_slots, max_batch_events=max_batch_events, device_budget=64 << 30)
+    sv = Server(dev, cfg, bounds, n_slots=n_slots, max_batch_events=max_batch_events, max_batch_hits=slot_hits,
+                device_budget=64 << 30)
     caps = sv.caps()
     _, st = sv.replay(hs, arrival)
     sv.close()
@@ -82,7 +83,8 @@ def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events):
             "events_per_s": round(st["events_per_s"], 1), "tracklets_per_s": round(st["tracklets_per_s"], 1),
             "batches": st["batches"], "mean_batch_events": round(st["mean_batch_events"], 2),
             "max_batch_events": st["max_batch_events"], "resubmits": st["resubmits"], "failed": st["failed"],
-            "slots": n_slots, "slot_hit_capacity": caps["max_hits"], "device_bytes": st["device_bytes"]}
+            "slots": n_slots, "slot_hit_capacity": caps["max_hits"], "device_bytes": st["device_bytes"],
+            "mean_ms": {k: round(st[k + "_ms"], 3) for k in ("queue", "pipeline", "copy", "split")}}
 
 
 def cpu_baseline(wl, events, cfg_cls, hitset_cls, budget_s=20.0):
@@ -118,6 +120,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="Poisson arrival rate of the stream (events/s)")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-max-batch", type=int, default=256)
+    ap.add_argument("--serve-slot-hits", type=int, default=2_000_000,
+                    help="hit capacity of each serving slot (its output capacities scale with it)")
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="processes generating the serving stream")
     args = ap.parse_args()
@@ -181,7 +185,7 @@ def main():
         b.close()  # the server holds its own workspaces
         print("serving stream", file=sys.stderr, flush=True)
         serving = serve(dev, stream[0], stream[1], bounds, args.serve_rate, SEED + 2, args.serve_slots,
-                        args.serve_max_batch)
+                        args.serve_max_batch, args.serve_slot_hits)
         b = Batch(dev, cfg, caps)
         b.submit(hs)
         b.wait()

@@ -228,6 +228,9 @@ typedef struct {
     uint64_t tracklets, track_entries, device_bytes;
     double wall_s, events_per_s, tracklets_per_s, mean_batch_events;
     double p50_ms, p99_ms, max_ms; /* latency: arrival to results on the host */
+    /* means over batches: arrival of the first event to submission, submission
+     * to the end of the pipeline, the result copy, the host-side split */
+    double queue_ms, pipeline_ms, copy_ms, split_ms;
 } trax_serve_stats;
 int trax_server_create(trax_ctx* ctx, const trax_config* cfg, const trax_serve_opts* opts, trax_server** out);
 void trax_server_destroy(trax_server* s);

@@ -56,7 +56,8 @@ class ServeStats(C.Structure):
     _fields_ = [(n, U32) for n in ("events", "failed", "batches", "resubmits", "max_batch_events", "slot_hits")] + \
                [(n, C.c_uint64) for n in ("tracklets", "track_entries", "device_bytes")] + \
                [(n, C.c_double) for n in ("wall_s", "events_per_s", "tracklets_per_s", "mean_batch_events",
-                                          "p50_ms", "p99_ms", "max_ms")]
+                                          "p50_ms", "p99_ms", "max_ms", "queue_ms", "pipeline_ms", "copy_ms",
+                                          "split_ms")]
 
 
 class DeviceInfo(C.Structure):

@@ -504,15 +504,16 @@ hipError_t launch_link_prepare(const TripletCols& tc, uint32_t n_hits_cap, uint3
                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
     (void)tc;
     const uint32_t blk = lc.link_block ? lc.link_block : 256;
-    hipError_t e = launch_fill(b.hit_count, 0u, (uint64_t)n_hits_cap + 1, st);
+    // fills cover the sizes on the device, not the capacities
+    hipError_t e = launch_fill_dn(b.hit_count, 0u, &cnt->n_hits, 1u, (uint64_t)n_hits_cap + 1, st);
     if (e != hipSuccess) return e;
-    e = launch_fill(b.hit_cursor, 0u, n_hits_cap, st);
+    e = launch_fill_dn(b.hit_cursor, 0u, &cnt->n_hits, 0u, n_hits_cap, st);
     if (e != hipSuccess) return e;
     TRAX_LAUNCH(k_eta, n_trk_cap, blk, hx, hy, hz, th1, th3, b.eta, b.hit_count, cnt);
     e = launch_scan(b.hit_count, &cnt->n_hits, 1u, (uint64_t)n_hits_cap + 1, nullptr, ss, st);
     if (e != hipSuccess) return e;
     TRAX_LAUNCH(k_csr_fill, n_trk_cap, blk, th1, b.hit_count, b.hit_cursor, b.csr, cnt);
-    return launch_fill(b.conn, 0u, (uint64_t)n_trk_cap + 1, st);
+    return launch_fill_dn(b.conn, 0u, &cnt->n_tracklets, 1u, (uint64_t)n_trk_cap + 1, st);
 }
 
 hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
@@ -564,7 +565,7 @@ hipError_t launch_link_compact(uint32_t n_trk_cap, const LinkBufs& b, DevCounts*
 hipError_t launch_circle_fit(uint32_t n_trk_cap, const float* hx, const float* hy, const uint32_t* th1,
                              const uint32_t* th2, const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
                              const LaunchCfg& lc, hipStream_t st) {
-    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(b.pt), 0u, n_trk_cap, st);
+    hipError_t e = launch_fill_dn(reinterpret_cast<uint32_t*>(b.pt), 0u, &cnt->n_tracklets, 0u, n_trk_cap, st);
     if (e != hipSuccess) return e;
     TRAX_LAUNCH(k_circle, n_trk_cap, 256u, hx, hy, th1, th2, th3, b, cnt);
     return hipSuccess;
@@ -586,9 +587,9 @@ hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, cons
 hipError_t launch_ca_basis_count(uint32_t n_trk_cap, uint32_t n_conn_cap, const uint32_t* basis,
                                  const uint32_t* foll, const CABufs& b, DevCounts* cnt, const LaunchCfg& lc,
                                  hipStream_t st) {
-    hipError_t e = launch_fill(b.fbc, 0u, (uint64_t)n_trk_cap + 1, st);
+    hipError_t e = launch_fill_dn(b.fbc, 0u, &cnt->n_tracklets, 1u, (uint64_t)n_trk_cap + 1, st);
     if (e != hipSuccess) return e;
-    e = launch_fill(b.cursor, 0u, n_trk_cap, st);
+    e = launch_fill_dn(b.cursor, 0u, &cnt->n_tracklets, 0u, n_trk_cap, st);
     if (e != hipSuccess) return e;
     TRAX_LAUNCH(k_ca_basis_count, n_conn_cap, 256u, basis, foll, b, cnt);
     return hipSuccess;
@@ -598,9 +599,9 @@ hipError_t launch_ca_basis_store(uint32_t n_trk_cap, uint32_t n_conn_cap, uint32
                                  const uint32_t* basis, const uint32_t* foll, const float* pt, const CABufs& b,
                                  DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     TRAX_LAUNCH(k_ca_basis_store, n_conn_cap, 256u, basis, foll, pt, b, n_basis_cap, cnt);
-    hipError_t e = launch_fill(b.best, kSentinel, n_trk_cap, st);
+    hipError_t e = launch_fill_dn(b.best, kSentinel, &cnt->n_tracklets, 0u, n_trk_cap, st);
     if (e != hipSuccess) return e;
-    e = launch_fill(b.isbest, 0u, n_trk_cap, st);
+    e = launch_fill_dn(b.isbest, 0u, &cnt->n_tracklets, 0u, n_trk_cap, st);
     if (e != hipSuccess) return e;
     TRAX_LAUNCH(k_ca_best, n_trk_cap, 256u, b, cnt);
     TRAX_LAUNCH(k_ca_handler, (uint64_t)n_trk_cap + 1, 256u, b, cnt);

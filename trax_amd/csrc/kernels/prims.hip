@@ -30,6 +30,10 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_excl(uint32_t* __restrict__
     const uint32_t tid = threadIdx.x;
     uint32_t n = (d_n ? *d_n : 0u) + n_add;
     if (n > n_cap) n = n_cap;  // a size past its buffer is reported by the stage kernels
+    // persistent: tiles are taken in increasing order from the counter, so a
+    // tile's look-back only waits on tiles held by running blocks
+    while (true) {
+    __syncthreads();  // s_items and s_prefix of the previous tile are consumed
     if (tid == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -95,6 +99,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_excl(uint32_t* __restrict__
         if (i < n) data[i] = s_items[pad_idx(li)];
     }
     if (d_total && tid == 0 && base + kScanTile >= n) *d_total = s_prefix + total;
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
@@ -122,9 +127,10 @@ __global__ void k_event_ids(uint32_t* __restrict__ evt, const uint32_t* __restri
 }
 
 __global__ void k_fill_u32_dn(uint32_t* __restrict__ p, uint32_t v, const uint32_t* __restrict__ d_n,
-                              uint32_t n_add) {
-    const uint32_t n = *d_n + n_add;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+                              uint32_t n_add, uint64_t n_cap) {
+    uint64_t n = (uint64_t)*d_n + n_add;
+    if (n > n_cap) n = n_cap;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
 
@@ -215,7 +221,7 @@ hipError_t launch_fill_dn(uint32_t* p, uint32_t v, const uint32_t* d_n, uint32_t
     uint64_t blocks = (n_cap + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_fill_u32_dn, dim3((uint32_t)blocks), dim3(256), 0, st, p, v, d_n, n_add);
+    hipLaunchKernelGGL(k_fill_u32_dn, dim3((uint32_t)blocks), dim3(256), 0, st, p, v, d_n, n_add, n_cap);
     return hipGetLastError();
 }
 
@@ -227,7 +233,9 @@ hipError_t launch_scan(uint32_t* data, const uint32_t* d_n, uint32_t n_add, uint
     // memset node, so that the sequence can be captured into a graph
     hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(s.status), 0u, 2ull * tiles + 2, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_excl, dim3(tiles), dim3(kScanBlock), 0, st, data, d_n, n_add, d_total,
+    LaunchCfg lc;
+    const uint32_t blocks = std::min<uint32_t>(tiles, stride_blocks((uint64_t)tiles * kScanBlock, kScanBlock, lc));
+    hipLaunchKernelGGL(k_scan_excl, dim3(blocks), dim3(kScanBlock), 0, st, data, d_n, n_add, d_total,
                        s.status, reinterpret_cast<uint32_t*>(s.status + tiles),
                        (uint32_t)(n_cap < 0xffffffffull ? n_cap : 0xffffffffull));
     return hipGetLastError();

@@ -43,6 +43,7 @@ struct Slot {
     std::vector<uint32_t> events;    // stream indices of the batch's events
     std::vector<uint32_t> hit_base;  // batch-relative first hit of each event, plus the total
     uint32_t R = 0, TE = 0, K = 0;
+    double t_submit = 0, t_pipe = 0;  // seconds from the replay's start
     uint32_t *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *handler = nullptr, *tracks = nullptr;  // pinned
 };
 
@@ -76,6 +77,7 @@ struct trax_server {
     // counters
     uint32_t batches = 0, resubmits = 0, max_batch = 0;
     uint64_t batch_event_sum = 0;
+    double sum_queue = 0, sum_pipe = 0, sum_copy = 0, sum_split = 0;
 
     ~trax_server() { release(); }
 
@@ -122,8 +124,10 @@ struct trax_server {
     }
 
     // stage the events into the slot's workspace and enqueue the pipeline
-    void submit(Slot& s, std::vector<uint32_t> evs) {
+    void submit(Slot& s, std::vector<uint32_t> evs, double now) {
         trax_batch* b = s.b;
+        s.t_submit = now;
+        sum_queue += now - arrival[evs[0]];
         const uint32_t E = (uint32_t)evs.size(), Ecap = b->E;
         s.events = std::move(evs);
         s.hit_base.assign(E + 1, 0);
@@ -357,6 +361,7 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
         sv->out.assign(sv->opts.keep_results ? n : 0, EventOut{});
         sv->batches = sv->resubmits = sv->max_batch = 0;
         sv->batch_event_sum = 0;
+        sv->sum_queue = sv->sum_pipe = sv->sum_copy = sv->sum_split = 0;
 
         std::deque<uint32_t> waiting;
         uint32_t next = 0, finished = 0, limit = sv->opts.max_batch_events;
@@ -369,6 +374,8 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
             for (Slot& s : sv->slots) {
                 if (s.state == 1 && hipEventQuery(s.ev_pipe) == hipSuccess) {
                     progressed = true;
+                    s.t_pipe = now_s();
+                    sv->sum_pipe += s.t_pipe - s.t_submit;
                     if (!sv->collect(s)) {  // overflow: re-queue in order, smaller batches
                         ++sv->resubmits;
                         if (s.events.size() == 1) {
@@ -385,7 +392,10 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
                 } else if (s.state == 2 && hipEventQuery(s.ev_dl) == hipSuccess) {
                     progressed = true;
                     finished += (uint32_t)s.events.size();
-                    sv->complete(s, now_s());
+                    const double t_dl = now_s();
+                    sv->sum_copy += t_dl - s.t_pipe;
+                    sv->complete(s, t_dl);
+                    sv->sum_split += now_s() - t_dl;
                     limit = std::min(sv->opts.max_batch_events, limit * 2);
                 }
             }
@@ -401,7 +411,7 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
                     evs.push_back(e);
                     waiting.pop_front();
                 }
-                sv->submit(s, std::move(evs));
+                sv->submit(s, std::move(evs), now_s());
                 progressed = true;
             }
             if (!progressed) {
@@ -438,6 +448,11 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
             stats->wall_s = wall;
             stats->events_per_s = wall > 0 ? n / wall : 0.0;
             stats->tracklets_per_s = wall > 0 ? trk / wall : 0.0;
+            const double nb = sv->batches ? (double)sv->batches : 1.0;
+            stats->queue_ms = sv->sum_queue / nb * 1e3;
+            stats->pipeline_ms = sv->sum_pipe / nb * 1e3;
+            stats->copy_ms = sv->sum_copy / nb * 1e3;
+            stats->split_ms = sv->sum_split / nb * 1e3;
             if (!lat.empty()) {
                 std::sort(lat.begin(), lat.end());
                 auto q = [&](double p) {  // nearest-rank percentile
