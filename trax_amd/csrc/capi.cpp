@@ -473,7 +473,7 @@ void trax_batch::enqueue_stage(int s) {
             ok(launch_scan(seg_base, &cnt->n_segments, 1u, nseg + 1ull, &cnt->n_pair_slots, scan, st), "scan");
             break;
         case 3:  // pair_count
-            ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, hout.phi, seg_base, slot_counts,
+            ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, hout.zphi, seg_base, slot_counts,
                                  cnt, lc, st),
                "pair count");
             break;
@@ -481,13 +481,13 @@ void trax_batch::enqueue_stage(int s) {
             ok(launch_scan(slot_counts, &cnt->n_pair_slots, 1u, slot_cap + 1ull, &cnt->n_pairs, scan, st), "scan");
             break;
         case 5:  // pair_store
-            ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, hout.phi, seg_base,
+            ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, hout.zphi, seg_base,
                                  slot_counts, pairs, pair_off, cnt, lc, st),
                "pair store");
             break;
         case 6:  // predict: count, scan and store in one pass
             ok(launch_fill(cand_off, 0u, nseg + 1ull, st), "fill");
-            ok(launch_predict_fused(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.phi, hout.layer, hout.evt, pairs,
+            ok(launch_predict_fused(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.zphi, hout.layer, hout.evt, pairs,
                                     cands, cand_off, cnt, scan, lc, st),
                "predict");
             break;
@@ -585,7 +585,7 @@ void trax_batch::layout(Arena& a) {
         };
         cols(B.hin);
         cols(B.hout);
-        B.hout.phi = a.take<float>(N);
+        B.hout.zphi = a.take<float2>(N);
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
         B.seg_base = a.take<uint32_t>(E * T + 1);

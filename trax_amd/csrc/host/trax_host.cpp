@@ -182,7 +182,7 @@ GridGeom GridConfig::geom() const {
 ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const GeometrySupplement& g,
                            uint nThreads, float dEtaCut) {
     const uint T = t.size(), G = g.size();
-    const size_t words = 10 * (size_t)T + 2 * (size_t)G;
+    const size_t words = 10 * (size_t)T + 4 * (size_t)G;
     std::vector<uint32_t> host(words);
     auto put_u = [&](size_t at, const std::vector<uint>& v) { std::memcpy(&host[at], v.data(), v.size() * 4); };
     auto put_f = [&](size_t at, const std::vector<float>& v) { std::memcpy(&host[at], v.data(), v.size() * 4); };
@@ -190,6 +190,13 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
     put_f(3 * T, t.dThetaCut); put_f(4 * T, t.sigmaZ); put_f(5 * T, t.dPhiCut); put_f(6 * T, t.sigmaPhi);
     put_f(7 * T, t.tipCut); put_f(8 * T, t.z0); put_u(9 * T, t.pairSpreadPhi);
     put_f(10 * T, g.minRadius); put_f(10 * T + G, g.maxRadius);
+    const float mrc = t.minRadiusCurvature();
+    std::vector<float> acos_min(G), acos_max(G);
+    for (uint l = 0; l < G; ++l) {
+        acos_min[l] = tx_acosf(g.minRadius[l] / (2 * mrc));
+        acos_max[l] = tx_acosf(g.maxRadius[l] / (2 * mrc));
+    }
+    put_f(10 * T + 2 * G, acos_min); put_f(10 * T + 3 * G, acos_max);
     blob_.reset(new DeviceVector<uint8_t>(ctx, words * 4 + 4));
     hip_check(hipMemcpyAsync(blob_->get_mem(), host.data(), words * 4, hipMemcpyHostToDevice, ctx.stream()), "h2d cfg");
     ctx.finish_default_queue();
@@ -199,6 +206,7 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
     cols_.dtheta = f + 3 * T; cols_.sigma_z = f + 4 * T; cols_.dphi = f + 5 * T; cols_.sigma_phi = f + 6 * T;
     cols_.tip = f + 7 * T; cols_.z0 = f + 8 * T; cols_.pair_spread_phi = d + 9 * T;
     cols_.min_r = f + 10 * T; cols_.max_r = f + 10 * T + G;
+    cols_.acos_min_r = f + 10 * T + 2 * G; cols_.acos_max_r = f + 10 * T + 3 * G;
     cols_.n_trip = T;
     cols_.n_geom = G;
     cols_.min_radius_curvature = t.minRadiusCurvature();
@@ -206,7 +214,9 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
     cols_.d_eta_cut = dEtaCut;
 }
 
-HitMirror::HitMirror(Context& ctx, uint n) : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 8ull * (n ? n : 1))) {}
+// ten words per hit: seven columns, one spare word and the (z, phi) pair at
+// an 8-byte aligned offset
+HitMirror::HitMirror(Context& ctx, uint n) : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 10ull * (n ? n : 1))) {}
 
 HitColumns HitMirror::cols() const {
     HitColumns c;
@@ -214,7 +224,7 @@ HitColumns HitMirror::cols() const {
     const size_t n = n_ ? n_ : 1;
     c.x = reinterpret_cast<float*>(b); c.y = reinterpret_cast<float*>(b + n); c.z = reinterpret_cast<float*>(b + 2 * n);
     c.layer = b + 3 * n; c.det = b + 4 * n; c.id = b + 5 * n; c.evt = b + 6 * n;
-    c.phi = reinterpret_cast<float*>(b + 7 * n);
+    c.zphi = reinterpret_cast<float2*>(b + 8 * n);
     return c;
 }
 
@@ -311,13 +321,13 @@ Pairing* PairGeneratorBeamspot::run(const HitMirror& hits, const ConfigMirror& c
     const uint nslot = ctx.download_count(&cnt->n_pair_slots);
     UVec slot_counts(ctx, nslot + 1ull);
     const HitColumns hc = hits.cols();
-    hip_check(launch_pair_count(g, tc, nseg, nslot, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi, seg_base.get_mem(),
+    hip_check(launch_pair_count(g, tc, nseg, nslot, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.zphi, seg_base.get_mem(),
                                 slot_counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "pair count");
     PrefixSum(ctx).run(slot_counts.get_mem(), nslot + 1, nThreads, &cnt->n_pairs);
     const uint nPairs = ctx.download_count(&cnt->n_pairs);  // PairGeneratorBeamspot.cpp:124
     Pairing* p = new Pairing(ctx, nPairs, E, T);
-    hip_check(launch_pair_store(g, tc, nseg, nslot, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi,
+    hip_check(launch_pair_store(g, tc, nseg, nslot, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.zphi,
                                 seg_base.get_mem(), slot_counts.get_mem(), p->pairing.get_mem(),
                                 p->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "pair store");
@@ -336,14 +346,14 @@ Pairing* TripletThetaPhiPredictor::run(const HitMirror& hits, const DetectorGeom
     ctx.set_count(&cnt->n_pairs, pairs.n);
     UVec counts(ctx, nPairs + 1ull);
     const HitColumns hc = hits.cols();
-    hip_check(launch_predict_count(g, tc, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi, hc.layer, hc.evt,
+    hip_check(launch_predict_count(g, tc, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.zphi, hc.layer, hc.evt,
                                    pairs.pairing.get_mem(), counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "predict count");
     PrefixSum(ctx).run(counts.get_mem(), nPairs + 1, (uint)nThreads, &cnt->n_cands);
     const uint nCand = ctx.download_count(&cnt->n_cands);  // TripletThetaPhiPredictor.cpp:68
     Pairing* c = new Pairing(ctx, nCand, E, T);
     hip_check(launch_fill(c->pairingOffsets.get_mem(), 0u, c->pairingOffsets.get_count(), ctx.stream()), "fill");
-    hip_check(launch_predict_store(g, tc, nPairs, nCand, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.phi, hc.layer,
+    hip_check(launch_predict_store(g, tc, nPairs, nCand, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.zphi, hc.layer,
                                    hc.evt, pairs.pairing.get_mem(), counts.get_mem(), c->pairing.get_mem(),
                                    c->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "predict store");

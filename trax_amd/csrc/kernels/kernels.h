@@ -21,10 +21,11 @@ struct ScanScratch {
 struct HitColumns {
     float* x = nullptr; float* y = nullptr; float* z = nullptr;
     uint32_t* layer = nullptr; uint32_t* det = nullptr; uint32_t* id = nullptr; uint32_t* evt = nullptr;
-    // device-only column written by the grid scatter: phi = atan2(y, x) of
-    // every reordered hit, the value every window test of the reference
-    // recomputes (bit-identical: the same built-in on the same inputs)
-    float* phi = nullptr;
+    // device-only column written by the grid scatter: (z, phi) of every
+    // reordered hit with phi = atan2(y, x), the value every window test of
+    // the reference recomputes (bit-identical: the same built-in on the same
+    // inputs); one 8-byte load per visited hit in the window walks
+    float2* zphi = nullptr;
 };
 
 // per-kernel block sizes (0 = default); chosen by the autotuner
@@ -45,6 +46,10 @@ struct TripletCols {
     const float* dtheta; const float* sigma_z; const float* dphi; const float* sigma_phi;
     const float* tip; const float* z0; const uint32_t* pair_spread_phi;
     const float* min_r; const float* max_r;  // index layer - 1
+    // acos(radius / (2 minRadiusCurvature)) of every layer's min and max
+    // radius: the per-layer terms of the pair and predictor phi windows,
+    // computed once on the host with the same built-in
+    const float* acos_min_r; const float* acos_max_r;
     uint32_t n_trip;
     uint32_t n_geom;
     float min_radius_curvature;
@@ -84,23 +89,23 @@ hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n
                             const uint32_t* grid, uint32_t* seg_base, DevCounts* cnt, hipStream_t st);
 hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
-                             const float* hz, const float* hphi, const uint32_t* seg_base, uint32_t* slot_counts,
+                             const float* hz, const float2* hzp, const uint32_t* seg_base, uint32_t* slot_counts,
                              DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
-                             const float* hx, const float* hy, const float* hz, const float* hphi,
+                             const float* hx, const float* hy, const float* hz, const float2* hzp,
                              const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
                              uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 
 // TripletThetaPhiPredictor: count -> scan(counts) -> store (+ offsets, forward fill)
 hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc,
                                 hipStream_t st);
 hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 uint32_t n_cand_cap, const uint32_t* grid, const float* hx, const float* hy,
-                                const float* hz, const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float* hz, const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, const uint32_t* counts, uint32_t* cands,
                                 uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 // count + scan + store in one pass (the offsets array must be zeroed before);
@@ -108,7 +113,7 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
 uint32_t predict_fused_tiles(uint64_t n_pair_cap);
 hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
                                 const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
 

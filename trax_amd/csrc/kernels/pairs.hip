@@ -61,8 +61,9 @@ __device__ inline Window pair_window(const GridGeom& g, const TripletCols& tc, u
     w.zLo = (uint32_t)(zl > 0 ? zl : 0);
     const uint32_t zh = tx_f2u_sat(floorf((zHigh - g.min_z) / g.sector_z)) + 1u;
     w.zHi = zh < g.n_z ? zh : g.n_z;
-    tmp = fabsf(tx_acosf(r / (2 * mrc)) - tx_acosf(maxR1 / (2 * mrc)));
-    float dPhi = fabsf(tx_acosf(r / (2 * mrc)) - tx_acosf(minR1 / (2 * mrc)));
+    const float aR = tx_acosf(r / (2 * mrc));
+    tmp = fabsf(aR - tc.acos_max_r[lay1]);       // acos(maxR1 / (2 mrc))
+    float dPhi = fabsf(aR - tc.acos_min_r[lay1]);  // acos(minR1 / (2 mrc))
     dPhi = (float)(tmp < dPhi) * dPhi + (float)(tmp > dPhi) * tmp;
     tmp = tx_atanf(tip * (r - minR1)) / (r * minR1);
     float phiHigh = tx_atanf(tip * (r - maxR1)) / (r * maxR1);
@@ -84,8 +85,9 @@ __device__ inline Window pair_window(const GridGeom& g, const TripletCols& tc, u
     return w;
 }
 
-__device__ inline bool in_window(const Window& w, float hz, float actPhi) {
+__device__ inline bool in_window(const Window& w, float2 zphi) {
     const float PI = TX_PI_F;
+    const float hz = zphi.x, actPhi = zphi.y;
     bool valid = w.zLow <= hz && hz <= w.zHigh;
     return valid && ((!w.wrap && (w.phiLow <= actPhi && actPhi <= w.phiHigh)) ||
                      (w.wrap && ((w.phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= w.phiHigh))));
@@ -126,7 +128,7 @@ __device__ inline SlotInfo slot_info(const TripletCols& tc, const uint32_t* seg_
 
 __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                              const float* __restrict__ hx, const float* __restrict__ hy,
-                             const float* __restrict__ hz, const float* __restrict__ hphi,
+                             const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, uint32_t* __restrict__ slot_counts,
                              DevCounts* __restrict__ cnt) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
@@ -143,16 +145,22 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
         if (il < end - layer2Offset) {
             const uint32_t i = layer2Offset + il;
             const uint32_t* lg1 = grid + ((size_t)si.e * g.n_layers + lay1) * g.cells;
-            const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i], hphi[i]);
+            const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i], hzp[i].y);
             if (w.wrap && w.pHi > w.pLo) overlap++;
             for (uint32_t zs = w.zLo; zs < w.zHi; ++zs) {  // (.h:172-230)
                 const uint32_t zStart = lg1[zs * np1];
                 const uint32_t zEnd = lg1[(zs + 1) * np1 - 1];
                 const uint32_t zLen = zEnd - zStart;
                 const uint32_t end2 = (uint32_t)w.wrap * zEnd + lg1[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
-                for (uint32_t j = lg1[zs * np1 + w.pLo]; j < end2; ++j) {
-                    const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
-                    nFound += in_window(w, hz[index], hphi[index]);
+                for (uint32_t j = lg1[zs * np1 + w.pLo]; j < end2; j += 4) {  // four loads in flight
+                    float2 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t jj = j + u;
+                        q[u] = jj < end2 ? hzp[jj - (uint32_t)(jj >= zEnd) * zLen] : make_float2(0.0f, 0.0f);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) nFound += (j + u < end2) && in_window(w, q[u]);
                 }
             }
         }
@@ -163,7 +171,7 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
 
 __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                              const float* __restrict__ hx, const float* __restrict__ hy,
-                             const float* __restrict__ hz, const float* __restrict__ hphi,
+                             const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
                              uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
@@ -180,7 +188,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
         const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
         const uint32_t i = g2[0] + si.k * tc.n_threads + si.t;
         const uint32_t* lg1 = grid + ((size_t)si.e * g.n_layers + lay1) * g.cells;
-        const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i], hphi[i]);
+        const Window w = pair_window(g, tc, si.p, lay1, hx[i], hy[i], hz[i], hzp[i].y);
         // every z sector visits the index intervals [a1, b1) (j below the sector
         // end) and [a2, b2) (j past the end, wrapped back); the bitmap read of
         // the reference emits their union in ascending order
@@ -201,7 +209,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
             const uint32_t hi[2] = {b1, b2};
             for (int r = 0; r < 2; ++r)
                 for (uint32_t index = lo[r]; index < hi[r] && pos < next; ++index) {
-                    if (in_window(w, hz[index], hphi[index])) {
+                    if (in_window(w, hzp[index])) {
                         pairs[2 * (size_t)pos] = index;
                         pairs[2 * (size_t)pos + 1] = i;
                         pos++;
@@ -234,7 +242,7 @@ struct PWin {
 };
 
 __device__ inline bool pred_window(const GridGeom& g, const TripletCols& tc, const uint32_t* grid,
-                                   const float* hx, const float* hy, const float* hz, const float* hphi,
+                                   const float* hx, const float* hy, const float* hz, const float2* hzp,
                                    const uint32_t* hlayer, const uint32_t* hevt, uint32_t n_events,
                                    uint32_t firstHit, uint32_t secondHit, bool store_variant, PWin& pw) {
     const uint32_t ev = hevt[firstHit];
@@ -273,15 +281,16 @@ __device__ inline bool pred_window(const GridGeom& g, const TripletCols& tc, con
     w.zLo = (uint32_t)(zl > 0 ? zl : 0);
     const uint32_t zh = tx_f2u_sat(floorf((zHigh - g.min_z) / g.sector_z)) + 1u;
     w.zHi = zh < g.n_z ? zh : g.n_z;
-    const float phi = hphi[secondHit];
-    tmp = hphi[firstHit];
+    const float phi = hzp[secondHit].y;
+    tmp = hzp[firstHit].y;
     const float PI = TX_PI_F;
     float dPhi = phi - tmp;
     dPhi += (dPhi > PI) ? -2 * PI : (dPhi < -PI) ? 2 * PI : 0;
     dPhi = fabsf(dPhi);
     const float dHits = sqrtf((p2x - p1x) * (p2x - p1x) + (p2y - p1y) * (p2y - p1y));
-    tmp = fabsf(tx_acosf(dHits / (2 * mrc)) - tx_acosf(maxR / (2 * mrc)));
-    float phiLow = fabsf(tx_acosf(dHits / (2 * mrc)) - tx_acosf(minR / (2 * mrc)));
+    const float aD = tx_acosf(dHits / (2 * mrc));
+    tmp = fabsf(aD - tc.acos_max_r[layer]);         // acos(maxR / (2 mrc))
+    float phiLow = fabsf(aD - tc.acos_min_r[layer]);  // acos(minR / (2 mrc))
     const float m2 = (tmp < phiLow) ? phiLow : tmp;  // OpenCL max(x, y): y if x < y, else x
     dPhi = (dPhi < m2) ? m2 : dPhi;
     float phiHigh = phi + dPhi + sigmaPhi;
@@ -320,7 +329,7 @@ __device__ inline bool same_cells(const Window& a, const Window& b) {
 
 __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                                 const float* __restrict__ hx, const float* __restrict__ hy,
-                                const float* __restrict__ hz, const float* __restrict__ hphi,
+                                const float* __restrict__ hz, const float2* __restrict__ hzp,
                                 const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
                                 const uint32_t* __restrict__ pairs, uint32_t* __restrict__ counts,
                                 DevCounts* __restrict__ cnt) {
@@ -332,9 +341,9 @@ __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __re
             const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
             PWin pw;
             if (h1 != kSentinel && h2 != kSentinel &&
-                pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, false, pw)) {
+                pred_window(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, false, pw)) {
                 pred_walk(pw.w, pw.lg3, np1, [&](uint32_t index) {
-                    nFound += in_window(pw.w, hz[index], hphi[index]);
+                    nFound += in_window(pw.w, hzp[index]);
                     return true;
                 });
             }
@@ -360,7 +369,7 @@ __device__ inline void pred_group_end(const TripletCols& tc, const uint32_t* hla
 
 __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                                 const float* __restrict__ hx, const float* __restrict__ hy,
-                                const float* __restrict__ hz, const float* __restrict__ hphi,
+                                const float* __restrict__ hz, const float2* __restrict__ hzp,
                                 const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
                                 const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ counts,
                                 uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets,
@@ -376,11 +385,11 @@ __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __re
         const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
         PWin pw;
         const bool ok = h1 != kSentinel && h2 != kSentinel &&
-                        pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, true, pw);
+                        pred_window(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, true, pw);
         if (ok && pos < next) {
             bool more = false;
             pred_walk(pw.w, pw.lg3, np1, [&](uint32_t index) {
-                const bool v = in_window(pw.w, hz[index], hphi[index]);
+                const bool v = in_window(pw.w, hzp[index]);
                 if (!(pos < next)) {
                     more = v;
                     return !v;
@@ -406,25 +415,152 @@ __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __re
     if (over) atomicOr(&cnt->err, kErrCapacity);
 }
 
+// Both predictor windows of one pair (.h:77-188 count, .h:261-355 store).
+// They differ only in rOrigin (.h:130-131 vs .h:305-306), hence only in the z
+// window; the phi window and every term without rOrigin are computed once
+// (same operations in the same order, so the same bits as two separate
+// evaluations).
+struct PredWin2 {
+    const uint32_t* lg3;
+    float zLowA, zHighA, zLowB, zHighB, phiLow, phiHigh;
+    uint32_t zLoA, zHiA, zLoB, zHiB, pLo, pHi;
+    uint32_t ev, lt;
+    bool wrap;
+};
+
+__device__ inline void pred_z(const GridGeom& g, float p1z, float p2z, float s2, float sMax, float sMin,
+                              float sigmaZ, float rOrigin, float& zLow, float& zHigh, uint32_t& zLo, uint32_t& zHi) {
+    const float cotTheta = (p2z - p1z) / (s2 - rOrigin);
+    const float tmp = p1z + (sMax - rOrigin) * cotTheta;
+    float zl = p1z + (sMin - rOrigin) * cotTheta;
+    float zh = (float)(tmp > zl) * tmp + (float)(tmp < zl) * zl;
+    zl = (float)(tmp > zl) * zl + (float)(tmp < zl) * tmp;
+    zl -= sigmaZ;
+    zh += sigmaZ;
+    const int zlo = tx_f2i_sat(floorf((zl - g.min_z) / g.sector_z));
+    zLo = (uint32_t)(zlo > 0 ? zlo : 0);
+    const uint32_t zhi = tx_f2u_sat(floorf((zh - g.min_z) / g.sector_z)) + 1u;
+    zHi = zhi < g.n_z ? zhi : g.n_z;
+    zLow = zl;
+    zHigh = zh;
+}
+
+__device__ inline bool pred_windows(const GridGeom& g, const TripletCols& tc, const uint32_t* grid,
+                                    const float* hx, const float* hy, const float* hz, const float2* hzp,
+                                    const uint32_t* hlayer, const uint32_t* hevt, uint32_t n_events,
+                                    uint32_t firstHit, uint32_t secondHit, PredWin2& w) {
+    const uint32_t ev = hevt[firstHit];
+    const uint32_t lt = hlayer[firstHit] - 1;  // (.h:88): the triplet index is the inner layer - 1
+    if (ev >= n_events || lt >= tc.n_trip) return false;
+    const uint32_t layer = tc.l3[lt] - 1;
+    if (layer >= g.n_layers || layer >= tc.n_geom) return false;
+    w.ev = ev;
+    w.lt = lt;
+    w.lg3 = grid + ((size_t)ev * g.n_layers + layer) * g.cells;
+    const float sigmaZ = tc.sigma_z[lt], sigmaPhi = tc.sigma_phi[lt];
+    const float minR = tc.min_r[layer], maxR = tc.max_r[layer];
+    const float mrc = tc.min_radius_curvature;
+    const float p1x = hx[firstHit], p1y = hy[firstHit], p1z = hz[firstHit];
+    const float p2x = hx[secondHit], p2y = hy[secondHit], p2z = hz[secondHit];
+    float tip2 = p1y * (p2x - p1x) - p1x * (p2y - p1y);
+    tip2 *= tip2;
+    tip2 /= ((p2x - p1x) * (p2x - p1x) + (p2y - p1y) * (p2y - p1y));
+    const float s2 = sqrtf(p2x * p2x + p2y * p2y - tip2);
+    const float sMax = sqrtf(maxR * maxR - tip2), sMin = sqrtf(minR * minR - tip2);
+    const float r1sq = p1x * p1x + p1y * p1y;
+    const float rOriginA = sqrtf(r1sq - tip2);  // count kernel (.h:130-131)
+    const float r1 = sqrtf(r1sq);
+    const float rOriginB = sqrtf(r1 * r1 - tip2);  // store kernel (.h:305-306)
+    pred_z(g, p1z, p2z, s2, sMax, sMin, sigmaZ, rOriginA, w.zLowA, w.zHighA, w.zLoA, w.zHiA);
+    pred_z(g, p1z, p2z, s2, sMax, sMin, sigmaZ, rOriginB, w.zLowB, w.zHighB, w.zLoB, w.zHiB);
+    const float phi = hzp[secondHit].y;
+    const float PI = TX_PI_F;
+    float dPhi = phi - hzp[firstHit].y;
+    dPhi += (dPhi > PI) ? -2 * PI : (dPhi < -PI) ? 2 * PI : 0;
+    dPhi = fabsf(dPhi);
+    const float dHits = sqrtf((p2x - p1x) * (p2x - p1x) + (p2y - p1y) * (p2y - p1y));
+    const float aD = tx_acosf(dHits / (2 * mrc));
+    const float tmp = fabsf(aD - tc.acos_max_r[layer]);
+    float phiLow = fabsf(aD - tc.acos_min_r[layer]);
+    const float m2 = (tmp < phiLow) ? phiLow : tmp;  // OpenCL max(x, y): y if x < y, else x
+    dPhi = (dPhi < m2) ? m2 : dPhi;
+    float phiHigh = phi + dPhi + sigmaPhi;
+    phiLow = phi - dPhi - sigmaPhi;
+    w.wrap = phiLow < -PI || phiHigh > PI || phiLow > PI || phiHigh < -PI;
+    phiLow -= (float)((phiLow > PI) * 2) * PI;
+    phiLow += (float)((phiLow < -PI) * 2) * PI;
+    phiHigh -= (float)((phiHigh > PI) * 2) * PI;
+    phiHigh += (float)((phiHigh < -PI) * 2) * PI;
+    const uint32_t pl = tx_f2u_sat(floorf((phiLow - g.min_phi) / g.sector_phi));
+    const uint32_t ph = tx_f2u_sat(floorf((phiHigh - g.min_phi) / g.sector_phi)) + 1u;
+    w.pLo = pl < g.n_phi + 1 ? pl : g.n_phi + 1;
+    w.pHi = ph < g.n_phi ? ph : g.n_phi;
+    w.phiLow = phiLow;
+    w.phiHigh = phiHigh;
+    return true;
+}
+
+__device__ inline bool phi_in(const PredWin2& w, float actPhi) {
+    const float PI = TX_PI_F;
+    return (!w.wrap && (w.phiLow <= actPhi && actPhi <= w.phiHigh)) ||
+           (w.wrap && ((w.phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= w.phiHigh)));
+}
+
+// One walk over the z sectors of both windows (their phi cells are the same):
+// sector zs belongs to window A if zLoA <= zs < zHiA and to B likewise, and in
+// every sector the hits are visited in the reference's j order, four loads in
+// flight at a time.  visit(index, inA, inB) sees each hit once.
+template <class V>
+__device__ inline void pred_walk2(const PredWin2& w, uint32_t np1, const float2* __restrict__ hzp, V visit) {
+    const bool ea = w.zLoA < w.zHiA, eb = w.zLoB < w.zHiB;
+    if (!ea && !eb) return;
+    const uint32_t z0 = !ea ? w.zLoB : (!eb ? w.zLoA : (w.zLoA < w.zLoB ? w.zLoA : w.zLoB));
+    const uint32_t z1 = !ea ? w.zHiB : (!eb ? w.zHiA : (w.zHiA > w.zHiB ? w.zHiA : w.zHiB));
+    const uint32_t* lg3 = w.lg3;
+    for (uint32_t zs = z0; zs < z1; ++zs) {
+        const bool sa = ea && zs >= w.zLoA && zs < w.zHiA;
+        const bool sb = eb && zs >= w.zLoB && zs < w.zHiB;
+        if (!sa && !sb) continue;
+        const uint32_t zStart = lg3[zs * np1];
+        const uint32_t zEnd = lg3[(zs + 1) * np1];
+        const uint32_t zLen = zEnd - zStart;
+        const uint32_t end2 = (uint32_t)w.wrap * zEnd + lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+        for (uint32_t j = lg3[zs * np1 + w.pLo]; j < end2; j += 4) {
+            float2 q[4];
+            uint32_t idx[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t jj = j + u;
+                idx[u] = jj - (uint32_t)(jj >= zEnd) * zLen;
+                q[u] = jj < end2 ? hzp[idx[u]] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (j + u < end2) visit(idx[u], q[u], sa, sb);
+        }
+    }
+}
+
 // Count, scan and store of the predictor in one pass over tiles of
-// kPredBlock pairs (batched path).  Both window expressions are evaluated on
-// one walk when they cover the same cells (otherwise each is walked); the
-// store-side matches of every pair are kept in LDS (kPredList per pair; a pair
+// kPredBlock pairs (batched path).  One merged walk counts the count-window
+// matches and collects the store-window matches of every pair in LDS
+// (kPredList per pair, as 16-bit offsets from the layer's first hit; a pair
 // with more re-walks its window for the rest), the tile's output offset comes
 // from a decoupled look-back, and the tile's candidates are written as one
-// contiguous run by all threads.
+// contiguous, coalesced run by all threads.
 constexpr int kPredBlock = 256;
 constexpr int kPredList = 24;
 
 __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
     GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
-    const float* __restrict__ hy, const float* __restrict__ hz, const float* __restrict__ hphi,
+    const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
     DevCounts* __restrict__ cnt, uint64_t* __restrict__ status, uint32_t* __restrict__ counter) {
-    __shared__ uint32_t s_list[kPredBlock * kPredList];
+    __shared__ uint16_t s_list[kPredBlock * kPredList];
     __shared__ uint32_t s_excl[kPredBlock + 1];
     __shared__ uint32_t s_nb[kPredBlock];
+    __shared__ uint32_t s_b3[kPredBlock];  // first hit of the pair's third layer; kSentinel: no list
     __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
     __shared__ uint32_t s_tile, s_base;
     const uint32_t tid = threadIdx.x;
@@ -439,42 +575,28 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
         const uint32_t tile = s_tile;
         if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
         const uint32_t gid = tile * kPredBlock + tid;
-        uint32_t countA = 0, nB = 0;
+        uint32_t countA = 0, nB = 0, b3 = kSentinel;
         uint32_t h1 = kSentinel;
-        PWin pa, pb;
-        bool ok = false, same = true;
+        PredWin2 pw;
         if (gid < n) {
             h1 = pairs[2 * (size_t)gid];
             const uint32_t h2 = pairs[2 * (size_t)gid + 1];
-            ok = h1 != kSentinel && h2 != kSentinel &&
-                 pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, false, pa);
-            if (ok) {
-                pred_window(g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, n_events, h1, h2, true, pb);
-                same = same_cells(pa.w, pb.w);
-                uint32_t* my = s_list + tid * kPredList;
-                if (same) {
-                    pred_walk(pa.w, pa.lg3, np1, [&](uint32_t index) {
-                        const float zz = hz[index], ph = hphi[index];
-                        countA += in_window(pa.w, zz, ph);
-                        if (in_window(pb.w, zz, ph)) {
-                            if (nB < kPredList) my[nB] = index;
-                            nB++;
-                        }
-                        return true;
-                    });
-                } else {
-                    pred_walk(pa.w, pa.lg3, np1, [&](uint32_t index) {
-                        countA += in_window(pa.w, hz[index], hphi[index]);
-                        return true;
-                    });
-                    pred_walk(pb.w, pb.lg3, np1, [&](uint32_t index) {
-                        if (in_window(pb.w, hz[index], hphi[index])) {
-                            if (nB < kPredList) my[nB] = index;
-                            nB++;
-                        }
-                        return true;
-                    });
-                }
+            if (h1 != kSentinel && h2 != kSentinel &&
+                pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw)) {
+                const uint32_t first = pw.lg3[0];
+                // the list holds 16-bit offsets from the layer's first hit
+                const bool listed = pw.lg3[g.cells - 1] - first <= 0xffffu;
+                b3 = listed ? first : kSentinel;
+                const uint32_t cap = listed ? (uint32_t)kPredList : 0u;
+                uint16_t* my = s_list + tid * kPredList;
+                pred_walk2(pw, np1, hzp, [&](uint32_t index, float2 q, bool sa, bool sb) {
+                    if (!phi_in(pw, q.y)) return;
+                    countA += sa && pw.zLowA <= q.x && q.x <= pw.zHighA;
+                    if (sb && pw.zLowB <= q.x && q.x <= pw.zHighB) {
+                        if (nB < cap) my[nB] = (uint16_t)(index - first);
+                        nB++;
+                    }
+                });
             }
         }
         uint32_t total;
@@ -485,6 +607,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
         }
         s_excl[tid] = excl;
         s_nb[tid] = nB;
+        s_b3[tid] = b3;
         if (tid == 0) s_excl[kPredBlock] = total;
         __syncthreads();
         const uint32_t base = s_base;
@@ -500,33 +623,42 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
                     const uint32_t mid = (lo + hi) >> 1;
                     if (s_excl[mid] <= s) lo = mid; else hi = mid;
                 }
-                const uint32_t k = s - s_excl[lo], nb = s_nb[lo];
+                const uint32_t k = s - s_excl[lo], nb = s_nb[lo], lb3 = s_b3[lo];
                 const uint64_t out = 2 * ((uint64_t)base + s);
                 if (k < nb) {
-                    if (k < (uint32_t)kPredList) {
+                    if (lb3 != kSentinel && k < (uint32_t)kPredList) {
                         cands[out] = tile * kPredBlock + lo;
-                        cands[out + 1] = s_list[lo * kPredList + k];
+                        cands[out + 1] = lb3 + s_list[lo * kPredList + k];
                     }
                 } else {
                     cands[out] = kSentinel;
                     cands[out + 1] = kSentinel;
                 }
             }
-            // pairs with more store-side matches than the LDS list holds
+            // pairs with more store-side matches than their LDS list holds
             const uint32_t keep = countA < nB ? countA : nB;
-            if (keep > (uint32_t)kPredList) {
+            const uint32_t listed = b3 != kSentinel ? (uint32_t)kPredList : 0u;
+            if (keep > listed) {
                 uint32_t k = 0;
                 const uint64_t o = (uint64_t)base + excl;
-                pred_walk(pb.w, pb.lg3, np1, [&](uint32_t index) {
-                    if (in_window(pb.w, hz[index], hphi[index])) {
-                        if (k >= (uint32_t)kPredList) {
-                            cands[2 * (o + k)] = gid;
-                            cands[2 * (o + k) + 1] = index;
+                for (uint32_t zs = pw.zLoB; zs < pw.zHiB && k < keep; ++zs) {
+                    const uint32_t zStart = pw.lg3[zs * np1];
+                    const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
+                    const uint32_t zLen = zEnd - zStart;
+                    const uint32_t end2 =
+                        (uint32_t)pw.wrap * zEnd + pw.lg3[zs * np1 + pw.pHi] - (uint32_t)pw.wrap * zStart;
+                    for (uint32_t j = pw.lg3[zs * np1 + pw.pLo]; j < end2 && k < keep; ++j) {
+                        const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
+                        const float2 q = hzp[index];
+                        if (phi_in(pw, q.y) && pw.zLowB <= q.x && q.x <= pw.zHighB) {
+                            if (k >= listed) {
+                                cands[2 * (o + k)] = gid;
+                                cands[2 * (o + k) + 1] = index;
+                            }
+                            k++;
                         }
-                        k++;
                     }
-                    return k < keep;
-                });
+                }
             }
         }
         if (gid < n) {
@@ -552,23 +684,23 @@ hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n
 
 hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
-                             const float* hz, const float* hphi, const uint32_t* seg_base, uint32_t* slot_counts,
+                             const float* hz, const float2* hzp, const uint32_t* seg_base, uint32_t* slot_counts,
                              DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     (void)n_seg_cap;
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     hipLaunchKernelGGL(k_pair_count, dim3(stride_blocks((uint64_t)n_slot_cap + 1, b, lc)), dim3(b), 0, st, g,
-                       tc, grid, hx, hy, hz, hphi, seg_base, slot_counts, cnt);
+                       tc, grid, hx, hy, hz, hzp, seg_base, slot_counts, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
-                             const float* hx, const float* hy, const float* hz, const float* hphi,
+                             const float* hx, const float* hy, const float* hz, const float2* hzp,
                              const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
                              uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     hipLaunchKernelGGL(k_pair_store, dim3(stride_blocks(n_slot_cap, b, lc)), dim3(b), 0, st, g, tc, grid, hx,
-                       hy, hz, hphi, seg_base, slot_counts, pairs, n_pair_cap, cnt);
+                       hy, hz, hzp, seg_base, slot_counts, pairs, n_pair_cap, cnt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pair_offsets, dim3((n_seg_cap + 1 + 255) / 256), dim3(256), 0, st, seg_base,
@@ -578,23 +710,23 @@ hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t 
 
 hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, uint32_t* counts, DevCounts* cnt, const LaunchCfg& lc,
                                 hipStream_t st) {
     const uint32_t b = lc.predict_block ? lc.predict_block : 256;
     hipLaunchKernelGGL(k_predict_count, dim3(stride_blocks((uint64_t)n_pair_cap + 1, b, lc)), dim3(b), 0, st,
-                       g, tc, grid, hx, hy, hz, hphi, hlayer, hevt, pairs, counts, cnt);
+                       g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, counts, cnt);
     return hipGetLastError();
 }
 
 hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 uint32_t n_cand_cap, const uint32_t* grid, const float* hx, const float* hy,
-                                const float* hz, const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float* hz, const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, const uint32_t* counts, uint32_t* cands,
                                 uint32_t* cand_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t b = lc.predict_block ? lc.predict_block : 256;
     hipLaunchKernelGGL(k_predict_store, dim3(stride_blocks(n_pair_cap, b, lc)), dim3(b), 0, st, g, tc, grid,
-                       hx, hy, hz, hphi, hlayer, hevt, pairs, counts, cands, cand_offsets, n_cand_cap, cnt);
+                       hx, hy, hz, hzp, hlayer, hevt, pairs, counts, cands, cand_offsets, n_cand_cap, cnt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
@@ -604,7 +736,7 @@ uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap
 
 hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const float* hphi, const uint32_t* hlayer, const uint32_t* hevt,
+                                const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                 const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
                                 const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t tiles = predict_fused_tiles(n_pair_cap);
@@ -612,7 +744,7 @@ hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32
     hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(ss.status), 0u, 2ull * tiles + 2, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_predict_fused, dim3(stride_blocks(n_pair_cap, kPredBlock, lc)), dim3(kPredBlock), 0, st, g,
-                       tc, grid, hx, hy, hz, hphi, hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt,
+                       tc, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt,
                        ss.status, reinterpret_cast<uint32_t*>(ss.status + tiles));
     e = hipGetLastError();
     if (e != hipSuccess) return e;
