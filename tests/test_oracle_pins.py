@@ -154,6 +154,64 @@ def test_pairs_brute_force_float64():
     assert near <= 3
 
 
+def test_predictor_brute_force_float64():
+    """TripletThetaPhiPredictor (.h:77-230): for every pair, the third hits the
+    oracle predicts are exactly the layer-3 hits of the same event inside the
+    window recomputed in float64 (z from the transverse impact and cot(theta),
+    phi from the hit-to-hit and curvature terms), up to hits within 1e-4 of a
+    window edge.  Both rOrigin forms coincide in float64."""
+    wl = small_workload(2, tracks=60, noise=600)
+    wl, cfg, hs, _ = (wl, *hitset(wl, seed=23))
+    res = oracle.run(hs.columns(), wl, cfg.min_radius_curvature, stop_after=3)
+    x, y, z = (res[k].astype(np.float64) for k in ("hits.x", "hits.y", "hits.z"))
+    layer = res["hits.layer"]
+    _, rmin, rmax = geometry_arrays()
+    mrc = float(cfg.min_radius_curvature)
+    pairs = res["pairs"].reshape(-1, 2).astype(np.int64)
+    cands = res["cands"].reshape(-1, 2).astype(np.int64)
+    ok_c = cands[:, 0] != 0xFFFFFFFF
+    got = {}
+    for pi, h3 in cands[ok_c]:
+        got.setdefault(int(pi), []).append(int(h3))
+    phi_all = np.arctan2(y, x)
+    near = missing = extra = checked = 0
+    for pi, (h1, h2) in enumerate(pairs):
+        if h1 == 0xFFFFFFFF:
+            continue
+        lt = int(layer[h1]) - 1
+        t = wl.triplets[lt]
+        l3 = t.layer3 - 1
+        p1x, p1y, p1z, p2x, p2y, p2z = x[h1], y[h1], z[h1], x[h2], y[h2], z[h2]
+        tip2 = (p1y * (p2x - p1x) - p1x * (p2y - p1y)) ** 2 / ((p2x - p1x) ** 2 + (p2y - p1y) ** 2)
+        r0 = np.sqrt(p1x * p1x + p1y * p1y - tip2)
+        cot = (p2z - p1z) / (np.sqrt(p2x * p2x + p2y * p2y - tip2) - r0)
+        za = p1z + (np.sqrt(float(rmax[l3]) ** 2 - tip2) - r0) * cot
+        zb = p1z + (np.sqrt(float(rmin[l3]) ** 2 - tip2) - r0) * cot
+        zl, zh = min(za, zb) - float(t.sigmaZ), max(za, zb) + float(t.sigmaZ)
+        phi = np.arctan2(p2y, p2x)
+        dphi = abs((phi - np.arctan2(p1y, p1x) + np.pi) % (2 * np.pi) - np.pi)
+        a = np.arccos(np.hypot(p2x - p1x, p2y - p1y) / (2 * mrc))
+        m = max(abs(a - np.arccos(float(rmax[l3]) / (2 * mrc))), abs(a - np.arccos(float(rmin[l3]) / (2 * mrc))))
+        half = max(dphi, m) + float(t.sigmaPhi)
+        same = np.nonzero((layer == t.layer3) & (res["hits.event"] == res["hits.event"][h1]))[0]
+        zi, di = z[same], np.abs((phi_all[same] - phi + np.pi) % (2 * np.pi) - np.pi)
+        inside = (zi >= zl) & (zi <= zh) & (di <= half)
+        margin = np.minimum.reduce([np.abs(zi - zl), np.abs(zh - zi), np.abs(half - di)])
+        g = set(got.get(pi, []))
+        for k, h in enumerate(same):
+            checked += 1
+            if (int(h) in g) != bool(inside[k]):
+                if margin[k] < 1e-4:
+                    near += 1
+                elif int(h) in g:
+                    extra += 1
+                else:
+                    missing += 1
+    assert checked > 50_000
+    assert missing == 0 and extra == 0, (missing, extra, near)
+    assert near <= 5
+
+
 def test_pair_order_is_reference_work_item_order(dense):
     """pairStore (PairGeneratorBeamspot.h:488-521): per (event, layer pair) the
     pairs of work item t (outer hits t, t + nThreads, ...) come first, inner hits
