@@ -87,6 +87,26 @@ def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_h
             "mean_ms": {k: round(st[k + "_ms"], 3) for k in ("queue", "pipeline", "copy", "split")}}
 
 
+# the kernel each phase's time belongs to (the phase also holds small fills)
+PHASE_KERNEL = {"grid_count": "k_grid_count", "grid_store": "k_grid_store", "pair_count": "k_pair_count",
+                "pair_store": "k_pair_store", "predict": "k_predict_fused", "filter_count": "k_filter_count",
+                "filter_store": "k_filter_store", "link_count": "k_link_count", "link_store": "k_link_store"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, calibrated)
+    from the rocprofv3 counter summary committed under profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", "hbm-traffic-256-events.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if p and not line.startswith("#") and p[0] == kernel and len(p) == 4:
+                return (int(p[2]) + int(p[3])) // max(int(p[1]), 1)
+    return None
+
+
 def cpu_baseline(wl, events, cfg_cls, hitset_cls, budget_s=20.0):
     import oracle
     cores = min(16, os.cpu_count() or 1)
@@ -268,8 +288,9 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
         "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": dom, "algorithmic_bytes": int(pb[dom])},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": measured_traffic(PHASE_KERNEL.get(dom, dom)),
+                     "kernel": PHASE_KERNEL.get(dom, dom), "algorithmic_bytes": int(pb[dom])},
         "latency_ms": {"p50": serving["p50_ms"] if serving else None,
                        "p99": serving["p99_ms"] if serving else None},
         "serving": serving,
