@@ -134,6 +134,7 @@ def main():
     ap.add_argument("--plain-events", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--serve-events", type=int, default=10000, help="events of the serving stream (0 = skip)")
@@ -151,7 +152,7 @@ def main():
         sys.exit(2)
 
     from trax_amd import make_events, workload
-    from trax_amd.api import Batch, Caps, Config, Device, HitSet, OutputBounds, run_batch_fit
+    from trax_amd.api import Batch, Caps, Config, Device, HitSet, OutputBounds, autotune, run_batch_fit
 
     wl = workload(3)
     t_gen = time.perf_counter()
@@ -174,7 +175,12 @@ def main():
     head = lambda v: int(v * 1.15) + 4096
     caps = Caps(N_EVENTS, hs.n_hits, head(c["pairs"]), head(c["cands"]), head(c["tracklets"]), head(c["conn"]),
                 head(c["track_entries"]), use_graph=not args.no_graph, per_event=True)
-    b = Batch(dev, cfg, caps)
+    # launch shapes for this workload from the autotuner (results do not depend on them)
+    t_tune = time.perf_counter()
+    tuning, trials = autotune(dev, cfg, hs, caps, repeats=2) if not args.no_autotune else ({}, [])
+    tune_s = time.perf_counter() - t_tune
+    print(f"autotuned in {tune_s:.1f} s: {tuning}", file=sys.stderr, flush=True)
+    b = Batch(dev, cfg, caps, tuning)
     b.submit(hs)  # upload once; later steps run on the resident hits
     if b.wait() != 0:
         raise RuntimeError("capacity overflow after planning")
@@ -206,7 +212,7 @@ def main():
         print("serving stream", file=sys.stderr, flush=True)
         serving = serve(dev, stream[0], stream[1], bounds, args.serve_rate, SEED + 2, args.serve_slots,
                         args.serve_max_batch, args.serve_slot_hits)
-        b = Batch(dev, cfg, caps)
+        b = Batch(dev, cfg, caps, tuning)
         b.submit(hs)
         b.wait()
 
@@ -275,7 +281,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "events_per_step": N_EVENTS, "hits_per_step": hs.n_hits,
                    "layers": wl.n_layers, "layer_triplets": len(wl.triplets), "grid": [wl.n_z, wl.n_phi],
-                   "seed": SEED, "graph": not args.no_graph},
+                   "seed": SEED, "graph": not args.no_graph, "event_groups": "per-event",
+                   "tuning": tuning, "autotune_s": round(tune_s, 2)},
         "triplets_per_s": round(trip_s, 1),
         "triplets_per_step": counts["tracklets"],
         "pairs_per_step": counts["pairs"],

@@ -169,9 +169,14 @@ typedef struct {
      *    that event alone, whatever batch it shares */
     uint32_t per_event;
 } trax_batch_caps;
+/* launch shapes (0 = default): block sizes per kernel family, blocks per CU
+ * of the grid-stride kernels, and the fused predictor's variant (block 128,
+ * 256 or 512 with 16, 24 or 32 LDS list entries per pair); results do not
+ * depend on them */
 typedef struct {
     uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
     uint32_t blocks_per_cu;
+    uint32_t predict_list;
 } trax_tuning;
 /* device and pinned host bytes a batch of these capacities would hold, without
  * allocating (the planning step of buildTriplets' per-group buffer sizing,
@@ -181,6 +186,15 @@ int trax_batch_plan(const trax_config* cfg, const trax_batch_caps* caps, uint64_
 int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const trax_batch_caps* caps,
                       const trax_tuning* tuning, trax_stream* s, trax_batch** out);
 void trax_batch_destroy(trax_batch* b);
+/* Autotuner: picks every launch shape of trax_tuning for this workload by
+ * coordinate descent over the supported values, timing the phases each one
+ * drives with HIP events on a batch of the sample events (eager launches,
+ * best of `repeats` runs per candidate) and checking that every candidate
+ * gives the same tracklets.  *best receives the fastest shapes; report (if
+ * not NULL) receives one JSON line per trial.  Returns TRAX_ERR_STATE if a
+ * candidate changed the results. */
+int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_hits* sample, const trax_batch_caps* caps,
+                  uint32_t repeats, trax_tuning* best, char* report, uint64_t report_bytes);
 /* upload a HitCollection and enqueue the whole pipeline; asynchronous */
 int trax_batch_submit(trax_batch* b, const trax_hits* hits);
 /* enqueue the pipeline again on the inputs already resident on the device */

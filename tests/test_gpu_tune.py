@@ -1,0 +1,63 @@
+"""Autotuner and launch shapes: every supported launch shape gives the same
+results as the oracle, and the autotuner's choice is one of them."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import compare, hitset, small_workload
+from trax_amd.api import Batch, Caps, autotune, run_batch_fit
+
+pytestmark = pytest.mark.gpu
+
+PREDICT_VARIANTS = [(128, 32), (256, 16), (256, 24), (256, 32), (512, 16), (512, 24)]
+
+
+def _case():
+    wl = small_workload(3, events=4, tracks_min=20, tracks_max=400)
+    return (wl, *hitset(wl, seed=11))
+
+
+@pytest.mark.parametrize("block,lst", PREDICT_VARIANTS)
+def test_predictor_variants_match_oracle(dev, block, lst):
+    wl, cfg, hs, _ = _case()
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits),
+                           tuning=dict(predict_block=block, predict_list=lst))
+    b.close()
+    bad = compare(got, want)
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("shape", [dict(filter_block=64, pair_block=64, link_block=64, grid_store_block=128,
+                                        grid_count_block=128, blocks_per_cu=2),
+                                   dict(filter_block=1024, pair_block=512, link_block=512, grid_store_block=1024,
+                                        grid_count_block=1024, blocks_per_cu=16)])
+def test_block_shapes_match_oracle(dev, shape):
+    wl, cfg, hs, _ = _case()
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits), tuning=shape)
+    b.close()
+    bad = compare(got, want)
+    assert not bad, "\n".join(bad)
+
+
+def test_autotune_picks_supported_shapes_with_same_results(dev):
+    wl, cfg, hs, _ = _case()
+    caps = Caps.for_hits(hs.n_events, hs.n_hits, per_event=True)
+    b, _ = run_batch_fit(dev, cfg, hs, caps)
+    caps = b.caps
+    b.close()
+    tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
+    assert len(trials) == 6 + 5 + 4 + 4 + 4 + 4 + 4
+    assert all(t["same_results"] for t in trials)
+    assert (tuning["predict_block"], tuning["predict_list"]) in PREDICT_VARIANTS
+    assert tuning["blocks_per_cu"] in (2, 4, 8, 16)
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    tb = Batch(dev, cfg, caps, tuning)
+    tb.submit(hs)
+    assert tb.wait() == 0
+    got = tb.result()
+    tb.close()
+    per_event = {k: got[k] for k in ("tracklets.h1", "tracklets.h2", "tracklets.h3", "pairs", "cands")}
+    bad = compare(per_event, want, keys=list(per_event))
+    assert not bad, "\n".join(bad)

@@ -42,7 +42,7 @@ class BatchCaps(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [(n, U32) for n in ("grid_count_block", "grid_store_block", "pair_block", "predict_block",
-                                   "filter_block", "link_block", "blocks_per_cu")]
+                                   "filter_block", "link_block", "blocks_per_cu", "predict_list")]
 
 
 class ServeOpts(C.Structure):
@@ -102,6 +102,7 @@ def lib():
         "trax_config_set_exec": (C.c_int, [P, U32, C.c_float]),
         "trax_config_min_radius_curvature": (C.c_float, [P]),
         "trax_run_plain": (C.c_int, [P, P, P, C.c_int, C.POINTER(P)]),
+        "trax_autotune": (C.c_int, [P, P, P, C.POINTER(BatchCaps), U32, C.POINTER(Tuning), C.c_char_p, C.c_uint64]),
         "trax_batch_plan": (C.c_int, [P, C.POINTER(BatchCaps), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "trax_server_create": (C.c_int, [P, P, C.POINTER(ServeOpts), C.POINTER(P)]),
         "trax_server_destroy": (None, [P]),
@@ -480,3 +481,16 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def autotune(dev: Device, cfg: Config, hits: HitSet, caps: Caps, repeats: int = 2):
+    """Launch shapes for this workload from the library's autotuner: returns
+    (tuning dict for Batch(..., tuning=...), list of trial records)."""
+    import json
+    c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
+                  caps.max_conn, caps.max_track_entries, 0, caps.ca_iterations, int(caps.per_event))
+    t = Tuning()
+    buf = C.create_string_buffer(1 << 16)
+    check(lib().trax_autotune(dev.h, cfg.h, hits.h, C.byref(c), repeats, C.byref(t), buf, len(buf)))
+    trials = [json.loads(l) for l in buf.value.decode().splitlines() if l.strip()]
+    return {n: int(getattr(t, n)) for n, _ in Tuning._fields_}, trials

@@ -24,6 +24,7 @@ SOURCES = [
     "host/trax_host.cpp",
     "capi.cpp",
     "serve.cpp",
+    "tune.cpp",
 ]
 
 FLAGS = [

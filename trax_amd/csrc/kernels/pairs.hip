@@ -548,9 +548,7 @@ __device__ inline void pred_walk2(const PredWin2& w, uint32_t np1, const float2*
 // with more re-walks its window for the rest), the tile's output offset comes
 // from a decoupled look-back, and the tile's candidates are written as one
 // contiguous, coalesced run by all threads.
-constexpr int kPredBlock = 256;
-constexpr int kPredList = 24;
-
+template <int kPredBlock, int kPredList>
 __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
     GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
     const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
@@ -732,7 +730,61 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
     return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
 }
 
-uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap / kPredBlock) + 2; }
+uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap / 128) + 2; }  // every variant
+
+namespace {
+
+// blocks of a variant that stay resident on the device (persistent launch);
+// first called outside any graph capture (predict_fused_prepare)
+template <int B, int L>
+uint32_t resident_blocks() {
+    static uint32_t resident = 0;
+    if (resident == 0) {
+        int per_cu = 0, dev = 0, n_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_fused<B, L>, B, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu < 1)
+            n_cu = 256;
+        resident = (uint32_t)(per_cu * n_cu);
+    }
+    return resident;
+}
+
+template <int B, int L>
+hipError_t predict_fused_variant(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
+                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
+                                 const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
+                                 const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
+                                 uint64_t* status, uint32_t* counter, hipStream_t st) {
+    // persistent: as many blocks as stay resident, each loops over tiles
+    const uint32_t resident = resident_blocks<B, L>();
+    const uint64_t tiles = (uint64_t)n_pair_cap / B + 1;
+    const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
+    hipLaunchKernelGGL((k_predict_fused<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp, hlayer,
+                       hevt, pairs, cands, cand_offsets, n_cand_cap, cnt, status, counter);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool predict_variant_supported(uint32_t block, uint32_t list) {
+    block = block ? block : 256;
+    list = list ? list : 24;
+    return (block == 128 && list == 32) || (block == 256 && (list == 16 || list == 24 || list == 32)) ||
+           (block == 512 && (list == 16 || list == 24));
+}
+
+void predict_fused_prepare(const LaunchCfg& lc) {
+    const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
+    if (b == 128 && l == 32) (void)resident_blocks<128, 32>();
+    if (b == 256 && l == 16) (void)resident_blocks<256, 16>();
+    if (b == 256 && l == 24) (void)resident_blocks<256, 24>();
+    if (b == 256 && l == 32) (void)resident_blocks<256, 32>();
+    if (b == 512 && l == 16) (void)resident_blocks<512, 16>();
+    if (b == 512 && l == 24) (void)resident_blocks<512, 24>();
+}
 
 hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
@@ -743,10 +795,22 @@ hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32
     if (tiles > ss.max_tiles) return hipErrorInvalidValue;
     hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(ss.status), 0u, 2ull * tiles + 2, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_predict_fused, dim3(stride_blocks(n_pair_cap, kPredBlock, lc)), dim3(kPredBlock), 0, st, g,
-                       tc, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt,
-                       ss.status, reinterpret_cast<uint32_t*>(ss.status + tiles));
-    e = hipGetLastError();
+    uint64_t* status = ss.status;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(ss.status + tiles);
+    const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
+#define TRAX_PRED_VARIANT(B, L)                                                                                   \
+    if (b == B && l == L)                                                                                        \
+        e = predict_fused_variant<B, L>(g, tc, n_pair_cap, n_cand_cap, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, \
+                                        cands, cand_offsets, cnt, status, counter, st);                          \
+    else
+    TRAX_PRED_VARIANT(128, 32)
+    TRAX_PRED_VARIANT(256, 16)
+    TRAX_PRED_VARIANT(256, 24)
+    TRAX_PRED_VARIANT(256, 32)
+    TRAX_PRED_VARIANT(512, 16)
+    TRAX_PRED_VARIANT(512, 24)
+    return hipErrorInvalidConfiguration;
+#undef TRAX_PRED_VARIANT
     if (e != hipSuccess) return e;
     return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
 }

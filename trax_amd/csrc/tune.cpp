@@ -1,0 +1,164 @@
+// Autotuner of the batched pipeline's launch shapes.
+//
+// Every launch shape is a performance choice only: the kernels read sizes
+// from the device, tile boundaries come from scans, and the output order is
+// fixed by the reference's work-item order, not by the launch.  The tuner
+// therefore picks each shape by timing alone, by coordinate descent in a
+// fixed order (each parameter in turn, the others at their best so far).
+// Each candidate is timed on the phases that parameter drives, with the HIP
+// events recorded between the phases on the batch's stream, as the best of
+// `repeats` eager runs.  As a guard it checks that every candidate reproduces
+// the first run's tracklets and counts, and it fails if one does not.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+
+using namespace trax;
+using namespace trax_capi;
+
+namespace {
+
+struct Param {
+    const char* name;
+    std::vector<uint32_t> values;        // predictor: (block << 16) | list
+    std::vector<const char*> phases;     // empty: the whole sequence
+};
+
+void set_param(trax_tuning& t, const char* name, uint32_t v) {
+    const std::string n(name);
+    if (n == "grid_count_block") t.grid_count_block = v;
+    else if (n == "grid_store_block") t.grid_store_block = v;
+    else if (n == "pair_block") t.pair_block = v;
+    else if (n == "predict") { t.predict_block = v >> 16; t.predict_list = v & 0xffffu; }
+    else if (n == "filter_block") t.filter_block = v;
+    else if (n == "link_block") t.link_block = v;
+    else if (n == "blocks_per_cu") t.blocks_per_cu = v;
+}
+
+LaunchCfg to_launch(const trax_tuning& t) {
+    LaunchCfg lc;
+    lc.grid_count_block = t.grid_count_block;
+    lc.grid_store_block = t.grid_store_block;
+    lc.pair_block = t.pair_block;
+    lc.predict_block = t.predict_block;
+    lc.filter_block = t.filter_block;
+    lc.link_block = t.link_block;
+    lc.blocks_per_cu = t.blocks_per_cu;
+    lc.predict_list = t.predict_list;
+    return lc;
+}
+
+uint64_t mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return h * 0xff51afd7ed558ccdull;
+}
+
+// digest of a completed run: the counts and every tracklet
+uint64_t run_digest(trax_batch* b) {
+    const DevCounts& c = *b->h_counts;
+    uint64_t h = 0;
+    for (uint32_t v : {c.n_pairs, c.n_cands, c.n_tracklets, c.n_conn, c.n_track_entries, c.err}) h = mix(h, v);
+    const uint32_t R = std::min(c.n_tracklets, b->caps.max_tracklets);
+    std::vector<uint32_t> t(R);
+    for (const uint32_t* col : {b->th1, b->th2, b->th3}) {
+        if (R) hip_check(hipMemcpy(t.data(), col, R * 4ull, hipMemcpyDeviceToHost), "d2h");
+        for (uint32_t i = 0; i < R; i += 1) h = mix(h, t[i]);
+    }
+    return h;
+}
+
+float phases_ms(trax_batch* b, const std::vector<const char*>& phases) {
+    float sum = 0.0f;
+    if (phases.empty()) {
+        hip_check(hipEventElapsedTime(&sum, b->ev[0], b->ev[kNumStages]), "event time");
+        return sum;
+    }
+    for (const char* p : phases)
+        for (int k = 0; k < kNumStages; ++k)
+            if (std::strcmp(p, kPhases[k]) == 0) {
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]), "event time");
+                sum += ms;
+            }
+    return sum;
+}
+
+}  // namespace
+
+extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_hits* sample,
+                             const trax_batch_caps* caps_in, uint32_t repeats, trax_tuning* best, char* report,
+                             uint64_t report_bytes) {
+    return guarded([&] {
+        if (!ctx || !cfg || !sample || !caps_in || !best) return fail(TRAX_ERR_ARG, "null argument");
+        if (repeats == 0) repeats = 1;
+        trax_batch_caps caps = *caps_in;
+        caps.use_graph = 0;  // the launch shape changes per trial; eager launches read it each time
+        trax_batch* b = nullptr;
+        trax_tuning cur{};
+        int rc = trax_batch_create(ctx, cfg, &caps, &cur, nullptr, &b);
+        if (rc != TRAX_OK) return rc;
+        std::unique_ptr<trax_batch, void (*)(trax_batch*)> hold(b, trax_batch_destroy);
+        rc = trax_batch_submit(b, sample);
+        if (rc != TRAX_OK) return rc;
+        rc = b->finish();
+        if (rc != TRAX_OK) return rc;
+        const uint64_t ref = run_digest(b);
+        const std::vector<Param> params = {
+            {"predict", {(128u << 16) | 32u, (256u << 16) | 16u, (256u << 16) | 24u, (256u << 16) | 32u,
+                         (512u << 16) | 16u, (512u << 16) | 24u}, {"predict"}},
+            {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
+            {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
+            {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
+            {"grid_store_block", {128, 256, 512, 1024}, {"grid_store"}},
+            {"grid_count_block", {128, 256, 512, 1024}, {"grid_count"}},
+            {"blocks_per_cu", {2, 4, 8, 16}, {}},
+        };
+        std::string log;
+        bool changed = false;
+        for (const Param& p : params) {
+            uint32_t best_v = 0;
+            float best_ms = 0.0f;
+            for (uint32_t v : p.values) {
+                trax_tuning t = cur;
+                set_param(t, p.name, v);
+                b->lc = to_launch(t);
+                predict_fused_prepare(b->lc);
+                float ms = 0.0f;
+                bool same = true;
+                for (uint32_t r = 0; r < repeats; ++r) {
+                    b->launch();
+                    rc = b->finish();
+                    if (rc != TRAX_OK) return rc;
+                    const float m = phases_ms(b, p.phases);
+                    ms = r == 0 ? m : std::min(ms, m);
+                    if (r == 0) same = run_digest(b) == ref;
+                }
+                changed |= !same;
+                char line[256];
+                std::snprintf(line, sizeof(line),
+                              "{\"param\": \"%s\", \"value\": %u, \"ms\": %.4f, \"same_results\": %s}\n", p.name,
+                              p.name == std::string("predict") ? (v >> 16) * 1000 + (v & 0xffffu) : v, ms,
+                              same ? "true" : "false");
+                log += line;
+                if (best_v == 0 || ms < best_ms) {
+                    best_v = v;
+                    best_ms = ms;
+                }
+            }
+            set_param(cur, p.name, best_v);
+        }
+        b->lc = to_launch(cur);
+        *best = cur;
+        if (report && report_bytes) {
+            const size_t n = std::min<size_t>(log.size(), report_bytes - 1);
+            std::memcpy(report, log.data(), n);
+            report[n] = 0;
+        }
+        if (changed) return fail(TRAX_ERR_STATE, "a launch shape changed the results");
+        return TRAX_OK;
+    });
+}
