@@ -118,6 +118,12 @@ void trax_hits_destroy(trax_hits* h);
 int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_t n, const float* x, const float* y,
                         const float* z, const uint32_t* layer, const uint32_t* det_id,
                         const uint32_t* track_id, const float* track_pt, int max_tracks, int only_tracks);
+/* EventLoader (src/triplet_builder/EventLoader.h:15-67) + addEvent: the
+ * events of a serialized PB_Event::PEventContainer (Event.proto:1-37),
+ * decoded from the wire format without libprotobuf, each added as by
+ * trax_hits_add_event; max_events = -1 for all.  *n_added: events added. */
+int trax_hits_add_pb(trax_hits* h, const trax_config* cfg, const void* data, uint64_t size, int max_events,
+                     int max_tracks, int only_tracks, uint32_t* n_added);
 uint32_t trax_hits_count(const trax_hits* h);
 uint32_t trax_hits_events(const trax_hits* h);
 /* column access: "x","y","z","layer","det_id","hit_id","event",
@@ -203,6 +209,9 @@ int trax_batch_rerun(trax_batch* b);
 int trax_batch_wait(trax_batch* b);
 /* results of the last completed run (downloads them) */
 int trax_batch_result(trax_batch* b, trax_result** out);
+/* the same for the named arrays only (comma-separated, names as in
+ * trax_result_get): large batches without copying every intermediate */
+int trax_batch_result_select(trax_batch* b, const char* names, trax_result** out);
 /* counts of the last completed run: pairs, candidates, tracklets,
  * connections, track entries, CA iterations */
 int trax_batch_counts(trax_batch* b, uint32_t out[8]);
@@ -272,11 +281,32 @@ int trax_result_get(const trax_result* r, const char* name, const void** data, u
 double trax_result_timing_ms(const trax_result* r, const char* stage);
 void trax_result_destroy(trax_result* r);
 
+/* PhysicsRecord::fillData (src/triplet_builder/PhysicsRecord.cpp:86-255):
+ * tracklets against the simulation truth per (event, layer triplet).  Inputs
+ * are the grid-ordered hits' simulated track ids, layers and events, the
+ * tracklets and their (reference) offsets.  Findable tracks: ids whose hits
+ * cover every layer the configuration's triplets use (not id 0 with
+ * only_tracks).  rows: n_events * layer triplets entries, event-major. */
+typedef struct {
+    uint32_t event, layer_triplet, n_tracklets, n_findable, found, fakes, clones, misc;
+    double efficiency, fake_rate, clone_rate;
+} trax_physics_row;
+int trax_physics_eval(const trax_config* cfg, uint32_t n_events, uint32_t n_hits, const uint32_t* hit_id,
+                      const uint32_t* hit_layer, const uint32_t* hit_event, uint32_t n_tracklets, const uint32_t* h1,
+                      const uint32_t* h2, const uint32_t* h3, const uint32_t* tracklet_offsets, int only_tracks,
+                      trax_physics_row* rows);
+
 /* host and device evaluation of the shared float built-ins (atan2, acos,
  * atan, asinh), for accuracy and host/device agreement tests.
  * fn: 0 atan2(y, x), 1 acos(x), 2 atan(x), 3 asinh(x) */
 int trax_math_eval_host(int fn, const float* x, const float* y, float* out, uint64_t n);
 int trax_math_eval_device(trax_ctx* ctx, int fn, const float* x, const float* y, float* out, uint64_t n);
+/* host evaluation of the filter's phi cut (TripletThetaPhiFilter.h:95-103)
+ * on hit differences u = p2 - p1, v = p3 - p2: passes[i] = the reference's
+ * decision, surely_outside[i] = the prefilter's proof of rejection (which
+ * must imply !passes[i]); for tests */
+int trax_filter_phi_decide_host(uint64_t n, const float* ux, const float* uy, const float* vx, const float* vy,
+                                const float* cut, uint8_t* surely_outside, uint8_t* passes);
 
 #ifdef __cplusplus
 }

@@ -25,8 +25,12 @@ CASES = {
     "dense-2000-tracks-20000-noise": (2, {}, 11),
     "four-uneven-events": (3, dict(events=4, tracks_min=20, tracks_max=400), 11),
     "two-events-ca-density": (5, dict(events=2, tracks=500, noise=5000), 11),
+    # one event at the full density of the CA workload (5000 tracks + 50,000
+    # noise hits): about 7 minutes of oracle time on 8 threads
+    "ca-scale-one-event": (5, dict(events=1, tracks=5000, noise=50000), 11),
 }
 FULL = {"one-event-100-tracks"}
+SLOW = {"ca-scale-one-event"}  # not re-run by the CPU tests
 
 
 def case_workload(name):
@@ -46,21 +50,25 @@ def digest(a):
     return {"dtype": str(a.dtype), "n": int(a.size), "sha256": hashlib.sha256(a.tobytes()).hexdigest()}
 
 
-def main():
+def main(names=None):
     import oracle
+    path = os.path.join(HERE, "pipeline_digests.json")
     digests = {}
-    for name in CASES:
+    if os.path.exists(path):
+        with open(path) as f:
+            digests = json.load(f)
+    for name in names or CASES:
         wl, seed = case_workload(name)
         cfg, hs, _ = hitset(wl, seed=seed)
-        res = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+        res = oracle.run(hs.columns(), wl, cfg.min_radius_curvature, workers=os.cpu_count() or 1)
         keys = [k for k in PIPELINE_KEYS if k in res]
         digests[name] = {k: digest(res[k]) for k in keys}
         if name in FULL:
             np.savez_compressed(os.path.join(HERE, name + ".npz"), **{k: res[k] for k in keys})
         print(name, hs.n_hits, "hits,", res["tracklets.h1"].size, "tracklets")
-    with open(os.path.join(HERE, "pipeline_digests.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(digests, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)  # optional case names: regenerate only those

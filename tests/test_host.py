@@ -121,3 +121,37 @@ def test_abi_exports_every_declared_symbol():
     lib = C.CDLL(LIB_PATH)
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_filter_phi_prefilter_never_rejects_an_accepted_candidate():
+    """The filter's phi-cut prefilter (tx_phi_surely_outside) may only reject
+    candidates that the reference's expression rejects: random angles around
+    the cut, angles within a few ulp of it, wraps across +-pi, and tiny, huge
+    and degenerate vectors."""
+    from trax_amd.api import filter_phi_decide_host
+    rng = np.random.default_rng(7)
+    n = 400_000
+    cuts = rng.choice(np.array([0.002, 0.01, 0.03, 0.05, 0.1, 0.3, 1.2], np.float32), n)
+    a = rng.uniform(-np.pi, np.pi, n)
+    # angles: half spread around the cut, half within 1e-5 rad of +-cut
+    off = np.where(rng.random(n) < 0.5, rng.uniform(-0.5, 0.5, n),
+                   np.sign(rng.random(n) - 0.5) * (cuts + rng.uniform(-1e-5, 1e-5, n)))
+    r1 = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), n))
+    r2 = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), n))
+    ux, uy = (r1 * np.cos(a)).astype(np.float32), (r1 * np.sin(a)).astype(np.float32)
+    vx, vy = (r2 * np.cos(a + off)).astype(np.float32), (r2 * np.sin(a + off)).astype(np.float32)
+    so, ps = filter_phi_decide_host(ux, uy, vx, vy, cuts)
+    assert not np.any(so & ps)
+    # it decides almost everything clearly outside (by more than 1e-4 rad)
+    clear = np.abs(off) > cuts + 1e-4
+    assert so[clear].mean() > 0.999
+    # degenerate and extreme vectors go to the reference expression
+    special = np.array([0.0, -0.0, 1e-38, 1e-30, 1.0, 3e38, np.inf, -np.inf, np.nan], np.float32)
+    g = np.array(np.meshgrid(special, special, special, special)).reshape(4, -1)
+    c = np.full(g.shape[1], 0.05, np.float32)
+    so, ps = filter_phi_decide_host(g[0], g[1], g[2], g[3], c)
+    assert not np.any(so & ps)
+    with np.errstate(invalid="ignore"):
+        undecided = (~np.all(np.isfinite(g), 0) | (np.maximum(np.abs(g[0]), np.abs(g[1])) < 1e-30) |
+                     (np.maximum(np.abs(g[2]), np.abs(g[3])) < 1e-30))
+    assert not np.any(so & undecided)

@@ -60,6 +60,12 @@ class ServeStats(C.Structure):
                                           "split_ms")]
 
 
+class PhysicsRow(C.Structure):
+    _fields_ = [(n, U32) for n in ("event", "layer_triplet", "n_tracklets", "n_findable", "found", "fakes",
+                                   "clones", "misc")] + [(n, C.c_double) for n in ("efficiency", "fake_rate",
+                                                                                   "clone_rate")]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("cu_count", U32), ("lds_bytes_per_block", U32), ("max_block", U32),
                 ("wave_size", U32), ("global_mem_bytes", C.c_uint64)]
@@ -91,6 +97,7 @@ def lib():
         "trax_hits_create": (C.c_int, [U32, C.POINTER(P)]),
         "trax_hits_destroy": (None, [P]),
         "trax_hits_add_event": (C.c_int, [P, P, U32, P, P, P, P, P, P, P, C.c_int, C.c_int]),
+        "trax_hits_add_pb": (C.c_int, [P, P, C.c_char_p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(U32)]),
         "trax_hits_count": (U32, [P]),
         "trax_hits_events": (U32, [P]),
         "trax_hits_column": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64)]),
@@ -116,6 +123,7 @@ def lib():
         "trax_batch_rerun": (C.c_int, [P]),
         "trax_run_plain_bench": (C.c_int, [P, P, P, C.POINTER(C.c_double)]),
         "trax_batch_result": (C.c_int, [P, C.POINTER(P)]),
+        "trax_batch_result_select": (C.c_int, [P, C.c_char_p, C.POINTER(P)]),
         "trax_batch_counts": (C.c_int, [P, C.POINTER(U32)]),
         "trax_batch_footprint": (C.c_uint64, [P]),
         "trax_batch_stage_ms": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_float)]),
@@ -123,6 +131,8 @@ def lib():
         "trax_result_timing_ms": (C.c_double, [P, C.c_char_p]),
         "trax_result_destroy": (None, [P]),
         "trax_math_eval_host": (C.c_int, [C.c_int, P, P, P, C.c_uint64]),
+        "trax_filter_phi_decide_host": (C.c_int, [C.c_uint64, P, P, P, P, P, P, P]),
+        "trax_physics_eval": (C.c_int, [P, U32, U32, P, P, P, U32, P, P, P, P, C.c_int, C.POINTER(PhysicsRow)]),
         "trax_math_eval_device": (C.c_int, [P, C.c_int, P, P, P, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
@@ -187,6 +197,13 @@ class HitSet:
         a = [np.ascontiguousarray(v) for v in (ev.x, ev.y, ev.z, ev.layer, ev.det_id, ev.track_id, ev.track_pt)]
         check(lib().trax_hits_add_event(self.h, self.cfg.h, a[0].size, *[ptr(v) for v in a], max_tracks,
                                         int(only_tracks)))
+
+    def add_pb(self, data: bytes, max_events: int = -1, max_tracks: int = -1, only_tracks: bool = False) -> int:
+        """Add the events of a serialized PEventContainer; returns how many."""
+        n = U32()
+        check(lib().trax_hits_add_pb(self.h, self.cfg.h, data, len(data), max_events, max_tracks, int(only_tracks),
+                                     C.byref(n)))
+        return int(n.value)
 
     @property
     def n_hits(self) -> int:
@@ -371,9 +388,13 @@ class Batch:
         keys = ("pairs", "cands", "tracklets", "conn", "track_entries", "ca_iterations", "connectable", "err")
         return dict(zip(keys, list(a)))
 
-    def result(self) -> Dict[str, np.ndarray]:
+    def result(self, names: Optional[List[str]] = None) -> Dict[str, np.ndarray]:
+        """Result arrays of the last run (all, or only `names`)."""
         rh = P()
-        check(lib().trax_batch_result(self.h, C.byref(rh)))
+        if names is None:
+            check(lib().trax_batch_result(self.h, C.byref(rh)))
+        else:
+            check(lib().trax_batch_result_select(self.h, ",".join(names).encode(), C.byref(rh)))
         return _result_dict(rh)
 
     def stage_ms(self, stage: str) -> float:
@@ -494,3 +515,27 @@ def autotune(dev: Device, cfg: Config, hits: HitSet, caps: Caps, repeats: int = 
     check(lib().trax_autotune(dev.h, cfg.h, hits.h, C.byref(c), repeats, C.byref(t), buf, len(buf)))
     trials = [json.loads(l) for l in buf.value.decode().splitlines() if l.strip()]
     return {n: int(getattr(t, n)) for n, _ in Tuning._fields_}, trials
+
+
+def filter_phi_decide_host(ux, uy, vx, vy, cut):
+    """(surely_outside, passes) of the filter's phi cut, evaluated on the host."""
+    a = [np.ascontiguousarray(v, dtype=np.float32) for v in (ux, uy, vx, vy, cut)]
+    n = a[0].size
+    so, ps = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    check(lib().trax_filter_phi_decide_host(n, *[ptr(v) for v in a], ptr(so), ptr(ps)))
+    return so.astype(bool), ps.astype(bool)
+
+
+def physics_eval(cfg: Config, res: Dict[str, np.ndarray], n_events: int, only_tracks: bool = False):
+    """PhysicsRecord per (event, layer triplet) of a result (HIP or oracle):
+    list of dicts with found/findable/fakes/clones and the three rates."""
+    T = len(cfg.wl.triplets)
+    a = {k: np.ascontiguousarray(res[k], dtype=np.uint32) for k in
+         ("hits.hit_id", "hits.layer", "hits.event", "tracklets.h1", "tracklets.h2", "tracklets.h3",
+          "tracklet_offsets")}
+    rows = (PhysicsRow * (n_events * T))()
+    check(lib().trax_physics_eval(cfg.h, n_events, a["hits.hit_id"].size, ptr(a["hits.hit_id"]), ptr(a["hits.layer"]),
+                                  ptr(a["hits.event"]), a["tracklets.h1"].size, ptr(a["tracklets.h1"]),
+                                  ptr(a["tracklets.h2"]), ptr(a["tracklets.h3"]), ptr(a["tracklet_offsets"]),
+                                  int(only_tracks), rows))
+    return [{n: getattr(r, n) for n, _ in PhysicsRow._fields_} for r in rows]

@@ -25,6 +25,7 @@ SOURCES = [
     "capi.cpp",
     "serve.cpp",
     "tune.cpp",
+    "physics.cpp",
 ]
 
 FLAGS = [

@@ -39,9 +39,11 @@ BARREL_HALF_LENGTH_10 = [26.5, 26.5, 26.5, 70.0, 70.0, 70.0, 70.0, 110.0, 110.0,
 _XML_ENTRIES = [
     (0.01, 0.037, 0.075, 0.032, 0.5, 2, 10, 2, 4),
     (0.2, 0.207, 0.2, 0.102, 0.5, 2, 10, 2, 16),
-    (0.4, 0.207, 0.4, 0.102, 1.0, 3, 15, 2, 0),
-    (0.5, 0.207, 0.4, 0.102, 3.5, 4, 30, 2, 0),
+    (0.4, 0.207, 0.4, 0.102, 1.0, 3, 15, 2, 64),
+    (0.5, 0.207, 0.4, 0.102, 3.5, 4, 30, 2, 128),
 ]
+# the layers of the four entries in configs/layerTriplets.xml
+_XML_LAYERS = [(1, 2, 3), (2, 3, 4), (3, 4, 5), (4, 5, 8)]
 
 MIN_PT_GEV = 1.0
 B_FIELD_T = 3.8
@@ -89,6 +91,20 @@ class Workload:
     @property
     def n_layers(self) -> int:
         return len(self.radii)
+
+
+def ttbar_workload() -> Workload:
+    """The reference's ttbar reconstruction run on recorded simulation
+    (configs/ttBar.reco.conf): the four layer triplets of
+    configs/layerTriplets.xml with their cuts, CMS barrel layers 1-5 and 8,
+    a 50 z x 8 phi grid, minPt 1 GeV, only findable tracks (onlyTracks).  The
+    events come from a serialized PEventContainer, not from the generator;
+    `radii` only sizes the layer count (8)."""
+    trip = [TripletConfig(l1, l2, l3, e[0], e[1], e[2], e[3], e[4], e[5], float(e[6]), e[7], e[8])
+            for (l1, l2, l3), e in zip(_XML_LAYERS, _XML_ENTRIES)]
+    radii = [0.5 * (lo + hi) for _, lo, hi in LAYER_RADIUS_BRACKETS[:8]]
+    return Workload("ttbar-recorded-events", radii, [0.0] * 8, trip, 50, 8,
+                    extra=dict(only_tracks=True))
 
 
 def workload(cfg: int) -> Workload:

@@ -171,6 +171,9 @@ extern "C" int trax_hits_create(uint32_t max_layer, trax_hits** out) {
 
 extern "C" void trax_hits_destroy(trax_hits* h) { delete h; }
 
+static void add_event(trax_hits* h, const trax_config* cfg, const std::vector<PHit>& ev, int max_tracks,
+                      int only_tracks);
+
 extern "C" int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_t n, const float* x, const float* y,
                                    const float* z, const uint32_t* layer, const uint32_t* det_id,
                                    const uint32_t* track_id, const float* track_pt, int max_tracks, int only_tracks) {
@@ -179,6 +182,15 @@ extern "C" int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_
             return fail(TRAX_ERR_ARG, "configuration's highest layer differs from the collection's");
         std::vector<PHit> ev(n);
         for (uint32_t i = 0; i < n; ++i) ev[i] = PHit{x[i], y[i], z[i], layer[i], det_id[i], track_id[i], track_pt[i]};
+        add_event(h, cfg, ev, max_tracks, only_tracks);
+        return TRAX_OK;
+    });
+}
+
+// HitCollection::addEvent on the collection's growing supplements
+static void add_event(trax_hits* h, const trax_config* cfg, const std::vector<PHit>& ev, int max_tracks,
+                      int only_tracks) {
+    {
         const uint32_t e = h->n_events, L = h->max_layer;
         // the supplements grow by one event in place (moved in and out, not copied)
         EventSupplement es(0);
@@ -204,6 +216,23 @@ extern "C" int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_
         h->evt_nhits = std::move(es.nHits); h->evt_offset = std::move(es.offset);
         h->lay_nhits = std::move(ls.nHits); h->lay_offset = std::move(ls.offset);
         h->n_events = e + 1;
+    }
+}
+
+extern "C" int trax_hits_add_pb(trax_hits* h, const trax_config* cfg, const void* data, uint64_t size, int max_events,
+                                int max_tracks, int only_tracks, uint32_t* n_added) {
+    return guarded([&] {
+        if (!h || !cfg || (!data && size)) return fail(TRAX_ERR_ARG, "null argument");
+        if (cfg->trip.maxLayer() != h->max_layer)
+            return fail(TRAX_ERR_ARG, "configuration's highest layer differs from the collection's");
+        const std::vector<PEventData> evs = decodeEventContainer(static_cast<const uint8_t*>(data), size);
+        uint32_t added = 0;
+        for (const PEventData& ev : evs) {
+            if (max_events >= 0 && added >= (uint32_t)max_events) break;
+            add_event(h, cfg, ev.hits, max_tracks, only_tracks);
+            ++added;
+        }
+        if (n_added) *n_added = added;
         return TRAX_OK;
     });
 }
@@ -810,13 +839,16 @@ extern "C" int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms) 
     });
 }
 
-extern "C" int trax_batch_result(trax_batch* b, trax_result** out) {
+// names: comma-separated result arrays to download (null: all of them)
+static int batch_result_impl(trax_batch* b, const char* names, trax_result** out) {
     return guarded([&] {
         if (!b->have_counts) return fail(TRAX_ERR_STATE, "no completed run");
         const DevCounts& c = *b->h_counts;
         auto* r = new trax_result;
         std::unique_ptr<trax_result> guard(r);
         const uint32_t E = b->n_events, N = b->n_hits, T = b->T;
+        const std::string sel = names ? std::string(",") + names + "," : std::string();
+        auto want = [&](const char* k) { return !names || sel.find(std::string(",") + k + ",") != std::string::npos; };
         auto dl = [&](const void* src, uint64_t n) {
             std::vector<uint32_t> v(n);
             if (n) hip_check(hipMemcpy(v.data(), src, n * 4, hipMemcpyDeviceToHost), "d2h");
@@ -829,34 +861,41 @@ extern "C" int trax_batch_result(trax_batch* b, trax_result** out) {
         };
         const uint32_t P = std::min(c.n_pairs, b->caps.max_pairs), Q = std::min(c.n_cands, b->caps.max_cands),
                        R = std::min(c.n_tracklets, b->caps.max_tracklets), K = std::min(c.n_conn, b->caps.max_conn);
-        r->put("grid", dl(b->grid, (uint64_t)E * b->L * b->C));
-        r->putf("hits.x", dlf(b->hout.x, N)); r->putf("hits.y", dlf(b->hout.y, N)); r->putf("hits.z", dlf(b->hout.z, N));
-        r->put("hits.layer", dl(b->hout.layer, N)); r->put("hits.det_id", dl(b->hout.det, N));
-        r->put("hits.hit_id", dl(b->hout.id, N)); r->put("hits.event", dl(b->hout.evt, N));
-        r->put("pairs", dl(b->pairs, 2ull * P));
-        r->put("pair_offsets", dl(b->pair_off, (uint64_t)E * T + 1));
-        r->put("cands", dl(b->cands, 2ull * Q));
-        r->put("cand_offsets", dl(b->cand_off, (uint64_t)E * T + 1));
-        r->put("tracklets.h1", dl(b->th1, R)); r->put("tracklets.h2", dl(b->th2, R)); r->put("tracklets.h3", dl(b->th3, R));
-        r->put("tracklet_offsets", dl(b->t_off, (uint64_t)E * T + 1));
-        r->putf("eta", dlf(b->lb.eta, R));
-        r->put("conn.basis", dl(b->lb.basis, K));
-        r->put("conn.followers", dl(b->lb.foll, K));
-        r->put("conn.event", dl(b->lb.pev, K));
-        r->put("conn.connectable", dl(b->lb.connectable, K ? std::min(c.n_connectable, R) : 0));
-        r->put("conn.nonconnectable", dl(b->lb.nonconnectable, K ? std::min(c.n_nonconnectable, R) : 0));
+        if (want("grid")) r->put("grid", dl(b->grid, (uint64_t)E * b->L * b->C));
+        if (want("hits.x")) r->putf("hits.x", dlf(b->hout.x, N)); if (want("hits.y")) r->putf("hits.y", dlf(b->hout.y, N)); if (want("hits.z")) r->putf("hits.z", dlf(b->hout.z, N));
+        if (want("hits.layer")) r->put("hits.layer", dl(b->hout.layer, N)); if (want("hits.det_id")) r->put("hits.det_id", dl(b->hout.det, N));
+        if (want("hits.hit_id")) r->put("hits.hit_id", dl(b->hout.id, N)); if (want("hits.event")) r->put("hits.event", dl(b->hout.evt, N));
+        if (want("pairs")) r->put("pairs", dl(b->pairs, 2ull * P));
+        if (want("pair_offsets")) r->put("pair_offsets", dl(b->pair_off, (uint64_t)E * T + 1));
+        if (want("cands")) r->put("cands", dl(b->cands, 2ull * Q));
+        if (want("cand_offsets")) r->put("cand_offsets", dl(b->cand_off, (uint64_t)E * T + 1));
+        if (want("tracklets.h1")) r->put("tracklets.h1", dl(b->th1, R)); if (want("tracklets.h2")) r->put("tracklets.h2", dl(b->th2, R)); if (want("tracklets.h3")) r->put("tracklets.h3", dl(b->th3, R));
+        if (want("tracklet_offsets")) r->put("tracklet_offsets", dl(b->t_off, (uint64_t)E * T + 1));
+        if (want("eta")) r->putf("eta", dlf(b->lb.eta, R));
+        if (want("conn.basis")) r->put("conn.basis", dl(b->lb.basis, K));
+        if (want("conn.followers")) r->put("conn.followers", dl(b->lb.foll, K));
+        if (want("conn.event")) r->put("conn.event", dl(b->lb.pev, K));
+        if (want("conn.connectable")) r->put("conn.connectable", dl(b->lb.connectable, K ? std::min(c.n_connectable, R) : 0));
+        if (want("conn.nonconnectable")) r->put("conn.nonconnectable", dl(b->lb.nonconnectable, K ? std::min(c.n_nonconnectable, R) : 0));
         if (K) {
-            r->putf("pt", dlf(b->lb.pt, R));
-            r->put("ca.states", dl(b->cb.st_a, R));
-            r->put("ca.best_basis", dl(b->cb.best, R));
-            r->put("tracks", dl(b->cb.tracks, std::min(c.n_track_entries, b->caps.max_track_entries)));
-            r->put("track_offsets", dl(b->cb.handler, R + 1ull));
-            r->put("ca.iterations", {c.ca_iterations});
+            if (want("pt")) r->putf("pt", dlf(b->lb.pt, R));
+            if (want("ca.states")) r->put("ca.states", dl(b->cb.st_a, R));
+            if (want("ca.best_basis")) r->put("ca.best_basis", dl(b->cb.best, R));
+            if (want("tracks")) r->put("tracks", dl(b->cb.tracks, std::min(c.n_track_entries, b->caps.max_track_entries)));
+            if (want("track_offsets")) r->put("track_offsets", dl(b->cb.handler, R + 1ull));
+            if (want("ca.iterations")) r->put("ca.iterations", {c.ca_iterations});
         }
-        r->put("diag", {c.diag_pair_short, c.diag_pred_short, c.diag_pred_trunc, c.diag_wrap_overlap});
+        if (want("diag")) r->put("diag", {c.diag_pair_short, c.diag_pred_short, c.diag_pred_trunc, c.diag_wrap_overlap});
         *out = guard.release();
         return TRAX_OK;
     });
+}
+
+extern "C" int trax_batch_result(trax_batch* b, trax_result** out) { return batch_result_impl(b, nullptr, out); }
+
+extern "C" int trax_batch_result_select(trax_batch* b, const char* names, trax_result** out) {
+    if (!names) return fail(TRAX_ERR_ARG, "null names");
+    return batch_result_impl(b, names, out);
 }
 
 // ---- results --------------------------------------------------------------------
@@ -903,6 +942,22 @@ __global__ void k_math(int fn, const float* x, const float* y, float* out, uint6
 extern "C" int trax_math_eval_host(int fn, const float* x, const float* y, float* out, uint64_t n) {
     if (fn < 0 || fn > 3) return fail(TRAX_ERR_ARG, "unknown function");
     for (uint64_t i = 0; i < n; ++i) out[i] = math_fn(fn, x[i], y ? y[i] : 0.0f);
+    return TRAX_OK;
+}
+
+extern "C" int trax_filter_phi_decide_host(uint64_t n, const float* ux, const float* uy, const float* vx,
+                                           const float* vy, const float* cut, uint8_t* surely_outside,
+                                           uint8_t* passes) {
+    if (!ux || !uy || !vx || !vy || !cut || !surely_outside || !passes) return fail(TRAX_ERR_ARG, "null argument");
+    const float PI = TX_PI_F;
+    for (uint64_t i = 0; i < n; ++i) {
+        surely_outside[i] = tx_phi_surely_outside(ux[i], uy[i], vx[i], vy[i], tx_phi_prefilter_tan(cut[i])) ? 1 : 0;
+        const float angle1 = tx_atan2f(uy[i], ux[i]);  // TripletThetaPhiFilter.h:95-103
+        const float angle2 = tx_atan2f(vy[i], vx[i]);
+        float delta = angle2 - angle1;
+        delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
+        passes[i] = fabsf(delta) <= cut[i] ? 1 : 0;
+    }
     return TRAX_OK;
 }
 

@@ -104,6 +104,110 @@ uint TripletConfigurations::maxLayer() const {
     return m;
 }
 
+// ---- protocol-buffer wire format (PEventContainer, Event.proto:1-37) ------------
+namespace {
+
+struct WireReader {
+    const uint8_t* p;
+    const uint8_t* end;
+    bool done() const { return p >= end; }
+    uint64_t varint() {
+        uint64_t v = 0;
+        for (int shift = 0; shift < 64; shift += 7) {
+            if (p >= end) throw InputError("truncated varint");
+            const uint8_t b = *p++;
+            v |= (uint64_t)(b & 0x7f) << shift;
+            if (b < 0x80) return v;
+        }
+        throw InputError("varint longer than 10 bytes");
+    }
+    float fixed32f() {
+        if (end - p < 4) throw InputError("truncated fixed32");
+        uint32_t u = (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+        p += 4;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    }
+    WireReader sub() {  // a length-delimited field's payload
+        const uint64_t n = varint();
+        if ((uint64_t)(end - p) < n) throw InputError("truncated length-delimited field");
+        WireReader r{p, p + n};
+        p += n;
+        return r;
+    }
+    void skip(uint32_t wire) {
+        switch (wire) {
+            case 0: (void)varint(); break;
+            case 1: if (end - p < 8) throw InputError("truncated fixed64"); p += 8; break;
+            case 2: (void)sub(); break;
+            case 5: if (end - p < 4) throw InputError("truncated fixed32"); p += 4; break;
+            default: throw InputError("unsupported wire type");
+        }
+    }
+};
+
+// PHit: 1 position (PGlobalPoint: 1 x, 2 y, 3 z as float), 2 detectorType,
+// 3 layer, 4 detectorId, 5 hitId, 6 simTrackID, 7 simTrackPt
+PHit decodeHit(WireReader r) {
+    PHit h{0.0f, 0.0f, 0.0f, 0u, 0u, 0u, 0.0f};
+    while (!r.done()) {
+        const uint64_t key = r.varint();
+        const uint32_t field = (uint32_t)(key >> 3), wire = (uint32_t)(key & 7);
+        if (field == 1 && wire == 2) {
+            WireReader q = r.sub();
+            while (!q.done()) {
+                const uint64_t k = q.varint();
+                const uint32_t f = (uint32_t)(k >> 3), w = (uint32_t)(k & 7);
+                if (w == 5 && f >= 1 && f <= 3) {
+                    const float v = q.fixed32f();
+                    (f == 1 ? h.x : f == 2 ? h.y : h.z) = v;
+                } else {
+                    q.skip(w);
+                }
+            }
+        } else if (field == 3 && wire == 0) {
+            h.layer = (uint32_t)r.varint();
+        } else if (field == 4 && wire == 0) {
+            h.detectorId = (uint32_t)r.varint();
+        } else if (field == 6 && wire == 0) {
+            h.simTrackId = (uint32_t)r.varint();
+        } else if (field == 7 && wire == 5) {
+            h.simTrackPt = r.fixed32f();
+        } else {
+            r.skip(wire);
+        }
+    }
+    return h;
+}
+
+}  // namespace
+
+std::vector<PEventData> decodeEventContainer(const uint8_t* data, size_t size) {
+    std::vector<PEventData> out;
+    WireReader r{data, data + size};
+    while (!r.done()) {
+        const uint64_t key = r.varint();
+        if ((key >> 3) != 1 || (key & 7) != 2) {
+            r.skip((uint32_t)(key & 7));
+            continue;
+        }
+        WireReader e = r.sub();
+        PEventData ev;
+        while (!e.done()) {
+            const uint64_t k = e.varint();
+            const uint32_t f = (uint32_t)(k >> 3), w = (uint32_t)(k & 7);
+            if (f == 4 && w == 2) ev.hits.push_back(decodeHit(e.sub()));
+            else if (f == 1 && w == 0) ev.runNumber = e.varint();
+            else if (f == 2 && w == 0) ev.lumiSection = e.varint();
+            else if (f == 3 && w == 0) ev.eventNumber = e.varint();
+            else e.skip(w);
+        }
+        out.push_back(std::move(ev));
+    }
+    return out;
+}
+
 HitCollection::tTrackList HitCollection::addEvent(const std::vector<PHit>& ev, EventSupplement& eventSupplement,
                                                   uint evtInGroup, LayerSupplement& layerSupplement,
                                                   const TripletConfigurations& layerTriplets, int numTracks,
@@ -182,7 +286,8 @@ GridGeom GridConfig::geom() const {
 ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const GeometrySupplement& g,
                            uint nThreads, float dEtaCut) {
     const uint T = t.size(), G = g.size();
-    const size_t words = 10 * (size_t)T + 4 * (size_t)G;
+    const size_t dbl_at = (10 * (size_t)T + 4 * (size_t)G + 1) & ~(size_t)1;  // 8-byte aligned doubles
+    const size_t words = dbl_at + 2 * (size_t)T;
     std::vector<uint32_t> host(words);
     auto put_u = [&](size_t at, const std::vector<uint>& v) { std::memcpy(&host[at], v.data(), v.size() * 4); };
     auto put_f = [&](size_t at, const std::vector<float>& v) { std::memcpy(&host[at], v.data(), v.size() * 4); };
@@ -197,6 +302,9 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
         acos_max[l] = tx_acosf(g.maxRadius[l] / (2 * mrc));
     }
     put_f(10 * T + 2 * G, acos_min); put_f(10 * T + 3 * G, acos_max);
+    std::vector<double> tan_dphi(T);
+    for (uint i = 0; i < T; ++i) tan_dphi[i] = tx_phi_prefilter_tan(t.dPhiCut[i]);
+    if (T) std::memcpy(&host[dbl_at], tan_dphi.data(), T * sizeof(double));
     blob_.reset(new DeviceVector<uint8_t>(ctx, words * 4 + 4));
     hip_check(hipMemcpyAsync(blob_->get_mem(), host.data(), words * 4, hipMemcpyHostToDevice, ctx.stream()), "h2d cfg");
     ctx.finish_default_queue();
@@ -207,6 +315,7 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
     cols_.tip = f + 7 * T; cols_.z0 = f + 8 * T; cols_.pair_spread_phi = d + 9 * T;
     cols_.min_r = f + 10 * T; cols_.max_r = f + 10 * T + G;
     cols_.acos_min_r = f + 10 * T + 2 * G; cols_.acos_max_r = f + 10 * T + 3 * G;
+    cols_.tan_dphi = reinterpret_cast<const double*>(d + dbl_at);
     cols_.n_trip = T;
     cols_.n_geom = G;
     cols_.min_radius_curvature = t.minRadiusCurvature();

@@ -120,6 +120,18 @@ struct PHit {  // PB_Event::PHit (src/datastructures/serialize/Event.proto)
     float simTrackPt;
 };
 
+// One decoded PEvent of a PEventContainer (Event.proto:1-37): its header
+// numbers and hits.
+struct PEventData {
+    uint64_t runNumber = 0, lumiSection = 0, eventNumber = 0;
+    std::vector<PHit> hits;
+};
+// Decodes a serialized PB_Event::PEventContainer (the input the reference's
+// EventLoader parses, src/triplet_builder/EventLoader.h:15-67) from the
+// protocol-buffer wire format, without libprotobuf.  Throws InputError on
+// malformed input.
+std::vector<PEventData> decodeEventContainer(const uint8_t* data, size_t size);
+
 // TripletConfigurations (src/datastructures/TripletConfiguration.h:39-72)
 class TripletConfigurations {
 public:

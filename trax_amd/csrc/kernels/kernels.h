@@ -51,6 +51,9 @@ struct TripletCols {
     // radius: the per-layer terms of the pair and predictor phi windows,
     // computed once on the host with the same built-in
     const float* acos_min_r; const float* acos_max_r;
+    // per layer triplet: tan(dPhiCut + 1e-5) of the filter's phi-cut prefilter
+    // (tx_phi_surely_outside), non-positive when disabled
+    const double* tan_dphi;
     uint32_t n_trip;
     uint32_t n_geom;
     float min_radius_curvature;

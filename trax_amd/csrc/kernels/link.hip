@@ -33,7 +33,9 @@ __device__ inline bool filter_valid(const TripletCols& tc, const float* hx, cons
     const float x2 = hx[h2], y2 = hy[h2], z2 = hz[h2];
     const float x3 = hx[h3], y3 = hy[h3], z3 = hz[h3];
     // the four cuts are a conjunction of pure expressions: a candidate is
-    // rejected at the first cut it fails (the result is the reference's)
+    // rejected at the first cut it fails (the result is the reference's);
+    // most fail the phi cut by far, which the prefilter proves without atan2
+    if (tx_phi_surely_outside(x2 - x1, y2 - y1, x3 - x2, y3 - y2, tc.tan_dphi[lt])) return false;
     float angle1 = tx_atan2f((y2 - y1), (x2 - x1));
     float angle2 = tx_atan2f((y3 - y2), (x3 - x2));
     float delta = angle2 - angle1;

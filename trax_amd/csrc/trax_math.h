@@ -185,3 +185,35 @@ TX_HD float tx_asinhf(float x) {
     }
     return tx_copysign(r, x);
 }
+
+// ---- filter phi-cut prefilter ------------------------------------------------
+// The filter's phi cut (TripletThetaPhiFilter.h:95-103) accepts a candidate
+// when |wrap(atan2(v) - atan2(u))| <= dPhiCut for the float hit differences
+// u = p2 - p1, v = p3 - p2.  That float expression differs from the exact
+// angle from u to v by less than 2e-6 rad (two atan2 of <= 2 ulp of pi, the
+// subtraction and the wrap by 2 pi).  In double the cross and dot products of
+// u and v are exact up to one rounding each, so
+//     d <= 0  or  |c| > tan(dPhiCut + 1e-5) * d
+// proves the exact angle exceeds the cut by more than 1e-5 rad, and with it
+// that the reference's expression rejects the candidate.  Candidates it does
+// not decide (near the cut, tiny or non-finite vectors, wide cuts) go
+// through the reference's expression; the decision is the same either way.
+// tan_cut: tan(dPhiCut + 1e-5) in double, or a non-positive value to disable
+// (cuts with dPhiCut + 1e-5 >= 1.5 rad).
+TX_HD bool tx_phi_surely_outside(float ux, float uy, float vx, float vy, double tan_cut) {
+    if (!(tan_cut > 0.0)) return false;
+    const float big = 3.0e38f;
+    if (!(fabsf(ux) < big && fabsf(uy) < big && fabsf(vx) < big && fabsf(vy) < big)) return false;
+    // zero, denormal or tiny vectors: left to the reference expression
+    const float tiny = 1.0e-30f;
+    if (fmaxf(fabsf(ux), fabsf(uy)) < tiny || fmaxf(fabsf(vx), fabsf(vy)) < tiny) return false;
+    const double c = (double)ux * (double)vy - (double)uy * (double)vx;
+    const double d = (double)ux * (double)vx + (double)uy * (double)vy;
+    return d <= 0.0 || fabs(c) > tan_cut * d;
+}
+
+// host-side value of tan_cut for a phi cut (see tx_phi_surely_outside)
+static inline double tx_phi_prefilter_tan(float d_phi_cut) {
+    const double a = (double)d_phi_cut + 1e-5;
+    return (a > 0.0 && a < 1.5) ? tan(a) : -1.0;
+}
