@@ -176,13 +176,15 @@ typedef struct {
     uint32_t per_event;
 } trax_batch_caps;
 /* launch shapes (0 = default): block sizes per kernel family, blocks per CU
- * of the grid-stride kernels, and the fused predictor's variant (block 128,
- * 256 or 512 with 16, 24 or 32 LDS list entries per pair); results do not
- * depend on them */
+ * of the grid-stride kernels, and the fused predictor's variant: (block,
+ * LDS list entries per pair, walk order) one of (128, 32|64), (256,
+ * 24|32|48|64), (512, 24|32) in pair order or (256, 32) longest walk first;
+ * 0 = (256, 24, pair order).  Results do not depend on them. */
 typedef struct {
     uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
     uint32_t blocks_per_cu;
     uint32_t predict_list;
+    uint32_t predict_sort;
 } trax_tuning;
 /* device and pinned host bytes a batch of these capacities would hold, without
  * allocating (the planning step of buildTriplets' per-group buffer sizing,

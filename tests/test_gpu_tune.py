@@ -9,7 +9,8 @@ from trax_amd.api import Batch, Caps, autotune, run_batch_fit
 
 pytestmark = pytest.mark.gpu
 
-PREDICT_VARIANTS = [(128, 32), (256, 16), (256, 24), (256, 32), (512, 16), (512, 24)]
+PREDICT_VARIANTS = [(128, 32, 0), (128, 64, 0), (256, 24, 0), (256, 32, 0), (256, 48, 0), (256, 64, 0),
+                    (512, 24, 0), (512, 32, 0), (256, 32, 1)]
 
 
 def _case():
@@ -17,12 +18,12 @@ def _case():
     return (wl, *hitset(wl, seed=11))
 
 
-@pytest.mark.parametrize("block,lst", PREDICT_VARIANTS)
-def test_predictor_variants_match_oracle(dev, block, lst):
+@pytest.mark.parametrize("block,lst,srt", PREDICT_VARIANTS)
+def test_predictor_variants_match_oracle(dev, block, lst, srt):
     wl, cfg, hs, _ = _case()
     want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
     b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits),
-                           tuning=dict(predict_block=block, predict_list=lst))
+                           tuning=dict(predict_block=block, predict_list=lst, predict_sort=srt))
     b.close()
     bad = compare(got, want)
     assert not bad, "\n".join(bad)
@@ -48,9 +49,9 @@ def test_autotune_picks_supported_shapes_with_same_results(dev):
     caps = b.caps
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
-    assert len(trials) == 6 + 5 + 4 + 4 + 4 + 4 + 4
+    assert len(trials) == 9 + 5 + 4 + 4 + 4 + 4 + 4
     assert all(t["same_results"] for t in trials)
-    assert (tuning["predict_block"], tuning["predict_list"]) in PREDICT_VARIANTS
+    assert (tuning["predict_block"], tuning["predict_list"], tuning["predict_sort"]) in PREDICT_VARIANTS
     assert tuning["blocks_per_cu"] in (2, 4, 8, 16)
     want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
     tb = Batch(dev, cfg, caps, tuning)

@@ -686,9 +686,10 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
             b->lc.link_block = tuning->link_block;
             b->lc.blocks_per_cu = tuning->blocks_per_cu;
             b->lc.predict_list = tuning->predict_list;
+            b->lc.predict_sort = tuning->predict_sort;
         }
-        if (!predict_variant_supported(b->lc.predict_block, b->lc.predict_list))
-            throw InputError("unsupported fused predictor variant (block, list)");
+        if (!predict_variant_supported(b->lc.predict_block, b->lc.predict_list, b->lc.predict_sort))
+            throw InputError("unsupported fused predictor variant (block, list, sort)");
         for (uint32_t v : {b->lc.grid_count_block, b->lc.grid_store_block, b->lc.pair_block, b->lc.filter_block,
                            b->lc.link_block})
             if (v % 64 != 0 || v > 1024) throw InputError("block sizes must be multiples of 64 up to 1024");

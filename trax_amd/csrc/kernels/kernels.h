@@ -38,6 +38,7 @@ struct LaunchCfg {
     uint32_t link_block = 0;
     uint32_t blocks_per_cu = 0;  // grid-stride kernels: blocks per CU (0 = 8)
     uint32_t predict_list = 0;   // fused predictor: LDS list entries per pair (0 = 24)
+    uint32_t predict_sort = 0;   // fused predictor: walk a tile's pairs longest first
 };
 
 // layer-triplet configuration columns on the device (TripletConfigurations,
@@ -116,7 +117,7 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
 // uses predict_fused_tiles(n_pair_cap) look-back words of the scan scratch
 uint32_t predict_fused_tiles(uint64_t n_pair_cap);
 // fused predictor variants (block size, LDS list entries per pair)
-bool predict_variant_supported(uint32_t block, uint32_t list);
+bool predict_variant_supported(uint32_t block, uint32_t list, uint32_t sort);
 // resolves the variant's resident-block count (call outside graph capture)
 void predict_fused_prepare(const LaunchCfg& lc);
 hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,

@@ -541,6 +541,72 @@ __device__ inline void pred_walk2(const PredWin2& w, uint32_t np1, const float2*
     }
 }
 
+// The merged walk of both windows as one flat loop per lane: each iteration
+// takes four hits of the current z sector (loads always issued, from a valid
+// index, and masked), evaluates both windows with branch-free predicates,
+// and moves to the next sector of either window when the current one ends.
+// countA counts window A's matches; window B's matches go to `my` (16-bit
+// offsets from `first`, at most cap of them) in the reference's visiting
+// order, and nB counts them all.
+__device__ inline void pred_walk_flat(const PredWin2& w, uint32_t np1, const float2* __restrict__ hzp, uint32_t cap,
+                                      uint32_t first, uint16_t* my, uint32_t& countA, uint32_t& nB) {
+    const bool ea = w.zLoA < w.zHiA, eb = w.zLoB < w.zHiB;
+    uint32_t z0 = 0, z1 = 0;
+    if (ea || eb) {
+        z0 = !ea ? w.zLoB : (!eb ? w.zLoA : (w.zLoA < w.zLoB ? w.zLoA : w.zLoB));
+        z1 = !ea ? w.zHiB : (!eb ? w.zHiA : (w.zHiA > w.zHiB ? w.zHiA : w.zHiB));
+    }
+    const uint32_t* lg3 = w.lg3;
+    const float PI = TX_PI_F;
+    const uint32_t wr = (uint32_t)w.wrap;
+    uint32_t zs = z0, j = 0, end2 = 0, zEnd = 0, zLen = 0;
+    bool sa = false, sb = false;
+    // the next sector at or after zs that either window walks and that is not empty
+    auto next_sector = [&]() {
+        for (; zs < z1; ++zs) {
+            sa = ea && zs >= w.zLoA && zs < w.zHiA;
+            sb = eb && zs >= w.zLoB && zs < w.zHiB;
+            if (!(sa || sb)) continue;
+            const uint32_t zStart = lg3[zs * np1];
+            zEnd = lg3[(zs + 1) * np1];
+            zLen = zEnd - zStart;
+            end2 = wr * zEnd + lg3[zs * np1 + w.pHi] - wr * zStart;
+            j = lg3[zs * np1 + w.pLo];
+            if (j < end2) return true;
+        }
+        return false;
+    };
+    bool active = next_sector();
+    while (active) {
+        float2 q[4];
+        uint32_t idx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t jj = j + u < end2 ? j + u : j;  // j < end2: always a hit of the sector
+            idx[u] = jj - (uint32_t)(jj >= zEnd) * zLen;
+            q[u] = hzp[idx[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float z = q[u].x, ph = q[u].y;
+            const bool inr = j + u < end2;
+            const bool p1 = (w.phiLow <= ph) & (ph <= w.phiHigh);
+            const bool p2 = ((w.phiLow <= ph) & (ph <= PI)) | ((-PI <= ph) & (ph <= w.phiHigh));
+            const bool pin = inr & ((w.wrap & p2) | (!w.wrap & p1));
+            countA += (uint32_t)(pin & sa & (w.zLowA <= z) & (z <= w.zHighA));
+            if (pin & sb & (w.zLowB <= z) & (z <= w.zHighB)) {
+                if (nB < cap) my[nB] = (uint16_t)(idx[u] - first);
+                nB++;
+            }
+        }
+        j += 4;
+        if (j >= end2) {
+            ++zs;
+            active = next_sector();
+        }
+    }
+}
+
 // Count, scan and store of the predictor in one pass over tiles of
 // kPredBlock pairs (batched path).  One merged walk counts the count-window
 // matches and collects the store-window matches of every pair in LDS
@@ -548,19 +614,70 @@ __device__ inline void pred_walk2(const PredWin2& w, uint32_t np1, const float2*
 // with more re-walks its window for the rest), the tile's output offset comes
 // from a decoupled look-back, and the tile's candidates are written as one
 // contiguous, coalesced run by all threads.
-template <int kPredBlock, int kPredList>
+// Windows of a tile kept in LDS while the pairs are walked in another order.
+template <int B>
+struct TileWindows {
+    float zLowA[B], zHighA[B], zLowB[B], zHighB[B], phiLow[B], phiHigh[B];
+    uint32_t lg3[B];              // cell offset of the third layer's grid; kSentinel: no window
+    uint32_t zA[B], zB[B], pr[B];  // zLo | zHi << 16 for A and B; pLo | pHi << 16 | wrap << 31
+    uint32_t work[B];
+};
+
+template <int B>
+__device__ inline void tile_store(TileWindows<B>& t, uint32_t i, const PredWin2& w, const uint32_t* grid) {
+    t.zLowA[i] = w.zLowA; t.zHighA[i] = w.zHighA; t.zLowB[i] = w.zLowB; t.zHighB[i] = w.zHighB;
+    t.phiLow[i] = w.phiLow; t.phiHigh[i] = w.phiHigh;
+    t.lg3[i] = (uint32_t)(w.lg3 - grid);
+    t.zA[i] = w.zLoA | w.zHiA << 16;
+    t.zB[i] = w.zLoB | w.zHiB << 16;
+    t.pr[i] = w.pLo | w.pHi << 16 | (uint32_t)w.wrap << 31;
+}
+
+template <int B>
+__device__ inline void tile_load(const TileWindows<B>& t, uint32_t i, PredWin2& w, const uint32_t* grid) {
+    w.zLowA = t.zLowA[i]; w.zHighA = t.zHighA[i]; w.zLowB = t.zLowB[i]; w.zHighB = t.zHighB[i];
+    w.phiLow = t.phiLow[i]; w.phiHigh = t.phiHigh[i];
+    w.lg3 = grid + t.lg3[i];
+    w.zLoA = t.zA[i] & 0xffffu; w.zHiA = t.zA[i] >> 16;
+    w.zLoB = t.zB[i] & 0xffffu; w.zHiB = t.zB[i] >> 16;
+    w.pLo = t.pr[i] & 0xffffu; w.pHi = (t.pr[i] >> 16) & 0x7fffu; w.wrap = (t.pr[i] >> 31) != 0u;
+}
+
+// hits a window walk visits (both windows' z sectors)
+__device__ inline uint32_t walk_work(const PredWin2& w, uint32_t np1) {
+    const bool ea = w.zLoA < w.zHiA, eb = w.zLoB < w.zHiB;
+    if (!ea && !eb) return 0;
+    const uint32_t z0 = !ea ? w.zLoB : (!eb ? w.zLoA : (w.zLoA < w.zLoB ? w.zLoA : w.zLoB));
+    const uint32_t z1 = !ea ? w.zHiB : (!eb ? w.zHiA : (w.zHiA > w.zHiB ? w.zHiA : w.zHiB));
+    uint32_t work = 0;
+    for (uint32_t zs = z0; zs < z1; ++zs) {
+        const uint32_t zStart = w.lg3[zs * np1], zEnd = w.lg3[(zs + 1) * np1];
+        const uint32_t end2 = (uint32_t)w.wrap * zEnd + w.lg3[zs * np1 + w.pHi] - (uint32_t)w.wrap * zStart;
+        const uint32_t j0 = w.lg3[zs * np1 + w.pLo];
+        work += end2 > j0 ? end2 - j0 : 0u;
+    }
+    return work;
+}
+
+// kSort: the tile's pairs are walked in decreasing order of their walk
+// length (16 logarithmic buckets), so that the lanes of a wave walk windows
+// of similar length; each pair's results stay indexed by the pair.
+template <int kPredBlock, int kPredList, bool kSort>
 __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
     GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
     const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
     DevCounts* __restrict__ cnt, uint64_t* __restrict__ status, uint32_t* __restrict__ counter) {
+    constexpr int kSortB = kSort ? kPredBlock : 1;
     __shared__ uint16_t s_list[kPredBlock * kPredList];
     __shared__ uint32_t s_excl[kPredBlock + 1];
     __shared__ uint32_t s_nb[kPredBlock];
     __shared__ uint32_t s_b3[kPredBlock];  // first hit of the pair's third layer; kSentinel: no list
     __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
     __shared__ uint32_t s_tile, s_base;
+    __shared__ TileWindows<kSortB> s_win;
+    __shared__ uint32_t s_cnt[kSortB], s_perm[kSortB], s_hist[16], s_cur[16];
     const uint32_t tid = threadIdx.x;
     const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
     const uint32_t n_tiles = (n + kPredBlock - 1) / kPredBlock;
@@ -569,6 +686,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
     bool over = false;
     while (true) {
         if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        if (kSort && tid < 16) s_hist[tid] = 0;
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
@@ -576,26 +694,62 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
         uint32_t countA = 0, nB = 0, b3 = kSentinel;
         uint32_t h1 = kSentinel;
         PredWin2 pw;
+        bool have = false;
         if (gid < n) {
             h1 = pairs[2 * (size_t)gid];
             const uint32_t h2 = pairs[2 * (size_t)gid + 1];
-            if (h1 != kSentinel && h2 != kSentinel &&
-                pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw)) {
-                const uint32_t first = pw.lg3[0];
-                // the list holds 16-bit offsets from the layer's first hit
-                const bool listed = pw.lg3[g.cells - 1] - first <= 0xffffu;
-                b3 = listed ? first : kSentinel;
-                const uint32_t cap = listed ? (uint32_t)kPredList : 0u;
-                uint16_t* my = s_list + tid * kPredList;
-                pred_walk2(pw, np1, hzp, [&](uint32_t index, float2 q, bool sa, bool sb) {
-                    if (!phi_in(pw, q.y)) return;
-                    countA += sa && pw.zLowA <= q.x && q.x <= pw.zHighA;
-                    if (sb && pw.zLowB <= q.x && q.x <= pw.zHighB) {
-                        if (nB < cap) my[nB] = (uint16_t)(index - first);
-                        nB++;
-                    }
-                });
+            have = h1 != kSentinel && h2 != kSentinel &&
+                   pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw);
+        }
+        // walk one pair's windows: count window A, list window B (by pair index)
+        auto walk = [&](uint32_t pidx, const PredWin2& w, uint32_t& cA, uint32_t& nb, uint32_t& lb3) {
+            const uint32_t first = w.lg3[0];
+            // the list holds 16-bit offsets from the layer's first hit
+            const bool listed = w.lg3[g.cells - 1] - first <= 0xffffu;
+            lb3 = listed ? first : kSentinel;
+            const uint32_t cap = listed ? (uint32_t)kPredList : 0u;
+            uint16_t* my = s_list + pidx * kPredList;
+            pred_walk_flat(w, np1, hzp, cap, first, my, cA, nb);
+        };
+        if (!kSort) {
+            if (have) walk(tid, pw, countA, nB, b3);
+        } else {
+            // bucket the pairs by walk length, longest first
+            uint32_t bucket = 0;
+            if (have) {
+                tile_store(s_win, tid, pw, grid);
+                const uint32_t work = walk_work(pw, np1);
+                bucket = work ? 31u - (uint32_t)__clz(work) + 1u : 0u;
+                bucket = bucket > 15u ? 15u : bucket;
+            } else {
+                s_win.lg3[tid] = kSentinel;
             }
+            atomicAdd(&s_hist[15 - bucket], 1u);
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t run = 0;
+                for (int b = 0; b < 16; ++b) {
+                    s_cur[b] = run;
+                    run += s_hist[b];
+                }
+            }
+            __syncthreads();
+            s_perm[atomicAdd(&s_cur[15 - bucket], 1u)] = tid;
+            __syncthreads();
+            const uint32_t q = s_perm[tid];
+            uint32_t cA = 0, nb = 0, lb3 = kSentinel;
+            if (s_win.lg3[q] != kSentinel) {
+                PredWin2 w;
+                tile_load(s_win, q, w, grid);
+                walk(q, w, cA, nb, lb3);
+            }
+            s_cnt[q] = cA;
+            s_nb[q] = nb;
+            s_b3[q] = lb3;
+            __syncthreads();
+            countA = s_cnt[tid];
+            nB = s_nb[tid];
+            b3 = s_b3[tid];
         }
         uint32_t total;
         const uint32_t excl = block_excl_scan(countA, s_wsum, &total);
@@ -604,8 +758,10 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
             if (tid == 0) s_base = b;
         }
         s_excl[tid] = excl;
-        s_nb[tid] = nB;
-        s_b3[tid] = b3;
+        if (!kSort) {
+            s_nb[tid] = nB;
+            s_b3[tid] = b3;
+        }
         if (tid == 0) s_excl[kPredBlock] = total;
         __syncthreads();
         const uint32_t base = s_base;
@@ -736,12 +892,12 @@ namespace {
 
 // blocks of a variant that stay resident on the device (persistent launch);
 // first called outside any graph capture (predict_fused_prepare)
-template <int B, int L>
+template <int B, int L, bool S>
 uint32_t resident_blocks() {
     static uint32_t resident = 0;
     if (resident == 0) {
         int per_cu = 0, dev = 0, n_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_fused<B, L>, B, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_fused<B, L, S>, B, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -752,38 +908,43 @@ uint32_t resident_blocks() {
     return resident;
 }
 
-template <int B, int L>
+template <int B, int L, bool S>
 hipError_t predict_fused_variant(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
                                  const uint32_t* grid, const float* hx, const float* hy, const float* hz,
                                  const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                  const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
                                  uint64_t* status, uint32_t* counter, hipStream_t st) {
     // persistent: as many blocks as stay resident, each loops over tiles
-    const uint32_t resident = resident_blocks<B, L>();
+    const uint32_t resident = resident_blocks<B, L, S>();
     const uint64_t tiles = (uint64_t)n_pair_cap / B + 1;
     const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
-    hipLaunchKernelGGL((k_predict_fused<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp, hlayer,
-                       hevt, pairs, cands, cand_offsets, n_cand_cap, cnt, status, counter);
+    hipLaunchKernelGGL((k_predict_fused<B, L, S>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp,
+                       hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt, status, counter);
     return hipGetLastError();
 }
 
+// the variants built: (block, list entries, sorted walks)
+#define TRAX_PRED_VARIANTS(X)                                                                                 \
+    X(128, 32, false) X(128, 64, false) X(256, 24, false) X(256, 32, false) X(256, 48, false) X(256, 64, false) \
+    X(512, 24, false) X(512, 32, false) X(256, 32, true)
+
 }  // namespace
 
-bool predict_variant_supported(uint32_t block, uint32_t list) {
+bool predict_variant_supported(uint32_t block, uint32_t list, uint32_t sort) {
     block = block ? block : 256;
     list = list ? list : 24;
-    return (block == 128 && list == 32) || (block == 256 && (list == 16 || list == 24 || list == 32)) ||
-           (block == 512 && (list == 16 || list == 24));
+#define TRAX_PRED_IS(B, L, S) if (block == B && list == L && (sort != 0) == S) return true;
+    TRAX_PRED_VARIANTS(TRAX_PRED_IS)
+#undef TRAX_PRED_IS
+    return false;
 }
 
 void predict_fused_prepare(const LaunchCfg& lc) {
     const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
-    if (b == 128 && l == 32) (void)resident_blocks<128, 32>();
-    if (b == 256 && l == 16) (void)resident_blocks<256, 16>();
-    if (b == 256 && l == 24) (void)resident_blocks<256, 24>();
-    if (b == 256 && l == 32) (void)resident_blocks<256, 32>();
-    if (b == 512 && l == 16) (void)resident_blocks<512, 16>();
-    if (b == 512 && l == 24) (void)resident_blocks<512, 24>();
+    const bool s = lc.predict_sort != 0;
+#define TRAX_PRED_PREP(B, L, S) if (b == B && l == L && s == S) (void)resident_blocks<B, L, S>();
+    TRAX_PRED_VARIANTS(TRAX_PRED_PREP)
+#undef TRAX_PRED_PREP
 }
 
 hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
@@ -798,19 +959,14 @@ hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32
     uint64_t* status = ss.status;
     uint32_t* counter = reinterpret_cast<uint32_t*>(ss.status + tiles);
     const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
-#define TRAX_PRED_VARIANT(B, L)                                                                                   \
-    if (b == B && l == L)                                                                                        \
-        e = predict_fused_variant<B, L>(g, tc, n_pair_cap, n_cand_cap, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, \
-                                        cands, cand_offsets, cnt, status, counter, st);                          \
-    else
-    TRAX_PRED_VARIANT(128, 32)
-    TRAX_PRED_VARIANT(256, 16)
-    TRAX_PRED_VARIANT(256, 24)
-    TRAX_PRED_VARIANT(256, 32)
-    TRAX_PRED_VARIANT(512, 16)
-    TRAX_PRED_VARIANT(512, 24)
-    return hipErrorInvalidConfiguration;
-#undef TRAX_PRED_VARIANT
+    const bool s = lc.predict_sort != 0;
+    e = hipErrorInvalidConfiguration;
+#define TRAX_PRED_RUN(B, L, S)                                                                                   \
+    if (b == B && l == L && s == S)                                                                              \
+        e = predict_fused_variant<B, L, S>(g, tc, n_pair_cap, n_cand_cap, grid, hx, hy, hz, hzp, hlayer, hevt,    \
+                                           pairs, cands, cand_offsets, cnt, status, counter, st);
+    TRAX_PRED_VARIANTS(TRAX_PRED_RUN)
+#undef TRAX_PRED_RUN
     if (e != hipSuccess) return e;
     return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
 }

@@ -33,7 +33,11 @@ void set_param(trax_tuning& t, const char* name, uint32_t v) {
     if (n == "grid_count_block") t.grid_count_block = v;
     else if (n == "grid_store_block") t.grid_store_block = v;
     else if (n == "pair_block") t.pair_block = v;
-    else if (n == "predict") { t.predict_block = v >> 16; t.predict_list = v & 0xffffu; }
+    else if (n == "predict") {
+        t.predict_block = (v >> 16) & 0x7fffu;
+        t.predict_list = v & 0xffffu;
+        t.predict_sort = v >> 31;
+    }
     else if (n == "filter_block") t.filter_block = v;
     else if (n == "link_block") t.link_block = v;
     else if (n == "blocks_per_cu") t.blocks_per_cu = v;
@@ -49,6 +53,7 @@ LaunchCfg to_launch(const trax_tuning& t) {
     lc.link_block = t.link_block;
     lc.blocks_per_cu = t.blocks_per_cu;
     lc.predict_list = t.predict_list;
+    lc.predict_sort = t.predict_sort;
     return lc;
 }
 
@@ -108,8 +113,9 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
         if (rc != TRAX_OK) return rc;
         const uint64_t ref = run_digest(b);
         const std::vector<Param> params = {
-            {"predict", {(128u << 16) | 32u, (256u << 16) | 16u, (256u << 16) | 24u, (256u << 16) | 32u,
-                         (512u << 16) | 16u, (512u << 16) | 24u}, {"predict"}},
+            {"predict", {(128u << 16) | 32u, (128u << 16) | 64u, (256u << 16) | 24u, (256u << 16) | 32u,
+                         (256u << 16) | 48u, (256u << 16) | 64u, (512u << 16) | 24u, (512u << 16) | 32u,
+                         1u << 31 | (256u << 16) | 32u}, {"predict"}},
             {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
             {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
@@ -141,7 +147,10 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
                 char line[256];
                 std::snprintf(line, sizeof(line),
                               "{\"param\": \"%s\", \"value\": %u, \"ms\": %.4f, \"same_results\": %s}\n", p.name,
-                              p.name == std::string("predict") ? (v >> 16) * 1000 + (v & 0xffffu) : v, ms,
+                              p.name == std::string("predict")
+                                  ? (v >> 31) * 1000000 + ((v >> 16) & 0x7fffu) * 1000 + (v & 0xffffu)
+                                  : v,
+                              ms,
                               same ? "true" : "false");
                 log += line;
                 if (best_v == 0 || ms < best_ms) {
