@@ -72,37 +72,86 @@ __device__ inline bool filter_valid(const TripletCols& tc, const float* hx, cons
     return valid;
 }
 
+// makes a wave's LDS writes visible to its other lanes (waves run their own
+// loops here, so no block barrier)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A candidate's hits (false for sentinel slots, which yield nothing)
+__device__ inline bool cand_hits(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* pairs,
+                                 const uint32_t* cands, uint32_t gid, uint32_t& h1, uint32_t& h2, uint32_t& h3,
+                                 uint32_t& lt) {
+    const uint32_t pi = cands[2 * (size_t)gid];
+    h3 = cands[2 * (size_t)gid + 1];
+    if (pi == kSentinel || h3 == kSentinel) return false;
+    h1 = pairs[2 * (size_t)pi];
+    h2 = pairs[2 * (size_t)pi + 1];
+    if (h1 == kSentinel || h2 == kSentinel) return false;
+    lt = hlayer[h1] - 1;
+    return lt < tc.n_trip;
+}
+
+// filterCount (TripletThetaPhiFilter.h:50-162) with the validity bitmap built
+// per 64-candidate word.  Each wave takes kFilterWords words at a time: the
+// phi-cut prefilter (tx_phi_surely_outside) settles most candidates, the
+// rest are queued in the wave's LDS and evaluated with the reference's cuts
+// 64 at a time, so that the full evaluation runs on whole waves of
+// candidates that need it.  Dynamic LDS: (kFilterWords * 64 + 2 * kFilterWords)
+// words per wave.
+constexpr int kFilterWords = 8;
+constexpr uint32_t kFilterWaveLds = kFilterWords * 64 + 2 * kFilterWords;
+
 __global__ void k_filter_count(TripletCols tc, const float* __restrict__ hx, const float* __restrict__ hy,
                                const float* __restrict__ hz, const uint32_t* __restrict__ hlayer,
                                const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ cands,
                                uint64_t* __restrict__ bitmap, uint32_t* __restrict__ word_counts,
                                DevCounts* __restrict__ cnt) {
+    extern __shared__ uint32_t lds[];
     const uint32_t n = bounded(cnt->n_cands, cnt->cap_cands, cnt);
     const uint32_t nwords = (n + 63) / 64;
-    if (gtid() == 0) cnt->n_filter_words = nwords;
+    if (gtid() == 0) {
+        cnt->n_filter_words = nwords;
+        word_counts[nwords] = 0;
+    }
     const uint32_t lane = lane_id();
-    for (uint32_t word = gtid() / kWave; word <= nwords; word += gstride() / kWave) {
-        if (word == nwords) {
-            if (lane == 0) word_counts[nwords] = 0;
-            continue;
+    uint32_t* queue = lds + (threadIdx.x / kWave) * kFilterWaveLds;
+    uint64_t* masks = reinterpret_cast<uint64_t*>(queue + kFilterWords * 64);
+    const uint32_t n_batches = (nwords + kFilterWords - 1) / kFilterWords;
+    for (uint32_t bt = gtid() / kWave; bt < n_batches; bt += gstride() / kWave) {
+        const uint32_t w0 = bt * kFilterWords;
+        uint32_t qn = 0;
+        for (int w = 0; w < kFilterWords; ++w) {
+            const uint32_t gid = (w0 + w) * 64 + lane;
+            bool undecided = false;
+            uint32_t h1, h2, h3, lt;
+            if (gid < n && cand_hits(tc, hlayer, pairs, cands, gid, h1, h2, h3, lt))
+                undecided = !tx_phi_surely_outside(hx[h2] - hx[h1], hy[h2] - hy[h1], hx[h3] - hx[h2],
+                                                   hy[h3] - hy[h2], tc.tan_dphi[lt]);
+            const uint64_t m = __ballot(undecided);
+            if (undecided) queue[qn + (uint32_t)__popcll(m & lanemask_lt())] = gid;
+            qn += (uint32_t)__popcll(m);
         }
-        const uint32_t gid = word * 64 + lane;
-        bool valid = false;
-        if (gid < n) {
-            const uint32_t pi = cands[2 * (size_t)gid], h3 = cands[2 * (size_t)gid + 1];
-            if (pi != kSentinel && h3 != kSentinel) {
-                const uint32_t h1 = pairs[2 * (size_t)pi], h2 = pairs[2 * (size_t)pi + 1];
-                if (h1 != kSentinel && h2 != kSentinel) {
-                    const uint32_t lt = hlayer[h1] - 1;
-                    if (lt < tc.n_trip) valid = filter_valid(tc, hx, hy, hz, h1, h2, h3, lt);
-                }
+        if (lane < kFilterWords) masks[lane] = 0ull;
+        wave_sync();
+        for (uint32_t k = 0; k < qn; k += 64) {
+            if (k + lane < qn) {
+                const uint32_t gid = queue[k + lane];
+                uint32_t h1, h2, h3, lt;
+                if (cand_hits(tc, hlayer, pairs, cands, gid, h1, h2, h3, lt) &&
+                    filter_valid(tc, hx, hy, hz, h1, h2, h3, lt))
+                    atomicOr(reinterpret_cast<unsigned long long*>(&masks[gid / 64 - w0]), 1ull << (gid & 63));
             }
         }
-        const uint64_t m = __ballot(valid);
-        if (lane == 0) {
-            bitmap[word] = m;
-            word_counts[word] = (uint32_t)__popcll(m);
+        wave_sync();
+        if (lane < kFilterWords && w0 + lane < nwords) {
+            const uint64_t m = masks[lane];
+            bitmap[w0 + lane] = m;
+            word_counts[w0 + lane] = (uint32_t)__popcll(m);
         }
+        wave_sync();  // the queue and masks are reused by the next batch
     }
 }
 
@@ -286,22 +335,47 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
         const uint32_t h = th2[t], h3 = th3[t];
         const float et = b.eta[t];
         const uint32_t q0 = b.hit_count[h], q1 = b.hit_count[h + 1];
-        uint32_t last = 0;
-        bool first = true;
-        for (; so < sn; ++so) {  // emit the followers in ascending index
-            uint32_t best = kSentinel;
-            for (uint32_t q = q0; q < q1; ++q) {
+        auto follows = [&](uint32_t i) {
+            return i >= r.begin && i < r.end && th2[i] == h3 && fabsf(et - b.eta[i]) < tc.d_eta_cut;
+        };
+        // the bucket is in fill (atomic) order: collect the followers, sort
+        // them and emit them in ascending index, the reference's order
+        constexpr uint32_t kFew = 16;
+        if (sn - so <= kFew) {
+            uint32_t f[kFew];
+            uint32_t k = 0;
+            for (uint32_t q = q0; q < q1 && k < sn - so; ++q) {
                 const uint32_t i = b.csr[q];
-                if ((first || i > last) && i < best && i >= r.begin && i < r.end && th2[i] == h3 &&
-                    fabsf(et - b.eta[i]) < tc.d_eta_cut)
-                    best = i;
+                if (follows(i)) {
+                    uint32_t p = k++;
+                    while (p > 0 && f[p - 1] > i) {
+                        f[p] = f[p - 1];
+                        --p;
+                    }
+                    f[p] = i;
+                }
             }
-            if (best == kSentinel) break;
-            b.basis[so] = t;
-            b.foll[so] = best;
-            b.pev[so] = r.ev;
-            last = best;
-            first = false;
+            for (uint32_t x = 0; x < k; ++x, ++so) {
+                b.basis[so] = t;
+                b.foll[so] = f[x];
+                b.pev[so] = r.ev;
+            }
+        } else {
+            uint32_t last = 0;
+            bool first = true;
+            for (; so < sn; ++so) {  // many followers: select them one by one in ascending index
+                uint32_t best = kSentinel;
+                for (uint32_t q = q0; q < q1; ++q) {
+                    const uint32_t i = b.csr[q];
+                    if ((first || i > last) && i < best && follows(i)) best = i;
+                }
+                if (best == kSentinel) break;
+                b.basis[so] = t;
+                b.foll[so] = best;
+                b.pev[so] = r.ev;
+                last = best;
+                first = false;
+            }
         }
     }
     if (over) atomicOr(&cnt->err, kErrCapacity);
@@ -483,9 +557,11 @@ hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const
                                const uint32_t* cands, uint64_t* bitmap, uint32_t* word_counts, DevCounts* cnt,
                                const LaunchCfg& lc, hipStream_t st) {
     const uint32_t b = lc.filter_block ? lc.filter_block : 256;
-    TRAX_LAUNCH(k_filter_count, (uint64_t)n_cand_cap + 64, b, tc, hx, hy, hz, hlayer, pairs, cands, bitmap,
-                word_counts, cnt);
-    return hipSuccess;
+    const size_t lds = (size_t)(b / kWave) * kFilterWaveLds * sizeof(uint32_t);
+    const uint64_t words = ((uint64_t)n_cand_cap + 63) / 64;
+    hipLaunchKernelGGL(k_filter_count, dim3(stride_blocks((words / kFilterWords + 1) * kWave, b, lc)), dim3(b), lds,
+                       st, tc, hx, hy, hz, hlayer, pairs, cands, bitmap, word_counts, cnt);
+    return hipGetLastError();
 }
 
 hipError_t launch_filter_store(const TripletCols& tc, uint32_t n_cand_cap, uint32_t n_trk_cap,
