@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="fixed launch shapes as JSON (e.g. the autotuner's choice), skipping the autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--serve-events", type=int, default=10000, help="events of the serving stream (0 = skip)")
@@ -177,7 +179,12 @@ def main():
                 head(c["track_entries"]), use_graph=not args.no_graph, per_event=True)
     # launch shapes for this workload from the autotuner (results do not depend on them)
     t_tune = time.perf_counter()
-    tuning, trials = autotune(dev, cfg, hs, caps, repeats=2) if not args.no_autotune else ({}, [])
+    if args.tuning:
+        tuning, trials = json.loads(args.tuning), []
+    elif args.no_autotune:
+        tuning, trials = {}, []
+    else:
+        tuning, trials = autotune(dev, cfg, hs, caps, repeats=2)
     tune_s = time.perf_counter() - t_tune
     print(f"autotuned in {tune_s:.1f} s: {tuning}", file=sys.stderr, flush=True)
     b = Batch(dev, cfg, caps, tuning)
