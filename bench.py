@@ -30,13 +30,28 @@ SEED = 2024
 HBM_PEAK = 8.0e12
 
 
-def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads):
+def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads, fused):
     """Algorithmic (compulsory) HBM bytes of each phase: every input stream read
     once, every output written once; the per-unit figures of the measurement
-    plan (grid 68 B/hit + 16 B/cell, scan 8 B/element, ...)."""
+    plan (grid 68 B/hit + 16 B/cell, scan 8 B/element, ...).  With the filter
+    fused into the predictor, the predict phase reads the pairs and the hits
+    of the three layers and writes the tracklets (8 B/pair + 16 B/hit + 12 B
+    per tracklet); the filter phases only place the tracklet offsets."""
     P, Q, R, K = counts["pairs"], counts["cands"], counts["tracklets"], counts["conn"]
     slots = counts.get("slots", 0)
     words = (Q + 63) // 64
+    if fused:
+        return {
+            "grid_count": 12 * n_hits + 12 * cells,
+            "grid_store": 56 * n_hits + 4 * cells,
+            "pair_count": 12 * seg_hits + 4 * slots,
+            "pair_scan": 8 * slots,
+            "pair_store": 12 * seg_hits + 4 * slots + 8 * P,
+            "predict": 8 * P + 16 * trip_hits + 12 * R,
+            "filter_store": 12 * R,
+            "link_count": 12 * R + 4 * R + 8 * R,
+            "link_store": 12 * R + 4 * R + 12 * K,
+        }
     return {
         "grid_count": 12 * n_hits + 12 * cells,       # count (12 B/hit, 4 B/cell) + scan (8 B/cell)
         "grid_store": 56 * n_hits + 4 * cells,        # 28 B/hit read + 28 B/hit write + cells
@@ -65,7 +80,7 @@ def serving_stream(wl, n_events, seed, workers):
     return cfg, hs
 
 
-def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_hits):
+def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_hits, fused):
     """Replay the stream with Poisson arrivals at rate_hz through the
     continuous-batching server; latency = arrival to results on the host."""
     import numpy as np
@@ -74,7 +89,7 @@ def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_h
     arrival = np.cumsum(rng.exponential(1.0 / rate_hz, hs.n_events))
     arrival -= arrival[0]
     sv = Server(dev, cfg, bounds, n_slots=n_slots, max_batch_events=max_batch_events, max_batch_hits=slot_hits,
-                device_budget=64 << 30)
+                device_budget=64 << 30, fuse_filter=fused)
     caps = sv.caps()
     _, st = sv.replay(hs, arrival)
     sv.close()
@@ -91,6 +106,7 @@ def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_h
 PHASE_KERNEL = {"grid_count": "k_grid_count", "grid_store": "k_grid_store", "pair_count": "k_pair_count",
                 "pair_store": "k_pair_store", "predict": "k_predict_fused", "filter_count": "k_filter_count",
                 "filter_store": "k_filter_store", "link_count": "k_link_count", "link_store": "k_link_store"}
+PHASE_KERNEL_FUSED = dict(PHASE_KERNEL, predict="k_predict_filter", filter_store="k_filter_offsets")
 
 
 def measured_traffic(kernel):
@@ -135,6 +151,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes instead of the autotuner's")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="predictor and filter as separate passes (the candidates stored) instead of one")
     ap.add_argument("--tuning", type=str, default=None,
                     help="fixed launch shapes as JSON (e.g. the autotuner's choice), skipping the autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -176,7 +194,7 @@ def main():
     probe.close()
     head = lambda v: int(v * 1.15) + 4096
     caps = Caps(N_EVENTS, hs.n_hits, head(c["pairs"]), head(c["cands"]), head(c["tracklets"]), head(c["conn"]),
-                head(c["track_entries"]), use_graph=not args.no_graph, per_event=True)
+                head(c["track_entries"]), use_graph=not args.no_graph, per_event=True, fuse_filter=not args.no_fuse)
     # launch shapes for this workload from the autotuner (results do not depend on them)
     t_tune = time.perf_counter()
     if args.tuning:
@@ -218,7 +236,7 @@ def main():
         b.close()  # the server holds its own workspaces
         print("serving stream", file=sys.stderr, flush=True)
         serving = serve(dev, stream[0], stream[1], bounds, args.serve_rate, SEED + 2, args.serve_slots,
-                        args.serve_max_batch, args.serve_slot_hits)
+                        args.serve_max_batch, args.serve_slot_hits, not args.no_fuse)
         b = Batch(dev, cfg, caps, tuning)
         b.submit(hs)
         b.wait()
@@ -266,7 +284,8 @@ def main():
                     for t in wl.triplets)
     cells = N_EVENTS * wl.n_layers * (wl.n_z + 1) * (wl.n_phi + 1)
     counts["slots"] = hs.n_hits + N_EVENTS * len(wl.triplets) * wl.n_threads
-    pb = phase_bytes(hs.n_hits, seg_hits, trip_hits, counts, cells, wl.n_threads)
+    pb = phase_bytes(hs.n_hits, seg_hits, trip_hits, counts, cells, wl.n_threads, not args.no_fuse)
+    kernel_of = PHASE_KERNEL if args.no_fuse else PHASE_KERNEL_FUSED
     gbs = {p: pb[p] / (phase_ms[p] * 1e-3) / 1e9 for p in pb if phase_ms[p] > 0}
     dom = max(pb, key=lambda p: phase_ms[p])
     achieved = pb[dom] / (phase_ms[dom] * 1e-3)
@@ -289,6 +308,7 @@ def main():
         "config": {"workload": wl.name, "events_per_step": N_EVENTS, "hits_per_step": hs.n_hits,
                    "layers": wl.n_layers, "layer_triplets": len(wl.triplets), "grid": [wl.n_z, wl.n_phi],
                    "seed": SEED, "graph": not args.no_graph, "event_groups": "per-event",
+                   "predictor_filter": "separate" if args.no_fuse else "fused",
                    "tuning": tuning, "autotune_s": round(tune_s, 2)},
         "triplets_per_s": round(trip_s, 1),
         "triplets_per_step": counts["tracklets"],
@@ -303,8 +323,8 @@ def main():
         "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": measured_traffic(PHASE_KERNEL.get(dom, dom)),
-                     "kernel": PHASE_KERNEL.get(dom, dom), "algorithmic_bytes": int(pb[dom])},
+                     "traffic": measured_traffic(kernel_of.get(dom, dom)),
+                     "kernel": kernel_of.get(dom, dom), "algorithmic_bytes": int(pb[dom])},
         "latency_ms": {"p50": serving["p50_ms"] if serving else None,
                        "p99": serving["p99_ms"] if serving else None},
         "serving": serving,

@@ -174,17 +174,26 @@ typedef struct {
      * 1: every event is its own group, so each event's results equal a run of
      *    that event alone, whatever batch it shares */
     uint32_t per_event;
+    /* 1: the predictor and the filter run as one pass: the tracklets come
+     *    straight from the window walks and the candidates (TripletThetaPhi-
+     *    Predictor's output) are counted, with their group offsets, but not
+     *    stored, so max_cands does not size any buffer and the "cands" result
+     *    is absent; every other result is the same */
+    uint32_t fuse_filter;
 } trax_batch_caps;
 /* launch shapes (0 = default): block sizes per kernel family, blocks per CU
  * of the grid-stride kernels, and the fused predictor's variant: (block,
  * LDS list entries per pair, walk order) one of (128, 32|64), (256,
  * 24|32|48|64), (512, 24|32) in pair order or (256, 32) longest walk first;
- * 0 = (256, 24, pair order).  Results do not depend on them. */
+ * 0 = (256, 24, pair order); and the z sub-rows per grid row of the fused
+ * predictor's walk structure (1, 2, 4, 8 or 16; 0 = 8).  Results do not
+ * depend on them. */
 typedef struct {
     uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
     uint32_t blocks_per_cu;
     uint32_t predict_list;
     uint32_t predict_sort;
+    uint32_t predict_fine;
 } trax_tuning;
 /* device and pinned host bytes a batch of these capacities would hold, without
  * allocating (the planning step of buildTriplets' per-group buffer sizing,
@@ -247,6 +256,7 @@ typedef struct {
     float pairs_per_hit, cands_per_hit, tracklets_per_hit, conn_per_hit, track_entries_per_hit;
     uint32_t keep_results; /* keep each event's tracklets and tracks (trax_server_event_result) */
     uint32_t use_graph;
+    uint32_t fuse_filter;  /* the slots' batches run the predictor and the filter as one pass */
 } trax_serve_opts;
 typedef struct {
     uint32_t events, failed, batches, resubmits, max_batch_events, slot_hits;

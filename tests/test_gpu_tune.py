@@ -5,7 +5,7 @@ import pytest
 
 import oracle
 from helpers import compare, hitset, small_workload
-from trax_amd.api import Batch, Caps, autotune, run_batch_fit
+from trax_amd.api import Batch, Caps, TraxError, autotune, run_batch_fit
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,41 @@ def test_predictor_variants_match_oracle(dev, block, lst, srt):
     assert not bad, "\n".join(bad)
 
 
+@pytest.mark.parametrize("fine", [1, 2, 4, 8, 16])
+def test_fine_walk_sub_rows_match_oracle(dev, fine):
+    """The fused predictor walks z sub-rows of the grid rows: any sub-row count
+    gives the grid walk's candidates in the grid walk's order."""
+    wl, cfg, hs, _ = _case()
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits),
+                           tuning=dict(predict_block=256, predict_list=24, predict_fine=fine))
+    b.close()
+    bad = compare(got, want)
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("fine,lst", [(16, 24), (4, 24), (8, 64)])
+def test_fine_walk_dense_event_full_lists(dev, fine, lst):
+    """A dense event, where many pairs match more hits than their LDS list
+    holds: the list keeps the first matches in the grid walk's order and the
+    rest come from the re-walk."""
+    wl = small_workload(2, tracks=1500, noise=20000, events=1)  # 109 pairs match more than 24 hits
+    cfg, hs, _ = hitset(wl, seed=5)
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits),
+                           tuning=dict(predict_block=256, predict_list=lst, predict_fine=fine))
+    b.close()
+    bad = compare(got, want)
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("fine", [3, 32])
+def test_fine_sub_rows_must_be_a_supported_power_of_two(dev, fine):
+    wl, cfg, hs, _ = _case()
+    with pytest.raises(TraxError):
+        Batch(dev, cfg, Caps.for_hits(hs.n_events, hs.n_hits), dict(predict_fine=fine))
+
+
 @pytest.mark.parametrize("shape", [dict(filter_block=64, pair_block=64, link_block=64, grid_store_block=128,
                                         grid_count_block=128, blocks_per_cu=2),
                                    dict(filter_block=1024, pair_block=512, link_block=512, grid_store_block=1024,
@@ -49,10 +84,11 @@ def test_autotune_picks_supported_shapes_with_same_results(dev):
     caps = b.caps
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
-    assert len(trials) == 9 + 5 + 4 + 4 + 4 + 4 + 4
+    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 4 + 4 + 4
     assert all(t["same_results"] for t in trials)
     assert (tuning["predict_block"], tuning["predict_list"], tuning["predict_sort"]) in PREDICT_VARIANTS
     assert tuning["blocks_per_cu"] in (2, 4, 8, 16)
+    assert tuning["predict_fine"] in (4, 8, 16)
     want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
     tb = Batch(dev, cfg, caps, tuning)
     tb.submit(hs)

@@ -37,19 +37,21 @@ class HitCols(C.Structure):
 
 class BatchCaps(C.Structure):
     _fields_ = [(n, U32) for n in ("max_events", "max_hits", "max_pairs", "max_cands", "max_tracklets",
-                                   "max_conn", "max_track_entries", "use_graph", "ca_iterations", "per_event")]
+                                   "max_conn", "max_track_entries", "use_graph", "ca_iterations", "per_event",
+                                   "fuse_filter")]
 
 
 class Tuning(C.Structure):
     _fields_ = [(n, U32) for n in ("grid_count_block", "grid_store_block", "pair_block", "predict_block",
-                                   "filter_block", "link_block", "blocks_per_cu", "predict_list", "predict_sort")]
+                                   "filter_block", "link_block", "blocks_per_cu", "predict_list", "predict_sort",
+                                   "predict_fine")]
 
 
 class ServeOpts(C.Structure):
     _fields_ = [("n_slots", U32), ("max_batch_events", U32), ("max_batch_hits", U32), ("device_budget", C.c_uint64),
                 ("pairs_per_hit", C.c_float), ("cands_per_hit", C.c_float), ("tracklets_per_hit", C.c_float),
                 ("conn_per_hit", C.c_float), ("track_entries_per_hit", C.c_float), ("keep_results", U32),
-                ("use_graph", U32)]
+                ("use_graph", U32), ("fuse_filter", U32)]
 
 
 class ServeStats(C.Structure):
@@ -335,20 +337,29 @@ class Caps:
     use_graph: bool = True
     ca_iterations: int = 0
     per_event: bool = False  # each event its own group (results equal per-event runs)
+    fuse_filter: bool = False  # predictor and filter in one pass; the candidates are not stored
 
     @staticmethod
     def for_hits(max_events: int, max_hits: int, pairs_per_hit: float = 8.0, cands_per_pair: float = 4.0,
-                 trk_per_hit: float = 1.0, use_graph: bool = True, per_event: bool = False) -> "Caps":
+                 trk_per_hit: float = 1.0, use_graph: bool = True, per_event: bool = False,
+                 fuse_filter: bool = False) -> "Caps":
         p = int(max_hits * pairs_per_hit) + 1024
         q = int(p * cands_per_pair) + 1024
         r = int(max_hits * trk_per_hit) + 1024
-        return Caps(max_events, max_hits, p, q, r, 4 * r + 1024, 2 * r + 1024, use_graph, 0, per_event)
+        return Caps(max_events, max_hits, p, q, r, 4 * r + 1024, 2 * r + 1024, use_graph, 0, per_event, fuse_filter)
 
     def grown(self, factor: float = 2.0) -> "Caps":
         g = lambda v: int(v * factor) + 1024
         return Caps(self.max_events, self.max_hits, g(self.max_pairs), g(self.max_cands), g(self.max_tracklets),
                     g(self.max_conn), g(self.max_track_entries), self.use_graph, self.ca_iterations,
-                    self.per_event)
+                    self.per_event, self.fuse_filter)
+
+    def native(self, use_graph: Optional[bool] = None) -> BatchCaps:
+        """The C struct of these capacities."""
+        return BatchCaps(self.max_events, self.max_hits, self.max_pairs, self.max_cands, self.max_tracklets,
+                         self.max_conn, self.max_track_entries,
+                         int(self.use_graph if use_graph is None else use_graph), self.ca_iterations,
+                         int(self.per_event), int(self.fuse_filter))
 
 
 class Batch:
@@ -356,9 +367,7 @@ class Batch:
 
     def __init__(self, dev: Device, cfg: Config, caps: Caps, tuning: Optional[Dict[str, int]] = None,
                  stream: Optional[Stream] = None):
-        c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
-                      caps.max_conn, caps.max_track_entries, int(caps.use_graph), caps.ca_iterations,
-                      int(caps.per_event))
+        c = caps.native()
         t = Tuning(**(tuning or {}))
         h = P()
         check(lib().trax_batch_create(dev.h, cfg.h, C.byref(c), C.byref(t), None if stream is None else stream.h,
@@ -429,9 +438,7 @@ def run_batch_fit(dev: Device, cfg: Config, hits: HitSet, caps: Caps, tuning=Non
 
 def plan_bytes(cfg: Config, caps: Caps):
     """(device bytes, pinned host bytes) a batch of these capacities holds."""
-    c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
-                  caps.max_conn, caps.max_track_entries, int(caps.use_graph), caps.ca_iterations,
-                      int(caps.per_event))
+    c = caps.native()
     d, h = C.c_uint64(), C.c_uint64()
     check(lib().trax_batch_plan(cfg.h, C.byref(c), C.byref(d), C.byref(h)))
     return int(d.value), int(h.value)
@@ -460,9 +467,10 @@ class Server:
 
     def __init__(self, dev: Device, cfg: Config, bounds: OutputBounds, n_slots: int = 2, max_batch_events: int = 256,
                  max_batch_hits: int = 0, device_budget: int = 0, keep_results: bool = False,
-                 use_graph: bool = True):
+                 use_graph: bool = True, fuse_filter: bool = False):
         o = ServeOpts(n_slots, max_batch_events, max_batch_hits, device_budget, bounds.pairs, bounds.cands,
-                      bounds.tracklets, bounds.conn, bounds.track_entries, int(keep_results), int(use_graph))
+                      bounds.tracklets, bounds.conn, bounds.track_entries, int(keep_results), int(use_graph),
+                      int(fuse_filter))
         h = P()
         check(lib().trax_server_create(dev.h, cfg.h, C.byref(o), C.byref(h)))
         self.h = h
@@ -508,8 +516,7 @@ def autotune(dev: Device, cfg: Config, hits: HitSet, caps: Caps, repeats: int = 
     """Launch shapes for this workload from the library's autotuner: returns
     (tuning dict for Batch(..., tuning=...), list of trial records)."""
     import json
-    c = BatchCaps(caps.max_events, caps.max_hits, caps.max_pairs, caps.max_cands, caps.max_tracklets,
-                  caps.max_conn, caps.max_track_entries, 0, caps.ca_iterations, int(caps.per_event))
+    c = caps.native(use_graph=False)
     t = Tuning()
     buf = C.create_string_buffer(1 << 16)
     check(lib().trax_autotune(dev.h, cfg.h, hits.h, C.byref(c), repeats, C.byref(t), buf, len(buf)))

@@ -514,25 +514,36 @@ void trax_batch::enqueue_stage(int s) {
                                  slot_counts, pairs, pair_off, cnt, lc, st),
                "pair store");
             break;
-        case 6:  // predict: count, scan and store in one pass
+        case 6:  // predict: count, scan and store in one pass (with the filter when fused)
             ok(launch_fill(cand_off, 0u, nseg + 1ull, st), "fill");
-            ok(launch_predict_fused(g, tc, P, Q, grid, hout.x, hout.y, hout.z, hout.zphi, hout.layer, hout.evt, pairs,
-                                    cands, cand_off, cnt, scan, lc, st),
-               "predict");
+            if (caps.fuse_filter)
+                ok(launch_predict_filter(g, tc, E * L, N, P, R, grid, hout.x, hout.y, hout.z, hout.zphi, hout.layer,
+                                         hout.evt, pairs, cand_off, th1, th2, th3, cnt, fine, scan, lc, st),
+                   "predict + filter");
+            else
+                ok(launch_predict_fused(g, tc, E * L, N, P, Q, grid, hout.x, hout.y, hout.z, hout.zphi, hout.layer,
+                                        hout.evt, pairs, cands, cand_off, cnt, fine, scan, lc, st),
+                   "predict");
             break;
         case 7:  // filter_count
+            if (caps.fuse_filter) break;
             ok(launch_filter_count(tc, Q, hout.x, hout.y, hout.z, hout.layer, pairs, cands, bitmap, word_counts, cnt,
                                    lc, st),
                "filter count");
             break;
         case 8:  // filter_scan
-            ok(launch_scan(word_counts, &cnt->n_filter_words, 1u, Q / 64 + 2ull, &cnt->n_tracklets, scan, st), "scan");
+            if (!caps.fuse_filter)
+                ok(launch_scan(word_counts, &cnt->n_filter_words, 1u, Q / 64 + 2ull, &cnt->n_tracklets, scan, st),
+                   "scan");
             ok(launch_fill(t_off, 0u, nseg + 1ull, st), "fill");
             break;
         case 9:  // filter_store
-            ok(launch_filter_store(tc, Q, R, hout.layer, hout.evt, pairs, cands, bitmap, word_counts, th1, th2, th3,
-                                   t_off, cnt, lc, st),
-               "filter store");
+            if (caps.fuse_filter)
+                ok(launch_filter_offsets(tc, R, hout.layer, hout.evt, th1, t_off, cnt, lc, st), "filter offsets");
+            else
+                ok(launch_filter_store(tc, Q, R, hout.layer, hout.evt, pairs, cands, bitmap, word_counts, th1, th2,
+                                       th3, t_off, cnt, lc, st),
+                   "filter store");
             break;
         case 10:  // link_prepare
             ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
@@ -615,6 +626,19 @@ void trax_batch::layout(Arena& a) {
         cols(B.hin);
         cols(B.hout);
         B.hout.zphi = a.take<float2>(N);
+        B.fine.grid = a.take<uint32_t>(E * L * fine_geom(B.cfg.grid.geom(), fine_max(B.cfg.grid.geom())).cells);
+        B.fine.hits = a.take<float4>(N);
+        if (caps.fuse_filter) {
+            B.fine.pool_cap = kFusedPoolCap;
+            B.fine.pool_blocks = kFusedPoolBlocks;
+            B.fine.pool = a.take<uint2>((uint64_t)kFusedPoolCap * kFusedPoolBlocks);
+            const uint64_t tiles = predict_fused_tiles(P) + 1ull;
+            B.fine.tile_cands = a.take<uint32_t>(tiles);
+            B.fine.tile_trk = a.take<uint32_t>(tiles);
+            B.fine.tile_at = a.take<uint32_t>(tiles);
+            B.fine.group_tile = a.take<uint32_t>(E * T + 1);
+            B.fine.scratch = a.take<uint32_t>(3 * R);
+        }
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
         B.seg_base = a.take<uint32_t>(E * T + 1);
@@ -622,10 +646,11 @@ void trax_batch::layout(Arena& a) {
         B.pairs = a.take<uint32_t>(2 * P);
         B.pair_off = a.take<uint32_t>(E * T + 1);
         B.pred_counts = a.take<uint32_t>(P + 1);
-        B.cands = a.take<uint32_t>(2 * Q);
+        const uint64_t Qs = caps.fuse_filter ? 0 : Q;  // candidates are stored only when not fused
+        B.cands = a.take<uint32_t>(2 * Qs);
         B.cand_off = a.take<uint32_t>(E * T + 1);
-        B.bitmap = a.take<uint64_t>(Q / 64 + 2);
-        B.word_counts = a.take<uint32_t>(Q / 64 + 2);
+        B.bitmap = a.take<uint64_t>(Qs / 64 + 2);
+        B.word_counts = a.take<uint32_t>(Qs / 64 + 2);
         B.th1 = a.take<uint32_t>(R); B.th2 = a.take<uint32_t>(R); B.th3 = a.take<uint32_t>(R);
         B.t_off = a.take<uint32_t>(E * T + 1);
         LinkBufs& lb = B.lb;
@@ -687,6 +712,7 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
             b->lc.blocks_per_cu = tuning->blocks_per_cu;
             b->lc.predict_list = tuning->predict_list;
             b->lc.predict_sort = tuning->predict_sort;
+            b->lc.predict_fine = tuning->predict_fine;
         }
         if (!predict_variant_supported(b->lc.predict_block, b->lc.predict_list, b->lc.predict_sort))
             throw InputError("unsupported fused predictor variant (block, list, sort)");
@@ -696,6 +722,12 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         predict_fused_prepare(b->lc);  // before any capture
         auto& B = *b;
         batch_shape(B, cfg, caps);
+        {
+            const uint32_t F = b->lc.predict_fine ? b->lc.predict_fine : 8, Fmax = fine_max(B.cfg.grid.geom());
+            if ((F & (F - 1)) != 0 || F > Fmax)
+                throw InputError("predict_fine must be a power of two up to " + std::to_string(Fmax) +
+                                 " for this grid");
+        }
         const uint64_t E = B.E, L = B.L, N = caps->max_hits, P = caps->max_pairs, Q = caps->max_cands,
                        R = caps->max_tracklets, K = caps->max_conn, TE = caps->max_track_entries;
         B.cm.reset(new ConfigMirror(*ctx->c, cfg->trip, cfg->geom, cfg->n_threads, cfg->d_eta_cut));
@@ -868,7 +900,7 @@ static int batch_result_impl(trax_batch* b, const char* names, trax_result** out
         if (want("hits.hit_id")) r->put("hits.hit_id", dl(b->hout.id, N)); if (want("hits.event")) r->put("hits.event", dl(b->hout.evt, N));
         if (want("pairs")) r->put("pairs", dl(b->pairs, 2ull * P));
         if (want("pair_offsets")) r->put("pair_offsets", dl(b->pair_off, (uint64_t)E * T + 1));
-        if (want("cands")) r->put("cands", dl(b->cands, 2ull * Q));
+        if (want("cands") && !b->caps.fuse_filter) r->put("cands", dl(b->cands, 2ull * Q));
         if (want("cand_offsets")) r->put("cand_offsets", dl(b->cand_off, (uint64_t)E * T + 1));
         if (want("tracklets.h1")) r->put("tracklets.h1", dl(b->th1, R)); if (want("tracklets.h2")) r->put("tracklets.h2", dl(b->th2, R)); if (want("tracklets.h3")) r->put("tracklets.h3", dl(b->th3, R));
         if (want("tracklet_offsets")) r->put("tracklet_offsets", dl(b->t_off, (uint64_t)E * T + 1));

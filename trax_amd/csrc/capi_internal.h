@@ -137,6 +137,7 @@ struct trax_batch {
     uint64_t* bitmap = nullptr;
     trax::LinkBufs lb;
     trax::CABufs cb;
+    trax::FineBufs fine;  // the fused predictor's walk structure
     hipGraph_t graphs[trax_capi::kNumStages] = {};
     hipGraphExec_t execs[trax_capi::kNumStages] = {};
     bool captured = false;

@@ -44,7 +44,9 @@ struct DevCounts {
     uint32_t diag_pred_short;   // candidate slots counted but not written
     uint32_t diag_pred_trunc;   // pairs whose store found more than counted
     uint32_t diag_wrap_overlap; // outer hits whose wrapped phi window overlaps itself
-    uint32_t pad[5];
+    uint32_t trk_alloc;         // fused predictor + filter: tracklet scratch handed out to tiles
+    uint32_t n_pred_tiles;      // fused predictor + filter: tiles of the run
+    uint32_t pad[3];
 };
 constexpr uint32_t kCountsShapeWords = 9;  // n_events, n_hits and the seven capacities
 
@@ -59,6 +61,14 @@ __device__ inline uint32_t bounded(uint32_t v, uint32_t cap, DevCounts* cnt) {
 }
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// makes a wave's LDS writes visible to its other lanes (for waves that run
+// their own loops, without a block barrier)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ inline uint64_t lanemask_lt() {
     const uint32_t l = lane_id();

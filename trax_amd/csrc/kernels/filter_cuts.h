@@ -1,0 +1,58 @@
+// The cuts of TripletThetaPhiFilter (src/algorithms/TripletThetaPhiFilter.h:81-152)
+// on one candidate, shared by the filter kernels and the fused predictor.
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+
+namespace trax {
+
+// TripletThetaPhiFilter.h:81-152
+__device__ inline bool filter_valid(const TripletCols& tc, const float* hx, const float* hy, const float* hz,
+                                    uint32_t h1, uint32_t h2, uint32_t h3, uint32_t lt) {
+    const float dThetaCut = tc.dtheta[lt], dPhiCut = tc.dphi[lt], tipCut = tc.tip[lt];
+    const float minRadius = tc.min_radius_curvature;
+    const float x1 = hx[h1], y1 = hy[h1], z1 = hz[h1];
+    const float x2 = hx[h2], y2 = hy[h2], z2 = hz[h2];
+    const float x3 = hx[h3], y3 = hy[h3], z3 = hz[h3];
+    // the four cuts are a conjunction of pure expressions: a candidate is
+    // rejected at the first cut it fails (the result is the reference's);
+    // most fail the phi cut by far, which the prefilter proves without atan2
+    if (tx_phi_surely_outside(x2 - x1, y2 - y1, x3 - x2, y3 - y2, tc.tan_dphi[lt])) return false;
+    float angle1 = tx_atan2f((y2 - y1), (x2 - x1));
+    float angle2 = tx_atan2f((y3 - y2), (x3 - x2));
+    float delta = angle2 - angle1;
+    const float PI = TX_PI_F;
+    delta += (delta > PI) ? -2 * PI : (delta < -PI) ? 2 * PI : 0;
+    if (!(fabsf(delta) <= dPhiCut)) return false;
+    angle1 = tx_atan2f(sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
+    angle2 = tx_atan2f(sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
+    delta = fabsf(angle2 / angle1);
+    if (!(1 - dThetaCut <= delta && delta <= 1 + dThetaCut)) return false;
+    bool valid = true;
+    const float p1x = x1, p1y = y1, p1z = x1 * x1 + y1 * y1;
+    const float p2x = x2, p2y = y2, p2z = x2 * x2 + y2 * y2;
+    const float p3x = x3, p3y = y3, p3z = x3 * x3 + y3 * y3;
+    const float ax = p2x - p1x, ay = p2y - p1y, az = p2z - p1z;
+    const float bx = p3x - p1x, by = p3y - p1y, bz = p3z - p1z;
+    float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+    {
+        const float d = nx * nx + ny * ny + nz * nz;
+        if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
+    }
+    const float cox = (-nx) / (2 * nz), coy = (-ny) / (2 * nz);
+    const float cc = -(nx * p1x + ny * p1y + nz * p1z);
+    const float cR = sqrtf((1 - nz * nz - 4 * cc * nz) / (4 * nz * nz));
+    float vx = -cox, vy = -coy;
+    {
+        const float d = vx * vx + vy * vy;
+        if (d != 0.0f) { const float l = sqrtf(d); vx = vx / l; vy = vy / l; }
+    }
+    const float pcax = cox + cR * vx, pcay = coy + cR * vy;
+    const float tip_squared = pcax * pcax + pcay * pcay;
+    valid = valid && (tip_squared <= (tipCut * tipCut));
+    valid = valid && (cR >= minRadius);
+    return valid;
+}
+
+}  // namespace trax

@@ -39,7 +39,40 @@ struct LaunchCfg {
     uint32_t blocks_per_cu = 0;  // grid-stride kernels: blocks per CU (0 = 8)
     uint32_t predict_list = 0;   // fused predictor: LDS list entries per pair (0 = 24)
     uint32_t predict_sort = 0;   // fused predictor: walk a tile's pairs longest first
+    uint32_t predict_fine = 0;   // fused predictor: z sub-rows per grid row (power of two; 0 = 8)
 };
+
+// The fused predictor's own walk structure: every z row of every (event,
+// layer) grid split into F sub-rows by z within the row, the hits of a row
+// reordered (stable) by (sub-row, phi cell), each with its grid index.  Sub-row
+// s of row r holds the hits whose fine_bin(z) is s; the prefix entries are laid
+// out like the grid's, (n_z * F) rows of n_phi + 1 cell starts and one final
+// entry, and use the grid's hit positions (the reordering stays inside a row).
+struct FineGeom {
+    uint32_t F = 1, shift = 0;  // sub-rows per row, log2 F
+    float inv_sub = 0.0f;       // F / sector_z
+    uint32_t cells = 0;         // n_z * F * (n_phi + 1) + 1 prefix entries per (event, layer)
+};
+FineGeom fine_geom(const GridGeom& g, uint32_t F);
+constexpr uint32_t kFineMax = 16;
+// largest sub-row count the launch supports for this grid (power of two <= kFineMax)
+uint32_t fine_max(const GridGeom& g);
+struct FineBufs {
+    uint32_t* grid = nullptr;  // E * L * fine_geom(g, kFineMax).cells prefix entries
+    float4* hits = nullptr;    // (z, phi, grid index as bits, 0) at fine positions; N entries
+    // the fused predictor + filter: per resident block, a pool of pool_cap
+    // store-window matches beyond a pair's LDS list
+    uint2* pool = nullptr;
+    uint32_t pool_cap = 0, pool_blocks = 0;
+    // ... per tile: candidates, tracklets, and where its tracklets went in
+    // the scratch (predict_fused_tiles(n_pair_cap) + 1 entries each)
+    uint32_t* tile_cands = nullptr;
+    uint32_t* tile_trk = nullptr;
+    uint32_t* tile_at = nullptr;
+    uint32_t* group_tile = nullptr;  // E * T + 1: the tile of a candidate group's end
+    uint32_t* scratch = nullptr;     // 3 * n_trk_cap: the tiles' tracklets in arrival order
+};
+constexpr uint32_t kFusedPoolCap = 8192, kFusedPoolBlocks = 2048;
 
 // layer-triplet configuration columns on the device (TripletConfigurations,
 // src/datastructures/TripletConfiguration.h:33) and GeometrySupplement radii
@@ -120,11 +153,23 @@ uint32_t predict_fused_tiles(uint64_t n_pair_cap);
 bool predict_variant_supported(uint32_t block, uint32_t list, uint32_t sort);
 // resolves the variant's resident-block count (call outside graph capture)
 void predict_fused_prepare(const LaunchCfg& lc);
-hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
-                                const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
-                                const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
-                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
+// builds the fine walk structure of every (event, layer) first, then walks it
+hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
+                                uint32_t n_pair_cap, uint32_t n_cand_cap, const uint32_t* grid, const float* hx,
+                                const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,
+                                const uint32_t* hevt, const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets,
+                                DevCounts* cnt, const FineBufs& fb, const ScanScratch& ss, const LaunchCfg& lc,
+                                hipStream_t st);
+
+// predictor and filter fused (batched path): tracklets straight from the
+// walk, candidates counted (and their group offsets) but not stored; the
+// tracklet offsets come from launch_filter_offsets afterwards
+hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
+                                 uint32_t n_pair_cap, uint32_t n_trk_cap, const uint32_t* grid, const float* hx,
+                                 const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,
+                                 const uint32_t* hevt, const uint32_t* pairs, uint32_t* cand_offsets, uint32_t* th1,
+                                 uint32_t* th2, uint32_t* th3, DevCounts* cnt, const FineBufs& fb,
+                                 const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
 
 // TripletThetaPhiFilter: count (bitmap + per-word counts) -> scan -> store (+ offsets)
 hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const float* hx,
@@ -136,6 +181,10 @@ hipError_t launch_filter_store(const TripletCols& tc, uint32_t n_cand_cap, uint3
                                const uint32_t* cands, const uint64_t* bitmap, const uint32_t* word_counts,
                                uint32_t* th1, uint32_t* th2, uint32_t* th3, uint32_t* t_off,
                                DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+// filterOffsetStore alone, over tracklets already stored (the offsets array zeroed before)
+hipError_t launch_filter_offsets(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
+                                 const uint32_t* hevt, const uint32_t* th1, uint32_t* t_off, DevCounts* cnt,
+                                 const LaunchCfg& lc, hipStream_t st);
 
 // TripletConnectivityTight
 struct LinkBufs {

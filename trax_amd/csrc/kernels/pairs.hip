@@ -24,6 +24,9 @@
 //
 // Every window walk reads the hit's phi from the column the grid scatter
 // wrote (the same atan2 of the same coordinates) instead of recomputing it.
+#include <type_traits>
+
+#include "filter_cuts.h"
 #include "kernels.h"
 
 namespace trax {
@@ -353,18 +356,21 @@ __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __re
 }
 
 // triplet offsets at the end of each (event, layer triplet) group (.h:404-418)
-__device__ inline void pred_group_end(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
-                                      const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
-                                      uint32_t h1, uint32_t next, uint32_t* cand_offsets) {
-    if (h1 == kSentinel) return;
+// returns the offsets entry written, kSentinel when none
+__device__ inline uint32_t pred_group_end(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
+                                          const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
+                                          uint32_t h1, uint32_t next, uint32_t* cand_offsets) {
+    if (h1 == kSentinel) return kSentinel;
     const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
-    if (ev >= n_events || lt >= tc.n_trip) return;
+    if (ev >= n_events || lt >= tc.n_trip) return kSentinel;
     bool last = true;
     if (gid + 1 < n) {
         const uint32_t nh = pairs[2 * (size_t)(gid + 1)];
         if (nh != kSentinel) last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
     }
-    if (last) cand_offsets[ev * tc.n_trip + lt + 1] = next;
+    if (!last) return kSentinel;
+    cand_offsets[ev * tc.n_trip + lt + 1] = next;
+    return ev * tc.n_trip + lt + 1;
 }
 
 __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
@@ -424,7 +430,7 @@ struct PredWin2 {
     const uint32_t* lg3;
     float zLowA, zHighA, zLowB, zHighB, phiLow, phiHigh;
     uint32_t zLoA, zHiA, zLoB, zHiB, pLo, pHi;
-    uint32_t ev, lt;
+    uint32_t ev, lt, el;  // el: the (event, layer) item of the third layer's grid
     bool wrap;
 };
 
@@ -456,7 +462,8 @@ __device__ inline bool pred_windows(const GridGeom& g, const TripletCols& tc, co
     if (layer >= g.n_layers || layer >= tc.n_geom) return false;
     w.ev = ev;
     w.lt = lt;
-    w.lg3 = grid + ((size_t)ev * g.n_layers + layer) * g.cells;
+    w.el = ev * g.n_layers + layer;
+    w.lg3 = grid + (size_t)w.el * g.cells;
     const float sigmaZ = tc.sigma_z[lt], sigmaPhi = tc.sigma_phi[lt];
     const float minR = tc.min_r[layer], maxR = tc.max_r[layer];
     const float mrc = tc.min_radius_curvature;
@@ -607,6 +614,161 @@ __device__ inline void pred_walk_flat(const PredWin2& w, uint32_t np1, const flo
     }
 }
 
+// Sub-row of a hit at height z in grid row zs: floor((z - row start) * F /
+// sector_z) clamped to [0, F).  For a fixed row it is a non-decreasing function
+// of z, which is all the fine walk relies on: every hit of row zs with
+// zLow <= z <= zHigh lies in the sub-rows fine_bin(zLow) .. fine_bin(zHigh).
+__device__ inline float fine_row_lo(const GridGeom& g, uint32_t zs) { return g.min_z + (float)zs * g.sector_z; }
+__device__ inline uint32_t fine_bin(const FineGeom& fg, float row_lo, float z) {
+    const int32_t s = tx_f2i_sat(floorf((z - row_lo) * fg.inv_sub));
+    return s < 0 ? 0u : (s >= (int32_t)fg.F ? fg.F - 1u : (uint32_t)s);
+}
+
+// the fine rows [fLo, fHi) of a window whose grid rows are [zLo, zHi) and
+// whose z test is zLow <= z <= zHigh: they hold every hit of those grid rows
+// that passes the z test
+__device__ inline void fine_range(const GridGeom& g, const FineGeom& fg, uint32_t zLo, uint32_t zHi, float zLow,
+                                  float zHigh, uint32_t& fLo, uint32_t& fHi) {
+    fLo = fHi = 0;
+    if (zLo >= zHi) return;
+    fLo = (zLo << fg.shift) + fine_bin(fg, fine_row_lo(g, zLo), zLow);
+    fHi = ((zHi - 1) << fg.shift) + fine_bin(fg, fine_row_lo(g, zHi - 1), zHigh) + 1u;
+    if (fHi < fLo) fHi = fLo;
+}
+
+// Fine structure of one (event, layer, grid row): the row's hits counted per
+// (sub-row, phi cell) by the thread that owns the cell (cells hold their hits
+// in input order, so a sequential pass keeps the order stable), an LDS scan
+// in (sub-row, cell) order offset by the row's first hit, then the scatter.
+// One block per item; items are (event, layer) * n_z + row.
+__global__ __launch_bounds__(256) void k_fine_grid(GridGeom g, FineGeom fg, const uint32_t* __restrict__ grid,
+                                                  const float2* __restrict__ hzp, uint32_t n_hit_cap,
+                                                  const DevCounts* __restrict__ cnt, uint32_t* __restrict__ fgrid,
+                                                  float4* __restrict__ fh) {
+    extern __shared__ uint32_t s_cnt[];
+    const uint32_t np1 = g.n_phi + 1, nb = fg.F * np1;
+    uint32_t* wsum = s_cnt + nb;
+    const uint32_t n_items = cnt->n_events * g.n_layers * g.n_z;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t el = item / g.n_z, zs = item - el * g.n_z;
+        const uint32_t* lg = grid + (size_t)el * g.cells + (size_t)zs * np1;
+        uint32_t* fo = fgrid + (size_t)el * fg.cells + (size_t)zs * nb;
+        const float row_lo = fine_row_lo(g, zs);
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) s_cnt[i] = 0;
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < np1; c += blockDim.x) {
+            const uint32_t p1 = lg[c + 1] < n_hit_cap ? lg[c + 1] : n_hit_cap;
+            for (uint32_t p = lg[c]; p < p1; ++p) s_cnt[fine_bin(fg, row_lo, hzp[p].x) * np1 + c] += 1u;
+        }
+        __syncthreads();
+        lds_excl_scan(s_cnt, nb, wsum);
+        const uint32_t base = lg[0];
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) fo[i] = base + s_cnt[i];
+        if (zs + 1 == g.n_z && threadIdx.x == 0) fo[nb] = lg[np1];  // the final entry: end of the last row
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c < np1; c += blockDim.x) {
+            const uint32_t p1 = lg[c + 1] < n_hit_cap ? lg[c + 1] : n_hit_cap;
+            for (uint32_t p = lg[c]; p < p1; ++p) {
+                const float2 q = hzp[p];
+                uint32_t& cur = s_cnt[fine_bin(fg, row_lo, q.x) * np1 + c];
+                const uint32_t pos = base + cur;
+                cur += 1u;
+                if (pos < n_hit_cap) fh[pos] = make_float4(q.x, q.y, __uint_as_float(p), 0.0f);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The merged walk of both windows over the fine rows (as pred_walk_flat over
+// the grid rows, but each grid row is visited only in the sub-rows that can
+// hold a hit passing the window's z test).  countA is the same number: the
+// same hits pass.  Window B's matches are appended to `my` (at most cap, as
+// 16-bit offsets of their grid indices from `first`) in walk order,
+// (sub-row, phi segment, cell, grid index); the reference's order is (grid
+// row, phi segment, grid index), the hits of the first phi segment (cells
+// pLo..) before those of the wrapped one (cells ..pHi).  Entry k's key in that
+// order is built from the bits the walk records: `rmask` bit k marks the first
+// entry of a grid row (grid rows arrive in order), `smask` bit k an entry of
+// the wrapped segment; entry k's grid-row ordinal is the number of rmask bits
+// at or below k.  Matches beyond cap go to over(ordinal, wrapped, grid index).
+// nB counts every match.
+template <class M>
+__device__ inline M fine_bit(uint32_t k) { return (M)1 << k; }
+template <class M, class OV>
+__device__ inline void pred_walk_fine(const PredWin2& w, const GridGeom& g, const FineGeom& fg,
+                                      const uint32_t* __restrict__ fl, const float4* __restrict__ fh, uint32_t cap,
+                                      uint32_t first, uint16_t* my, uint32_t& countA, uint32_t& nB, M& rmask,
+                                      M& smask, OV&& over) {
+    uint32_t aLo, aHi, bLo, bHi;
+    fine_range(g, fg, w.zLoA, w.zHiA, w.zLowA, w.zHighA, aLo, aHi);
+    fine_range(g, fg, w.zLoB, w.zHiB, w.zLowB, w.zHighB, bLo, bHi);
+    const bool ea = aLo < aHi, eb = bLo < bHi;
+    uint32_t f = 0, f1 = 0;
+    if (ea || eb) {
+        f = !ea ? bLo : (!eb ? aLo : (aLo < bLo ? aLo : bLo));
+        f1 = !ea ? bHi : (!eb ? aHi : (aHi > bHi ? aHi : bHi));
+    }
+    const uint32_t np1 = g.n_phi + 1;
+    const float PI = TX_PI_F;
+    const uint32_t wr = (uint32_t)w.wrap;
+    uint32_t j = 0, end2 = 0, fEnd = 0, fLen = 0;
+    bool sa = false, sb = false;
+    uint32_t row = 0xffffffffu, ord = 0;  // grid row of the last match, grid rows with matches so far
+    auto next_row = [&]() {
+        for (; f < f1; ++f) {
+            sa = ea && f >= aLo && f < aHi;
+            sb = eb && f >= bLo && f < bHi;
+            if (!(sa || sb)) continue;
+            const uint32_t* r = fl + (size_t)f * np1;
+            const uint32_t fStart = r[0];
+            fEnd = r[np1];
+            fLen = fEnd - fStart;
+            end2 = wr * fEnd + r[w.pHi] - wr * fStart;
+            j = r[w.pLo];
+            if (j < end2) return true;
+        }
+        return false;
+    };
+    bool active = next_row();
+    while (active) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t jj = j + u < end2 ? j + u : j;  // j < end2: always a hit of the row
+            q[u] = fh[jj - (uint32_t)(jj >= fEnd) * fLen];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float z = q[u].x, ph = q[u].y;
+            const bool inr = j + u < end2;
+            const bool p1 = (w.phiLow <= ph) & (ph <= w.phiHigh);
+            const bool p2 = ((w.phiLow <= ph) & (ph <= PI)) | ((-PI <= ph) & (ph <= w.phiHigh));
+            const bool pin = inr & ((w.wrap & p2) | (!w.wrap & p1));
+            countA += (uint32_t)(pin & sa & (w.zLowA <= z) & (z <= w.zHighA));
+            if (pin & sb & (w.zLowB <= z) & (z <= w.zHighB)) {
+                const uint32_t rf = f >> fg.shift;
+                const bool new_row = rf != row, seg2 = j + u >= fEnd;
+                row = rf;
+                ord += (uint32_t)new_row;
+                if (nB < cap) {
+                    rmask |= new_row ? fine_bit<M>(nB) : (M)0;
+                    smask |= seg2 ? fine_bit<M>(nB) : (M)0;
+                    my[nB] = (uint16_t)(__float_as_uint(q[u].z) - first);
+                } else {
+                    over(ord, seg2, __float_as_uint(q[u].z));
+                }
+                nB++;
+            }
+        }
+        j += 4;
+        if (j >= end2) {
+            ++f;
+            active = next_row();
+        }
+    }
+}
+
 // Count, scan and store of the predictor in one pass over tiles of
 // kPredBlock pairs (batched path).  One merged walk counts the count-window
 // matches and collects the store-window matches of every pair in LDS
@@ -634,13 +796,34 @@ __device__ inline void tile_store(TileWindows<B>& t, uint32_t i, const PredWin2&
 }
 
 template <int B>
-__device__ inline void tile_load(const TileWindows<B>& t, uint32_t i, PredWin2& w, const uint32_t* grid) {
+__device__ inline void tile_load(const TileWindows<B>& t, uint32_t i, PredWin2& w, const uint32_t* grid,
+                                 uint32_t g_cells) {
     w.zLowA = t.zLowA[i]; w.zHighA = t.zHighA[i]; w.zLowB = t.zLowB[i]; w.zHighB = t.zHighB[i];
     w.phiLow = t.phiLow[i]; w.phiHigh = t.phiHigh[i];
     w.lg3 = grid + t.lg3[i];
+    w.el = t.lg3[i] / g_cells;
     w.zLoA = t.zA[i] & 0xffffu; w.zHiA = t.zA[i] >> 16;
     w.zLoB = t.zB[i] & 0xffffu; w.zHiB = t.zB[i] >> 16;
     w.pLo = t.pr[i] & 0xffffu; w.pHi = (t.pr[i] >> 16) & 0x7fffu; w.wrap = (t.pr[i] >> 31) != 0u;
+}
+
+// The store window's first `keep` matches in the reference's order (the grid
+// walk of TripletThetaPhiPredictor's store kernel): f(k, grid index).
+template <class F>
+__device__ inline void grid_walk_store(const PredWin2& pw, uint32_t np1, const float2* __restrict__ hzp, uint32_t keep,
+                                       F f) {
+    uint32_t k = 0;
+    for (uint32_t zs = pw.zLoB; zs < pw.zHiB && k < keep; ++zs) {
+        const uint32_t zStart = pw.lg3[zs * np1];
+        const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
+        const uint32_t zLen = zEnd - zStart;
+        const uint32_t end2 = (uint32_t)pw.wrap * zEnd + pw.lg3[zs * np1 + pw.pHi] - (uint32_t)pw.wrap * zStart;
+        for (uint32_t j = pw.lg3[zs * np1 + pw.pLo]; j < end2 && k < keep; ++j) {
+            const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
+            const float2 q = hzp[index];
+            if (phi_in(pw, q.y) && pw.zLowB <= q.x && q.x <= pw.zHighB) f(k++, index);
+        }
+    }
 }
 
 // hits a window walk visits (both windows' z sectors)
@@ -662,18 +845,23 @@ __device__ inline uint32_t walk_work(const PredWin2& w, uint32_t np1) {
 // kSort: the tile's pairs are walked in decreasing order of their walk
 // length (16 logarithmic buckets), so that the lanes of a wave walk windows
 // of similar length; each pair's results stay indexed by the pair.
+// Six waves per SIMD: the walks are latency-bound, so the register budget is
+// capped at 80 VGPRs (the variants that fit without spilling).
 template <int kPredBlock, int kPredList, bool kSort>
-__global__ __launch_bounds__(kPredBlock) void k_predict_fused(
+__global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_predict_fused(
     GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
     const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
-    DevCounts* __restrict__ cnt, uint64_t* __restrict__ status, uint32_t* __restrict__ counter) {
+    DevCounts* __restrict__ cnt, uint64_t* __restrict__ status, uint32_t* __restrict__ counter, FineGeom fg,
+    const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh) {
     constexpr int kSortB = kSort ? kPredBlock : 1;
+    using Mask = typename std::conditional<(kPredList <= 32), uint32_t, uint64_t>::type;
+    static_assert(kPredList <= 64, "list masks hold 64 entries");
     __shared__ uint16_t s_list[kPredBlock * kPredList];
-    __shared__ uint32_t s_excl[kPredBlock + 1];
+    __shared__ uint32_t s_excl[kPredBlock + 1];  // output slots (count window) before each pair
     __shared__ uint32_t s_nb[kPredBlock];
-    __shared__ uint32_t s_b3[kPredBlock];  // first hit of the pair's third layer; kSentinel: no list
+    __shared__ uint32_t s_b3[kPredBlock];  // first hit of the pair's third layer; kSentinel: not listed
     __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
     __shared__ uint32_t s_tile, s_base;
     __shared__ TileWindows<kSortB> s_win;
@@ -709,7 +897,36 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
             lb3 = listed ? first : kSentinel;
             const uint32_t cap = listed ? (uint32_t)kPredList : 0u;
             uint16_t* my = s_list + pidx * kPredList;
-            pred_walk_flat(w, np1, hzp, cap, first, my, cA, nb);
+            Mask rm = 0, sm = 0;
+            pred_walk_fine<Mask>(w, g, fg, fgrid + (size_t)w.el * fg.cells, fh, cap, first, my, cA, nb, rm, sm,
+                                 [](uint32_t, bool, uint32_t) {});
+            // the list into the reference's order: each grid row's entries
+            // (rows arrive in order) by (segment, grid index), insertion
+            // sort carrying the segment bits; runs arrive sorted, so the
+            // cost is the entries plus their inversions
+            const uint32_t m = nb < cap ? nb : cap;
+            uint32_t row0 = 0;
+            for (uint32_t i = 1; i < m; ++i) {
+                if ((rm >> i) & 1u) {
+                    row0 = i;
+                    continue;
+                }
+                const uint32_t x = my[i], xs = (uint32_t)((sm >> i) & 1u), kx = xs << 16 | x;
+                uint32_t j = i;
+                while (j > row0) {
+                    const uint32_t y = my[j - 1], ys = (uint32_t)((sm >> (j - 1)) & 1u);
+                    if ((ys << 16 | y) < kx) break;
+                    my[j] = (uint16_t)y;
+                    sm = (sm & ~fine_bit<Mask>(j)) | ((Mask)ys << j);
+                    --j;
+                }
+                if (j != i) {
+                    my[j] = (uint16_t)x;
+                    sm = (sm & ~fine_bit<Mask>(j)) | ((Mask)xs << j);
+                }
+            }
+            // a list without every match is not used: the grid walk writes them
+            if (nb > cap) lb3 = kSentinel;
         };
         if (!kSort) {
             if (have) walk(tid, pw, countA, nB, b3);
@@ -740,7 +957,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
             uint32_t cA = 0, nb = 0, lb3 = kSentinel;
             if (s_win.lg3[q] != kSentinel) {
                 PredWin2 w;
-                tile_load(s_win, q, w, grid);
+                tile_load(s_win, q, w, grid, g.cells);
                 walk(q, w, cA, nb, lb3);
             }
             s_cnt[q] = cA;
@@ -789,30 +1006,17 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
                     cands[out + 1] = kSentinel;
                 }
             }
-            // pairs with more store-side matches than their LDS list holds
+            const uint64_t o = (uint64_t)base + excl;
+            // pairs whose matches are not all listed: the grid walk in the
+            // reference's order
             const uint32_t keep = countA < nB ? countA : nB;
-            const uint32_t listed = b3 != kSentinel ? (uint32_t)kPredList : 0u;
-            if (keep > listed) {
-                uint32_t k = 0;
-                const uint64_t o = (uint64_t)base + excl;
-                for (uint32_t zs = pw.zLoB; zs < pw.zHiB && k < keep; ++zs) {
-                    const uint32_t zStart = pw.lg3[zs * np1];
-                    const uint32_t zEnd = pw.lg3[(zs + 1) * np1];
-                    const uint32_t zLen = zEnd - zStart;
-                    const uint32_t end2 =
-                        (uint32_t)pw.wrap * zEnd + pw.lg3[zs * np1 + pw.pHi] - (uint32_t)pw.wrap * zStart;
-                    for (uint32_t j = pw.lg3[zs * np1 + pw.pLo]; j < end2 && k < keep; ++j) {
-                        const uint32_t index = j - (uint32_t)(j >= zEnd) * zLen;
-                        const float2 q = hzp[index];
-                        if (phi_in(pw, q.y) && pw.zLowB <= q.x && q.x <= pw.zHighB) {
-                            if (k >= listed) {
-                                cands[2 * (o + k)] = gid;
-                                cands[2 * (o + k) + 1] = index;
-                            }
-                            k++;
-                        }
-                    }
-                }
+            if (b3 == kSentinel && keep > 0) {
+                // the window again (rare path; keeps it out of registers during the walk)
+                pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, pairs[2 * (size_t)gid + 1], pw);
+                grid_walk_store(pw, np1, hzp, keep, [&](uint32_t k, uint32_t index) {
+                    cands[2 * (o + k)] = gid;
+                    cands[2 * (o + k) + 1] = index;
+                });
             }
         }
         if (gid < n) {
@@ -821,6 +1025,283 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_fused(
             pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, base + excl + countA, cand_offsets);
         }
         __syncthreads();  // the LDS lists are reused by the next tile
+    }
+    if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
+    if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// Predictor and filter in one pass (batched path with the filter fused).
+// Per tile of kPredBlock pairs:
+//  1. each pair walks its windows as in k_predict_fused: the count-window
+//     matches are counted, the store-window matches (the candidates) are
+//     listed in LDS (kPredList per pair, 16-bit offsets from the layer's first
+//     hit) and the rest go to the block's pool in global memory (grid index,
+//     grid-row ordinal, phi segment), so no pair walks twice;
+//  2. a pair with more store-window matches than count-window slots keeps as
+//     candidates only the first countA in the reference's order (a rank
+//     over its matches; rare);
+//  3. the tile's candidates are tested against the filter's cuts, by waves
+//     of 64: the phi-cut prefilter first, the undecided ones queued in the
+//     wave's LDS and tested 64 at a time with the full cuts;
+//  4. the tile takes room for its tracklets in a scratch array with one
+//     atomic (tiles finish in any order, none waits for another) and every
+//     pair writes its valid triplets there in the reference's candidate
+//     order, (grid-row ordinal, phi segment, grid index); the tile records
+//     its candidate and tracklet counts and where its tracklets went.
+// The candidates are not stored.  Afterwards the tile counts are scanned and
+// each tile's tracklets are copied to their place in pair order
+// (k_tile_tracklets), and the candidate group offsets, written tile-local,
+// get their tile's offset (k_group_offsets).
+template <int kPredBlock, int kPredList>
+__global__ __launch_bounds__(kPredBlock) void k_predict_filter(
+    GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
+    const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
+    const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
+    uint32_t* __restrict__ cand_offsets, uint32_t* __restrict__ th1, uint32_t* __restrict__ th2,
+    uint32_t* __restrict__ th3, uint32_t n_trk_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
+    FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, uint2* __restrict__ pools,
+    uint32_t pool_cap, uint32_t* __restrict__ tile_cands, uint32_t* __restrict__ tile_trk,
+    uint32_t* __restrict__ tile_at, uint32_t* __restrict__ group_tile) {
+    using Mask = typename std::conditional<(kPredList <= 32), uint32_t, uint64_t>::type;
+    static_assert(kPredList <= 64, "list masks hold 64 entries");
+    constexpr uint32_t kWaves = kPredBlock / kWave, kQueue = 2 * kWave;
+    constexpr uint32_t kValidBit = 0x80000000u, kDeadBit = 0x40000000u;  // pool entry flags (x word)
+    __shared__ uint16_t s_list[kPredBlock * kPredList];
+    __shared__ uint32_t s_eex[kPredBlock + 1];  // listed candidates before each pair
+    __shared__ uint32_t s_first[kPredBlock];    // first hit of the pair's third layer
+    __shared__ uint32_t s_h1[kPredBlock], s_h2[kPredBlock];
+    __shared__ uint32_t s_lt[kPredBlock];       // layer triplet | pool entries << 8
+    __shared__ Mask s_valid[kPredBlock];        // listed candidates that pass the cuts
+    __shared__ Mask s_dead[kPredBlock];         // listed matches that are not candidates
+    __shared__ Mask s_rm[kPredBlock], s_sm[kPredBlock];
+    __shared__ uint32_t s_pool_valid[kPredBlock];
+    __shared__ uint32_t s_queue[kWaves * kQueue];
+    __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
+    __shared__ uint32_t s_tile, s_tbase, s_pool_n;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    const uint32_t n_tiles = (n + kPredBlock - 1) / kPredBlock;
+    if (blockIdx.x == 0 && tid == 0) cnt->n_pred_tiles = n_tiles;
+    uint2* pool = pools + (size_t)blockIdx.x * pool_cap;
+    uint32_t short_slots = 0, trunc = 0;
+    bool over = false;
+    while (true) {
+        if (tid == 0) {
+            s_tile = atomicAdd(counter, 1u);
+            s_pool_n = 0;
+        }
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
+        uint32_t countA = 0, nB = 0, listed = 0;
+        Mask rm = 0, sm = 0;
+        {
+            // the pair's hits and layer triplet go to LDS before the walk (fewer registers held across it)
+            const uint32_t gid = tile * kPredBlock + tid;
+            PredWin2 pw;
+            bool have = false;
+            uint32_t h1 = kSentinel, h2 = kSentinel, first = 0, lt = 0;
+            if (gid < n) {
+                h1 = pairs[2 * (size_t)gid];
+                h2 = pairs[2 * (size_t)gid + 1];
+                have = h1 != kSentinel && h2 != kSentinel &&
+                       pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw);
+            }
+            if (have) {
+                lt = pw.lt;
+                first = pw.lg3[0];
+            }
+            s_h1[tid] = h1;
+            s_h2[tid] = h2;
+            s_first[tid] = first;
+            s_lt[tid] = lt;
+            if (have) {
+                const uint32_t cap = pw.lg3[g.cells - 1] - first <= 0xffffu ? (uint32_t)kPredList : 0u;
+                pred_walk_fine<Mask>(pw, g, fg, fgrid + (size_t)pw.el * fg.cells, fh, cap, first,
+                                     s_list + tid * kPredList, countA, nB, rm, sm,
+                                     [&](uint32_t ord, bool seg2, uint32_t h3) {
+                                         const uint32_t slot = atomicAdd(&s_pool_n, 1u);
+                                         if (slot < pool_cap)
+                                             pool[slot] = make_uint2(tid | (uint32_t)seg2 << 16 | (ord & 0x1fffu) << 17,
+                                                                     h3);
+                                     });
+                listed = nB < cap ? nB : cap;
+            }
+        }
+        uint32_t total_c, total_e;
+        const uint32_t excl_c = block_excl_scan(countA, s_wsum, &total_c);
+        const uint32_t excl_e = block_excl_scan(listed, s_wsum, &total_e);
+        if (tid == 0 && tile < n_tiles) tile_cands[tile] = total_c;
+        s_eex[tid] = excl_e;
+        s_lt[tid] |= (nB - listed) << 8;
+        s_valid[tid] = 0;
+        s_dead[tid] = 0;
+        s_rm[tid] = rm;
+        s_sm[tid] = sm;
+        s_pool_valid[tid] = 0;
+        if (tid == 0) s_eex[kPredBlock] = total_e;
+        __syncthreads();
+        const uint32_t n_pool = s_pool_n < pool_cap ? s_pool_n : pool_cap;
+        if (tid == 0 && s_pool_n > pool_cap) atomicOr(&cnt->err, kErrCapacity);
+        if (tile * kPredBlock + tid < n) {
+            short_slots += countA > nB ? countA - nB : 0u;
+            trunc += (countA > 0 && nB > countA);
+            // the group's end, tile-local; k_group_offsets adds the tile's offset
+            const uint32_t g_end = pred_group_end(tc, hlayer, hevt, pairs, n, n_events, tile * kPredBlock + tid,
+                                                  s_h1[tid], excl_c + countA, cand_offsets);
+            if (g_end != kSentinel) group_tile[g_end] = tile;
+        }
+        // entry keys in the reference's order: (grid-row ordinal, phi segment, grid index)
+        auto list_key = [&](uint32_t lo, uint32_t i) {
+            const Mask r = s_rm[lo];
+            const uint64_t ord = (uint64_t)__builtin_popcountll((uint64_t)(r & (((Mask)2 << i) - 1)));
+            return ord << 33 | (uint64_t)((s_sm[lo] >> i) & 1u) << 32 | (s_first[lo] + s_list[lo * kPredList + i]);
+        };
+        auto pool_key = [](uint2 e) { return (uint64_t)((e.x >> 17) & 0x1fffu) << 33 | (uint64_t)((e.x >> 16) & 1u) << 32 | e.y; };
+        // 2. only the first countA matches (in the reference's order) are candidates
+        if (nB > countA && nB > 0) {
+            const uint32_t n_own_pool = nB - listed;
+            auto rank = [&](uint64_t k) {
+                uint32_t r = 0;
+                for (uint32_t i = 0; i < listed; ++i) r += list_key(tid, i) < k;
+                if (n_own_pool)
+                    for (uint32_t p = 0; p < n_pool; ++p) {
+                        const uint2 e = pool[p];
+                        r += (e.x & 0xffffu) == tid && pool_key(e) < k;
+                    }
+                return r;
+            };
+            Mask dead = 0;
+            for (uint32_t i = 0; i < listed; ++i) dead |= rank(list_key(tid, i)) >= countA ? (Mask)1 << i : (Mask)0;
+            s_dead[tid] = dead;
+            if (n_own_pool)
+                for (uint32_t p = 0; p < n_pool; ++p) {
+                    const uint2 e = pool[p];
+                    if ((e.x & 0xffffu) == tid && rank(pool_key(e)) >= countA) pool[p].x = e.x | kDeadBit;
+                }
+        }
+        __syncthreads();
+        // 3. the candidates against the filter's cuts, by waves: listed entries
+        //    [0, total_e), then the pool [total_e, total_e + n_pool)
+        {
+            uint32_t* q = s_queue + wave * kQueue;
+            uint32_t qn = 0;
+            const uint32_t n_all = total_e + n_pool;
+            const uint32_t per = (n_all + kWaves - 1) / kWaves;
+            const uint32_t e0 = wave * per, e1 = e0 + per < n_all ? e0 + per : n_all;
+            // a queued entry's hits: tag = lo << 16 | i (listed) or kValidBit | p (pool)
+            auto hits_of = [&](uint32_t tag, uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& l) {
+                uint32_t lo;
+                if (tag & kValidBit) {
+                    const uint2 e = pool[tag & ~kValidBit];
+                    lo = e.x & 0xffffu;
+                    c = e.y;
+                } else {
+                    lo = tag >> 16;
+                    c = s_first[lo] + s_list[lo * kPredList + (tag & 0xffffu)];
+                }
+                a = s_h1[lo];
+                b = s_h2[lo];
+                l = s_lt[lo] & 0xffu;
+                return lo;
+            };
+            auto test_queue = [&](uint32_t k0) {
+                const uint32_t tag = q[k0 + lane];
+                uint32_t a, b, c, l;
+                const uint32_t lo = hits_of(tag, a, b, c, l);
+                if (filter_valid(tc, hx, hy, hz, a, b, c, l)) {
+                    if (tag & kValidBit) {
+                        atomicAdd(&s_pool_valid[lo], 1u);
+                        pool[tag & ~kValidBit].x |= kValidBit;
+                    } else {
+                        atomicOr(&s_valid[lo], (Mask)1 << (tag & 0xffffu));
+                    }
+                }
+            };
+            for (uint32_t eb = e0; eb < e1; eb += kWave) {
+                const uint32_t e = eb + lane;
+                bool undecided = false;
+                uint32_t tag = 0;
+                if (e < e1) {
+                    if (e < total_e) {
+                        uint32_t lo = 0, hi = kPredBlock;
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_eex[mid] <= e) lo = mid; else hi = mid;
+                        }
+                        const uint32_t i = e - s_eex[lo];
+                        tag = lo << 16 | i;
+                        undecided = !((s_dead[lo] >> i) & 1u);
+                    } else {
+                        tag = kValidBit | (e - total_e);
+                        undecided = !(pool[e - total_e].x & kDeadBit);
+                    }
+                    if (undecided) {
+                        uint32_t a, b, c, l;
+                        hits_of(tag, a, b, c, l);
+                        undecided = !tx_phi_surely_outside(hx[b] - hx[a], hy[b] - hy[a], hx[c] - hx[b], hy[c] - hy[b],
+                                                           tc.tan_dphi[l]);
+                    }
+                }
+                const uint64_t m = __ballot(undecided);
+                if (undecided) q[qn + (uint32_t)__popcll(m & lanemask_lt())] = tag;
+                qn += (uint32_t)__popcll(m);
+                wave_sync();
+                if (qn >= kWave) {
+                    qn -= kWave;
+                    test_queue(qn);
+                    wave_sync();
+                }
+            }
+            if (lane < qn) test_queue(0);
+        }
+        __syncthreads();
+        // 4. the pair's valid triplets, scanned, then written in the reference's order
+        const Mask vm = s_valid[tid];
+        const uint32_t nv = (uint32_t)__builtin_popcountll((uint64_t)vm) + s_pool_valid[tid];
+        uint32_t total_t;
+        const uint32_t excl_t = block_excl_scan(nv, s_wsum, &total_t);
+        if (tid == 0) {
+            const uint32_t at = total_t ? atomicAdd(&cnt->trk_alloc, total_t) : 0u;
+            s_tbase = at;
+            if (tile < n_tiles) {
+                tile_trk[tile] = total_t;
+                tile_at[tile] = at;
+            }
+        }
+        __syncthreads();
+        const uint32_t tbase = s_tbase;
+        if (nv) {
+            const uint32_t o = tbase + excl_t, pv = s_pool_valid[tid];
+            auto write = [&](uint64_t k, uint32_t h3) {
+                uint32_t r = 0;
+                for (Mask m = vm; m; m &= m - 1) r += list_key(tid, (uint32_t)__builtin_ctzll((uint64_t)m)) < k;
+                if (pv)
+                    for (uint32_t p = 0; p < n_pool; ++p) {
+                        const uint2 e = pool[p];
+                        r += (e.x & 0xffffu) == tid && (e.x & kValidBit) && pool_key(e) < k;
+                    }
+                const uint64_t pos = (uint64_t)o + r;
+                if (pos < n_trk_cap) {
+                    th1[pos] = s_h1[tid];
+                    th2[pos] = s_h2[tid];
+                    th3[pos] = h3;
+                } else {
+                    over = true;
+                }
+            };
+            for (Mask m = vm; m; m &= m - 1) {
+                const uint32_t i = (uint32_t)__builtin_ctzll((uint64_t)m);
+                write(list_key(tid, i), s_first[tid] + s_list[tid * kPredList + i]);
+            }
+            if (pv)
+                for (uint32_t p = 0; p < n_pool; ++p) {
+                    const uint2 e = pool[p];
+                    if ((e.x & 0xffffu) == tid && (e.x & kValidBit)) write(pool_key(e), e.y);
+                }
+        }
+        __syncthreads();  // the LDS lists and the pool are reused by the next tile
     }
     if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
     if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
@@ -913,13 +1394,82 @@ hipError_t predict_fused_variant(const GridGeom& g, const TripletCols& tc, uint3
                                  const uint32_t* grid, const float* hx, const float* hy, const float* hz,
                                  const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                  const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
-                                 uint64_t* status, uint32_t* counter, hipStream_t st) {
+                                 uint64_t* status, uint32_t* counter, const FineGeom& fg, const FineBufs& fb,
+                                 hipStream_t st) {
     // persistent: as many blocks as stay resident, each loops over tiles
     const uint32_t resident = resident_blocks<B, L, S>();
     const uint64_t tiles = (uint64_t)n_pair_cap / B + 1;
     const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
     hipLaunchKernelGGL((k_predict_fused<B, L, S>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp,
-                       hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt, status, counter);
+                       hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt, status, counter, fg, fb.grid,
+                       fb.hits);
+    return hipGetLastError();
+}
+
+// each tile's tracklets from the scratch to their place in pair order: one
+// wave per tile (tile_trk scanned: a tile's count is the next entry minus its)
+__global__ void k_tile_tracklets(const uint32_t* __restrict__ tile_trk, const uint32_t* __restrict__ tile_at,
+                                 const uint32_t* __restrict__ scratch, uint32_t* __restrict__ th1,
+                                 uint32_t* __restrict__ th2, uint32_t* __restrict__ th3, uint32_t n_trk_cap,
+                                 DevCounts* __restrict__ cnt) {
+    const uint32_t n_tiles = cnt->n_pred_tiles;
+    const uint32_t lane = lane_id();
+    bool over = false;
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n_tiles;
+         t += gridDim.x * blockDim.x / kWave) {
+        const uint32_t dst = tile_trk[t], m = tile_trk[t + 1] - dst, src = tile_at[t];
+        for (uint32_t k = lane; k < m; k += kWave) {
+            if ((uint64_t)src + k < n_trk_cap && (uint64_t)dst + k < n_trk_cap) {
+                th1[dst + k] = scratch[src + k];
+                th2[dst + k] = scratch[(size_t)n_trk_cap + src + k];
+                th3[dst + k] = scratch[2 * (size_t)n_trk_cap + src + k];
+            } else {
+                over = true;
+            }
+        }
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// candidate group ends written tile-local get their tile's offset (tile_cands scanned)
+__global__ void k_group_offsets(uint32_t* __restrict__ cand_offsets, const uint32_t* __restrict__ group_tile,
+                                const uint32_t* __restrict__ tile_cands, uint32_t n_trip,
+                                const DevCounts* __restrict__ cnt) {
+    const uint32_t n = cnt->n_events * n_trip + 1;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (group_tile[i] != kSentinel) cand_offsets[i] += tile_cands[group_tile[i]];
+}
+
+template <int B, int L>
+uint32_t resident_blocks_filter() {
+    static uint32_t resident = 0;
+    if (resident == 0) {
+        int per_cu = 0, dev = 0, n_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_filter<B, L>, B, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu < 1)
+            n_cu = 256;
+        resident = (uint32_t)(per_cu * n_cu);
+    }
+    return resident;
+}
+
+template <int B, int L>
+hipError_t predict_filter_variant(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_trk_cap,
+                                  const uint32_t* grid, const float* hx, const float* hy, const float* hz,
+                                  const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
+                                  const uint32_t* pairs, uint32_t* cand_offsets, uint32_t* th1, uint32_t* th2,
+                                  uint32_t* th3, DevCounts* cnt, uint32_t* counter, const FineGeom& fg,
+                                  const FineBufs& fb, hipStream_t st) {
+    uint32_t resident = resident_blocks_filter<B, L>();
+    resident = resident < fb.pool_blocks ? resident : fb.pool_blocks;  // one pool per block
+    const uint64_t tiles = (uint64_t)n_pair_cap / B + 1;
+    const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
+    hipLaunchKernelGGL((k_predict_filter<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp, hlayer,
+                       hevt, pairs, cand_offsets, th1, th2, th3, n_trk_cap, cnt, counter, fg, fb.grid, fb.hits,
+                       fb.pool, fb.pool_cap, fb.tile_cands, fb.tile_trk, fb.tile_at, fb.group_tile);
     return hipGetLastError();
 }
 
@@ -942,19 +1492,107 @@ bool predict_variant_supported(uint32_t block, uint32_t list, uint32_t sort) {
 void predict_fused_prepare(const LaunchCfg& lc) {
     const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
     const bool s = lc.predict_sort != 0;
-#define TRAX_PRED_PREP(B, L, S) if (b == B && l == L && s == S) (void)resident_blocks<B, L, S>();
+#define TRAX_PRED_PREP(B, L, S)                              \
+    if (b == B && l == L && s == S) {                        \
+        (void)resident_blocks<B, L, S>();                    \
+        if (!S) (void)resident_blocks_filter<B, L>();        \
+    }
     TRAX_PRED_VARIANTS(TRAX_PRED_PREP)
 #undef TRAX_PRED_PREP
 }
 
-hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_cand_cap,
-                                const uint32_t* grid, const float* hx, const float* hy, const float* hz,
-                                const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
-                                const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets, DevCounts* cnt,
-                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
+FineGeom fine_geom(const GridGeom& g, uint32_t F) {
+    FineGeom fg;
+    fg.F = F;
+    fg.shift = 0;
+    while ((1u << fg.shift) < F) ++fg.shift;
+    fg.inv_sub = (float)F / g.sector_z;
+    fg.cells = g.n_z * F * (g.n_phi + 1) + 1;
+    return fg;
+}
+
+uint32_t fine_max(const GridGeom& g) {
+    uint32_t F = kFineMax;
+    while (F > 1 && (uint64_t)F * (g.n_phi + 1) > 8192) F >>= 1;  // the build's LDS counts: at most 32 KiB
+    return F;
+}
+
+namespace {
+
+// the fine walk structure of every (event, layer), for the sub-row count of the launch shape
+hipError_t launch_fine_grid(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit_cap, const uint32_t* grid,
+                            const float2* hzp, DevCounts* cnt, const FineBufs& fb, const LaunchCfg& lc, FineGeom& fg,
+                            hipStream_t st) {
+    const uint32_t F = lc.predict_fine ? lc.predict_fine : 8;
+    if (F > fine_max(g) || (F & (F - 1)) != 0 || !fb.grid || !fb.hits) return hipErrorInvalidValue;
+    fg = fine_geom(g, F);
+    const uint64_t items = (uint64_t)n_el_cap * g.n_z;
+    const uint32_t blocks = (uint32_t)(items < 65536 ? (items ? items : 1) : 65536);
+    const size_t lds = ((size_t)F * (g.n_phi + 1) + 256 / kWave + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_fine_grid, dim3(blocks), dim3(256), lds, st, g, fg, grid, hzp, n_hit_cap, cnt, fb.grid,
+                       fb.hits);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
+                                 uint32_t n_pair_cap, uint32_t n_trk_cap, const uint32_t* grid, const float* hx,
+                                 const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,
+                                 const uint32_t* hevt, const uint32_t* pairs, uint32_t* cand_offsets, uint32_t* th1,
+                                 uint32_t* th2, uint32_t* th3, DevCounts* cnt, const FineBufs& fb,
+                                 const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
+    const uint32_t tiles = predict_fused_tiles(n_pair_cap);
+    if (!fb.pool || !fb.pool_cap || !fb.pool_blocks || !fb.tile_cands || !fb.tile_trk || !fb.tile_at ||
+        !fb.group_tile || !fb.scratch)
+        return hipErrorInvalidValue;
+    FineGeom fg;
+    hipError_t e = launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
+    if (e != hipSuccess) return e;
+    // tile counts (one entry past the last tile stays 0 for the scans), group tiles, the tile counter
+    for (uint32_t* a : {fb.tile_cands, fb.tile_trk})
+        if ((e = launch_fill(a, 0u, tiles + 1ull, st)) != hipSuccess) return e;
+    if ((e = launch_fill(fb.group_tile, kSentinel, (uint64_t)n_el_cap / g.n_layers * tc.n_trip + 1, st)) != hipSuccess)
+        return e;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(ss.status);
+    if ((e = launch_fill(counter, 0u, 1, st)) != hipSuccess) return e;
+    const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
+    if (lc.predict_sort) return hipErrorInvalidConfiguration;
+    e = hipErrorInvalidConfiguration;
+#define TRAX_PF_RUN(B, L, S)                                                                                      \
+    if (!S && b == B && l == L)                                                                                   \
+        e = predict_filter_variant<B, L>(g, tc, n_pair_cap, n_trk_cap, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, \
+                                         cand_offsets, fb.scratch, fb.scratch + n_trk_cap,                         \
+                                         fb.scratch + 2 * (size_t)n_trk_cap, cnt, counter, fg, fb, st);
+    TRAX_PRED_VARIANTS(TRAX_PF_RUN)
+#undef TRAX_PF_RUN
+    if (e != hipSuccess) return e;
+    // the tiles' counts in tile order: candidate and tracklet offsets per tile, and the totals
+    if ((e = launch_scan(fb.tile_cands, &cnt->n_pred_tiles, 1u, tiles + 1ull, &cnt->n_cands, ss, st)) != hipSuccess)
+        return e;
+    if ((e = launch_scan(fb.tile_trk, &cnt->n_pred_tiles, 1u, tiles + 1ull, &cnt->n_tracklets, ss, st)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_tile_tracklets, dim3(stride_blocks((uint64_t)tiles * kWave, 256, lc)), dim3(256), 0, st,
+                       fb.tile_trk, fb.tile_at, fb.scratch, th1, th2, th3, n_trk_cap, cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_group_offsets, dim3(16), dim3(256), 0, st, cand_offsets, fb.group_tile, fb.tile_cands,
+                       tc.n_trip, cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
+}
+
+hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
+                                uint32_t n_pair_cap, uint32_t n_cand_cap, const uint32_t* grid, const float* hx,
+                                const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,
+                                const uint32_t* hevt, const uint32_t* pairs, uint32_t* cands, uint32_t* cand_offsets,
+                                DevCounts* cnt, const FineBufs& fb, const ScanScratch& ss, const LaunchCfg& lc,
+                                hipStream_t st) {
     const uint32_t tiles = predict_fused_tiles(n_pair_cap);
     if (tiles > ss.max_tiles) return hipErrorInvalidValue;
-    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(ss.status), 0u, 2ull * tiles + 2, st);
+    FineGeom fg;
+    hipError_t e = launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
+    if (e != hipSuccess) return e;
+    e = launch_fill(reinterpret_cast<uint32_t*>(ss.status), 0u, 2ull * tiles + 2, st);
     if (e != hipSuccess) return e;
     uint64_t* status = ss.status;
     uint32_t* counter = reinterpret_cast<uint32_t*>(ss.status + tiles);
@@ -964,7 +1602,7 @@ hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32
 #define TRAX_PRED_RUN(B, L, S)                                                                                   \
     if (b == B && l == L && s == S)                                                                              \
         e = predict_fused_variant<B, L, S>(g, tc, n_pair_cap, n_cand_cap, grid, hx, hy, hz, hzp, hlayer, hevt,    \
-                                           pairs, cands, cand_offsets, cnt, status, counter, st);
+                                           pairs, cands, cand_offsets, cnt, status, counter, fg, fb, st);
     TRAX_PRED_VARIANTS(TRAX_PRED_RUN)
 #undef TRAX_PRED_RUN
     if (e != hipSuccess) return e;

@@ -287,6 +287,7 @@ extern "C" int trax_server_create(trax_ctx* ctx, const trax_config* cfg, const t
             c.max_conn = cap(o.conn_per_hit);
             c.max_track_entries = cap(o.track_entries_per_hit);
             c.use_graph = o.use_graph;
+            c.fuse_filter = o.fuse_filter;
             c.per_event = 1;  // an event's results do not depend on the batch it shares
             return c;
         };

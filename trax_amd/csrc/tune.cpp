@@ -41,6 +41,7 @@ void set_param(trax_tuning& t, const char* name, uint32_t v) {
     else if (n == "filter_block") t.filter_block = v;
     else if (n == "link_block") t.link_block = v;
     else if (n == "blocks_per_cu") t.blocks_per_cu = v;
+    else if (n == "predict_fine") t.predict_fine = v;
 }
 
 LaunchCfg to_launch(const trax_tuning& t) {
@@ -54,6 +55,7 @@ LaunchCfg to_launch(const trax_tuning& t) {
     lc.blocks_per_cu = t.blocks_per_cu;
     lc.predict_list = t.predict_list;
     lc.predict_sort = t.predict_sort;
+    lc.predict_fine = t.predict_fine;
     return lc;
 }
 
@@ -116,6 +118,7 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
             {"predict", {(128u << 16) | 32u, (128u << 16) | 64u, (256u << 16) | 24u, (256u << 16) | 32u,
                          (256u << 16) | 48u, (256u << 16) | 64u, (512u << 16) | 24u, (512u << 16) | 32u,
                          1u << 31 | (256u << 16) | 32u}, {"predict"}},
+            {"predict_fine", {4, 8, 16}, {"predict"}},
             {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
             {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
@@ -128,7 +131,11 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
         for (const Param& p : params) {
             uint32_t best_v = 0;
             float best_ms = 0.0f;
+            // the fused predictor + filter has no sorted-walk variant and no separate filter kernels
+            if (caps.fuse_filter && std::strcmp(p.name, "filter_block") == 0) continue;
             for (uint32_t v : p.values) {
+                if (std::strcmp(p.name, "predict_fine") == 0 && v > fine_max(b->cfg.grid.geom())) continue;
+                if (caps.fuse_filter && std::strcmp(p.name, "predict") == 0 && (v >> 31)) continue;
                 trax_tuning t = cur;
                 set_param(t, p.name, v);
                 b->lc = to_launch(t);
