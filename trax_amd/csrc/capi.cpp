@@ -629,6 +629,8 @@ void trax_batch::layout(Arena& a) {
         B.fine.grid = a.take<uint32_t>(E * L * fine_geom(B.cfg.grid.geom(), fine_max(B.cfg.grid.geom())).cells);
         B.fine.hits = a.take<float4>(N);
         if (caps.fuse_filter) {
+            B.hout.xy = a.take<float2>(N);
+            B.fine.xy = B.hout.xy;
             B.fine.pool_cap = kFusedPoolCap;
             B.fine.pool_blocks = kFusedPoolBlocks;
             B.fine.pool = a.take<uint2>((uint64_t)kFusedPoolCap * kFusedPoolBlocks);

@@ -64,6 +64,7 @@ struct HitColsOut {
     float* x; float* y; float* z;
     uint32_t* layer; uint32_t* det; uint32_t* id; uint32_t* evt;
     float2* zphi;
+    float2* xy;
 };
 
 __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t* __restrict__ d_nevents,
@@ -127,6 +128,7 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
                 out.id[pos] = in.id[off + i];
                 out.evt[pos] = in.evt[off + i];
                 if (out.zphi) out.zphi[pos] = make_float2(z, phi);
+                if (out.xy) out.xy[pos] = make_float2(x, y);
             }
         }
         __syncthreads();
@@ -155,7 +157,7 @@ hipError_t launch_grid_store(const GridGeom& g, uint32_t n_items_cap, const uint
     const uint32_t blocks = n_items_cap ? n_items_cap : 1;
     const size_t lds = (size_t)g.cells * sizeof(uint32_t);
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
-    HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi};
+    HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};
     hipLaunchKernelGGL(k_grid_store, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset,
                        lay_nhits, lay_offset, grid, ic, oc, n_hits_cap);
     return hipGetLastError();

@@ -26,6 +26,9 @@ struct HitColumns {
     // the reference recomputes (bit-identical: the same built-in on the same
     // inputs); one 8-byte load per visited hit in the window walks
     float2* zphi = nullptr;
+    // device-only column written by the grid scatter: (x, y) of every
+    // reordered hit, one 8-byte load for the fused filter's phi prefilter
+    float2* xy = nullptr;
 };
 
 // per-kernel block sizes (0 = default); chosen by the autotuner
@@ -71,6 +74,7 @@ struct FineBufs {
     uint32_t* tile_at = nullptr;
     uint32_t* group_tile = nullptr;  // E * T + 1: the tile of a candidate group's end
     uint32_t* scratch = nullptr;     // 3 * n_trk_cap: the tiles' tracklets in arrival order
+    const float2* xy = nullptr;      // (x, y) of every grid-ordered hit (the grid scatter's column)
 };
 constexpr uint32_t kFusedPoolCap = 8192, kFusedPoolBlocks = 2048;
 

@@ -1060,7 +1060,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cand_offsets, uint32_t* __restrict__ th1, uint32_t* __restrict__ th2,
     uint32_t* __restrict__ th3, uint32_t n_trk_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
-    FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, uint2* __restrict__ pools,
+    FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
+    uint2* __restrict__ pools,
     uint32_t pool_cap, uint32_t* __restrict__ tile_cands, uint32_t* __restrict__ tile_trk,
     uint32_t* __restrict__ tile_at, uint32_t* __restrict__ group_tile) {
     using Mask = typename std::conditional<(kPredList <= 32), uint32_t, uint64_t>::type;
@@ -1070,7 +1071,10 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ uint16_t s_list[kPredBlock * kPredList];
     __shared__ uint32_t s_eex[kPredBlock + 1];  // listed candidates before each pair
     __shared__ uint32_t s_first[kPredBlock];    // first hit of the pair's third layer
-    __shared__ uint32_t s_h1[kPredBlock], s_h2[kPredBlock];
+    __shared__ float4 s_pair[kPredBlock];       // x2, y2, x2 - x1, y2 - y1: the prefilter's pair terms
+    // blocks up to 256 pairs: the pair of every listed candidate (one byte each)
+    constexpr bool kOwner = kPredBlock <= 256;
+    __shared__ uint8_t s_owner[kOwner ? kPredBlock * kPredList : 1];
     __shared__ uint32_t s_lt[kPredBlock];       // layer triplet | pool entries << 8
     __shared__ Mask s_valid[kPredBlock];        // listed candidates that pass the cuts
     __shared__ Mask s_dead[kPredBlock];         // listed matches that are not candidates
@@ -1112,10 +1116,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 lt = pw.lt;
                 first = pw.lg3[0];
             }
-            s_h1[tid] = h1;
-            s_h2[tid] = h2;
             s_first[tid] = first;
             s_lt[tid] = lt;
+            if (have) {
+                const float2 p1 = hxy[h1], p2 = hxy[h2];
+                s_pair[tid] = make_float4(p2.x, p2.y, p2.x - p1.x, p2.y - p1.y);
+            }
             if (have) {
                 const uint32_t cap = pw.lg3[g.cells - 1] - first <= 0xffffu ? (uint32_t)kPredList : 0u;
                 pred_walk_fine<Mask>(pw, g, fg, fgrid + (size_t)pw.el * fg.cells, fh, cap, first,
@@ -1133,6 +1139,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         const uint32_t excl_c = block_excl_scan(countA, s_wsum, &total_c);
         const uint32_t excl_e = block_excl_scan(listed, s_wsum, &total_e);
         if (tid == 0 && tile < n_tiles) tile_cands[tile] = total_c;
+        if (kOwner)
+            for (uint32_t i = 0; i < listed; ++i) s_owner[excl_e + i] = (uint8_t)tid;
         s_eex[tid] = excl_e;
         s_lt[tid] |= (nB - listed) << 8;
         s_valid[tid] = 0;
@@ -1149,7 +1157,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             trunc += (countA > 0 && nB > countA);
             // the group's end, tile-local; k_group_offsets adds the tile's offset
             const uint32_t g_end = pred_group_end(tc, hlayer, hevt, pairs, n, n_events, tile * kPredBlock + tid,
-                                                  s_h1[tid], excl_c + countA, cand_offsets);
+                                                  pairs[2 * (size_t)(tile * kPredBlock + tid)], excl_c + countA,
+                                                  cand_offsets);
             if (g_end != kSentinel) group_tile[g_end] = tile;
         }
         // entry keys in the reference's order: (grid-row ordinal, phi segment, grid index)
@@ -1190,27 +1199,24 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             const uint32_t n_all = total_e + n_pool;
             const uint32_t per = (n_all + kWaves - 1) / kWaves;
             const uint32_t e0 = wave * per, e1 = e0 + per < n_all ? e0 + per : n_all;
-            // a queued entry's hits: tag = lo << 16 | i (listed) or kValidBit | p (pool)
-            auto hits_of = [&](uint32_t tag, uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& l) {
-                uint32_t lo;
+            // a candidate's pair (in the tile) and third hit: tag = lo << 16 | i
+            // (listed) or kValidBit | p (pool)
+            auto third_of = [&](uint32_t tag, uint32_t& c) {
                 if (tag & kValidBit) {
                     const uint2 e = pool[tag & ~kValidBit];
-                    lo = e.x & 0xffffu;
                     c = e.y;
-                } else {
-                    lo = tag >> 16;
-                    c = s_first[lo] + s_list[lo * kPredList + (tag & 0xffffu)];
+                    return e.x & 0xffffu;
                 }
-                a = s_h1[lo];
-                b = s_h2[lo];
-                l = s_lt[lo] & 0xffu;
+                const uint32_t lo = tag >> 16;
+                c = s_first[lo] + s_list[lo * kPredList + (tag & 0xffffu)];
                 return lo;
             };
             auto test_queue = [&](uint32_t k0) {
                 const uint32_t tag = q[k0 + lane];
-                uint32_t a, b, c, l;
-                const uint32_t lo = hits_of(tag, a, b, c, l);
-                if (filter_valid(tc, hx, hy, hz, a, b, c, l)) {
+                uint32_t c;
+                const uint32_t lo = third_of(tag, c);
+                const size_t pi = tile * kPredBlock + lo;
+                if (filter_valid(tc, hx, hy, hz, pairs[2 * pi], pairs[2 * pi + 1], c, s_lt[lo] & 0xffu)) {
                     if (tag & kValidBit) {
                         atomicAdd(&s_pool_valid[lo], 1u);
                         pool[tag & ~kValidBit].x |= kValidBit;
@@ -1225,10 +1231,15 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 uint32_t tag = 0;
                 if (e < e1) {
                     if (e < total_e) {
-                        uint32_t lo = 0, hi = kPredBlock;
-                        while (hi - lo > 1) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (s_eex[mid] <= e) lo = mid; else hi = mid;
+                        uint32_t lo = 0;
+                        if (kOwner) {
+                            lo = s_owner[e];
+                        } else {
+                            uint32_t hi = kPredBlock;
+                            while (hi - lo > 1) {
+                                const uint32_t mid = (lo + hi) >> 1;
+                                if (s_eex[mid] <= e) lo = mid; else hi = mid;
+                            }
                         }
                         const uint32_t i = e - s_eex[lo];
                         tag = lo << 16 | i;
@@ -1238,10 +1249,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                         undecided = !(pool[e - total_e].x & kDeadBit);
                     }
                     if (undecided) {
-                        uint32_t a, b, c, l;
-                        hits_of(tag, a, b, c, l);
-                        undecided = !tx_phi_surely_outside(hx[b] - hx[a], hy[b] - hy[a], hx[c] - hx[b], hy[c] - hy[b],
-                                                           tc.tan_dphi[l]);
+                        uint32_t c;
+                        const uint32_t lo = third_of(tag, c);
+                        const float4 pr = s_pair[lo];
+                        const float2 p3 = hxy[c];
+                        undecided = !tx_phi_surely_outside(pr.z, pr.w, p3.x - pr.x, p3.y - pr.y,
+                                                           tc.tan_dphi[s_lt[lo] & 0xffu]);
                     }
                 }
                 const uint64_t m = __ballot(undecided);
@@ -1284,8 +1297,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                     }
                 const uint64_t pos = (uint64_t)o + r;
                 if (pos < n_trk_cap) {
-                    th1[pos] = s_h1[tid];
-                    th2[pos] = s_h2[tid];
+                    th1[pos] = pairs[2 * (size_t)(tile * kPredBlock + tid)];
+                    th2[pos] = pairs[2 * (size_t)(tile * kPredBlock + tid) + 1];
                     th3[pos] = h3;
                 } else {
                     over = true;
@@ -1469,7 +1482,7 @@ hipError_t predict_filter_variant(const GridGeom& g, const TripletCols& tc, uint
     const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
     hipLaunchKernelGGL((k_predict_filter<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp, hlayer,
                        hevt, pairs, cand_offsets, th1, th2, th3, n_trk_cap, cnt, counter, fg, fb.grid, fb.hits,
-                       fb.pool, fb.pool_cap, fb.tile_cands, fb.tile_trk, fb.tile_at, fb.group_tile);
+                       fb.xy, fb.pool, fb.pool_cap, fb.tile_cands, fb.tile_trk, fb.tile_at, fb.group_tile);
     return hipGetLastError();
 }
 
@@ -1544,7 +1557,7 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
                                  const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t tiles = predict_fused_tiles(n_pair_cap);
     if (!fb.pool || !fb.pool_cap || !fb.pool_blocks || !fb.tile_cands || !fb.tile_trk || !fb.tile_at ||
-        !fb.group_tile || !fb.scratch)
+        !fb.group_tile || !fb.scratch || !fb.xy)
         return hipErrorInvalidValue;
     FineGeom fg;
     hipError_t e = launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
