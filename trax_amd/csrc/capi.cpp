@@ -602,8 +602,8 @@ void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* c
         throw InputError("capacities too large");
     B.slot_cap = (uint32_t)slot_cap;
     const uint64_t P = caps->max_pairs, Q = caps->max_cands, R = caps->max_tracklets;
-    B.scan_cap = (uint32_t)std::max<uint64_t>(
-        {caps->max_hits + 1ull, slot_cap + 1, P + 1, Q / 64 + 2, R + 1, (uint64_t)B.E * B.T + 1});
+    B.scan_cap = (uint32_t)std::max<uint64_t>({caps->max_hits + 1ull, slot_cap + 1, P + 1, Q / 64 + 2, R + 1,
+                                               (uint64_t)B.E * B.T + 1, predict_fused_chunks(P) + 1});
 }
 
 uint32_t batch_scan_tiles(const trax_batch& B) {
@@ -634,12 +634,13 @@ void trax_batch::layout(Arena& a) {
             B.fine.pool_cap = kFusedPoolCap;
             B.fine.pool_blocks = kFusedPoolBlocks;
             B.fine.pool = a.take<uint2>((uint64_t)kFusedPoolCap * kFusedPoolBlocks);
-            const uint64_t tiles = predict_fused_tiles(P) + 1ull;
-            B.fine.tile_cands = a.take<uint32_t>(tiles);
-            B.fine.tile_trk = a.take<uint32_t>(tiles);
-            B.fine.tile_at = a.take<uint32_t>(tiles);
-            B.fine.group_tile = a.take<uint32_t>(E * T + 1);
-            B.fine.scratch = a.take<uint32_t>(3 * R);
+            const uint64_t chunks = predict_fused_chunks(P) + 1;
+            B.fine.chunk_cands = a.take<uint32_t>(chunks);
+            B.fine.chunk_trk = a.take<uint32_t>(chunks);
+            B.fine.chunk_at = a.take<uint32_t>(chunks);
+            B.fine.group_chunk = a.take<uint32_t>(E * T + 1);
+            B.fine.scratch_cap = (uint32_t)fused_scratch_cap(R);
+            B.fine.scratch = a.take<uint32_t>(3ull * B.fine.scratch_cap);
         }
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);

@@ -45,7 +45,7 @@ struct DevCounts {
     uint32_t diag_pred_trunc;   // pairs whose store found more than counted
     uint32_t diag_wrap_overlap; // outer hits whose wrapped phi window overlaps itself
     uint32_t trk_alloc;         // fused predictor + filter: tracklet scratch handed out to tiles
-    uint32_t n_pred_tiles;      // fused predictor + filter: tiles of the run
+    uint32_t n_pred_chunks;     // fused predictor + filter: 64-pair chunks of the run
     uint32_t pad[3];
 };
 constexpr uint32_t kCountsShapeWords = 9;  // n_events, n_hits and the seven capacities

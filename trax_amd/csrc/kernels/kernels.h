@@ -67,16 +67,20 @@ struct FineBufs {
     // store-window matches beyond a pair's LDS list
     uint2* pool = nullptr;
     uint32_t pool_cap = 0, pool_blocks = 0;
-    // ... per tile: candidates, tracklets, and where its tracklets went in
-    // the scratch (predict_fused_tiles(n_pair_cap) + 1 entries each)
-    uint32_t* tile_cands = nullptr;
-    uint32_t* tile_trk = nullptr;
-    uint32_t* tile_at = nullptr;
-    uint32_t* group_tile = nullptr;  // E * T + 1: the tile of a candidate group's end
-    uint32_t* scratch = nullptr;     // 3 * n_trk_cap: the tiles' tracklets in arrival order
+    // ... per chunk of 64 pairs: candidates, tracklets, and where its
+    // tracklets went in the scratch (predict_fused_chunks(n_pair_cap) + 1 entries each)
+    uint32_t* chunk_cands = nullptr;
+    uint32_t* chunk_trk = nullptr;
+    uint32_t* chunk_at = nullptr;
+    uint32_t* group_chunk = nullptr;  // E * T + 1: the chunk of a candidate group's end
+    uint32_t* scratch = nullptr;     // 3 * scratch_cap: the chunks' tracklets in arrival order
+    uint32_t scratch_cap = 0;        // fused_scratch_cap(n_trk_cap)
     const float2* xy = nullptr;      // (x, y) of every grid-ordered hit (the grid scatter's column)
 };
 constexpr uint32_t kFusedPoolCap = 8192, kFusedPoolBlocks = 2048;
+constexpr uint32_t kTrkRange = 256;  // tracklet scratch a wave takes per atomic
+// tracklet scratch of the fused predictor + filter for n_trk_cap tracklets
+uint64_t fused_scratch_cap(uint64_t n_trk_cap);
 
 // layer-triplet configuration columns on the device (TripletConfigurations,
 // src/datastructures/TripletConfiguration.h:33) and GeometrySupplement radii
@@ -153,6 +157,8 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
 // count + scan + store in one pass (the offsets array must be zeroed before);
 // uses predict_fused_tiles(n_pair_cap) look-back words of the scan scratch
 uint32_t predict_fused_tiles(uint64_t n_pair_cap);
+// chunks of 64 pairs of the fused predictor + filter
+uint64_t predict_fused_chunks(uint64_t n_pair_cap);
 // fused predictor variants (block size, LDS list entries per pair)
 bool predict_variant_supported(uint32_t block, uint32_t list, uint32_t sort);
 // resolves the variant's resident-block count (call outside graph capture)

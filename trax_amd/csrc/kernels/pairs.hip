@@ -1044,26 +1044,28 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6)))
 //  3. the tile's candidates are tested against the filter's cuts, by waves
 //     of 64: the phi-cut prefilter first, the undecided ones queued in the
 //     wave's LDS and tested 64 at a time with the full cuts;
-//  4. the tile takes room for its tracklets in a scratch array with one
-//     atomic (tiles finish in any order, none waits for another) and every
-//     pair writes its valid triplets there in the reference's candidate
-//     order, (grid-row ordinal, phi segment, grid index); the tile records
-//     its candidate and tracklet counts and where its tracklets went.
-// The candidates are not stored.  Afterwards the tile counts are scanned and
-// each tile's tracklets are copied to their place in pair order
-// (k_tile_tracklets), and the candidate group offsets, written tile-local,
-// get their tile's offset (k_group_offsets).
+//  4. every wave (a chunk of 64 consecutive pairs) takes room for its
+//     tracklets in a scratch array from a range it holds (refilled with one
+//     atomic per kTrkRange entries; chunks finish in any order, none waits
+//     for another) and its pairs write their valid triplets there in the
+//     reference's candidate order, (grid-row ordinal, phi segment, grid
+//     index); the chunk records its candidate and tracklet counts and where
+//     its tracklets went.
+// The candidates are not stored.  Afterwards the chunk counts are scanned and
+// each chunk's tracklets are copied to their place in pair order
+// (k_chunk_tracklets), and the candidate group offsets, written chunk-local,
+// get their chunk's offset (k_group_offsets).
 template <int kPredBlock, int kPredList>
 __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
     const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cand_offsets, uint32_t* __restrict__ th1, uint32_t* __restrict__ th2,
-    uint32_t* __restrict__ th3, uint32_t n_trk_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
+    uint32_t* __restrict__ th3, uint32_t scratch_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
     FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
     uint2* __restrict__ pools,
-    uint32_t pool_cap, uint32_t* __restrict__ tile_cands, uint32_t* __restrict__ tile_trk,
-    uint32_t* __restrict__ tile_at, uint32_t* __restrict__ group_tile) {
+    uint32_t pool_cap, uint32_t* __restrict__ chunk_cands, uint32_t* __restrict__ chunk_trk,
+    uint32_t* __restrict__ chunk_at, uint32_t* __restrict__ group_chunk) {
     using Mask = typename std::conditional<(kPredList <= 32), uint32_t, uint64_t>::type;
     static_assert(kPredList <= 64, "list masks hold 64 entries");
     constexpr uint32_t kWaves = kPredBlock / kWave, kQueue = 2 * kWave;
@@ -1082,18 +1084,20 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ uint32_t s_pool_valid[kPredBlock];
     __shared__ uint32_t s_queue[kWaves * kQueue];
     __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
-    __shared__ uint32_t s_tile, s_tbase, s_pool_n;
+    __shared__ uint32_t s_tile, s_pool_n, s_dead_any;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
     const uint32_t n_tiles = (n + kPredBlock - 1) / kPredBlock;
-    if (blockIdx.x == 0 && tid == 0) cnt->n_pred_tiles = n_tiles;
+    if (blockIdx.x == 0 && tid == 0) cnt->n_pred_chunks = n_tiles * kWaves;
     uint2* pool = pools + (size_t)blockIdx.x * pool_cap;
     uint32_t short_slots = 0, trunc = 0;
+    uint32_t range_at = 0, range_left = 0;  // the wave's scratch range (wave-uniform)
     bool over = false;
     while (true) {
         if (tid == 0) {
             s_tile = atomicAdd(counter, 1u);
             s_pool_n = 0;
+            s_dead_any = 0;
         }
         __syncthreads();
         const uint32_t tile = s_tile;
@@ -1135,10 +1139,13 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 listed = nB < cap ? nB : cap;
             }
         }
-        uint32_t total_c, total_e;
-        const uint32_t excl_c = block_excl_scan(countA, s_wsum, &total_c);
+        // the wave's candidates: chunk-local offsets and the chunk count
+        const uint32_t chunk = tile * kWaves + wave;
+        const uint32_t incl_c = wave_incl_scan(countA);
+        const uint32_t excl_c = incl_c - countA;
+        if (lane == kWave - 1 && tile < n_tiles) chunk_cands[chunk] = incl_c;
+        uint32_t total_e;
         const uint32_t excl_e = block_excl_scan(listed, s_wsum, &total_e);
-        if (tid == 0 && tile < n_tiles) tile_cands[tile] = total_c;
         if (kOwner)
             for (uint32_t i = 0; i < listed; ++i) s_owner[excl_e + i] = (uint8_t)tid;
         s_eex[tid] = excl_e;
@@ -1149,17 +1156,18 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         s_sm[tid] = sm;
         s_pool_valid[tid] = 0;
         if (tid == 0) s_eex[kPredBlock] = total_e;
+        if (nB > countA && nB > 0) s_dead_any = 1;
         __syncthreads();
         const uint32_t n_pool = s_pool_n < pool_cap ? s_pool_n : pool_cap;
         if (tid == 0 && s_pool_n > pool_cap) atomicOr(&cnt->err, kErrCapacity);
         if (tile * kPredBlock + tid < n) {
             short_slots += countA > nB ? countA - nB : 0u;
             trunc += (countA > 0 && nB > countA);
-            // the group's end, tile-local; k_group_offsets adds the tile's offset
+            // the group's end, chunk-local; k_group_offsets adds the chunk's offset
             const uint32_t g_end = pred_group_end(tc, hlayer, hevt, pairs, n, n_events, tile * kPredBlock + tid,
                                                   pairs[2 * (size_t)(tile * kPredBlock + tid)], excl_c + countA,
                                                   cand_offsets);
-            if (g_end != kSentinel) group_tile[g_end] = tile;
+            if (g_end != kSentinel) group_chunk[g_end] = chunk;
         }
         // entry keys in the reference's order: (grid-row ordinal, phi segment, grid index)
         auto list_key = [&](uint32_t lo, uint32_t i) {
@@ -1169,7 +1177,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         };
         auto pool_key = [](uint2 e) { return (uint64_t)((e.x >> 17) & 0x1fffu) << 33 | (uint64_t)((e.x >> 16) & 1u) << 32 | e.y; };
         // 2. only the first countA matches (in the reference's order) are candidates
-        if (nB > countA && nB > 0) {
+        const bool prune = s_dead_any != 0;  // uniform: rarely any pair of a tile needs it
+        if (prune && nB > countA && nB > 0) {
             const uint32_t n_own_pool = nB - listed;
             auto rank = [&](uint64_t k) {
                 uint32_t r = 0;
@@ -1190,7 +1199,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                     if ((e.x & 0xffffu) == tid && rank(pool_key(e)) >= countA) pool[p].x = e.x | kDeadBit;
                 }
         }
-        __syncthreads();
+        if (prune) __syncthreads();
         // 3. the candidates against the filter's cuts, by waves: listed entries
         //    [0, total_e), then the pool [total_e, total_e + n_pool)
         {
@@ -1270,23 +1279,28 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             if (lane < qn) test_queue(0);
         }
         __syncthreads();
-        // 4. the pair's valid triplets, scanned, then written in the reference's order
+        // 4. the pair's valid triplets: the wave's room in the scratch, then
+        //    every pair's triplets in the reference's order
         const Mask vm = s_valid[tid];
         const uint32_t nv = (uint32_t)__builtin_popcountll((uint64_t)vm) + s_pool_valid[tid];
-        uint32_t total_t;
-        const uint32_t excl_t = block_excl_scan(nv, s_wsum, &total_t);
-        if (tid == 0) {
-            const uint32_t at = total_t ? atomicAdd(&cnt->trk_alloc, total_t) : 0u;
-            s_tbase = at;
-            if (tile < n_tiles) {
-                tile_trk[tile] = total_t;
-                tile_at[tile] = at;
-            }
+        const uint32_t incl_t = wave_incl_scan(nv);
+        const uint32_t total_t = __shfl(incl_t, kWave - 1, kWave);
+        if (total_t > range_left) {  // uniform
+            const uint32_t size = total_t > kTrkRange ? total_t : kTrkRange;
+            uint32_t a = 0;
+            if (lane == 0) a = atomicAdd(&cnt->trk_alloc, size);
+            range_at = __shfl(a, 0, kWave);
+            range_left = size;
         }
-        __syncthreads();
-        const uint32_t tbase = s_tbase;
+        const uint32_t at = range_at;
+        range_at += total_t;
+        range_left -= total_t;
+        if (lane == kWave - 1 && tile < n_tiles) {
+            chunk_trk[chunk] = total_t;
+            chunk_at[chunk] = at;
+        }
         if (nv) {
-            const uint32_t o = tbase + excl_t, pv = s_pool_valid[tid];
+            const uint32_t o = at + incl_t - nv, pv = s_pool_valid[tid];
             auto write = [&](uint64_t k, uint32_t h3) {
                 uint32_t r = 0;
                 for (Mask m = vm; m; m &= m - 1) r += list_key(tid, (uint32_t)__builtin_ctzll((uint64_t)m)) < k;
@@ -1296,7 +1310,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                         r += (e.x & 0xffffu) == tid && (e.x & kValidBit) && pool_key(e) < k;
                     }
                 const uint64_t pos = (uint64_t)o + r;
-                if (pos < n_trk_cap) {
+                if (pos < scratch_cap) {
                     th1[pos] = pairs[2 * (size_t)(tile * kPredBlock + tid)];
                     th2[pos] = pairs[2 * (size_t)(tile * kPredBlock + tid) + 1];
                     th3[pos] = h3;
@@ -1381,6 +1395,11 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
 }
 
 uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap / 128) + 2; }  // every variant
+uint64_t predict_fused_chunks(uint64_t n_pair_cap) { return n_pair_cap / kWave + 8; }  // 64-pair chunks, every variant
+uint64_t fused_scratch_cap(uint64_t n_trk_cap) {
+    // every resident wave may leave the rest of its last range unused
+    return n_trk_cap + (uint64_t)kTrkRange * kFusedPoolBlocks * (512 / kWave);
+}
 
 namespace {
 
@@ -1419,23 +1438,23 @@ hipError_t predict_fused_variant(const GridGeom& g, const TripletCols& tc, uint3
     return hipGetLastError();
 }
 
-// each tile's tracklets from the scratch to their place in pair order: one
-// wave per tile (tile_trk scanned: a tile's count is the next entry minus its)
-__global__ void k_tile_tracklets(const uint32_t* __restrict__ tile_trk, const uint32_t* __restrict__ tile_at,
-                                 const uint32_t* __restrict__ scratch, uint32_t* __restrict__ th1,
-                                 uint32_t* __restrict__ th2, uint32_t* __restrict__ th3, uint32_t n_trk_cap,
-                                 DevCounts* __restrict__ cnt) {
-    const uint32_t n_tiles = cnt->n_pred_tiles;
+// each chunk's tracklets from the scratch to their place in pair order: one
+// wave per chunk (chunk_trk scanned: a chunk's count is the next entry minus its)
+__global__ void k_chunk_tracklets(const uint32_t* __restrict__ chunk_trk, const uint32_t* __restrict__ chunk_at,
+                                  const uint32_t* __restrict__ scratch, uint32_t scratch_cap,
+                                  uint32_t* __restrict__ th1, uint32_t* __restrict__ th2, uint32_t* __restrict__ th3,
+                                  uint32_t n_trk_cap, DevCounts* __restrict__ cnt) {
+    const uint32_t n_chunks = cnt->n_pred_chunks;
     const uint32_t lane = lane_id();
     bool over = false;
-    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n_tiles;
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n_chunks;
          t += gridDim.x * blockDim.x / kWave) {
-        const uint32_t dst = tile_trk[t], m = tile_trk[t + 1] - dst, src = tile_at[t];
+        const uint32_t dst = chunk_trk[t], m = chunk_trk[t + 1] - dst, src = chunk_at[t];
         for (uint32_t k = lane; k < m; k += kWave) {
-            if ((uint64_t)src + k < n_trk_cap && (uint64_t)dst + k < n_trk_cap) {
+            if ((uint64_t)src + k < scratch_cap && (uint64_t)dst + k < n_trk_cap) {
                 th1[dst + k] = scratch[src + k];
-                th2[dst + k] = scratch[(size_t)n_trk_cap + src + k];
-                th3[dst + k] = scratch[2 * (size_t)n_trk_cap + src + k];
+                th2[dst + k] = scratch[(size_t)scratch_cap + src + k];
+                th3[dst + k] = scratch[2 * (size_t)scratch_cap + src + k];
             } else {
                 over = true;
             }
@@ -1444,13 +1463,13 @@ __global__ void k_tile_tracklets(const uint32_t* __restrict__ tile_trk, const ui
     if (over) atomicOr(&cnt->err, kErrCapacity);
 }
 
-// candidate group ends written tile-local get their tile's offset (tile_cands scanned)
-__global__ void k_group_offsets(uint32_t* __restrict__ cand_offsets, const uint32_t* __restrict__ group_tile,
-                                const uint32_t* __restrict__ tile_cands, uint32_t n_trip,
+// candidate group ends written chunk-local get their chunk's offset (chunk_cands scanned)
+__global__ void k_group_offsets(uint32_t* __restrict__ cand_offsets, const uint32_t* __restrict__ group_chunk,
+                                const uint32_t* __restrict__ chunk_cands, uint32_t n_trip,
                                 const DevCounts* __restrict__ cnt) {
     const uint32_t n = cnt->n_events * n_trip + 1;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (group_tile[i] != kSentinel) cand_offsets[i] += tile_cands[group_tile[i]];
+        if (group_chunk[i] != kSentinel) cand_offsets[i] += chunk_cands[group_chunk[i]];
 }
 
 template <int B, int L>
@@ -1482,7 +1501,7 @@ hipError_t predict_filter_variant(const GridGeom& g, const TripletCols& tc, uint
     const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
     hipLaunchKernelGGL((k_predict_filter<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp, hlayer,
                        hevt, pairs, cand_offsets, th1, th2, th3, n_trk_cap, cnt, counter, fg, fb.grid, fb.hits,
-                       fb.xy, fb.pool, fb.pool_cap, fb.tile_cands, fb.tile_trk, fb.tile_at, fb.group_tile);
+                       fb.xy, fb.pool, fb.pool_cap, fb.chunk_cands, fb.chunk_trk, fb.chunk_at, fb.group_chunk);
     return hipGetLastError();
 }
 
@@ -1555,17 +1574,18 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
                                  const uint32_t* hevt, const uint32_t* pairs, uint32_t* cand_offsets, uint32_t* th1,
                                  uint32_t* th2, uint32_t* th3, DevCounts* cnt, const FineBufs& fb,
                                  const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
-    const uint32_t tiles = predict_fused_tiles(n_pair_cap);
-    if (!fb.pool || !fb.pool_cap || !fb.pool_blocks || !fb.tile_cands || !fb.tile_trk || !fb.tile_at ||
-        !fb.group_tile || !fb.scratch || !fb.xy)
+    const uint64_t chunks = predict_fused_chunks(n_pair_cap);
+    if (!fb.pool || !fb.pool_cap || !fb.pool_blocks || !fb.chunk_cands || !fb.chunk_trk || !fb.chunk_at ||
+        !fb.group_chunk || !fb.scratch || !fb.xy || fb.scratch_cap < fused_scratch_cap(n_trk_cap))
         return hipErrorInvalidValue;
     FineGeom fg;
     hipError_t e = launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
     if (e != hipSuccess) return e;
-    // tile counts (one entry past the last tile stays 0 for the scans), group tiles, the tile counter
-    for (uint32_t* a : {fb.tile_cands, fb.tile_trk})
-        if ((e = launch_fill(a, 0u, tiles + 1ull, st)) != hipSuccess) return e;
-    if ((e = launch_fill(fb.group_tile, kSentinel, (uint64_t)n_el_cap / g.n_layers * tc.n_trip + 1, st)) != hipSuccess)
+    // chunk counts (the entry after the last chunk stays 0 for the scans), group chunks, the tile counter
+    for (uint32_t* a : {fb.chunk_cands, fb.chunk_trk})
+        if ((e = launch_fill(a, 0u, chunks + 1, st)) != hipSuccess) return e;
+    if ((e = launch_fill(fb.group_chunk, kSentinel, (uint64_t)n_el_cap / g.n_layers * tc.n_trip + 1, st)) !=
+        hipSuccess)
         return e;
     uint32_t* counter = reinterpret_cast<uint32_t*>(ss.status);
     if ((e = launch_fill(counter, 0u, 1, st)) != hipSuccess) return e;
@@ -1574,21 +1594,21 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
     e = hipErrorInvalidConfiguration;
 #define TRAX_PF_RUN(B, L, S)                                                                                      \
     if (!S && b == B && l == L)                                                                                   \
-        e = predict_filter_variant<B, L>(g, tc, n_pair_cap, n_trk_cap, grid, hx, hy, hz, hzp, hlayer, hevt, pairs, \
-                                         cand_offsets, fb.scratch, fb.scratch + n_trk_cap,                         \
-                                         fb.scratch + 2 * (size_t)n_trk_cap, cnt, counter, fg, fb, st);
+        e = predict_filter_variant<B, L>(g, tc, n_pair_cap, fb.scratch_cap, grid, hx, hy, hz, hzp, hlayer, hevt,    \
+                                         pairs, cand_offsets, fb.scratch, fb.scratch + fb.scratch_cap,               \
+                                         fb.scratch + 2 * (size_t)fb.scratch_cap, cnt, counter, fg, fb, st);
     TRAX_PRED_VARIANTS(TRAX_PF_RUN)
 #undef TRAX_PF_RUN
     if (e != hipSuccess) return e;
-    // the tiles' counts in tile order: candidate and tracklet offsets per tile, and the totals
-    if ((e = launch_scan(fb.tile_cands, &cnt->n_pred_tiles, 1u, tiles + 1ull, &cnt->n_cands, ss, st)) != hipSuccess)
+    // the chunks' counts in pair order: candidate and tracklet offsets per chunk, and the totals
+    if ((e = launch_scan(fb.chunk_cands, &cnt->n_pred_chunks, 1u, chunks + 1, &cnt->n_cands, ss, st)) != hipSuccess)
         return e;
-    if ((e = launch_scan(fb.tile_trk, &cnt->n_pred_tiles, 1u, tiles + 1ull, &cnt->n_tracklets, ss, st)) != hipSuccess)
+    if ((e = launch_scan(fb.chunk_trk, &cnt->n_pred_chunks, 1u, chunks + 1, &cnt->n_tracklets, ss, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_tile_tracklets, dim3(stride_blocks((uint64_t)tiles * kWave, 256, lc)), dim3(256), 0, st,
-                       fb.tile_trk, fb.tile_at, fb.scratch, th1, th2, th3, n_trk_cap, cnt);
+    hipLaunchKernelGGL(k_chunk_tracklets, dim3(stride_blocks(chunks * kWave, 256, lc)), dim3(256), 0, st,
+                       fb.chunk_trk, fb.chunk_at, fb.scratch, fb.scratch_cap, th1, th2, th3, n_trk_cap, cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_group_offsets, dim3(16), dim3(256), 0, st, cand_offsets, fb.group_tile, fb.tile_cands,
+    hipLaunchKernelGGL(k_group_offsets, dim3(16), dim3(256), 0, st, cand_offsets, fb.group_chunk, fb.chunk_cands,
                        tc.n_trip, cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
