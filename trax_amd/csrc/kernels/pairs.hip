@@ -1074,9 +1074,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ uint32_t s_eex[kPredBlock + 1];  // listed candidates before each pair
     __shared__ uint32_t s_first[kPredBlock];    // first hit of the pair's third layer
     __shared__ float4 s_pair[kPredBlock];       // x2, y2, x2 - x1, y2 - y1: the prefilter's pair terms
-    // blocks up to 256 pairs: the pair of every listed candidate (one byte each)
+    __shared__ float2 s_angles[kPredBlock];     // the cuts' pair angles (phi, theta)
+    // blocks up to 256 pairs: the pair of each of the tile's first kOwnerCap
+    // listed candidates (one byte each; later ones search the offsets)
     constexpr bool kOwner = kPredBlock <= 256;
-    __shared__ uint8_t s_owner[kOwner ? kPredBlock * kPredList : 1];
+    constexpr uint32_t kOwnerCap = kOwner ? kPredBlock * 16 : 1;
+    __shared__ uint8_t s_owner[kOwnerCap];
     __shared__ uint32_t s_lt[kPredBlock];       // layer triplet | pool entries << 8
     __shared__ Mask s_valid[kPredBlock];        // listed candidates that pass the cuts
     __shared__ Mask s_dead[kPredBlock];         // listed matches that are not candidates
@@ -1147,7 +1150,14 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         uint32_t total_e;
         const uint32_t excl_e = block_excl_scan(listed, s_wsum, &total_e);
         if (kOwner)
-            for (uint32_t i = 0; i < listed; ++i) s_owner[excl_e + i] = (uint8_t)tid;
+            for (uint32_t i = excl_e; i < excl_e + listed && i < kOwnerCap; ++i) s_owner[i] = (uint8_t)tid;
+        if (nB) {
+            const size_t pi = tile * kPredBlock + tid;
+            const uint32_t a = pairs[2 * pi], b = pairs[2 * pi + 1];
+            const float2 p1 = hxy[a], p2 = hxy[b];
+            s_angles[tid] = make_float2(filter_pair_phi(p1.x, p1.y, p2.x, p2.y),
+                                        filter_pair_theta(p1.x, p1.y, hz[a], p2.x, p2.y, hz[b]));
+        }
         s_eex[tid] = excl_e;
         s_lt[tid] |= (nB - listed) << 8;
         s_valid[tid] = 0;
@@ -1225,7 +1235,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 uint32_t c;
                 const uint32_t lo = third_of(tag, c);
                 const size_t pi = tile * kPredBlock + lo;
-                if (filter_valid(tc, hx, hy, hz, pairs[2 * pi], pairs[2 * pi + 1], c, s_lt[lo] & 0xffu)) {
+                const uint32_t a = pairs[2 * pi], b = pairs[2 * pi + 1];
+                const float2 p1 = hxy[a], p2 = hxy[b], p3 = hxy[c];
+                const float2 ang = s_angles[lo];
+                const uint32_t l = s_lt[lo] & 0xffu;
+                if (filter_phi_cut(tc, l, p2.x, p2.y, p3.x, p3.y, ang.x) &&
+                    filter_theta_circle_cuts(tc, l, p1.x, p1.y, p2.x, p2.y, hz[b], p3.x, p3.y, hz[c], ang.y)) {
                     if (tag & kValidBit) {
                         atomicAdd(&s_pool_valid[lo], 1u);
                         pool[tag & ~kValidBit].x |= kValidBit;
@@ -1241,7 +1256,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 if (e < e1) {
                     if (e < total_e) {
                         uint32_t lo = 0;
-                        if (kOwner) {
+                        if (kOwner && e < kOwnerCap) {
                             lo = s_owner[e];
                         } else {
                             uint32_t hi = kPredBlock;
