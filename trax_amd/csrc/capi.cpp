@@ -549,12 +549,11 @@ void trax_batch::enqueue_stage(int s) {
             ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
             ok(launch_link_events(tc, E, R, hout.layer, hout.evt, th1, t_off, lb, cnt, st), "link events");
             break;
-        case 11:  // link_count
-            ok(launch_link_count(tc, R, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link count");
-            ok(launch_scan(lb.counts, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_conn, scan, st), "scan");
+        case 11:  // link_count: count and store in one pass, the chunk counts scanned
+            ok(launch_link_fused(tc, R, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, scan, lc, st), "link");
             break;
-        case 12:  // link_store
-            ok(launch_link_store(tc, R, K, hout.layer, hout.evt, th1, th2, th3, t_off, lb, cnt, lc, st), "link store");
+        case 12:  // link_store: the connections placed in base order
+            ok(launch_link_fused_place(R, K, lb, cnt, lc, st), "link place");
             ok(launch_link_flags(R, hout.evt, th1, lb, cnt, lc, st), "link flags");
             ok(launch_scan(lb.flag_a, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_connectable, scan, st), "scan");
             ok(launch_scan(lb.flag_b, &cnt->n_tracklets, 1u, R + 1ull, &cnt->n_nonconnectable, scan, st), "scan");
@@ -603,7 +602,8 @@ void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* c
     B.slot_cap = (uint32_t)slot_cap;
     const uint64_t P = caps->max_pairs, Q = caps->max_cands, R = caps->max_tracklets;
     B.scan_cap = (uint32_t)std::max<uint64_t>({caps->max_hits + 1ull, slot_cap + 1, P + 1, Q / 64 + 2, R + 1,
-                                               (uint64_t)B.E * B.T + 1, predict_fused_chunks(P) + 1});
+                                               (uint64_t)B.E * B.T + 1, predict_fused_chunks(P) + 1,
+                                               link_chunks(R) + 1});
 }
 
 uint32_t batch_scan_tiles(const trax_batch& B) {
@@ -663,6 +663,10 @@ void trax_batch::layout(Arena& a) {
         lb.csr = a.take<uint32_t>(R);
         lb.conn = a.take<uint32_t>(R + 1);
         lb.counts = a.take<uint32_t>(R + 1);
+        lb.chunk_n = a.take<uint32_t>(link_chunks(R) + 1);
+        lb.chunk_at = a.take<uint32_t>(link_chunks(R) + 1);
+        lb.scratch_cap = (uint32_t)link_scratch_cap(K);
+        lb.scratch = a.take<uint32_t>(3ull * lb.scratch_cap);
         lb.basis = a.take<uint32_t>(K); lb.foll = a.take<uint32_t>(K); lb.pev = a.take<uint32_t>(K);
         lb.flag_a = a.take<uint32_t>(R + 1); lb.flag_b = a.take<uint32_t>(R + 1);
         lb.connectable = a.take<uint32_t>(R); lb.nonconnectable = a.take<uint32_t>(R);

@@ -46,7 +46,9 @@ struct DevCounts {
     uint32_t diag_wrap_overlap; // outer hits whose wrapped phi window overlaps itself
     uint32_t trk_alloc;         // fused predictor + filter: tracklet scratch handed out to tiles
     uint32_t n_pred_chunks;     // fused predictor + filter: 64-pair chunks of the run
-    uint32_t pad[3];
+    uint32_t link_alloc;        // single-pass connectivity: connection scratch handed out to chunks
+    uint32_t n_link_chunks;     // single-pass connectivity: 64-tracklet chunks of the run
+    uint32_t pad[1];
 };
 constexpr uint32_t kCountsShapeWords = 9;  // n_events, n_hits and the seven capacities
 

@@ -78,7 +78,6 @@ struct FineBufs {
     const float2* xy = nullptr;      // (x, y) of every grid-ordered hit (the grid scatter's column)
 };
 constexpr uint32_t kFusedPoolCap = 8192, kFusedPoolBlocks = 2048;
-constexpr uint32_t kTrkRange = 256;  // tracklet scratch a wave takes per atomic
 // tracklet scratch of the fused predictor + filter for n_trk_cap tracklets
 uint64_t fused_scratch_cap(uint64_t n_trk_cap);
 
@@ -117,6 +116,16 @@ hipError_t launch_fill_dn(uint32_t* p, uint32_t v, const uint32_t* d_n, uint32_t
 // n_cap; writes the sum of all elements to *d_total when d_total is non-null
 hipError_t launch_scan(uint32_t* data, const uint32_t* d_n, uint32_t n_add, uint64_t n_cap,
                        uint32_t* d_total, const ScanScratch& s, hipStream_t st);
+// Output produced in chunks of 64 work items that finish in any order: each
+// chunk holds a range of a scratch taken kChunkRange entries per atomic, and
+// records its count and where its entries went; once the counts are scanned,
+// this copies every chunk's three columns to their place in chunk order
+// (*d_chunks chunks, at most max_chunks)
+constexpr uint32_t kChunkRange = 256;
+hipError_t launch_chunk_copy3(const uint32_t* d_chunks, uint64_t max_chunks, const uint32_t* chunk_n,
+                              const uint32_t* chunk_at, const uint32_t* scratch, uint32_t scratch_cap, uint32_t* o1,
+                              uint32_t* o2, uint32_t* o3, uint32_t out_cap, DevCounts* cnt, const LaunchCfg& lc,
+                              hipStream_t st);
 // forward fill of zero entries of an offsets array (E*T + 1 entries, E on device)
 hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_trip, hipStream_t st);
 
@@ -204,6 +213,12 @@ struct LinkBufs {
     uint32_t* csr = nullptr;         // R
     uint32_t* conn = nullptr;        // R: connectivity predicate
     uint32_t* counts = nullptr;      // R + 1: followers per base, scanned
+    // the batched path's single connectivity pass (launch_link_fused): per
+    // 64-tracklet chunk its connections and where they went in the scratch
+    uint32_t* chunk_n = nullptr;     // link_chunks(R) + 1
+    uint32_t* chunk_at = nullptr;    // link_chunks(R) + 1
+    uint32_t* scratch = nullptr;     // 3 * scratch_cap: basis, follower, event
+    uint32_t scratch_cap = 0;        // link_scratch_cap(K)
     uint32_t* basis = nullptr; uint32_t* foll = nullptr; uint32_t* pev = nullptr;  // K
     uint32_t* flag_a = nullptr; uint32_t* flag_b = nullptr;  // R + 1
     uint32_t* connectable = nullptr; uint32_t* nonconnectable = nullptr;  // R
@@ -225,6 +240,17 @@ hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const ui
                              const uint32_t* hevt, const uint32_t* th1, const uint32_t* th2,
                              const uint32_t* th3, const uint32_t* t_off, const LinkBufs& b, DevCounts* cnt,
                              const LaunchCfg& lc, hipStream_t st);
+// count and store in one pass (batched path): connections per 64-tracklet
+// chunk into the scratch, the chunk counts scanned (n_conn), then
+// launch_link_fused_place copies them to basis / followers / event
+uint64_t link_chunks(uint64_t n_trk_cap);
+uint64_t link_scratch_cap(uint64_t n_conn_cap);
+hipError_t launch_link_fused(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer, const uint32_t* hevt,
+                             const uint32_t* th1, const uint32_t* th2, const uint32_t* th3, const uint32_t* t_off,
+                             const LinkBufs& b, DevCounts* cnt, const ScanScratch& ss, const LaunchCfg& lc,
+                             hipStream_t st);
+hipError_t launch_link_fused_place(uint32_t n_trk_cap, uint32_t n_conn_cap, const LinkBufs& b, DevCounts* cnt,
+                                   const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_link_store(const TripletCols& tc, uint32_t n_trk_cap, uint32_t n_conn_cap,
                              const uint32_t* hlayer, const uint32_t* hevt, const uint32_t* th1,
                              const uint32_t* th2, const uint32_t* th3, const uint32_t* t_off,

@@ -326,6 +326,109 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
     if (over) atomicOr(&cnt->err, kErrCapacity);
 }
 
+// tripletConnectivityTightCount and Store in one pass (batched path): a wave
+// takes a chunk of 64 consecutive bases, each lane counts its base's
+// followers (setting the connectivity flags as the count kernel does), the
+// wave takes room for the chunk's connections in the scratch from a range it
+// holds, and each lane writes its (base, follower, event) entries there in
+// ascending follower index, the reference's order: the bucket (in fill order)
+// is read once and the followers are kept sorted in the lane's kLinkFew LDS
+// slots; a base with more is emitted by selection, smallest first.  The
+// chunks finish in any order; launch_link_fused scans their counts and
+// launch_link_fused_place copies them into base order.  Dynamic LDS:
+// kLinkFew words per thread.
+constexpr uint32_t kLinkFew = 16;
+__global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
+                             const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
+                             const uint32_t* __restrict__ th3, const uint32_t* __restrict__ t_off, LinkBufs b,
+                             DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    const uint32_t n_off = cnt->n_events * tc.n_trip + 1;
+    const uint32_t n_chunks = (R + kWave - 1) / kWave;
+    if (gtid() == 0) cnt->n_link_chunks = n_chunks;
+    extern __shared__ uint32_t s_few[];
+    uint32_t* few = s_few + threadIdx.x * kLinkFew;
+    const uint32_t lane = lane_id(), first_wave = gtid() / kWave, n_waves = gstride() / kWave;
+    uint32_t range_at = 0, range_left = 0;  // the wave's scratch range (wave-uniform)
+    bool over = false;
+    for (uint32_t chunk = first_wave; chunk < n_chunks; chunk += n_waves) {
+        const uint32_t t = chunk * kWave + lane;
+        uint32_t n = 0, q0 = 0, q1 = 0, h3 = 0;
+        float et = 0.0f;
+        LinkRange r{0, 0, 0};
+        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
+            const uint32_t h = th2[t];
+            h3 = th3[t];
+            et = b.eta[t];
+            q0 = b.hit_count[h];
+            q1 = b.hit_count[h + 1];
+            for (uint32_t q = q0; q < q1; ++q) {
+                const uint32_t i = b.csr[q];
+                if (i >= r.begin && i < r.end && th2[i] == h3 && fabsf(et - b.eta[i]) < tc.d_eta_cut) {
+                    if (n < kLinkFew) {  // insertion into the sorted slots
+                        uint32_t p = n;
+                        while (p > 0 && few[p - 1] > i) {
+                            few[p] = few[p - 1];
+                            --p;
+                        }
+                        few[p] = i;
+                    }
+                    n++;
+                    b.conn[i] = 1u;
+                }
+            }
+            if (n) {
+                b.conn[t] = 1u;
+                if (b.per_event) b.ev_has[r.ev] = 1u;
+            }
+        }
+        const uint32_t incl = wave_incl_scan(n);
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        if (total > range_left) {  // uniform
+            const uint32_t size = total > kChunkRange ? total : kChunkRange;
+            uint32_t a = 0;
+            if (lane == 0) a = atomicAdd(&cnt->link_alloc, size);
+            range_at = __shfl(a, 0, kWave);
+            range_left = size;
+        }
+        const uint32_t at = range_at;
+        range_at += total;
+        range_left -= total;
+        if (lane == kWave - 1) {
+            b.chunk_n[chunk] = total;
+            b.chunk_at[chunk] = at;
+        }
+        uint32_t o = at + incl - n;
+        auto emit = [&](uint32_t f) {
+            if (o < b.scratch_cap) {
+                b.scratch[o] = t;
+                b.scratch[(size_t)b.scratch_cap + o] = f;
+                b.scratch[2 * (size_t)b.scratch_cap + o] = r.ev;
+            } else {
+                over = true;
+            }
+            ++o;
+        };
+        if (n <= kLinkFew) {
+            for (uint32_t k = 0; k < n; ++k) emit(few[k]);
+        } else {
+            uint32_t last = 0;
+            for (uint32_t k = 0; k < n; ++k) {
+                uint32_t best = kSentinel;
+                for (uint32_t q = q0; q < q1; ++q) {
+                    const uint32_t i = b.csr[q];
+                    if ((k == 0 || i > last) && i < best && i >= r.begin && i < r.end && th2[i] == h3 &&
+                        fabsf(et - b.eta[i]) < tc.d_eta_cut)
+                        best = i;
+                }
+                emit(best);
+                last = best;
+            }
+        }
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
 // predicates over the first R - 1 tracklets (TripletConnectivityTight.cpp:184-256);
 // nothing when no connection exists (the reference returns null lists, .cpp:93-95).
 // With each event its own group: over every event's tracklets but its last,
@@ -553,6 +656,36 @@ hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const ui
     const uint32_t blk = lc.link_block ? lc.link_block : 256;
     TRAX_LAUNCH(k_link_count, (uint64_t)n_trk_cap + 1, blk, tc, hlayer, hevt, th1, th2, th3, t_off, b, cnt);
     return hipSuccess;
+}
+
+uint64_t link_chunks(uint64_t n_trk_cap) { return n_trk_cap / kWave + 2; }
+constexpr uint32_t kLinkMaxWaves = 65536;  // waves of the single-pass launch, at most
+uint64_t link_scratch_cap(uint64_t n_conn_cap) {
+    // every wave may leave the rest of its last range unused
+    return n_conn_cap + (uint64_t)kChunkRange * kLinkMaxWaves;
+}
+
+hipError_t launch_link_fused(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer, const uint32_t* hevt,
+                             const uint32_t* th1, const uint32_t* th2, const uint32_t* th3, const uint32_t* t_off,
+                             const LinkBufs& b, DevCounts* cnt, const ScanScratch& ss, const LaunchCfg& lc,
+                             hipStream_t st) {
+    if (!b.chunk_n || !b.chunk_at || !b.scratch || b.scratch_cap == 0) return hipErrorInvalidValue;
+    const uint64_t chunks = link_chunks(n_trk_cap);
+    hipError_t e = launch_fill(b.chunk_n, 0u, chunks + 1, st);
+    if (e != hipSuccess) return e;
+    const uint32_t blk = lc.link_block ? lc.link_block : 256;
+    uint32_t blocks = stride_blocks(chunks * kWave, blk, lc);
+    if ((uint64_t)blocks * (blk / kWave) > kLinkMaxWaves) blocks = kLinkMaxWaves / (blk / kWave);
+    hipLaunchKernelGGL(k_link_fused, dim3(blocks), dim3(blk), (size_t)blk * kLinkFew * sizeof(uint32_t), st, tc, hlayer,
+                       hevt, th1, th2, th3, t_off, b, cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return launch_scan(b.chunk_n, &cnt->n_link_chunks, 1u, chunks + 1, &cnt->n_conn, ss, st);
+}
+
+hipError_t launch_link_fused_place(uint32_t n_trk_cap, uint32_t n_conn_cap, const LinkBufs& b, DevCounts* cnt,
+                                   const LaunchCfg& lc, hipStream_t st) {
+    return launch_chunk_copy3(&cnt->n_link_chunks, link_chunks(n_trk_cap), b.chunk_n, b.chunk_at, b.scratch,
+                              b.scratch_cap, b.basis, b.foll, b.pev, n_conn_cap, cnt, lc, st);
 }
 
 hipError_t launch_link_store(const TripletCols& tc, uint32_t n_trk_cap, uint32_t n_conn_cap,

@@ -1046,14 +1046,14 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6)))
 //     wave's LDS and tested 64 at a time with the full cuts;
 //  4. every wave (a chunk of 64 consecutive pairs) takes room for its
 //     tracklets in a scratch array from a range it holds (refilled with one
-//     atomic per kTrkRange entries; chunks finish in any order, none waits
+//     atomic per kChunkRange entries; chunks finish in any order, none waits
 //     for another) and its pairs write their valid triplets there in the
 //     reference's candidate order, (grid-row ordinal, phi segment, grid
 //     index); the chunk records its candidate and tracklet counts and where
 //     its tracklets went.
 // The candidates are not stored.  Afterwards the chunk counts are scanned and
 // each chunk's tracklets are copied to their place in pair order
-// (k_chunk_tracklets), and the candidate group offsets, written chunk-local,
+// (launch_chunk_copy3), and the candidate group offsets, written chunk-local,
 // get their chunk's offset (k_group_offsets).
 template <int kPredBlock, int kPredList>
 __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
@@ -1301,7 +1301,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         const uint32_t incl_t = wave_incl_scan(nv);
         const uint32_t total_t = __shfl(incl_t, kWave - 1, kWave);
         if (total_t > range_left) {  // uniform
-            const uint32_t size = total_t > kTrkRange ? total_t : kTrkRange;
+            const uint32_t size = total_t > kChunkRange ? total_t : kChunkRange;
             uint32_t a = 0;
             if (lane == 0) a = atomicAdd(&cnt->trk_alloc, size);
             range_at = __shfl(a, 0, kWave);
@@ -1413,7 +1413,7 @@ uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap
 uint64_t predict_fused_chunks(uint64_t n_pair_cap) { return n_pair_cap / kWave + 8; }  // 64-pair chunks, every variant
 uint64_t fused_scratch_cap(uint64_t n_trk_cap) {
     // every resident wave may leave the rest of its last range unused
-    return n_trk_cap + (uint64_t)kTrkRange * kFusedPoolBlocks * (512 / kWave);
+    return n_trk_cap + (uint64_t)kChunkRange * kFusedPoolBlocks * (512 / kWave);
 }
 
 namespace {
@@ -1451,31 +1451,6 @@ hipError_t predict_fused_variant(const GridGeom& g, const TripletCols& tc, uint3
                        hlayer, hevt, pairs, cands, cand_offsets, n_cand_cap, cnt, status, counter, fg, fb.grid,
                        fb.hits);
     return hipGetLastError();
-}
-
-// each chunk's tracklets from the scratch to their place in pair order: one
-// wave per chunk (chunk_trk scanned: a chunk's count is the next entry minus its)
-__global__ void k_chunk_tracklets(const uint32_t* __restrict__ chunk_trk, const uint32_t* __restrict__ chunk_at,
-                                  const uint32_t* __restrict__ scratch, uint32_t scratch_cap,
-                                  uint32_t* __restrict__ th1, uint32_t* __restrict__ th2, uint32_t* __restrict__ th3,
-                                  uint32_t n_trk_cap, DevCounts* __restrict__ cnt) {
-    const uint32_t n_chunks = cnt->n_pred_chunks;
-    const uint32_t lane = lane_id();
-    bool over = false;
-    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n_chunks;
-         t += gridDim.x * blockDim.x / kWave) {
-        const uint32_t dst = chunk_trk[t], m = chunk_trk[t + 1] - dst, src = chunk_at[t];
-        for (uint32_t k = lane; k < m; k += kWave) {
-            if ((uint64_t)src + k < scratch_cap && (uint64_t)dst + k < n_trk_cap) {
-                th1[dst + k] = scratch[src + k];
-                th2[dst + k] = scratch[(size_t)scratch_cap + src + k];
-                th3[dst + k] = scratch[2 * (size_t)scratch_cap + src + k];
-            } else {
-                over = true;
-            }
-        }
-    }
-    if (over) atomicOr(&cnt->err, kErrCapacity);
 }
 
 // candidate group ends written chunk-local get their chunk's offset (chunk_cands scanned)
@@ -1620,9 +1595,9 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
         return e;
     if ((e = launch_scan(fb.chunk_trk, &cnt->n_pred_chunks, 1u, chunks + 1, &cnt->n_tracklets, ss, st)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_chunk_tracklets, dim3(stride_blocks(chunks * kWave, 256, lc)), dim3(256), 0, st,
-                       fb.chunk_trk, fb.chunk_at, fb.scratch, fb.scratch_cap, th1, th2, th3, n_trk_cap, cnt);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_chunk_copy3(&cnt->n_pred_chunks, chunks, fb.chunk_trk, fb.chunk_at, fb.scratch, fb.scratch_cap,
+                                th1, th2, th3, n_trk_cap, cnt, lc, st)) != hipSuccess)
+        return e;
     hipLaunchKernelGGL(k_group_offsets, dim3(16), dim3(256), 0, st, cand_offsets, fb.group_chunk, fb.chunk_cands,
                        tc.n_trip, cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -186,6 +186,45 @@ uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc) {
     return (uint32_t)want;
 }
 
+namespace {
+
+// each chunk's three output columns from a scratch to their place in chunk
+// order: one wave per chunk (chunk_n scanned: a chunk's count is the next
+// entry minus its own; chunk_at: where its entries are in the scratch)
+__global__ void k_chunk_copy3(const uint32_t* __restrict__ d_chunks, const uint32_t* __restrict__ chunk_n,
+                              const uint32_t* __restrict__ chunk_at, const uint32_t* __restrict__ scratch,
+                              uint32_t scratch_cap, uint32_t* __restrict__ o1, uint32_t* __restrict__ o2,
+                              uint32_t* __restrict__ o3, uint32_t out_cap, DevCounts* __restrict__ cnt) {
+    const uint32_t n_chunks = *d_chunks;
+    const uint32_t lane = lane_id();
+    bool over = false;
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n_chunks;
+         t += gridDim.x * blockDim.x / kWave) {
+        const uint32_t dst = chunk_n[t], m = chunk_n[t + 1] - dst, src = chunk_at[t];
+        for (uint32_t k = lane; k < m; k += kWave) {
+            if ((uint64_t)src + k < scratch_cap && (uint64_t)dst + k < out_cap) {
+                o1[dst + k] = scratch[src + k];
+                o2[dst + k] = scratch[(size_t)scratch_cap + src + k];
+                o3[dst + k] = scratch[2 * (size_t)scratch_cap + src + k];
+            } else {
+                over = true;
+            }
+        }
+    }
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+}  // namespace
+
+hipError_t launch_chunk_copy3(const uint32_t* d_chunks, uint64_t max_chunks, const uint32_t* chunk_n,
+                              const uint32_t* chunk_at, const uint32_t* scratch, uint32_t scratch_cap, uint32_t* o1,
+                              uint32_t* o2, uint32_t* o3, uint32_t out_cap, DevCounts* cnt, const LaunchCfg& lc,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_chunk_copy3, dim3(stride_blocks(max_chunks * kWave, 256, lc)), dim3(256), 0, st, d_chunks,
+                       chunk_n, chunk_at, scratch, scratch_cap, o1, o2, o3, out_cap, cnt);
+    return hipGetLastError();
+}
+
 hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_trip, hipStream_t st) {
     hipLaunchKernelGGL(k_forward_fill, dim3(1), dim3(256), 0, st, off, cnt, n_trip);
     return hipGetLastError();
