@@ -84,7 +84,7 @@ def test_autotune_picks_supported_shapes_with_same_results(dev):
     caps = b.caps
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
-    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 4 + 4 + 4
+    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 4 + 4
     assert all(t["same_results"] for t in trials)
     assert (tuning["predict_block"], tuning["predict_list"], tuning["predict_sort"]) in PREDICT_VARIANTS
     assert tuning["blocks_per_cu"] in (2, 4, 8, 16)

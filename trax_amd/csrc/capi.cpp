@@ -489,13 +489,13 @@ void trax_batch::enqueue_stage(int s) {
                    N = caps.max_hits;
     auto ok = [](hipError_t e, const char* w) { hip_check(e, w); };
     switch (s) {
-        case 0:  // grid_count
+        case 0:  // grid_count: the whole grid build (count, scan, stable scatter) in one kernel
             ok(launch_fill(reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords, 0u,
                            sizeof(DevCounts) / 4 - kCountsShapeWords, st), "reset");
-            ok(launch_grid_count(g, E * L, eo, ln, lo, hin.x, hin.y, hin.z, grid, cnt, lc, st), "grid count");
+            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st),
+               "grid build");
             break;
-        case 1:  // grid_store
-            ok(launch_grid_store(g, E * L, eo, ln, lo, grid, hin, hout, N, cnt, lc, st), "grid store");
+        case 1:  // grid_store: part of the grid build above
             break;
         case 2:  // pair_plan
             ok(launch_pair_plan(g, tc, nseg, grid, seg_base, cnt, st), "pair plan");
@@ -644,6 +644,7 @@ void trax_batch::layout(Arena& a) {
         }
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
+        B.grid_perm = a.take<uint32_t>(N);
         B.seg_base = a.take<uint32_t>(E * T + 1);
         B.slot_counts = a.take<uint32_t>(slot_cap + 1);
         B.pairs = a.take<uint32_t>(2 * P);

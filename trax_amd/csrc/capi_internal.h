@@ -138,6 +138,7 @@ struct trax_batch {
     trax::LinkBufs lb;
     trax::CABufs cb;
     trax::FineBufs fine;  // the fused predictor's walk structure
+    uint32_t* grid_perm = nullptr;  // the grid build's scatter scratch (N)
     hipGraph_t graphs[trax_capi::kNumStages] = {};
     hipGraphExec_t execs[trax_capi::kNumStages] = {};
     bool captured = false;

@@ -135,7 +135,100 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
     }
 }
 
+// The whole grid build of one (event, layer) in one block (batched path):
+// the LDS histogram of the cells and its scan as in k_grid_count (the grid
+// written out), then the scatter in three steps instead of k_grid_store's
+// wave-ordered cursors: every hit takes a position in its cell with an LDS
+// atomic and leaves its local index there (perm, in global memory, at the
+// output positions), each cell's few indices are sorted ascending (so a cell
+// holds its hits in input order: the same stable outcome), and the outputs
+// are gathered in output order, coalesced.  Reads 12 B per hit for the
+// count, 28 B per hit for the gather; writes 4 B per cell, 4 B per hit of
+// perm and 44 B per hit of output (52 B with the (x, y) column).
+__global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t* __restrict__ d_nevents,
+                                                   const uint32_t* __restrict__ evt_offset,
+                                                   const uint32_t* __restrict__ lay_nhits,
+                                                   const uint32_t* __restrict__ lay_offset, HitCols in,
+                                                   HitColsOut out, uint32_t* __restrict__ grid,
+                                                   uint32_t* __restrict__ perm, uint32_t n_hits_cap,
+                                                   DevCounts* __restrict__ cnt) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* hist = lds;
+    uint32_t* wsum = lds + g.cells;
+    const uint32_t n_items = *d_nevents * g.n_layers;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t e = item / g.n_layers;
+        for (uint32_t i = threadIdx.x; i < g.cells; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        const uint32_t n = lay_nhits[item];
+        const uint32_t off = evt_offset[e] + lay_offset[item];
+        const bool fits = (uint64_t)off + n <= n_hits_cap;
+        bool bad = false;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
+            if (c < g.cells) atomicAdd(&hist[c], 1u);
+            else bad = true;
+        }
+        if (bad) atomicOr(&cnt->err, kErrCellRange);
+        __syncthreads();
+        lds_excl_scan(hist, g.cells, wsum);
+        uint32_t* gout = grid + (size_t)item * g.cells;
+        for (uint32_t i = threadIdx.x; i < g.cells; i += blockDim.x) gout[i] = off + hist[i];
+        __syncthreads();
+        if (fits) {
+            // positions: hist becomes each cell's end
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
+                if (c < g.cells) perm[off + atomicAdd(&hist[c], 1u)] = i;
+            }
+            __syncthreads();
+            // input order within each cell
+            for (uint32_t c = threadIdx.x; c < g.cells; c += blockDim.x) {
+                const uint32_t lo = c ? hist[c - 1] : 0u, hi = hist[c];
+                for (uint32_t k = lo + 1; k < hi; ++k) {
+                    const uint32_t v = perm[off + k];
+                    uint32_t j = k;
+                    while (j > lo && perm[off + j - 1] > v) {
+                        perm[off + j] = perm[off + j - 1];
+                        --j;
+                    }
+                    perm[off + j] = v;
+                }
+            }
+            __syncthreads();
+            const uint32_t placed = g.cells ? hist[g.cells - 1] : 0u;  // hits with a cell
+            for (uint32_t p = threadIdx.x; p < placed; p += blockDim.x) {
+                const uint32_t i = off + perm[off + p], pos = off + p;
+                const float x = in.x[i], y = in.y[i], z = in.z[i];
+                out.x[pos] = x; out.y[pos] = y; out.z[pos] = z;
+                out.layer[pos] = in.layer[i];
+                out.det[pos] = in.det[i];
+                out.id[pos] = in.id[i];
+                out.evt[pos] = in.evt[i];
+                if (out.zphi) out.zphi[pos] = make_float2(z, tx_atan2f(y, x));
+                if (out.xy) out.xy[pos] = make_float2(x, y);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
+                             const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
+                             const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+    (void)grid_unused;
+    const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 256;
+    const uint32_t blocks = n_items_cap ? n_items_cap : 1;
+    const size_t lds = (size_t)(g.cells + bs / kWave + 1) * sizeof(uint32_t);
+    HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
+    HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};
+    hipLaunchKernelGGL(k_grid_build, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset, lay_nhits,
+                       lay_offset, ic, oc, grid, perm, n_hits_cap, cnt);
+    return hipGetLastError();
+}
 
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const float* hx,

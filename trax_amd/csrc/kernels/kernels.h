@@ -130,6 +130,11 @@ hipError_t launch_chunk_copy3(const uint32_t* d_chunks, uint64_t max_chunks, con
 hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_trip, hipStream_t st);
 
 // GridBuilder
+// count, scan and stable scatter in one kernel (batched path); perm: N u32 scratch
+hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
+                             const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
+                             const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const float* hx,
                              const float* hy, const float* hz, uint32_t* grid, DevCounts* cnt,

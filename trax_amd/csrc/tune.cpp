@@ -122,7 +122,6 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
             {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
             {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
-            {"grid_store_block", {128, 256, 512, 1024}, {"grid_store"}},
             {"grid_count_block", {128, 256, 512, 1024}, {"grid_count"}},
             {"blocks_per_cu", {2, 4, 8, 16}, {}},
         };
