@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <set>
 
 using namespace trax;
 using namespace trax_capi;
@@ -546,7 +547,8 @@ void trax_batch::enqueue_stage(int s) {
                    "filter store");
             break;
         case 10:  // link_prepare
-            ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th3, lb, cnt, scan, lc, st), "link prepare");
+            ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th2, th3, lb, cnt, scan, lc, st),
+               "link prepare");
             ok(launch_link_events(tc, E, R, hout.layer, hout.evt, th1, t_off, lb, cnt, st), "link events");
             break;
         case 11:  // link_count: count and store in one pass, the chunk counts scanned
@@ -673,6 +675,12 @@ void trax_batch::layout(Arena& a) {
         lb.connectable = a.take<uint32_t>(R); lb.nonconnectable = a.take<uint32_t>(R);
         lb.pt = a.take<float>(R);
         lb.per_event = caps.per_event ? 1u : 0u;
+        {
+            // one run of tracklets per middle hit when no layer is the middle layer of two triplets
+            std::set<uint32_t> mid;
+            for (uint32_t i = 0; i < cfg.trip.size(); ++i) mid.insert(cfg.trip.layer2[i]);
+            lb.runs = mid.size() == cfg.trip.size() ? 1u : 0u;
+        }
         if (lb.per_event) {
             lb.ev_first = a.take<uint32_t>(E); lb.ev_last = a.take<uint32_t>(E); lb.ev_has = a.take<uint32_t>(E);
             lb.ev_off = a.take<uint32_t>(E * (T + 1));

@@ -523,7 +523,8 @@ ConnectivityOut TripletConnectivityTight::run(const HitMirror& hits, TrackletCol
     b.hit_count = hit_count.get_mem(); b.hit_cursor = hit_cursor.get_mem(); b.csr = csr.get_mem();
     b.conn = conn.get_mem(); b.counts = counts.get_mem(); b.flag_a = flag_a.get_mem(); b.flag_b = flag_b.get_mem();
     const HitColumns hc = hits.cols();
-    hip_check(launch_link_prepare(tc, N, R, hc.x, hc.y, hc.z, tr.hit1.get_mem(), tr.hit3.get_mem(), b, cnt,
+    hip_check(launch_link_prepare(tc, N, R, hc.x, hc.y, hc.z, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(),
+                                  b, cnt,
                                   ctx.scan_scratch(N + 1ull), ctx.launch(), ctx.stream()),
               "link prepare");
     hip_check(launch_link_count(tc, R, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(),

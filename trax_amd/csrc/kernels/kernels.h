@@ -220,6 +220,11 @@ struct LinkBufs {
     uint32_t* counts = nullptr;      // R + 1: followers per base, scanned
     // the batched path's single connectivity pass (launch_link_fused): per
     // 64-tracklet chunk its connections and where they went in the scratch
+    // the tracklets come in one run per middle hit (each layer is the middle
+    // layer of at most one layer triplet): run_start / run_end per hit are
+    // hit_count / hit_cursor, and a base's followers are the run of its third
+    // hit (in ascending index) instead of a CSR bucket
+    uint32_t runs = 0;
     uint32_t* chunk_n = nullptr;     // link_chunks(R) + 1
     uint32_t* chunk_at = nullptr;    // link_chunks(R) + 1
     uint32_t* scratch = nullptr;     // 3 * scratch_cap: basis, follower, event
@@ -239,7 +244,7 @@ struct LinkBufs {
 };
 hipError_t launch_link_prepare(const TripletCols& tc, uint32_t n_hits_cap, uint32_t n_trk_cap,
                                const float* hx, const float* hy, const float* hz, const uint32_t* th1,
-                               const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
+                               const uint32_t* th2, const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
                              const uint32_t* hevt, const uint32_t* th1, const uint32_t* th2,

@@ -163,6 +163,25 @@ __global__ void k_eta(const float* __restrict__ hx, const float* __restrict__ hy
     }
 }
 
+// η per tracklet as in k_eta, and the run of every middle hit: run_start[h],
+// run_end[h] (tracklets of one middle hit are consecutive when each layer is
+// the middle layer of at most one layer triplet)
+__global__ void k_eta_runs(const float* __restrict__ hx, const float* __restrict__ hy, const float* __restrict__ hz,
+                           const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
+                           const uint32_t* __restrict__ th3, float* __restrict__ eta,
+                           uint32_t* __restrict__ run_start, uint32_t* __restrict__ run_end,
+                           DevCounts* __restrict__ cnt) {
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    for (uint32_t t = gtid(); t < R; t += gstride()) {  // tripletEtaCalculatorStore (.h:60-75)
+        const uint32_t a = th1[t], b = th3[t], m = th2[t];
+        const float px = hx[b] - hx[a], py = hy[b] - hy[a], pz = hz[b] - hz[a];
+        const float tt = pz / sqrtf(px * px + py * py);
+        eta[t] = tx_asinhf(tt);
+        if (t == 0 || th2[t - 1] != m) run_start[m] = t;
+        if (t + 1 == R || th2[t + 1] != m) run_end[m] = t + 1;
+    }
+}
+
 __global__ void k_csr_fill(const uint32_t* __restrict__ th1, const uint32_t* __restrict__ hit_start,
                            uint32_t* __restrict__ hit_cursor, uint32_t* __restrict__ csr,
                            DevCounts* __restrict__ cnt) {
@@ -356,7 +375,29 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
         uint32_t n = 0, q0 = 0, q1 = 0, h3 = 0;
         float et = 0.0f;
         LinkRange r{0, 0, 0};
-        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
+        if (t < R && b.runs && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
+            // the run of the third hit: tracklets i with th2[i] == h3, in ascending index
+            const uint32_t h = th2[t];
+            h3 = th3[t];
+            et = b.eta[t];
+            const uint32_t i0 = b.hit_count[h3], i1 = b.hit_cursor[h3];
+            const uint32_t lo = i0 > r.begin ? i0 : r.begin, hi = i1 < r.end ? i1 : r.end;
+            for (uint32_t i = lo; i < hi; ++i) {
+                if (th1[i] == h && fabsf(et - b.eta[i]) < tc.d_eta_cut) {
+                    if (n < kLinkFew) few[n] = i;
+                    n++;
+                    b.conn[i] = 1u;
+                }
+            }
+            if (n) {
+                b.conn[t] = 1u;
+                if (b.per_event) b.ev_has[r.ev] = 1u;
+            }
+            if (n > kLinkFew) {  // emitted by the selection below: the same tests over the CSR-less run
+                q0 = lo;
+                q1 = hi;
+            }
+        } else if (t < R && !b.runs && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
             const uint32_t h = th2[t];
             h3 = th3[t];
             et = b.eta[t];
@@ -413,16 +454,22 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
             for (uint32_t k = 0; k < n; ++k) emit(few[k]);
         } else {
             uint32_t last = 0;
-            for (uint32_t k = 0; k < n; ++k) {
-                uint32_t best = kSentinel;
-                for (uint32_t q = q0; q < q1; ++q) {
-                    const uint32_t i = b.csr[q];
-                    if ((k == 0 || i > last) && i < best && i >= r.begin && i < r.end && th2[i] == h3 &&
-                        fabsf(et - b.eta[i]) < tc.d_eta_cut)
-                        best = i;
+            if (b.runs) {  // the run is in ascending index: every follower in turn
+                const uint32_t h = th2[t];
+                for (uint32_t i = q0; i < q1; ++i)
+                    if (th1[i] == h && fabsf(et - b.eta[i]) < tc.d_eta_cut) emit(i);
+            } else {
+                for (uint32_t k = 0; k < n; ++k) {
+                    uint32_t best = kSentinel;
+                    for (uint32_t q = q0; q < q1; ++q) {
+                        const uint32_t i = b.csr[q];
+                        if ((k == 0 || i > last) && i < best && i >= r.begin && i < r.end && th2[i] == h3 &&
+                            fabsf(et - b.eta[i]) < tc.d_eta_cut)
+                            best = i;
+                    }
+                    emit(best);
+                    last = best;
                 }
-                emit(best);
-                last = best;
             }
         }
     }
@@ -633,10 +680,18 @@ hipError_t launch_filter_offsets(const TripletCols& tc, uint32_t n_trk_cap, cons
 
 hipError_t launch_link_prepare(const TripletCols& tc, uint32_t n_hits_cap, uint32_t n_trk_cap,
                                const float* hx, const float* hy, const float* hz, const uint32_t* th1,
-                               const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
+                               const uint32_t* th2, const uint32_t* th3, const LinkBufs& b, DevCounts* cnt,
                                const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
     (void)tc;
     const uint32_t blk = lc.link_block ? lc.link_block : 256;
+    if (b.runs) {  // runs per middle hit: no CSR (empty runs for hits without tracklets)
+        hipError_t e = launch_fill_dn(b.hit_count, 0u, &cnt->n_hits, 0u, n_hits_cap, st);
+        if (e != hipSuccess) return e;
+        e = launch_fill_dn(b.hit_cursor, 0u, &cnt->n_hits, 0u, n_hits_cap, st);
+        if (e != hipSuccess) return e;
+        TRAX_LAUNCH(k_eta_runs, n_trk_cap, blk, hx, hy, hz, th1, th2, th3, b.eta, b.hit_count, b.hit_cursor, cnt);
+        return launch_fill_dn(b.conn, 0u, &cnt->n_tracklets, 1u, (uint64_t)n_trk_cap + 1, st);
+    }
     // fills cover the sizes on the device, not the capacities
     hipError_t e = launch_fill_dn(b.hit_count, 0u, &cnt->n_hits, 1u, (uint64_t)n_hits_cap + 1, st);
     if (e != hipSuccess) return e;
