@@ -356,10 +356,10 @@ __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __re
 }
 
 // triplet offsets at the end of each (event, layer triplet) group (.h:404-418)
-// returns the offsets entry written, kSentinel when none
-__device__ inline uint32_t pred_group_end(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
-                                          const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
-                                          uint32_t h1, uint32_t next, uint32_t* cand_offsets) {
+// the offsets entry a pair writes as the last of its group, kSentinel when none
+__device__ inline uint32_t pred_group_end_index(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
+                                                const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
+                                                uint32_t h1) {
     if (h1 == kSentinel) return kSentinel;
     const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
     if (ev >= n_events || lt >= tc.n_trip) return kSentinel;
@@ -368,9 +368,16 @@ __device__ inline uint32_t pred_group_end(const TripletCols& tc, const uint32_t*
         const uint32_t nh = pairs[2 * (size_t)(gid + 1)];
         if (nh != kSentinel) last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
     }
-    if (!last) return kSentinel;
-    cand_offsets[ev * tc.n_trip + lt + 1] = next;
-    return ev * tc.n_trip + lt + 1;
+    return last ? ev * tc.n_trip + lt + 1 : kSentinel;
+}
+
+// returns the offsets entry written, kSentinel when none
+__device__ inline uint32_t pred_group_end(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
+                                          const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
+                                          uint32_t h1, uint32_t next, uint32_t* cand_offsets) {
+    const uint32_t g = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1);
+    if (g != kSentinel) cand_offsets[g] = next;
+    return g;
 }
 
 __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
@@ -1096,16 +1103,17 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     uint32_t short_slots = 0, trunc = 0;
     uint32_t range_at = 0, range_left = 0;  // the wave's scratch range (wave-uniform)
     bool over = false;
+    if (tid == 0) {
+        s_pool_n = 0;
+        s_dead_any = 0;
+    }
+    // tiles from a work queue: tiles differ in cost, so blocks take the next one when free
     while (true) {
-        if (tid == 0) {
-            s_tile = atomicAdd(counter, 1u);
-            s_pool_n = 0;
-            s_dead_any = 0;
-        }
+        if (tid == 0) s_tile = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
-        uint32_t countA = 0, nB = 0, listed = 0;
+        uint32_t countA = 0, nB = 0, listed = 0, g_end = kSentinel;
         Mask rm = 0, sm = 0;
         {
             // the pair's hits and layer triplet go to LDS before the walk (fewer registers held across it)
@@ -1116,6 +1124,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             if (gid < n) {
                 h1 = pairs[2 * (size_t)gid];
                 h2 = pairs[2 * (size_t)gid + 1];
+                // whether the pair ends its candidate group (loads issued with the window's)
+                g_end = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1);
                 have = h1 != kSentinel && h2 != kSentinel &&
                        pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw);
             }
@@ -1174,10 +1184,10 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             short_slots += countA > nB ? countA - nB : 0u;
             trunc += (countA > 0 && nB > countA);
             // the group's end, chunk-local; k_group_offsets adds the chunk's offset
-            const uint32_t g_end = pred_group_end(tc, hlayer, hevt, pairs, n, n_events, tile * kPredBlock + tid,
-                                                  pairs[2 * (size_t)(tile * kPredBlock + tid)], excl_c + countA,
-                                                  cand_offsets);
-            if (g_end != kSentinel) group_chunk[g_end] = chunk;
+            if (g_end != kSentinel) {
+                cand_offsets[g_end] = excl_c + countA;
+                group_chunk[g_end] = chunk;
+            }
         }
         // entry keys in the reference's order: (grid-row ordinal, phi segment, grid index)
         auto list_key = [&](uint32_t lo, uint32_t i) {
@@ -1342,6 +1352,10 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                     const uint2 e = pool[p];
                     if ((e.x & 0xffffu) == tid && (e.x & kValidBit)) write(pool_key(e), e.y);
                 }
+        }
+        if (tid == 0) {  // read into registers above; cleared for the next tile
+            s_pool_n = 0;
+            s_dead_any = 0;
         }
         __syncthreads();  // the LDS lists and the pool are reused by the next tile
     }
