@@ -67,6 +67,39 @@ def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads, fused):
     }
 
 
+def scan_hbm(dev, n=1 << 28, reps=10):
+    """HBM rate of the prefix-sum primitive alone (trax_prefix_sum, the
+    reference's PrefixSum::run) at a size where HBM, not launch latency,
+    bounds it: n u32 values read and written once (8 B per element), timed by
+    device events on the stream the scan runs on.  The scans inside the
+    pipeline cover a few MB each and are bound by launch latency instead."""
+    import ctypes as C
+    import numpy as np
+    from trax_amd.api import Timer, check, lib
+    L = lib()
+    buf, tot = C.c_void_p(), C.c_void_p()
+    check(L.trax_buf_alloc(dev.h, 4 * n, C.byref(buf)))
+    check(L.trax_buf_alloc(dev.h, 4, C.byref(tot)))
+    try:
+        check(L.trax_buf_fill_u32(dev.h, buf, 1, n, None))
+        check(L.trax_prefix_sum(dev.h, buf, n, tot, None))  # also sizes the scan's scratch
+        total = np.zeros(1, np.uint32)
+        check(L.trax_d2h(dev.h, total.ctypes.data, tot, 4, None))
+        check(L.trax_stream_sync(dev.h, None))
+        if int(total[0]) != n:
+            raise RuntimeError(f"scan total {int(total[0])} != {n}")
+        timer = Timer(dev)
+        timer.start()
+        for _ in range(reps):
+            check(L.trax_prefix_sum(dev.h, buf, n, tot, None))
+        ms = timer.stop() / reps
+    finally:
+        L.trax_buf_free(dev.h, buf)
+        L.trax_buf_free(dev.h, tot)
+    gbs = 8 * n / (ms * 1e-3) / 1e9
+    return {"elements": n, "ms": round(ms, 4), "gbs": round(gbs, 1), "frac": round(gbs * 1e9 / HBM_PEAK, 4)}
+
+
 def serving_stream(wl, n_events, seed, workers):
     """The 10,000-event stream of the serving measurement, generated on a
     process pool (before the process touches the GPU) into one HitSet."""
@@ -276,6 +309,9 @@ def main():
         lib().trax_buf_free(dev.h, a)
         lib().trax_buf_free(dev.h, bb)
 
+    scan = scan_hbm(dev)
+    print(f"scan alone: {scan}", file=sys.stderr, flush=True)
+
     # roofline of the dominant phase, HBM-bound
     cols = hs.columns()
     lay = cols["layer_nhits"].reshape(N_EVENTS, wl.n_layers).astype(int)
@@ -319,6 +355,8 @@ def main():
         "batched_vs_plain_same_mix": round(batch_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s else None,
         "pcie_inclusive_events_per_s": round(pcie_ev_s, 1),
         "grid_build_hbm_gbs": round(grid_gbs, 1),
+        "grid_build_hbm_frac": round(grid_gbs * 1e9 / HBM_PEAK, 4),
+        "scan_hbm": scan,
         "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
         "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,

@@ -66,6 +66,17 @@ int trax_stream_destroy(trax_stream* s);
 /* NULL = the context's default stream (clever::context::finish_default_queue) */
 int trax_stream_sync(trax_ctx* ctx, trax_stream* s);
 
+/* Device timer: events recorded on a stream around launches, read back as
+ * elapsed milliseconds.  Replaces the cl_events that the kernel classes
+ * collect and clever::context::getKernelPerf reads
+ * (TripletBuilder.cpp:491-507, RuntimeRecord.cpp:151-160).  stop records the
+ * end event, waits for it and writes the time since start to *ms. */
+typedef struct trax_timer trax_timer;
+int trax_timer_create(trax_ctx* ctx, trax_timer** out);
+int trax_timer_destroy(trax_timer* t);
+int trax_timer_start(trax_ctx* ctx, trax_timer* t, trax_stream* s);
+int trax_timer_stop(trax_ctx* ctx, trax_timer* t, trax_stream* s, float* ms);
+
 /* clever::vector<T,1>(count, ctx) / release_buffer */
 int trax_buf_alloc(trax_ctx* ctx, uint64_t bytes, void** dptr);
 int trax_buf_free(trax_ctx* ctx, void* dptr);

@@ -29,19 +29,22 @@ def test_builtins_bit_identical_host_device(dev):
         assert np.array_equal(h.view(np.uint32), d.view(np.uint32)), f"function {fn}"
 
 
-def _scan_device(dev, values):
+def _scan_device(dev, values, shift=0):
+    """The scan of values placed `shift` u32 past a 256-byte aligned
+    allocation (shift 1..3: the array is not 16-byte aligned)."""
     L = lib()
     v = np.ascontiguousarray(values, np.uint32)
     n = v.size
     buf, tot = C.c_void_p(), C.c_void_p()
-    check(L.trax_buf_alloc(dev.h, max(n, 1) * 4, C.byref(buf)))
+    check(L.trax_buf_alloc(dev.h, (max(n, 1) + shift) * 4, C.byref(buf)))
     check(L.trax_buf_alloc(dev.h, 4, C.byref(tot)))
+    data = C.c_void_p(buf.value + 4 * shift)
     try:
-        check(L.trax_h2d(dev.h, buf, ptr(v), n * 4, None))
-        check(L.trax_prefix_sum(dev.h, buf, n, tot, None))
+        check(L.trax_h2d(dev.h, data, ptr(v), n * 4, None))
+        check(L.trax_prefix_sum(dev.h, data, n, tot, None))
         out = np.empty(n, np.uint32)
         total = np.zeros(1, np.uint32)
-        check(L.trax_d2h(dev.h, ptr(out), buf, n * 4, None))
+        check(L.trax_d2h(dev.h, ptr(out), data, n * 4, None))
         check(L.trax_d2h(dev.h, ptr(total), tot, 4, None))
         check(L.trax_stream_sync(dev.h, None))
         return out, int(total[0])
@@ -50,11 +53,14 @@ def _scan_device(dev, values):
         L.trax_buf_free(dev.h, tot)
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 2047, 2048, 2049, 4096 * 64 + 17, 1 << 20, 3_000_001])
-def test_scan_matches_closed_form(dev, n):
+# sizes up to 2^25 take the reduce-then-scan form, larger ones the look-back form
+@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 63, 2047, 2048, 2049, 4095, 4096, 4097, 8191, 8192, 8193, 4096 * 64 + 17,
+                               1 << 20, 3_000_001, 1 << 25, (1 << 25) + 16384 * 3 + 5])
+@pytest.mark.parametrize("shift", [0, 1])
+def test_scan_matches_closed_form(dev, n, shift):
     rng = np.random.default_rng(n)
     v = rng.integers(0, 1 << 12, n, dtype=np.uint32)
-    out, total = _scan_device(dev, v)
+    out, total = _scan_device(dev, v, shift)
     run = np.concatenate([np.zeros(1, np.uint64), np.cumsum(v.astype(np.uint64))[:-1]])
     ref = (run & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     assert np.array_equal(out, ref)
@@ -73,3 +79,28 @@ def test_scan_wraps_modulo_2_32(dev):
     out, total = _scan_device(dev, v)
     ref = (np.arange(5000, dtype=np.uint64) * 0xF0000000) & 0xFFFFFFFF
     assert np.array_equal(out, ref.astype(np.uint32))
+
+
+def test_timer_measures_stream_work(dev):
+    """The device timer reads the time of the launches between start and stop
+    on a stream: positive, and longer for eight scans than for one."""
+    from trax_amd.api import Stream, Timer
+    L = lib()
+    n = 1 << 24
+    buf = C.c_void_p()
+    check(L.trax_buf_alloc(dev.h, 4 * n, C.byref(buf)))
+    try:
+        s = Stream(dev)
+        t = Timer(dev)
+        check(L.trax_buf_fill_u32(dev.h, buf, 1, n, s.h))
+        check(L.trax_prefix_sum(dev.h, buf, n, None, s.h))
+        s.sync()
+        times = []
+        for reps in (1, 8):
+            t.start(s)
+            for _ in range(reps):
+                check(L.trax_prefix_sum(dev.h, buf, n, None, s.h))
+            times.append(t.stop(s))
+        assert 0.0 < times[0] < times[1]
+    finally:
+        L.trax_buf_free(dev.h, buf)

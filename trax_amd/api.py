@@ -89,6 +89,10 @@ def lib():
         "trax_stream_create": (C.c_int, [P, C.POINTER(P)]),
         "trax_stream_destroy": (C.c_int, [P]),
         "trax_stream_sync": (C.c_int, [P, P]),
+        "trax_timer_create": (C.c_int, [P, C.POINTER(P)]),
+        "trax_timer_destroy": (C.c_int, [P]),
+        "trax_timer_start": (C.c_int, [P, P, P]),
+        "trax_timer_stop": (C.c_int, [P, P, P, C.POINTER(C.c_float)]),
         "trax_buf_alloc": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
         "trax_buf_free": (C.c_int, [P, P]),
         "trax_buf_fill_u32": (C.c_int, [P, P, U32, C.c_uint64, P]),
@@ -314,6 +318,30 @@ class Stream:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.trax_stream_destroy(self.h)
+            self.h = None
+
+
+class Timer:
+    """Device time between start() and stop() on a stream (None = the
+    context's default stream), from events recorded on that stream."""
+
+    def __init__(self, dev: Device):
+        h = P()
+        check(lib().trax_timer_create(dev.h, C.byref(h)))
+        self.h = h
+        self.dev = dev
+
+    def start(self, stream: Optional[Stream] = None):
+        check(lib().trax_timer_start(self.dev.h, self.h, stream.h if stream else None))
+
+    def stop(self, stream: Optional[Stream] = None) -> float:
+        ms = C.c_float()
+        check(lib().trax_timer_stop(self.dev.h, self.h, stream.h if stream else None, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.trax_timer_destroy(self.h)
             self.h = None
 
 

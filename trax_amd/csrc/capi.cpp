@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstring>
 #include <set>
+#include <stdexcept>
 
 using namespace trax;
 using namespace trax_capi;
@@ -68,6 +69,51 @@ static hipStream_t pick(trax_ctx* ctx, trax_stream* s) { return pick_stream(ctx,
 extern "C" int trax_stream_sync(trax_ctx* ctx, trax_stream* s) {
     return guarded([&] {
         hip_check(hipStreamSynchronize(pick(ctx, s)), "hipStreamSynchronize");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_timer_create(trax_ctx* ctx, trax_timer** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("trax_timer_create: out is NULL");
+        hip_check(hipSetDevice(ctx->c->device()), "hipSetDevice");
+        std::unique_ptr<trax_timer> t(new trax_timer);
+        hip_check(hipEventCreate(&t->start), "hipEventCreate");
+        const hipError_t e = hipEventCreate(&t->stop);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(t->start);
+            hip_check(e, "hipEventCreate");
+        }
+        *out = t.release();
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_timer_destroy(trax_timer* t) {
+    return guarded([&] {
+        if (t) {
+            hip_check(hipEventDestroy(t->start), "hipEventDestroy");
+            hip_check(hipEventDestroy(t->stop), "hipEventDestroy");
+            delete t;
+        }
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_timer_start(trax_ctx* ctx, trax_timer* t, trax_stream* s) {
+    return guarded([&] {
+        if (!t) throw std::invalid_argument("trax_timer_start: timer is NULL");
+        hip_check(hipEventRecord(t->start, pick(ctx, s)), "hipEventRecord");
+        return TRAX_OK;
+    });
+}
+
+extern "C" int trax_timer_stop(trax_ctx* ctx, trax_timer* t, trax_stream* s, float* ms) {
+    return guarded([&] {
+        if (!t || !ms) throw std::invalid_argument("trax_timer_stop: timer or ms is NULL");
+        hip_check(hipEventRecord(t->stop, pick(ctx, s)), "hipEventRecord");
+        hip_check(hipEventSynchronize(t->stop), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(ms, t->start, t->stop), "hipEventElapsedTime");
         return TRAX_OK;
     });
 }

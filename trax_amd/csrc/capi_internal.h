@@ -93,6 +93,9 @@ struct trax_ctx {
 struct trax_stream {
     hipStream_t s = nullptr;
 };
+struct trax_timer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 
 struct trax_config {
     trax::GridConfig grid;

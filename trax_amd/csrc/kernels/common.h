@@ -150,7 +150,8 @@ __device__ inline void atomic_store_u64(uint64_t* p, uint64_t v) {
 // started).  Called by all 64 lanes of one wave; publishes the tile's
 // aggregate, returns the exclusive prefix of the tile and publishes its
 // inclusive prefix.  status words: (flag << 32) | value, flag 1 = aggregate,
-// 2 = inclusive prefix.
+// 2 = inclusive prefix.  Each step reads the 64 nearest unread predecessors
+// and waits only for those up to the nearest inclusive prefix.
 __device__ inline uint32_t tile_lookback(uint64_t* status, uint32_t tile, uint32_t total) {
     const uint32_t lane = lane_id();
     if (tile == 0) {
@@ -161,15 +162,17 @@ __device__ inline uint32_t tile_lookback(uint64_t* status, uint32_t tile, uint32
     uint32_t excl = 0;
     int64_t j = (int64_t)tile - 1 - (int64_t)lane;
     while (true) {
-        uint64_t s;
+        uint64_t s, incl;
         uint32_t flag;
         while (true) {
             s = (j >= 0) ? atomic_load_u64(&status[j]) : (2ull << 32);
             flag = (uint32_t)(s >> 32);
-            if (__ballot(flag == 0u) == 0ull) break;
+            incl = __ballot(flag == 2u);
+            // only the tiles up to the nearest inclusive prefix are needed
+            const uint64_t need = incl ? ((incl & (~incl + 1ull)) << 1) - 1ull : ~0ull;
+            if ((__ballot(flag == 0u) & need) == 0ull) break;
             __builtin_amdgcn_s_sleep(1);
         }
-        const uint64_t incl = __ballot(flag == 2u);
         const uint32_t stop = incl ? (uint32_t)(__ffsll((unsigned long long)incl) - 1) : 63u;
         excl += wave_sum(lane <= stop ? (uint32_t)s : 0u);
         if (incl) break;

@@ -139,74 +139,133 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
 // the LDS histogram of the cells and its scan as in k_grid_count (the grid
 // written out), then the scatter in three steps instead of k_grid_store's
 // wave-ordered cursors: every hit takes a position in its cell with an LDS
-// atomic and leaves its local index there (perm, in global memory, at the
-// output positions), each cell's few indices are sorted ascending (so a cell
-// holds its hits in input order: the same stable outcome), and the outputs
-// are gathered in output order, coalesced.  Reads 12 B per hit for the
-// count, 28 B per hit for the gather; writes 4 B per cell, 4 B per hit of
-// perm and 44 B per hit of output (52 B with the (x, y) column).
+// atomic and leaves its local index there (perm), each cell's few indices are
+// sorted ascending (so a cell holds its hits in input order: the same stable
+// outcome), and the outputs are gathered in output order, coalesced.  A layer
+// of at most lds_hits hits keeps its cells and perm in LDS (2 B per hit each):
+// x, y, z are read from HBM once and the sort never leaves LDS.  A larger
+// layer re-reads x, y, z for the scatter and keeps perm in global memory at
+// the output positions.  Reads 12 B per hit for the count, 28 B per hit for
+// the gather; writes 4 B per cell and 44 B per hit of output (52 B with the
+// (x, y) column).
+
+// insertion sort of each cell's entries of perm; ends[c] is the end of cell c
+template <typename T>
+__device__ inline void sort_cells(T* perm, const uint32_t* ends, uint32_t cells) {
+    for (uint32_t c = threadIdx.x; c < cells; c += blockDim.x) {
+        const uint32_t lo = c ? ends[c - 1] : 0u, hi = ends[c];
+        for (uint32_t k = lo + 1; k < hi; ++k) {
+            const T v = perm[k];
+            uint32_t j = k;
+            while (j > lo && perm[j - 1] > v) {
+                perm[j] = perm[j - 1];
+                --j;
+            }
+            perm[j] = v;
+        }
+    }
+}
+
+constexpr uint32_t kGridUnroll = 4;       // hits per thread whose loads are issued together
+constexpr uint16_t kNoCell = 0xffffu;     // a hit outside the grid, in the LDS cell list
+
 __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t* __restrict__ d_nevents,
                                                    const uint32_t* __restrict__ evt_offset,
                                                    const uint32_t* __restrict__ lay_nhits,
                                                    const uint32_t* __restrict__ lay_offset, HitCols in,
                                                    HitColsOut out, uint32_t* __restrict__ grid,
                                                    uint32_t* __restrict__ perm, uint32_t n_hits_cap,
-                                                   DevCounts* __restrict__ cnt) {
+                                                   uint32_t lds_hits, DevCounts* __restrict__ cnt) {
     extern __shared__ uint32_t lds[];
     uint32_t* hist = lds;
     uint32_t* wsum = lds + g.cells;
+    uint16_t* lcell = reinterpret_cast<uint16_t*>(wsum + blockDim.x / kWave + 1);
+    uint16_t* lperm = lcell + lds_hits;
     const uint32_t n_items = *d_nevents * g.n_layers;
+    const uint32_t bs = blockDim.x;
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t e = item / g.n_layers;
-        for (uint32_t i = threadIdx.x; i < g.cells; i += blockDim.x) hist[i] = 0;
+        for (uint32_t i = threadIdx.x; i < g.cells; i += bs) hist[i] = 0;
         __syncthreads();
         const uint32_t n = lay_nhits[item];
         const uint32_t off = evt_offset[e] + lay_offset[item];
         const bool fits = (uint64_t)off + n <= n_hits_cap;
+        const bool local = n <= lds_hits;  // block-uniform
         bool bad = false;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
-            if (c < g.cells) atomicAdd(&hist[c], 1u);
-            else bad = true;
+        for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kGridUnroll * bs) {
+            float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll];
+#pragma unroll
+            for (uint32_t k = 0; k < kGridUnroll; ++k) {
+                const uint32_t i = i0 + k * bs;
+                if (i < n) { x[k] = in.x[off + i]; y[k] = in.y[off + i]; z[k] = in.z[off + i]; }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kGridUnroll; ++k) {
+                const uint32_t i = i0 + k * bs;
+                if (i < n) {
+                    const uint32_t c = hit_cell(g, x[k], y[k], z[k]);
+                    if (c < g.cells) atomicAdd(&hist[c], 1u);
+                    else bad = true;
+                    if (local) lcell[i] = c < g.cells ? (uint16_t)c : kNoCell;
+                }
+            }
         }
         if (bad) atomicOr(&cnt->err, kErrCellRange);
         __syncthreads();
         lds_excl_scan(hist, g.cells, wsum);
         uint32_t* gout = grid + (size_t)item * g.cells;
-        for (uint32_t i = threadIdx.x; i < g.cells; i += blockDim.x) gout[i] = off + hist[i];
+        for (uint32_t i = threadIdx.x; i < g.cells; i += bs) gout[i] = off + hist[i];
         __syncthreads();
         if (fits) {
             // positions: hist becomes each cell's end
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-                const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
-                if (c < g.cells) perm[off + atomicAdd(&hist[c], 1u)] = i;
-            }
-            __syncthreads();
-            // input order within each cell
-            for (uint32_t c = threadIdx.x; c < g.cells; c += blockDim.x) {
-                const uint32_t lo = c ? hist[c - 1] : 0u, hi = hist[c];
-                for (uint32_t k = lo + 1; k < hi; ++k) {
-                    const uint32_t v = perm[off + k];
-                    uint32_t j = k;
-                    while (j > lo && perm[off + j - 1] > v) {
-                        perm[off + j] = perm[off + j - 1];
-                        --j;
-                    }
-                    perm[off + j] = v;
+            if (local) {
+                for (uint32_t i = threadIdx.x; i < n; i += bs) {
+                    const uint16_t c = lcell[i];
+                    if (c != kNoCell) lperm[atomicAdd(&hist[c], 1u)] = (uint16_t)i;
                 }
+                __syncthreads();
+                sort_cells(lperm, hist, g.cells);
+            } else {
+                for (uint32_t i = threadIdx.x; i < n; i += bs) {
+                    const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
+                    if (c < g.cells) perm[off + atomicAdd(&hist[c], 1u)] = i;
+                }
+                __syncthreads();
+                sort_cells(perm + off, hist, g.cells);
             }
             __syncthreads();
             const uint32_t placed = g.cells ? hist[g.cells - 1] : 0u;  // hits with a cell
-            for (uint32_t p = threadIdx.x; p < placed; p += blockDim.x) {
-                const uint32_t i = off + perm[off + p], pos = off + p;
-                const float x = in.x[i], y = in.y[i], z = in.z[i];
-                out.x[pos] = x; out.y[pos] = y; out.z[pos] = z;
-                out.layer[pos] = in.layer[i];
-                out.det[pos] = in.det[i];
-                out.id[pos] = in.id[i];
-                out.evt[pos] = in.evt[i];
-                if (out.zphi) out.zphi[pos] = make_float2(z, tx_atan2f(y, x));
-                if (out.xy) out.xy[pos] = make_float2(x, y);
+            for (uint32_t p0 = threadIdx.x; p0 < placed; p0 += kGridUnroll * bs) {
+                uint32_t src[kGridUnroll];
+                float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll];
+                uint32_t la[kGridUnroll], de[kGridUnroll], id[kGridUnroll], ev[kGridUnroll];
+#pragma unroll
+                for (uint32_t k = 0; k < kGridUnroll; ++k) {
+                    const uint32_t p = p0 + k * bs;
+                    src[k] = p < placed ? off + (local ? (uint32_t)lperm[p] : perm[off + p]) : off;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kGridUnroll; ++k) {
+                    if (p0 + k * bs < placed) {
+                        const uint32_t i = src[k];
+                        x[k] = in.x[i]; y[k] = in.y[i]; z[k] = in.z[i];
+                        la[k] = in.layer[i]; de[k] = in.det[i]; id[k] = in.id[i]; ev[k] = in.evt[i];
+                    }
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kGridUnroll; ++k) {
+                    const uint32_t p = p0 + k * bs;
+                    if (p < placed) {
+                        const uint32_t pos = off + p;
+                        out.x[pos] = x[k]; out.y[pos] = y[k]; out.z[pos] = z[k];
+                        out.layer[pos] = la[k];
+                        out.det[pos] = de[k];
+                        out.id[pos] = id[k];
+                        out.evt[pos] = ev[k];
+                        if (out.zphi) out.zphi[pos] = make_float2(z[k], tx_atan2f(y[k], x[k]));
+                        if (out.xy) out.xy[pos] = make_float2(x[k], y[k]);
+                    }
+                }
             }
         }
         __syncthreads();
@@ -222,12 +281,23 @@ hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint
     (void)grid_unused;
     const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 256;
     const uint32_t blocks = n_items_cap ? n_items_cap : 1;
-    const size_t lds = (size_t)(g.cells + bs / kWave + 1) * sizeof(uint32_t);
+    const size_t base = (size_t)(g.cells + bs / kWave + 1) * sizeof(uint32_t);
+    const uint32_t lds_hits = grid_lds_hits(g, bs);
+    const size_t lds = base + (size_t)lds_hits * 2 * sizeof(uint16_t);
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
     HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};
     hipLaunchKernelGGL(k_grid_build, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset, lay_nhits,
-                       lay_offset, ic, oc, grid, perm, n_hits_cap, cnt);
+                       lay_offset, ic, oc, grid, perm, n_hits_cap, lds_hits, cnt);
     return hipGetLastError();
+}
+
+uint32_t grid_lds_hits(const GridGeom& g, uint32_t block) {
+    // LDS per block stays within 64 KiB; cells are kept as u16 with 0xffff free
+    constexpr size_t kBudget = 64 * 1024;
+    const size_t base = (size_t)(g.cells + block / kWave + 1) * sizeof(uint32_t);
+    if (g.cells >= kNoCell || base >= kBudget) return 0;
+    const size_t fit = (kBudget - base) / (2 * sizeof(uint16_t));
+    return (uint32_t)std::min<size_t>(fit & ~(size_t)63, kGridLdsHitsMax);
 }
 
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,

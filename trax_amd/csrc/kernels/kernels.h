@@ -131,6 +131,11 @@ hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_t
 
 // GridBuilder
 // count, scan and stable scatter in one kernel (batched path); perm: N u32 scratch
+// hits of one (event, layer) that the one-kernel grid build keeps in LDS at a
+// block size (0 when the grid's cells do not fit a u16 or LDS); larger layers
+// take its global-memory path
+constexpr uint32_t kGridLdsHitsMax = 8192;
+uint32_t grid_lds_hits(const GridGeom& g, uint32_t block);
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,

@@ -3,103 +3,222 @@
 //
 // The reference scans with a recursive Blelloch tree (2*wg elements per
 // work-group, one launch per level plus a uniform-add pass per level).  Here
-// the scan is a single-pass decoupled look-back scan: each 256-thread block
-// owns a tile of 2048 elements, publishes its aggregate, and wave 0 looks back
-// over up to 64 predecessor tiles at a time.  Every element is read once and
-// written once (8 bytes per element of HBM traffic).  The result is the same
-// exclusive running sum modulo 2^32.
+// an array whose capacity is at most kRtsMaxN elements is scanned in three
+// launches (reduce, then scan): the sums of 8192-element tiles, one block
+// scanning those sums, then every tile scanned again from its tile's prefix
+// (12 bytes per element of HBM traffic, no tile waits on another).  A larger
+// one takes a single-pass decoupled look-back scan: 1024 persistent blocks of
+// 512 threads take 16384-element tiles in increasing order from a counter,
+// publish each tile's aggregate and look back over their predecessors (8
+// bytes per element).  In both, each wave owns consecutive elements and moves
+// them in rows of 64 16-byte vectors (1 KB contiguous per instruction); a row
+// is scanned across the wave's lanes.  Measured on MI355X: reduce-then-scan
+// is the faster of the two up to about 32 M elements, look-back beyond.  The
+// result is the same exclusive running sum modulo 2^32 either way.
 #include "kernels.h"
 
 namespace trax {
 namespace {
 
-constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanBlock * kScanItems;
+constexpr int kScanItems = 32;               // elements per thread in both forms
+constexpr int kRtsBlock = 256;
+constexpr uint32_t kRtsTile = kRtsBlock * kScanItems;
+constexpr int kLbBlock = 512;
+constexpr uint32_t kLbTile = kLbBlock * kScanItems;
+constexpr uint32_t kLbBlocks = 1024;         // persistent look-back blocks
+constexpr uint64_t kRtsMaxN = 1ull << 25;
 
-__device__ inline uint32_t pad_idx(uint32_t i) { return i + (i >> 5); }
+__device__ inline uint32_t scan_size(const uint32_t* d_n, uint32_t n_add, uint32_t n_cap) {
+    const uint32_t n = (d_n ? *d_n : 0u) + n_add;
+    return n > n_cap ? n_cap : n;  // a size past its buffer is reported by the stage kernels
+}
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_excl(uint32_t* __restrict__ data,
-                                                        const uint32_t* __restrict__ d_n, uint32_t n_add,
-                                                        uint32_t* __restrict__ d_total,
-                                                        uint64_t* __restrict__ status,
-                                                        uint32_t* __restrict__ counter, uint32_t n_cap) {
-    __shared__ uint32_t s_items[kScanTile + kScanTile / 32];
-    __shared__ uint32_t s_wsum[kScanBlock / kWave + 1];
-    __shared__ uint32_t s_tile, s_prefix;
-    const uint32_t tid = threadIdx.x;
-    uint32_t n = (d_n ? *d_n : 0u) + n_add;
-    if (n > n_cap) n = n_cap;  // a size past its buffer is reported by the stage kernels
-    // persistent: tiles are taken in increasing order from the counter, so a
-    // tile's look-back only waits on tiles held by running blocks
+// One wave's segment of kWave*kScanItems elements from seg: row q, lane l is
+// the 16-byte vector q*64+l (vector loads when vec; element loads, bounded by
+// n, otherwise)
+template <int R>
+__device__ inline void seg_load(const uint32_t* data, uint64_t seg, uint32_t n, bool vec, uint4 (&a)[R]) {
+    const uint32_t lane = lane_id();
+    if (vec) {
+        const uint4* p = reinterpret_cast<const uint4*>(data + seg);
+#pragma unroll
+        for (int q = 0; q < R; ++q) a[q] = p[q * kWave + lane];
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const uint64_t e = seg + 4ull * (q * kWave + lane);
+            a[q].x = e < n ? data[e] : 0u;
+            a[q].y = e + 1 < n ? data[e + 1] : 0u;
+            a[q].z = e + 2 < n ? data[e + 2] : 0u;
+            a[q].w = e + 3 < n ? data[e + 3] : 0u;
+        }
+    }
+}
+template <int R>
+__device__ inline void seg_store(uint32_t* data, uint64_t seg, uint32_t n, bool vec, const uint4 (&a)[R]) {
+    const uint32_t lane = lane_id();
+    if (vec) {
+        uint4* p = reinterpret_cast<uint4*>(data + seg);
+#pragma unroll
+        for (int q = 0; q < R; ++q) p[q * kWave + lane] = a[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const uint64_t e = seg + 4ull * (q * kWave + lane);
+            if (e < n) data[e] = a[q].x;
+            if (e + 1 < n) data[e + 1] = a[q].y;
+            if (e + 2 < n) data[e + 2] = a[q].z;
+            if (e + 3 < n) data[e + 3] = a[q].w;
+        }
+    }
+}
+// exclusive prefix of each row within the wave's segment; returns the segment sum
+template <int R>
+__device__ inline uint32_t seg_scan(const uint4 (&a)[R], uint32_t (&rex)[R]) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const uint32_t v = a[q].x + a[q].y + a[q].z + a[q].w;
+        const uint32_t inc = wave_incl_scan(v);
+        rex[q] = run + inc - v;
+        run += __shfl(inc, kWave - 1, kWave);
+    }
+    return run;
+}
+// the segment's elements replaced by their exclusive running sums from p
+template <int R>
+__device__ inline void seg_apply(uint4 (&a)[R], const uint32_t (&rex)[R], uint32_t p) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        uint32_t r = p + rex[q];
+        uint4 o;
+        o.x = r; r += a[q].x;
+        o.y = r; r += a[q].y;
+        o.z = r; r += a[q].z;
+        o.w = r;
+        a[q] = o;
+    }
+}
+
+// look-back form: persistent blocks, tiles in increasing order from counter
+template <bool kVec>
+__global__ __launch_bounds__(kLbBlock) void k_scan_lookback(uint32_t* __restrict__ data,
+                                                            const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                            uint32_t* __restrict__ d_total,
+                                                            uint64_t* __restrict__ status,
+                                                            uint32_t* __restrict__ counter, uint32_t n_cap) {
+    constexpr int R = kScanItems / 4;
+    constexpr int kWaves = kLbBlock / kWave;
+    __shared__ uint32_t s_wp[kWaves];
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x, w = tid / kWave;
+    const uint32_t n = scan_size(d_n, n_add, n_cap);
     while (true) {
-    __syncthreads();  // s_items and s_prefix of the previous tile are consumed
-    if (tid == 0) s_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint64_t base = (uint64_t)tile * kScanTile;
-    if (base >= n) {
-        if (tile == 0 && tid == 0 && d_total) *d_total = 0;
-        return;
+        __syncthreads();  // s_tile and s_wp of the previous tile are consumed
+        if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        const uint64_t base = (uint64_t)tile * kLbTile;
+        if (base >= n) {
+            if (tile == 0 && tid == 0 && d_total) *d_total = 0;
+            return;
+        }
+        const uint64_t seg = base + (uint64_t)w * (kWave * kScanItems);
+        const bool vec = kVec && base + kLbTile <= n;
+        uint4 a[R];
+        uint32_t rex[R];
+        seg_load(data, seg, n, vec, a);
+        const uint32_t wsum = seg_scan(a, rex);
+        if (lane_id() == kWave - 1) s_wp[w] = wsum;
+        __syncthreads();
+        if (tid < kWave) {
+            const uint32_t wv = tid < kWaves ? s_wp[tid] : 0u;
+            const uint32_t winc = wave_incl_scan(wv);
+            const uint32_t total = __shfl(winc, kWaves - 1, kWave);
+            const uint32_t excl = tile_lookback(status, tile, total);
+            if (tid < kWaves) s_wp[tid] = excl + winc - wv;
+            if (d_total && tid == 0 && base + kLbTile >= n) *d_total = excl + total;
+        }
+        __syncthreads();
+        seg_apply(a, rex, s_wp[w]);
+        seg_store(data, seg, n, vec, a);
     }
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint32_t li = k * kScanBlock + tid;
-        const uint64_t i = base + li;
-        s_items[pad_idx(li)] = i < n ? data[i] : 0u;
-    }
-    __syncthreads();
-    uint32_t v[kScanItems];
+}
+
+// reduce-then-scan form: 1. tile sums
+template <bool kVec>
+__global__ __launch_bounds__(kRtsBlock) void k_scan_tile_sums(const uint32_t* __restrict__ data,
+                                                              const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                              uint32_t* __restrict__ sums, uint32_t n_cap) {
+    constexpr int R = kScanItems / 4;
+    __shared__ uint32_t s_wsum[kRtsBlock / kWave + 1];
+    const uint32_t n = scan_size(d_n, n_add, n_cap);
+    const uint64_t base = (uint64_t)blockIdx.x * kRtsTile;
+    if (base >= n) return;  // block-uniform
     uint32_t sum = 0;
+    if (kVec && base + kRtsTile <= n) {
+        const uint4* p = reinterpret_cast<const uint4*>(data + base);
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = s_items[pad_idx(tid * kScanItems + k)];
-        sum += v[k];
+        for (int q = 0; q < R; ++q) {
+            const uint4 a = p[q * kRtsBlock + threadIdx.x];
+            sum += a.x + a.y + a.z + a.w;
+        }
+    } else {
+        const uint64_t end = base + kRtsTile < n ? base + kRtsTile : n;
+        for (uint64_t e = base + threadIdx.x; e < end; e += kRtsBlock) sum += data[e];
     }
     uint32_t total;
-    const uint32_t ex = block_excl_scan(sum, s_wsum, &total);
+    block_excl_scan(sum, s_wsum, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// 2. exclusive scan of the tile sums in one block; the total to d_total
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ sums, const uint32_t* __restrict__ d_n,
+                                                    uint32_t n_add, uint32_t* __restrict__ d_total, uint32_t n_cap) {
+    __shared__ uint32_t s_wsum[1024 / kWave + 1];
+    const uint32_t n = scan_size(d_n, n_add, n_cap);
+    const uint32_t m = (uint32_t)(((uint64_t)n + kRtsTile - 1) / kRtsTile);
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < m; b += blockDim.x) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < m ? sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, s_wsum, &total);
+        if (i < m) sums[i] = carry + ex;
+        carry += total;
+        __syncthreads();  // s_wsum of this round is consumed
+    }
+    if (d_total && threadIdx.x == 0) *d_total = carry;
+}
+
+// 3. every tile scanned from its prefix
+template <bool kVec>
+__global__ __launch_bounds__(kRtsBlock) void k_scan_down(uint32_t* __restrict__ data,
+                                                         const uint32_t* __restrict__ d_n, uint32_t n_add,
+                                                         const uint32_t* __restrict__ sums, uint32_t n_cap) {
+    constexpr int R = kScanItems / 4;
+    constexpr int kWaves = kRtsBlock / kWave;
+    __shared__ uint32_t s_wp[kWaves];
+    const uint32_t tid = threadIdx.x, w = tid / kWave;
+    const uint32_t n = scan_size(d_n, n_add, n_cap);
+    const uint64_t base = (uint64_t)blockIdx.x * kRtsTile;
+    if (base >= n) return;  // block-uniform
+    const uint64_t seg = base + (uint64_t)w * (kWave * kScanItems);
+    const bool vec = kVec && base + kRtsTile <= n;
+    uint4 a[R];
+    uint32_t rex[R];
+    seg_load(data, seg, n, vec, a);
+    const uint32_t wsum = seg_scan(a, rex);
+    if (lane_id() == kWave - 1) s_wp[w] = wsum;
+    __syncthreads();
     if (tid < kWave) {
-        uint32_t excl = 0;
-        if (tile == 0) {
-            if (tid == 0) atomic_store_u64(&status[0], (2ull << 32) | total);
-        } else {
-            if (tid == 0) atomic_store_u64(&status[tile], (1ull << 32) | total);
-            int64_t j = (int64_t)tile - 1 - (int64_t)tid;
-            while (true) {
-                uint64_t s;
-                uint32_t flag;
-                while (true) {
-                    s = (j >= 0) ? atomic_load_u64(&status[j]) : (2ull << 32);
-                    flag = (uint32_t)(s >> 32);
-                    if (__ballot(flag == 0u) == 0ull) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                const uint64_t incl = __ballot(flag == 2u);
-                const uint32_t stop = incl ? (uint32_t)(__ffsll((unsigned long long)incl) - 1) : 63u;
-                excl += wave_sum(tid <= stop ? (uint32_t)s : 0u);
-                if (incl) break;
-                j -= kWave;
-            }
-            if (tid == 0) atomic_store_u64(&status[tile], (2ull << 32) | (uint32_t)(excl + total));
-        }
-        if (tid == 0) s_prefix = excl;
+        const uint32_t wv = tid < kWaves ? s_wp[tid] : 0u;
+        const uint32_t winc = wave_incl_scan(wv);
+        if (tid < kWaves) s_wp[tid] = sums[blockIdx.x] + winc - wv;
     }
     __syncthreads();
-    uint32_t run = s_prefix + ex;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        s_items[pad_idx(tid * kScanItems + k)] = run;
-        run += v[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint32_t li = k * kScanBlock + tid;
-        const uint64_t i = base + li;
-        if (i < n) data[i] = s_items[pad_idx(li)];
-    }
-    if (d_total && tid == 0 && base + kScanTile >= n) *d_total = s_prefix + total;
-    }
+    seg_apply(a, rex, s_wp[w]);
+    seg_store(data, seg, n, vec, a);
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
@@ -230,7 +349,9 @@ hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_t
     return hipGetLastError();
 }
 
-uint32_t scan_tiles(uint64_t n_cap) { return (uint32_t)((n_cap + kScanTile - 1) / kScanTile) + 1; }
+// status words (u64) the scan needs at a capacity: tile sums of the
+// reduce-then-scan form, or the look-back form's tile status and counter
+uint32_t scan_tiles(uint64_t n_cap) { return (uint32_t)((n_cap + kRtsTile - 1) / kRtsTile) + 2; }
 
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
@@ -268,15 +389,36 @@ hipError_t launch_scan(uint32_t* data, const uint32_t* d_n, uint32_t n_add, uint
                        uint32_t* d_total, const ScanScratch& s, hipStream_t st) {
     const uint32_t tiles = scan_tiles(n_cap);
     if (tiles > s.max_tiles) return hipErrorInvalidValue;
+    const uint32_t cap = (uint32_t)(n_cap < 0xffffffffull ? n_cap : 0xffffffffull);
+    const bool vec = (reinterpret_cast<uintptr_t>(data) & 15u) == 0;
+    if (n_cap <= kRtsMaxN) {
+        const uint32_t blocks = std::max<uint32_t>(1u, (uint32_t)((n_cap + kRtsTile - 1) / kRtsTile));
+        uint32_t* sums = reinterpret_cast<uint32_t*>(s.status);
+        if (vec)
+            hipLaunchKernelGGL(k_scan_tile_sums<true>, dim3(blocks), dim3(kRtsBlock), 0, st, data, d_n, n_add, sums, cap);
+        else
+            hipLaunchKernelGGL(k_scan_tile_sums<false>, dim3(blocks), dim3(kRtsBlock), 0, st, data, d_n, n_add, sums,
+                               cap);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, sums, d_n, n_add, d_total, cap);
+        if (vec)
+            hipLaunchKernelGGL(k_scan_down<true>, dim3(blocks), dim3(kRtsBlock), 0, st, data, d_n, n_add, sums, cap);
+        else
+            hipLaunchKernelGGL(k_scan_down<false>, dim3(blocks), dim3(kRtsBlock), 0, st, data, d_n, n_add, sums, cap);
+        return hipGetLastError();
+    }
     // status words and the tile counter are cleared by a kernel, not a
     // memset node, so that the sequence can be captured into a graph
-    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(s.status), 0u, 2ull * tiles + 2, st);
+    const uint32_t lb_tiles = (uint32_t)((n_cap + kLbTile - 1) / kLbTile) + 1;
+    hipError_t e = launch_fill(reinterpret_cast<uint32_t*>(s.status), 0u, 2ull * lb_tiles + 2, st);
     if (e != hipSuccess) return e;
-    LaunchCfg lc;
-    const uint32_t blocks = std::min<uint32_t>(tiles, stride_blocks((uint64_t)tiles * kScanBlock, kScanBlock, lc));
-    hipLaunchKernelGGL(k_scan_excl, dim3(blocks), dim3(kScanBlock), 0, st, data, d_n, n_add, d_total,
-                       s.status, reinterpret_cast<uint32_t*>(s.status + tiles),
-                       (uint32_t)(n_cap < 0xffffffffull ? n_cap : 0xffffffffull));
+    const uint32_t blocks = std::min<uint32_t>(lb_tiles, kLbBlocks);
+    uint32_t* counter = reinterpret_cast<uint32_t*>(s.status + lb_tiles);
+    if (vec)
+        hipLaunchKernelGGL(k_scan_lookback<true>, dim3(blocks), dim3(kLbBlock), 0, st, data, d_n, n_add, d_total,
+                           s.status, counter, cap);
+    else
+        hipLaunchKernelGGL(k_scan_lookback<false>, dim3(blocks), dim3(kLbBlock), 0, st, data, d_n, n_add, d_total,
+                           s.status, counter, cap);
     return hipGetLastError();
 }
 
