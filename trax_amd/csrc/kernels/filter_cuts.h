@@ -7,7 +7,7 @@
 
 namespace trax {
 
-// The pair terms of the cuts (TripletThetaPhiFilter.h:96, :104): the phi and
+// The pair terms of the cuts (TripletThetaPhiFilter.h:99-100, :84-88): the phi and
 // theta angles of the first hit pair, the same for every candidate of a pair.
 __device__ inline float filter_pair_phi(float x1, float y1, float x2, float y2) {
     return tx_atan2f((y2 - y1), (x2 - x1));
@@ -16,7 +16,7 @@ __device__ inline float filter_pair_theta(float x1, float y1, float z1, float x2
     return tx_atan2f(sqrtf((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1)), (z2 - z1));
 }
 
-// The phi cut, TripletThetaPhiFilter.h:94-102, given the pair's phi angle.
+// The phi cut, TripletThetaPhiFilter.h:98-107, given the pair's phi angle.
 __device__ inline bool filter_phi_cut(const TripletCols& tc, uint32_t lt, float x2, float y2, float x3, float y3,
                                       float phi1) {
     const float angle2 = tx_atan2f((y3 - y2), (x3 - x2));
@@ -26,17 +26,22 @@ __device__ inline bool filter_phi_cut(const TripletCols& tc, uint32_t lt, float 
     return fabsf(delta) <= tc.dphi[lt];
 }
 
-// The theta and circle cuts, TripletThetaPhiFilter.h:104-152, given the
-// pair's theta angle.  With the phi cut these are a conjunction of pure
-// expressions: a candidate is rejected at the first cut it fails (the result
-// is the reference's).
-__device__ inline bool filter_theta_circle_cuts(const TripletCols& tc, uint32_t lt, float x1, float y1, float x2,
-                                                float y2, float z2, float x3, float y3, float z3, float theta1) {
-    const float dThetaCut = tc.dtheta[lt], tipCut = tc.tip[lt];
-    const float minRadius = tc.min_radius_curvature;
+// The theta cut, TripletThetaPhiFilter.h:83-96, given the pair's theta angle.
+__device__ inline bool filter_theta_cut(const TripletCols& tc, uint32_t lt, float x2, float y2, float z2, float x3,
+                                        float y3, float z3, float theta1) {
+    const float dThetaCut = tc.dtheta[lt];
     const float angle2 = tx_atan2f(sqrtf((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2)), (z3 - z2));
     const float delta = fabsf(angle2 / theta1);
-    if (!(1 - dThetaCut <= delta && delta <= 1 + dThetaCut)) return false;
+    return 1 - dThetaCut <= delta && delta <= 1 + dThetaCut;
+}
+
+// The circle cuts, TripletThetaPhiFilter.h:109-150: the transverse impact
+// parameter and the radius of the circle through the three hits (from the
+// plane through their lifts onto the paraboloid z = x^2 + y^2).
+__device__ inline bool filter_circle_cuts(const TripletCols& tc, uint32_t lt, float x1, float y1, float x2, float y2,
+                                          float x3, float y3) {
+    const float tipCut = tc.tip[lt];
+    const float minRadius = tc.min_radius_curvature;
     bool valid = true;
     const float p1x = x1, p1y = y1, p1z = x1 * x1 + y1 * y1;
     const float p2x = x2, p2y = y2, p2z = x2 * x2 + y2 * y2;
@@ -61,6 +66,16 @@ __device__ inline bool filter_theta_circle_cuts(const TripletCols& tc, uint32_t 
     valid = valid && (tip_squared <= (tipCut * tipCut));
     valid = valid && (cR >= minRadius);
     return valid;
+}
+
+// The theta and circle cuts, TripletThetaPhiFilter.h:83-96 and 109-150, given the
+// pair's theta angle.  With the phi cut these are a conjunction of pure
+// expressions: a candidate is rejected at the first cut it fails, and the
+// cuts may be tested in any order (the result is the reference's).
+__device__ inline bool filter_theta_circle_cuts(const TripletCols& tc, uint32_t lt, float x1, float y1, float x2,
+                                                float y2, float z2, float x3, float y3, float z3, float theta1) {
+    return filter_theta_cut(tc, lt, x2, y2, z2, x3, y3, z3, theta1) &&
+           filter_circle_cuts(tc, lt, x1, y1, x2, y2, x3, y3);
 }
 
 // TripletThetaPhiFilter.h:81-152 on one candidate: the phi prefilter (most

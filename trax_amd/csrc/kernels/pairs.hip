@@ -363,10 +363,17 @@ __device__ inline uint32_t pred_group_end_index(const TripletCols& tc, const uin
     if (h1 == kSentinel) return kSentinel;
     const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
     if (ev >= n_events || lt >= tc.n_trip) return kSentinel;
+    // against the next pair that is not padding: padding pairs (short slots)
+    // sit inside groups, and the reference's comparison with one (an
+    // out-of-range hit read) lets several pairs of a group write its end, the
+    // last one (in pair order) last.  Only that pair writes here, so the
+    // result does not depend on the order in which pairs are processed.
+    uint32_t j = gid + 1;
+    while (j < n && pairs[2 * (size_t)j] == kSentinel) ++j;
     bool last = true;
-    if (gid + 1 < n) {
-        const uint32_t nh = pairs[2 * (size_t)(gid + 1)];
-        if (nh != kSentinel) last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
+    if (j < n) {
+        const uint32_t nh = pairs[2 * (size_t)j];
+        last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
     }
     return last ? ev * tc.n_trip + lt + 1 : kSentinel;
 }
@@ -1093,6 +1100,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ Mask s_rm[kPredBlock], s_sm[kPredBlock];
     __shared__ uint32_t s_pool_valid[kPredBlock];
     __shared__ uint32_t s_queue[kWaves * kQueue];
+    __shared__ uint32_t s_queue2[kWaves * kQueue];
     __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
     __shared__ uint32_t s_tile, s_pool_n, s_dead_any;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
@@ -1240,23 +1248,51 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 c = s_first[lo] + s_list[lo * kPredList + (tag & 0xffffu)];
                 return lo;
             };
-            auto test_queue = [&](uint32_t k0) {
-                const uint32_t tag = q[k0 + lane];
+            // The cuts are a conjunction of pure expressions, tested in two
+            // compacted stages: the circle cuts first (they reject most of
+            // what the phi prefilter keeps), then the phi and theta cuts (two
+            // atan2) on the candidates left, 64 at a time from a second queue.
+            uint32_t* q2 = s_queue2 + wave * kQueue;
+            uint32_t qn2 = 0;
+            auto test_angles = [&](uint32_t k0) {
+                const uint32_t tag = q2[k0 + lane];
                 uint32_t c;
                 const uint32_t lo = third_of(tag, c);
-                const size_t pi = tile * kPredBlock + lo;
-                const uint32_t a = pairs[2 * pi], b = pairs[2 * pi + 1];
-                const float2 p1 = hxy[a], p2 = hxy[b], p3 = hxy[c];
+                const uint32_t b = pairs[2 * (size_t)(tile * kPredBlock + lo) + 1];
+                const float2 p2 = hxy[b], p3 = hxy[c];
                 const float2 ang = s_angles[lo];
                 const uint32_t l = s_lt[lo] & 0xffu;
                 if (filter_phi_cut(tc, l, p2.x, p2.y, p3.x, p3.y, ang.x) &&
-                    filter_theta_circle_cuts(tc, l, p1.x, p1.y, p2.x, p2.y, hz[b], p3.x, p3.y, hz[c], ang.y)) {
+                    filter_theta_cut(tc, l, p2.x, p2.y, hz[b], p3.x, p3.y, hz[c], ang.y)) {
                     if (tag & kValidBit) {
                         atomicAdd(&s_pool_valid[lo], 1u);
                         pool[tag & ~kValidBit].x |= kValidBit;
                     } else {
                         atomicOr(&s_valid[lo], (Mask)1 << (tag & 0xffffu));
                     }
+                }
+            };
+            // called by the whole wave: the circle cuts on q[k0 + lane] for
+            // the active lanes, those passing appended to the second queue
+            auto test_queue = [&](uint32_t k0, bool active) {
+                bool pass = false;
+                uint32_t tag = 0;
+                if (active) {
+                    tag = q[k0 + lane];
+                    uint32_t c;
+                    const uint32_t lo = third_of(tag, c);
+                    const size_t pi = tile * kPredBlock + lo;
+                    const float2 p1 = hxy[pairs[2 * pi]], p2 = hxy[pairs[2 * pi + 1]], p3 = hxy[c];
+                    pass = filter_circle_cuts(tc, s_lt[lo] & 0xffu, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+                }
+                const uint64_t m = __ballot(pass);
+                if (pass) q2[qn2 + (uint32_t)__popcll(m & lanemask_lt())] = tag;
+                qn2 += (uint32_t)__popcll(m);
+                wave_sync();
+                if (qn2 >= kWave) {
+                    qn2 -= kWave;
+                    test_angles(qn2);
+                    wave_sync();
                 }
             };
             for (uint32_t eb = e0; eb < e1; eb += kWave) {
@@ -1297,11 +1333,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 wave_sync();
                 if (qn >= kWave) {
                     qn -= kWave;
-                    test_queue(qn);
+                    test_queue(qn, true);
                     wave_sync();
                 }
             }
-            if (lane < qn) test_queue(0);
+            test_queue(0, lane < qn);
+            if (lane < qn2) test_angles(0);
         }
         __syncthreads();
         // 4. the pair's valid triplets: the wave's room in the scratch, then

@@ -127,6 +127,7 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
         };
         std::string log;
         bool changed = false;
+        char first_change[96] = "";
         for (const Param& p : params) {
             uint32_t best_v = 0;
             float best_ms = 0.0f;
@@ -148,6 +149,12 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
                     const float m = phases_ms(b, p.phases);
                     ms = r == 0 ? m : std::min(ms, m);
                     if (r == 0) same = run_digest(b) == ref;
+                }
+                if (!same && !changed) {
+                    std::snprintf(first_change, sizeof(first_change), "%s = %u", p.name,
+                                  p.name == std::string("predict")
+                                      ? (v >> 31) * 1000000 + ((v >> 16) & 0x7fffu) * 1000 + (v & 0xffffu)
+                                      : v);
                 }
                 changed |= !same;
                 char line[256];
@@ -173,7 +180,10 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
             std::memcpy(report, log.data(), n);
             report[n] = 0;
         }
-        if (changed) return fail(TRAX_ERR_STATE, "a launch shape changed the results");
+        if (changed) {
+            const std::string msg = std::string("a launch shape changed the results (first: ") + first_change + ")";
+            return fail(TRAX_ERR_STATE, msg.c_str());
+        }
         return TRAX_OK;
     });
 }
