@@ -100,6 +100,56 @@ def scan_hbm(dev, n=1 << 28, reps=10):
     return {"elements": n, "ms": round(ms, 4), "gbs": round(gbs, 1), "frac": round(gbs * 1e9 / HBM_PEAK, 4)}
 
 
+def fp64_check(dev, wl, cfg, events, n_events=4):
+    """The filter's float32 decisions against a float64 recomputation: the
+    first n_events events run through the unfused batched path (which stores
+    the candidates), every candidate's cuts (TripletThetaPhiFilter.h:83-150)
+    evaluated in float64, and the candidates whose float64 decision differs
+    from the GPU's float32 one counted.  Such candidates sit within float32
+    rounding of a cut (the circle fit of near-straight tracks is
+    ill-conditioned); the unit tests explain each one by float32 Monte Carlo
+    arithmetic."""
+    import numpy as np
+    from trax_amd.api import Caps, HitSet, run_batch_fit
+    from trax_amd.config import triplet_arrays
+    hs = HitSet(cfg, events[:n_events])
+    b, res = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, per_event=True))
+    b.close()
+    x, y, z = (res[k].astype(np.float64) for k in ("hits.x", "hits.y", "hits.z"))
+    pairs = res["pairs"].reshape(-1, 2).astype(np.int64)
+    cands = res["cands"].reshape(-1, 2).astype(np.int64)
+    cands = cands[(cands[:, 0] != 0xFFFFFFFF) & (cands[:, 1] != 0xFFFFFFFF)]
+    h1, h2, h3 = pairs[cands[:, 0], 0], pairs[cands[:, 0], 1], cands[:, 1]
+    lt = res["hits.layer"][h1].astype(np.int64) - 1
+    t = triplet_arrays(wl.triplets)
+    dphi, dth, tip = (t[k][lt].astype(np.float64) for k in ("dPhiCut", "dThetaCut", "tipCut"))
+    th1 = np.arctan2(np.hypot(x[h2] - x[h1], y[h2] - y[h1]), z[h2] - z[h1])
+    th2 = np.arctan2(np.hypot(x[h3] - x[h2], y[h3] - y[h2]), z[h3] - z[h2])
+    r = np.abs(th2 / th1)
+    ok = (1 - dth <= r) & (r <= 1 + dth)
+    d = np.arctan2(y[h3] - y[h2], x[h3] - x[h2]) - np.arctan2(y[h2] - y[h1], x[h2] - x[h1])
+    d = np.where(d > np.pi, d - 2 * np.pi, np.where(d < -np.pi, d + 2 * np.pi, d))
+    ok &= np.abs(d) <= dphi
+    p1 = np.stack([x[h1], y[h1], x[h1] ** 2 + y[h1] ** 2], 1)
+    p2 = np.stack([x[h2], y[h2], x[h2] ** 2 + y[h2] ** 2], 1)
+    p3 = np.stack([x[h3], y[h3], x[h3] ** 2 + y[h3] ** 2], 1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        n = np.cross(p2 - p1, p3 - p1)
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        cx, cy = -n[:, 0] / (2 * n[:, 2]), -n[:, 1] / (2 * n[:, 2])
+        c = -(n * p1).sum(1)
+        radius = np.sqrt((1 - n[:, 2] ** 2 - 4 * c * n[:, 2]) / (4 * n[:, 2] ** 2))
+        v = np.stack([-cx, -cy], 1)
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        pca = np.stack([cx, cy], 1) + radius[:, None] * v
+        ok &= ((pca ** 2).sum(1) <= tip ** 2) & (radius >= cfg.min_radius_curvature)
+    key = lambda a, b_, c_: (a << 42) | (b_ << 21) | c_
+    kept = np.isin(key(h1, h2, h3), key(res["tracklets.h1"].astype(np.int64), res["tracklets.h2"].astype(np.int64),
+                                        res["tracklets.h3"].astype(np.int64)))
+    return {"events": hs.n_events, "candidates": int(h3.size), "tracklets": int(kept.sum()),
+            "float32_float64_disagreements": int((kept != ok).sum())}
+
+
 def serving_stream(wl, n_events, seed, workers):
     """The 10,000-event stream of the serving measurement, generated on a
     process pool (before the process touches the GPU) into one HitSet."""
@@ -311,6 +361,8 @@ def main():
 
     scan = scan_hbm(dev)
     print(f"scan alone: {scan}", file=sys.stderr, flush=True)
+    fp64 = fp64_check(dev, wl, cfg, events)
+    print(f"float64 check: {fp64}", file=sys.stderr, flush=True)
 
     # roofline of the dominant phase, HBM-bound
     cols = hs.columns()
@@ -357,6 +409,7 @@ def main():
         "grid_build_hbm_gbs": round(grid_gbs, 1),
         "grid_build_hbm_frac": round(grid_gbs * 1e9 / HBM_PEAK, 4),
         "scan_hbm": scan,
+        "fp64_check": fp64,
         "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
         "phase_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
