@@ -178,7 +178,9 @@ typedef struct {
     uint32_t max_conn;
     uint32_t max_track_entries;
     uint32_t use_graph;    /* capture the sequence once, replay per batch */
-    uint32_t ca_iterations; /* CA iterations in the sequence (0 = layer triplets + 2) */
+    uint32_t ca_iterations; /* CA iterations in the sequence; 0 = one pass per layer triplet when every
+                               triplet's second layer is above its first (same states and iteration
+                               count), else layer triplets + 2 iterations */
     /* 0: the batch is one event group, as buildTriplets runs a group of events
      *    (the connectivity stage leaves out the group's last tracklet,
      *    TripletConnectivityTight.cpp:277 and TripletThetaPhiFilter.h:250-253);

@@ -612,7 +612,10 @@ void trax_batch::enqueue_stage(int s) {
             break;
         case 14:  // ca_forward
             ok(launch_ca_init(R, cb, cnt, st), "ca init");
-            ok(launch_ca_iterations(K, lb.basis, lb.foll, cb, cnt, 0, ca_iter, lc, st), "ca iterations");
+            if (ca_layered)
+                ok(launch_ca_layers(K, T, lb.basis, lb.foll, th1, hout.layer, cb, cnt, lc, st), "ca layers");
+            else
+                ok(launch_ca_iterations(K, lb.basis, lb.foll, cb, cnt, 0, ca_iter, lc, st), "ca iterations");
             break;
         case 15:  // ca_backward
             enqueue_ca_tail();
@@ -644,6 +647,10 @@ void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* c
     B.cfg.grid.nLayers = B.L;
     B.C = B.cfg.grid.cells();
     B.ca_iter = caps->ca_iterations ? caps->ca_iterations : B.T + 2;
+    // every connection leads to a higher layer triplet when each triplet's
+    // second layer is above the layer that indexes it (its first)
+    B.ca_layered = caps->ca_iterations == 0 && B.T <= 256;
+    for (uint32_t i = 0; i < B.T; ++i) B.ca_layered = B.ca_layered && cfg->trip.layer2[i] >= i + 2;
     const uint64_t slot_cap = (uint64_t)caps->max_hits + (uint64_t)B.E * B.T * cfg->n_threads;
     if (slot_cap >= 0xffffffffull || (uint64_t)B.E * B.L * B.C >= 0xffffffffull)
         throw InputError("capacities too large");
@@ -739,6 +746,7 @@ void trax_batch::layout(Arena& a) {
         cb.handler = a.take<uint32_t>(R + 1);
         cb.tracks = a.take<uint32_t>(TE);
         cb.flags = a.take<uint32_t>(3);
+        cb.clt = B.ca_layered ? a.take<uint8_t>(K) : nullptr;
 }
 
 extern "C" int trax_batch_plan(const trax_config* cfg, const trax_batch_caps* caps, uint64_t* device_bytes,

@@ -124,6 +124,7 @@ struct trax_batch {
     trax_batch_caps caps{};
     trax::LaunchCfg lc;
     uint32_t E = 0, L = 0, T = 0, C = 0, slot_cap = 0, scan_cap = 0, ca_iter = 0;
+    bool ca_layered = false;  // CA forward phase by layer triplets (launch_ca_layers)
     std::unique_ptr<trax::ConfigMirror> cm;
     trax::DevCounts* cnt = nullptr;
     trax::ScanScratch scan;

@@ -295,6 +295,7 @@ struct CABufs {
     uint32_t* handler = nullptr;   // R + 1
     uint32_t* tracks = nullptr;    // track-entry capacity
     uint32_t* flags = nullptr;     // 3 rotating "alive" flags
+    uint8_t* clt = nullptr;        // connection capacity: the layer triplet of each connection's basis
 };
 hipError_t launch_ca_init(uint32_t n_trk_cap, const CABufs& b, DevCounts* cnt, hipStream_t st);
 // iterations first_iter .. first_iter + n_iter - 1; an iteration after the
@@ -302,6 +303,14 @@ hipError_t launch_ca_init(uint32_t n_trk_cap, const CABufs& b, DevCounts* cnt, h
 hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, const uint32_t* foll,
                                 const CABufs& b, DevCounts* cnt, uint32_t first_iter, uint32_t n_iter,
                                 const LaunchCfg& lc, hipStream_t st);
+// The forward phase in one pass per layer triplet instead of iterations, for
+// configurations where every connection leads to a higher layer triplet
+// (each triplet's second layer above its first): the same states and the
+// same iteration count (the iteration that finds no living cell included).
+// th1 and hlayer give each basis's layer triplet; n_trip passes.
+hipError_t launch_ca_layers(uint32_t n_conn_cap, uint32_t n_trip, const uint32_t* basis, const uint32_t* foll,
+                            const uint32_t* th1, const uint32_t* hlayer, const CABufs& b, DevCounts* cnt,
+                            const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_ca_basis_count(uint32_t n_trk_cap, uint32_t n_conn_cap, const uint32_t* basis,
                                  const uint32_t* foll, const CABufs& b, DevCounts* cnt, const LaunchCfg& lc,
                                  hipStream_t st);

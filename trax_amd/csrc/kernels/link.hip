@@ -557,6 +557,44 @@ __global__ void k_ca_iter(const uint32_t* __restrict__ basis, const uint32_t* __
     if (__ballot(alive) != 0ull && lane_id() == 0) b.flags[k % 3] = 1u;
 }
 
+// The forward phase by layers.  From all states 1, the synchronous update
+// (.h:52-100) raises a follower by one while some basis has its state, so at
+// iteration k a tracklet holds min(k, L) + 1, L the longest chain of
+// connections ending at it, and the iterations stop at max L + 1.  When every
+// connection goes to a higher layer triplet, st[f] = max(st[f], st[b] + 1)
+// over the connections taken in layer-triplet order of their bases gives the
+// same final states: each pass reads bases that no later pass writes.
+__global__ void k_ca_conn_layer(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ th1,
+                                const uint32_t* __restrict__ hlayer, uint8_t* __restrict__ clt,
+                                DevCounts* __restrict__ cnt) {
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    for (uint32_t p = gtid(); p < K; p += gstride()) clt[p] = (uint8_t)(hlayer[th1[basis[p]]] - 1u);
+    if (gtid() == 0) cnt->ca_max_state = 1u;
+}
+
+__global__ void k_ca_layer(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll,
+                           const uint8_t* __restrict__ clt, uint32_t lt, CABufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    uint32_t mx = 0;
+    for (uint32_t p = gtid(); p < K; p += gstride()) {
+        if (clt[p] != lt) continue;
+        const uint32_t s = b.st_a[basis[p]] + 1u;
+        atomicMax(&b.st_a[foll[p]], s);
+        mx = s > mx ? s : mx;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(mx, d, kWave);
+        mx = o > mx ? o : mx;
+    }
+    if (lane_id() == 0 && mx) atomicMax(&cnt->ca_max_state, mx);
+}
+
+__global__ void k_ca_layers_done(CABufs b, DevCounts* __restrict__ cnt) {
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    cnt->ca_iterations = K ? cnt->ca_max_state : 0u;  // max L + 1
+    b.flags[0] = 0u; b.flags[1] = 0u; b.flags[2] = 0u;  // nothing left for the host to iterate
+}
+
 __global__ void k_ca_basis_count(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll, CABufs b,
                                  DevCounts* __restrict__ cnt) {
     const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
@@ -800,6 +838,16 @@ hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, cons
                                 const LaunchCfg& lc, hipStream_t st) {
     for (uint32_t k = first_iter; k < first_iter + n_iter; ++k) TRAX_LAUNCH(k_ca_iter, n_conn_cap, 256u, basis, foll, b, k, cnt);
     return hipSuccess;
+}
+
+hipError_t launch_ca_layers(uint32_t n_conn_cap, uint32_t n_trip, const uint32_t* basis, const uint32_t* foll,
+                            const uint32_t* th1, const uint32_t* hlayer, const CABufs& b, DevCounts* cnt,
+                            const LaunchCfg& lc, hipStream_t st) {
+    if (!b.clt || n_trip > 256) return hipErrorInvalidValue;
+    TRAX_LAUNCH(k_ca_conn_layer, n_conn_cap, 256u, basis, th1, hlayer, b.clt, cnt);
+    for (uint32_t lt = 0; lt < n_trip; ++lt) TRAX_LAUNCH(k_ca_layer, n_conn_cap, 256u, basis, foll, b.clt, lt, b, cnt);
+    hipLaunchKernelGGL(k_ca_layers_done, dim3(1), dim3(1), 0, st, b, cnt);
+    return hipGetLastError();
 }
 
 hipError_t launch_ca_basis_count(uint32_t n_trk_cap, uint32_t n_conn_cap, const uint32_t* basis,
