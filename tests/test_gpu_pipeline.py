@@ -80,6 +80,25 @@ def test_batched_eager_matches_graph(dev):
     g.close()
 
 
+@pytest.mark.parametrize("iterations", [1, 3])
+def test_ca_iterations_match_layer_passes(dev, iterations):
+    """The CA forward phase as fixed iterations on the device (with the host
+    continuing while cells live) gives the states, tracks and iteration count
+    of the default one pass per layer triplet, and both the oracle's."""
+    wl, cfg, hs, _ = make_case("two-events-ca-density")
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    caps = Caps.for_hits(hs.n_events, hs.n_hits)
+    b, layered = run_batch_fit(dev, cfg, hs, caps)
+    b.close()
+    caps.ca_iterations = iterations
+    b, iterated = run_batch_fit(dev, cfg, hs, caps)
+    b.close()
+    assert int(want["ca.iterations"][0]) > iterations  # the host loop ran
+    for got in (layered, iterated):
+        bad = compare(got, want)
+        assert not bad, "\n".join(bad)
+
+
 def _per_event_triplets(res, n_events):
     """Tracklets of each event as sorted (h1, h2, h3) with event-local hit indices."""
     ev = res["hits.event"]

@@ -405,6 +405,28 @@ def test_connectivity_tight_four_layer_chain():
     assert last not in res["conn.connectable"] and last not in res["conn.nonconnectable"]
 
 
+def test_ca_states_are_longest_chains(dense):
+    """The CA forward phase (CellularAutomaton.h:52-100), iterated until no
+    cell lives, leaves each tracklet at 1 + the length of the longest chain of
+    connections ending at it, and stops after max + 1 iterations: a closed
+    form, computed here over the connections in the layer order of their
+    bases (every connection leads to a higher layer triplet)."""
+    wl, cfg, cols, res, _ = dense
+    basis = res["conn.basis"].astype(np.int64)
+    foll = res["conn.followers"].astype(np.int64)
+    th1 = res["tracklets.h1"].astype(np.int64)
+    lay = res["hits.layer"].astype(np.int64)
+    assert basis.size > 1000
+    lt_b, lt_f = lay[th1[basis]], lay[th1[foll]]
+    assert np.all(lt_f > lt_b)
+    st = np.ones(th1.size, np.int64)
+    for lt in np.unique(lt_b):
+        m = lt_b == lt
+        np.maximum.at(st, foll[m], st[basis[m]] + 1)
+    assert np.array_equal(st, res["ca.states"].astype(np.int64))
+    assert int(res["ca.iterations"][0]) == int(st.max())
+
+
 def test_ca_best_basis_breaks_pt_ties_by_lowest_index():
     # two bases with the same circle (identical hits) feeding one follower
     wl = small_workload(2)

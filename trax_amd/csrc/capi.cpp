@@ -613,7 +613,7 @@ void trax_batch::enqueue_stage(int s) {
         case 14:  // ca_forward
             ok(launch_ca_init(R, cb, cnt, st), "ca init");
             if (ca_layered)
-                ok(launch_ca_layers(K, T, lb.basis, lb.foll, th1, hout.layer, cb, cnt, lc, st), "ca layers");
+                ok(launch_ca_layers(K, T, lb.basis, lb.foll, th1, hout.layer, cb, cnt, scan, lc, st), "ca layers");
             else
                 ok(launch_ca_iterations(K, lb.basis, lb.foll, cb, cnt, 0, ca_iter, lc, st), "ca iterations");
             break;
@@ -658,7 +658,7 @@ void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* c
     const uint64_t P = caps->max_pairs, Q = caps->max_cands, R = caps->max_tracklets;
     B.scan_cap = (uint32_t)std::max<uint64_t>({caps->max_hits + 1ull, slot_cap + 1, P + 1, Q / 64 + 2, R + 1,
                                                (uint64_t)B.E * B.T + 1, predict_fused_chunks(P) + 1,
-                                               link_chunks(R) + 1});
+                                               link_chunks(R) + 1, (uint64_t)B.T * kCaTableBlocks});
 }
 
 uint32_t batch_scan_tiles(const trax_batch& B) {
@@ -747,6 +747,8 @@ void trax_batch::layout(Arena& a) {
         cb.tracks = a.take<uint32_t>(TE);
         cb.flags = a.take<uint32_t>(3);
         cb.clt = B.ca_layered ? a.take<uint8_t>(K) : nullptr;
+        cb.lt_conn = B.ca_layered ? a.take<uint2>(K) : nullptr;
+        cb.lt_tab = B.ca_layered ? a.take<uint32_t>((uint64_t)B.T * kCaTableBlocks + 1) : nullptr;
 }
 
 extern "C" int trax_batch_plan(const trax_config* cfg, const trax_batch_caps* caps, uint64_t* device_bytes,

@@ -48,7 +48,7 @@ struct DevCounts {
     uint32_t n_pred_chunks;     // fused predictor + filter: 64-pair chunks of the run
     uint32_t link_alloc;        // single-pass connectivity: connection scratch handed out to chunks
     uint32_t n_link_chunks;     // single-pass connectivity: 64-tracklet chunks of the run
-    uint32_t ca_max_state;      // CA forward phase by layers: the largest state
+    uint32_t pad[1];
 };
 constexpr uint32_t kCountsShapeWords = 9;  // n_events, n_hits and the seven capacities
 

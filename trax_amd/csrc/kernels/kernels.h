@@ -296,7 +296,10 @@ struct CABufs {
     uint32_t* tracks = nullptr;    // track-entry capacity
     uint32_t* flags = nullptr;     // 3 rotating "alive" flags
     uint8_t* clt = nullptr;        // connection capacity: the layer triplet of each connection's basis
+    uint2* lt_conn = nullptr;      // connection capacity: (basis, follower) bucketed by that layer triplet
+    uint32_t* lt_tab = nullptr;    // layer triplets * kCaTableBlocks + 1: per-block bucket counts, then starts
 };
+constexpr uint32_t kCaTableBlocks = 1024;
 hipError_t launch_ca_init(uint32_t n_trk_cap, const CABufs& b, DevCounts* cnt, hipStream_t st);
 // iterations first_iter .. first_iter + n_iter - 1; an iteration after the
 // one that found no living cell returns at once
@@ -310,7 +313,7 @@ hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, cons
 // th1 and hlayer give each basis's layer triplet; n_trip passes.
 hipError_t launch_ca_layers(uint32_t n_conn_cap, uint32_t n_trip, const uint32_t* basis, const uint32_t* foll,
                             const uint32_t* th1, const uint32_t* hlayer, const CABufs& b, DevCounts* cnt,
-                            const LaunchCfg& lc, hipStream_t st);
+                            const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
 hipError_t launch_ca_basis_count(uint32_t n_trk_cap, uint32_t n_conn_cap, const uint32_t* basis,
                                  const uint32_t* foll, const CABufs& b, DevCounts* cnt, const LaunchCfg& lc,
                                  hipStream_t st);

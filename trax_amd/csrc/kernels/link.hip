@@ -564,35 +564,101 @@ __global__ void k_ca_iter(const uint32_t* __restrict__ basis, const uint32_t* __
 // connection goes to a higher layer triplet, st[f] = max(st[f], st[b] + 1)
 // over the connections taken in layer-triplet order of their bases gives the
 // same final states: each pass reads bases that no later pass writes.
-__global__ void k_ca_conn_layer(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ th1,
-                                const uint32_t* __restrict__ hlayer, uint8_t* __restrict__ clt,
-                                DevCounts* __restrict__ cnt) {
-    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
-    for (uint32_t p = gtid(); p < K; p += gstride()) clt[p] = (uint8_t)(hlayer[th1[basis[p]]] - 1u);
-    if (gtid() == 0) cnt->ca_max_state = 1u;
+// each connection's basis layer triplet, and per block of a fixed grid (each
+// a contiguous range of connections) its connections per layer triplet, in a
+// table tab[lt * gridDim.x + block] that a scan turns into bucket positions
+__device__ inline void ca_block_range(uint32_t K, uint32_t& lo, uint32_t& hi) {
+    const uint32_t per = (K + gridDim.x - 1) / gridDim.x;
+    lo = blockIdx.x * per < K ? blockIdx.x * per : K;
+    hi = lo + per < K ? lo + per : K;
 }
 
-__global__ void k_ca_layer(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll,
-                           const uint8_t* __restrict__ clt, uint32_t lt, CABufs b, DevCounts* __restrict__ cnt) {
+__global__ void k_ca_conn_layer(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ th1,
+                                const uint32_t* __restrict__ hlayer, uint32_t n_trip, uint8_t* __restrict__ clt,
+                                uint32_t* __restrict__ tab, DevCounts* __restrict__ cnt) {
+    __shared__ uint32_t hist[256];
+    for (uint32_t i = threadIdx.x; i < n_trip; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
     const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
-    uint32_t mx = 0;
-    for (uint32_t p = gtid(); p < K; p += gstride()) {
-        if (clt[p] != lt) continue;
-        const uint32_t s = b.st_a[basis[p]] + 1u;
-        atomicMax(&b.st_a[foll[p]], s);
-        mx = s > mx ? s : mx;
+    uint32_t lo, hi;
+    ca_block_range(K, lo, hi);
+    bool bad = false;
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+        const uint32_t lt = hlayer[th1[basis[p]]] - 1u;
+        clt[p] = (uint8_t)lt;
+        if (lt < n_trip) atomicAdd(&hist[lt], 1u);
+        else bad = true;
     }
+    if (bad) atomicOr(&cnt->err, kErrIndexRange);  // a basis outside the configured layer triplets
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n_trip; i += blockDim.x) tab[i * gridDim.x + blockIdx.x] = hist[i];
+}
+
+// the block's connections, as (basis, follower), to their bucket positions
+// (order within a bucket free)
+__global__ void k_ca_conn_scatter(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll,
+                                  const uint8_t* __restrict__ clt, uint32_t n_trip, const uint32_t* __restrict__ tab,
+                                  uint2* __restrict__ lt_conn, DevCounts* __restrict__ cnt) {
+    __shared__ uint32_t cur[256];
+    for (uint32_t i = threadIdx.x; i < n_trip; i += blockDim.x) cur[i] = tab[i * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    uint32_t lo, hi;
+    ca_block_range(K, lo, hi);
+    for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
+        const uint32_t lt = clt[p];
+        if (lt < n_trip) lt_conn[atomicAdd(&cur[lt], 1u)] = make_uint2(basis[p], foll[p]);
+    }
+}
+
+__global__ void k_ca_layer(const uint2* __restrict__ lt_conn, const uint32_t* __restrict__ tab, uint32_t tab_blocks,
+                           uint32_t n_trip, uint32_t lt, CABufs b) {
+    // the bucket: from its first table entry to the next bucket's (the total after the last)
+    const uint32_t lo = tab[lt * tab_blocks], hi = tab[lt + 1 < n_trip ? (lt + 1) * tab_blocks : n_trip * tab_blocks];
+    for (uint32_t v = lo + gtid(); v < hi; v += gstride()) {
+        const uint2 c = lt_conn[v];
+        atomicMax(&b.st_a[c.y], b.st_a[c.x] + 1u);
+    }
+}
+
+// the largest state: per-block maxima (part, gridDim.x entries), combined by k_ca_layers_done
+__global__ void k_ca_state_max(CABufs b, uint32_t* __restrict__ part, DevCounts* __restrict__ cnt) {
+    __shared__ uint32_t s_w[1024 / kWave];
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    uint32_t lo, hi;
+    ca_block_range(R, lo, hi);
+    uint32_t mx = 0;
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += blockDim.x) mx = b.st_a[t] > mx ? b.st_a[t] : mx;
     for (int d = 32; d >= 1; d >>= 1) {
         const uint32_t o = __shfl_xor(mx, d, kWave);
         mx = o > mx ? o : mx;
     }
-    if (lane_id() == 0 && mx) atomicMax(&cnt->ca_max_state, mx);
+    if (lane_id() == 0) s_w[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / kWave; ++w) mx = s_w[w] > mx ? s_w[w] : mx;
+        part[blockIdx.x] = mx;
+    }
 }
 
-__global__ void k_ca_layers_done(CABufs b, DevCounts* __restrict__ cnt) {
-    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
-    cnt->ca_iterations = K ? cnt->ca_max_state : 0u;  // max L + 1
-    b.flags[0] = 0u; b.flags[1] = 0u; b.flags[2] = 0u;  // nothing left for the host to iterate
+// one block: the iteration count max L + 1 (the largest state) when there are connections
+__global__ void k_ca_layers_done(CABufs b, const uint32_t* __restrict__ part, uint32_t n_part,
+                                 DevCounts* __restrict__ cnt) {
+    __shared__ uint32_t s_w[1024 / kWave];
+    uint32_t mx = 0;
+    for (uint32_t i = threadIdx.x; i < n_part; i += blockDim.x) mx = part[i] > mx ? part[i] : mx;
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = __shfl_xor(mx, d, kWave);
+        mx = o > mx ? o : mx;
+    }
+    if (lane_id() == 0) s_w[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / kWave; ++w) mx = s_w[w] > mx ? s_w[w] : mx;
+        const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+        cnt->ca_iterations = K ? mx : 0u;
+        b.flags[0] = 0u; b.flags[1] = 0u; b.flags[2] = 0u;  // nothing left for the host to iterate
+    }
 }
 
 __global__ void k_ca_basis_count(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll, CABufs b,
@@ -842,11 +908,23 @@ hipError_t launch_ca_iterations(uint32_t n_conn_cap, const uint32_t* basis, cons
 
 hipError_t launch_ca_layers(uint32_t n_conn_cap, uint32_t n_trip, const uint32_t* basis, const uint32_t* foll,
                             const uint32_t* th1, const uint32_t* hlayer, const CABufs& b, DevCounts* cnt,
-                            const LaunchCfg& lc, hipStream_t st) {
-    if (!b.clt || n_trip > 256) return hipErrorInvalidValue;
-    TRAX_LAUNCH(k_ca_conn_layer, n_conn_cap, 256u, basis, th1, hlayer, b.clt, cnt);
-    for (uint32_t lt = 0; lt < n_trip; ++lt) TRAX_LAUNCH(k_ca_layer, n_conn_cap, 256u, basis, foll, b.clt, lt, b, cnt);
-    hipLaunchKernelGGL(k_ca_layers_done, dim3(1), dim3(1), 0, st, b, cnt);
+                            const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st) {
+    if (!b.clt || !b.lt_conn || !b.lt_tab || n_trip == 0 || n_trip > 256) return hipErrorInvalidValue;
+    const uint32_t nb = kCaTableBlocks;
+    hipLaunchKernelGGL(k_ca_conn_layer, dim3(nb), dim3(256), 0, st, basis, th1, hlayer, n_trip, b.clt, b.lt_tab, cnt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // bucket positions: the table scanned in (layer triplet, block) order; its total after the last entry
+    e = launch_scan(b.lt_tab, nullptr, n_trip * nb, (uint64_t)n_trip * nb, b.lt_tab + (size_t)n_trip * nb, ss, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ca_conn_scatter, dim3(nb), dim3(256), 0, st, basis, foll, b.clt, n_trip, b.lt_tab, b.lt_conn,
+                       cnt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (uint32_t lt = 0; lt < n_trip; ++lt)
+        TRAX_LAUNCH(k_ca_layer, n_conn_cap, 256u, b.lt_conn, b.lt_tab, nb, n_trip, lt, b);
+    // the table's first nb entries are free now: the per-block maxima
+    hipLaunchKernelGGL(k_ca_state_max, dim3(nb), dim3(256), 0, st, b, b.lt_tab, cnt);
+    hipLaunchKernelGGL(k_ca_layers_done, dim3(1), dim3(1024), 0, st, b, b.lt_tab, nb, cnt);
     return hipGetLastError();
 }
 
