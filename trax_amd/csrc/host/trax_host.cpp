@@ -324,8 +324,9 @@ ConfigMirror::ConfigMirror(Context& ctx, const TripletConfigurations& t, const G
 }
 
 // ten words per hit: seven columns, one spare word and the (z, phi) pair at
-// an 8-byte aligned offset
-HitMirror::HitMirror(Context& ctx, uint n) : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 10ull * (n ? n : 1))) {}
+// a 16-byte aligned offset, then kZphiSlack pairs of room past its end
+HitMirror::HitMirror(Context& ctx, uint n)
+    : n_(n), buf_(new DeviceVector<uint32_t>(ctx, 10ull * (n ? n : 1) + 2ull * kZphiSlack)) {}
 
 HitColumns HitMirror::cols() const {
     HitColumns c;

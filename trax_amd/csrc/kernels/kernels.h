@@ -57,6 +57,9 @@ struct FineGeom {
     uint32_t cells = 0;         // n_z * F * (n_phi + 1) + 1 prefix entries per (event, layer)
 };
 FineGeom fine_geom(const GridGeom& g, uint32_t F);
+// entries of room past the last hit of every (z, phi) column: the pair walks
+// read it in aligned 16-byte blocks of four entries
+constexpr uint32_t kZphiSlack = 4;
 constexpr uint32_t kFineMax = 16;
 // largest sub-row count the launch supports for this grid (power of two <= kFineMax)
 uint32_t fine_max(const GridGeom& g);

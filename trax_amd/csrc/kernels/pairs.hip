@@ -129,6 +129,28 @@ __device__ inline SlotInfo slot_info(const TripletCols& tc, const uint32_t* seg_
     return si;
 }
 
+// The entries [a, b) of the (z, phi) column in ascending order, read as
+// aligned 16-byte pairs, four entries a step (half the loads of one entry a
+// load; a wave's lanes each walk their own rows; the pair store's walk).  The column has
+// kZphiSlack entries of room past its last hit, so a step's tail read stays
+// inside the allocation.  f(j, entry) returns false to stop; returns whether
+// the range was finished.
+template <class F>
+__device__ inline bool zphi_scan(const float2* __restrict__ hzp, uint32_t a, uint32_t b, F&& f) {
+    const float4* p = reinterpret_cast<const float4*>(hzp);
+    for (uint32_t jb = a & ~3u; jb < b; jb += 4) {
+        const float4 v0 = p[jb >> 1], v1 = p[(jb >> 1) + 1];
+        const float2 e[4] = {make_float2(v0.x, v0.y), make_float2(v0.z, v0.w), make_float2(v1.x, v1.y),
+                             make_float2(v1.z, v1.w)};
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t j = jb + u;
+            if (j >= a && j < b && !f(j, e[u])) return false;
+        }
+    }
+    return true;
+}
+
 __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                              const float* __restrict__ hx, const float* __restrict__ hy,
                              const float* __restrict__ hz, const float2* __restrict__ hzp,
@@ -210,14 +232,15 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
             }
             const uint32_t lo[2] = {a1, b1 > a2 ? b1 : a2};
             const uint32_t hi[2] = {b1, b2};
-            for (int r = 0; r < 2; ++r)
-                for (uint32_t index = lo[r]; index < hi[r] && pos < next; ++index) {
-                    if (in_window(w, hzp[index])) {
+            for (int r = 0; r < 2 && pos < next; ++r)
+                zphi_scan(hzp, lo[r], hi[r], [&](uint32_t index, float2 q) {
+                    if (in_window(w, q)) {
                         pairs[2 * (size_t)pos] = index;
                         pairs[2 * (size_t)pos + 1] = i;
                         pos++;
                     }
-                }
+                    return pos < next;
+                });
         }
         for (uint32_t q = pos; q < next; ++q) {
             pairs[2 * (size_t)q] = kSentinel;
