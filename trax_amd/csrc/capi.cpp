@@ -477,6 +477,12 @@ const trax_capi::StageRange trax_capi::kStageRanges[6] = {{"grid", 0, 1},   {"pa
                                                           {"predict", 6, 6}, {"filter", 7, 9},
                                                           {"connectivity", 10, 13}, {"ca", 14, 15}};
 
+void trax_batch::enqueue_fine(hipStream_t s) {
+    FineGeom fg;
+    hip_check(launch_fine_grid(cfg.grid.geom(), E * L, caps.max_hits, grid, hout.zphi, cnt, fine, lc, fg, s),
+              "fine grid");
+}
+
 void trax_batch::launch() {
     if (caps.use_graph && !captured) {
         for (int k = 0; k < kNumStages; ++k) {
@@ -485,12 +491,24 @@ void trax_batch::launch() {
             hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
             hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
         }
+        hip_check(hipStreamBeginCapture(st2, hipStreamCaptureModeThreadLocal), "begin capture");
+        enqueue_fine(st2);
+        hip_check(hipStreamEndCapture(st2, &fine_graph), "end capture");
+        hip_check(hipGraphInstantiate(&fine_exec, fine_graph, nullptr, nullptr, 0), "instantiate");
         captured = true;
     }
     for (int k = 0; k < kNumStages; ++k) {
         hip_check(hipEventRecord(ev[k], st), "event");
+        if (k == kPredictStage) hip_check(hipStreamWaitEvent(st, ev_fine, 0), "wait fine grid");
         if (caps.use_graph) hip_check(hipGraphLaunch(execs[k], st), "graph launch");
         else enqueue_stage(k);
+        if (k == kGridStage) {  // fork: the fine walk structure beside the pair stages
+            hip_check(hipEventRecord(ev_grid, st), "event");
+            hip_check(hipStreamWaitEvent(st2, ev_grid, 0), "wait grid");
+            if (caps.use_graph) hip_check(hipGraphLaunch(fine_exec, st2), "graph launch");
+            else enqueue_fine(st2);
+            hip_check(hipEventRecord(ev_fine, st2), "event");
+        }
     }
     hip_check(hipEventRecord(ev[kNumStages], st), "event");
     hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
@@ -825,6 +843,10 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&B.h_counts), sizeof(DevCounts), hipHostMallocDefault),
                   "hipHostMalloc");
         for (auto& e : B.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipStreamCreateWithFlags(&B.st2, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&B.ev_grid, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&B.ev_fine, hipEventDisableTiming), "hipEventCreate");
+        B.fine.prebuilt = true;
         (void)stride_blocks(1, 256, B.lc);  // resolve the CU count before any capture
         *out = b.release();
         return TRAX_OK;
@@ -834,10 +856,16 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
 extern "C" void trax_batch_destroy(trax_batch* b) {
     if (!b) return;
     (void)hipStreamSynchronize(b->st);
+    if (b->st2) (void)hipStreamSynchronize(b->st2);
     for (int i = 0; i < kNumStages; ++i) {
         if (b->execs[i]) (void)hipGraphExecDestroy(b->execs[i]);
         if (b->graphs[i]) (void)hipGraphDestroy(b->graphs[i]);
     }
+    if (b->fine_exec) (void)hipGraphExecDestroy(b->fine_exec);
+    if (b->fine_graph) (void)hipGraphDestroy(b->fine_graph);
+    if (b->ev_grid) (void)hipEventDestroy(b->ev_grid);
+    if (b->ev_fine) (void)hipEventDestroy(b->ev_fine);
+    if (b->st2) (void)hipStreamDestroy(b->st2);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->arena_mem) (void)hipFree(b->arena_mem);

@@ -64,6 +64,7 @@ struct Arena {
 // phases of the batched launch sequence: one graph each, HIP events between
 // them time every phase on the batch's stream
 constexpr int kNumStages = 16;
+constexpr int kGridStage = 0, kPredictStage = 6;  // the grid build, and the stage that reads the fine walk structure
 extern const char* const kPhases[kNumStages];
 // the reference's algorithm classes as ranges of phases
 struct StageRange {
@@ -147,10 +148,18 @@ struct trax_batch {
     hipGraphExec_t execs[trax_capi::kNumStages] = {};
     bool captured = false;
     hipEvent_t ev[trax_capi::kNumStages + 1] = {};
+    // the predictor's fine walk structure is built on a second stream, beside
+    // the pair stages: it needs only the grid build (ev_grid), and the
+    // predict stage waits for it (ev_fine)
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_grid = nullptr, ev_fine = nullptr;
+    hipGraph_t fine_graph = nullptr;
+    hipGraphExec_t fine_exec = nullptr;
     uint32_t n_events = 0, n_hits = 0;
     bool submitted = false, have_counts = false;
 
     void enqueue_stage(int s);
+    void enqueue_fine(hipStream_t s);
     void enqueue_ca_tail();
     // carve every buffer out of the arena (run on a measuring arena first)
     void layout(trax_capi::Arena& a);

@@ -79,6 +79,7 @@ struct FineBufs {
     uint32_t* scratch = nullptr;     // 3 * scratch_cap: the chunks' tracklets in arrival order
     uint32_t scratch_cap = 0;        // fused_scratch_cap(n_trk_cap)
     const float2* xy = nullptr;      // (x, y) of every grid-ordered hit (the grid scatter's column)
+    bool prebuilt = false;           // grid and hits built by launch_fine_grid before the predictor's launch
 };
 constexpr uint32_t kFusedPoolCap = 8192, kFusedPoolBlocks = 2048;
 // tracklet scratch of the fused predictor + filter for n_trk_cap tracklets
@@ -196,6 +197,12 @@ hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32
 // predictor and filter fused (batched path): tracklets straight from the
 // walk, candidates counted (and their group offsets) but not stored; the
 // tracklet offsets come from launch_filter_offsets afterwards
+// the fine walk structure (fb.grid, fb.hits) of the grid-ordered hits for the
+// launch's sub-row count; the predictor launches build it themselves unless
+// fb.prebuilt (a caller may build it on another stream, after the grid build)
+hipError_t launch_fine_grid(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit_cap, const uint32_t* grid,
+                            const float2* hzp, DevCounts* cnt, const FineBufs& fb, const LaunchCfg& lc, FineGeom& fg,
+                            hipStream_t st);
 hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
                                  uint32_t n_pair_cap, uint32_t n_trk_cap, const uint32_t* grid, const float* hx,
                                  const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,

@@ -1616,6 +1616,20 @@ uint32_t fine_max(const GridGeom& g) {
 namespace {
 
 // the fine walk structure of every (event, layer), for the sub-row count of the launch shape
+// the fine walk structure's geometry for the launch's sub-row count, and the
+// structure itself unless the caller built it already (fb.prebuilt)
+hipError_t fine_grid_for_predict(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit_cap, const uint32_t* grid,
+                                 const float2* hzp, DevCounts* cnt, const FineBufs& fb, const LaunchCfg& lc,
+                                 FineGeom& fg, hipStream_t st) {
+    if (!fb.prebuilt) return launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
+    const uint32_t F = lc.predict_fine ? lc.predict_fine : 8;
+    if (F > fine_max(g) || (F & (F - 1)) != 0 || !fb.grid || !fb.hits) return hipErrorInvalidValue;
+    fg = fine_geom(g, F);
+    return hipSuccess;
+}
+
+}  // namespace
+
 hipError_t launch_fine_grid(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit_cap, const uint32_t* grid,
                             const float2* hzp, DevCounts* cnt, const FineBufs& fb, const LaunchCfg& lc, FineGeom& fg,
                             hipStream_t st) {
@@ -1630,8 +1644,6 @@ hipError_t launch_fine_grid(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit
     return hipGetLastError();
 }
 
-}  // namespace
-
 hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
                                  uint32_t n_pair_cap, uint32_t n_trk_cap, const uint32_t* grid, const float* hx,
                                  const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,
@@ -1643,7 +1655,7 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
         !fb.group_chunk || !fb.scratch || !fb.xy || fb.scratch_cap < fused_scratch_cap(n_trk_cap))
         return hipErrorInvalidValue;
     FineGeom fg;
-    hipError_t e = launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
+    hipError_t e = fine_grid_for_predict(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
     if (e != hipSuccess) return e;
     // chunk counts (the entry after the last chunk stays 0 for the scans), group chunks, the tile counter
     for (uint32_t* a : {fb.chunk_cands, fb.chunk_trk})
@@ -1687,7 +1699,7 @@ hipError_t launch_predict_fused(const GridGeom& g, const TripletCols& tc, uint32
     const uint32_t tiles = predict_fused_tiles(n_pair_cap);
     if (tiles > ss.max_tiles) return hipErrorInvalidValue;
     FineGeom fg;
-    hipError_t e = launch_fine_grid(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
+    hipError_t e = fine_grid_for_predict(g, n_el_cap, n_hit_cap, grid, hzp, cnt, fb, lc, fg, st);
     if (e != hipSuccess) return e;
     e = launch_fill(reinterpret_cast<uint32_t*>(ss.status), 0u, 2ull * tiles + 2, st);
     if (e != hipSuccess) return e;
