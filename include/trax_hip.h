@@ -135,6 +135,17 @@ int trax_hits_add_event(trax_hits* h, const trax_config* cfg, uint32_t n, const 
  * trax_hits_add_event; max_events = -1 for all.  *n_added: events added. */
 int trax_hits_add_pb(trax_hits* h, const trax_config* cfg, const void* data, uint64_t size, int max_events,
                      int max_tracks, int only_tracks, uint32_t* n_added);
+/* The same from an event store (GenericOutputStore / GenericInputStore,
+ * src/datastructures/serialize/PEventStore.h:29-248, read as EventStoreLoader
+ * does, src/triplet_builder/EventLoader.h:171-206): a little-endian u32 event
+ * count, then per event an entry type varint (1), a size varint and the
+ * serialized PB_Event::PEvent.  Reading stops at the count, at another entry
+ * type, at a zero size or at a truncated entry; the first skip_events are
+ * dropped and at most max_events (-1 = all) added.  *n_added: events added;
+ * *n_stored: the store's event count (either may be NULL).  A malformed event
+ * gives TRAX_ERR_INPUT. */
+int trax_hits_add_pb_store(trax_hits* h, const trax_config* cfg, const void* data, uint64_t size, int skip_events,
+                           int max_events, int max_tracks, int only_tracks, uint32_t* n_added, uint32_t* n_stored);
 uint32_t trax_hits_count(const trax_hits* h);
 uint32_t trax_hits_events(const trax_hits* h);
 /* column access: "x","y","z","layer","det_id","hit_id","event",

@@ -103,6 +103,56 @@ def test_pb_loader_matches_independent_decoder(ttbar):
     assert hs.n_hits > 1000
 
 
+def _varint_bytes(v):
+    out = bytearray()
+    while True:
+        b = v & 0x7F
+        v >>= 7
+        out.append(b | (0x80 if v else 0))
+        if not v:
+            return bytes(out)
+
+
+def _store(events_bytes, count=None):
+    """An event store as GenericOutputStore writes one (PEventStore.h:29-140):
+    the event count (u32, little endian), then per event the entry type 1, the
+    size and the serialized PEvent."""
+    body = b"".join(_varint_bytes(1) + _varint_bytes(len(e)) + e for e in events_bytes)
+    return struct.pack("<I", len(events_bytes) if count is None else count) + body
+
+
+def test_pb_store_matches_container(ttbar):
+    """The ttbar events rewritten as an event store load into the same hits as
+    the container, and the store's reading rules hold: the first events can
+    be skipped, the count bounds the read, and an entry of another type or a
+    cut-short entry ends it."""
+    wl, cfg, hs, n = ttbar
+    with open(PB, "rb") as f:
+        raw = [ev for fld, ev in _fields(f.read()) if fld == 1]
+    assert len(raw) == n == 10
+    st = HitSet(cfg)
+    added, stored = st.add_pb_store(_store(raw), only_tracks=True)
+    assert (added, stored) == (10, 10)
+    a, b = hs.columns(), st.columns()
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    # skip the first three, as EventStoreLoader's skipEvents; the rest equal the container's last seven
+    tail = HitSet(cfg)
+    assert tail.add_pb_store(_store(raw), skip_events=3, only_tracks=True) == (7, 10)
+    ref = HitSet(cfg)
+    for e in _decode(b"".join(_varint_bytes(0x0A) + _varint_bytes(len(x)) + x for x in raw[3:])):
+        ref.add(e, only_tracks=True)
+    for k in ("x", "y", "z", "hit_id", "event_nhits"):
+        assert np.array_equal(tail.column(k).view(np.uint32), ref.column(k).view(np.uint32)), k
+    # the header's count bounds the read; a foreign entry type or a short entry ends it
+    assert HitSet(cfg).add_pb_store(_store(raw, count=4), only_tracks=True) == (4, 4)
+    bad_type = _store(raw[:2]) + _varint_bytes(2) + _varint_bytes(len(raw[2])) + raw[2]
+    assert HitSet(cfg).add_pb_store(struct.pack("<I", 3) + bad_type[4:], only_tracks=True) == (2, 3)
+    cut = _store(raw[:3])[:-5]
+    assert HitSet(cfg).add_pb_store(cut, only_tracks=True) == (2, 3)
+    assert HitSet(cfg).add_pb_store(b"\x00\x00", only_tracks=True) == (0, 0)
+
+
 def test_oracle_physics_on_ttbar_matches_published_efficiency(ttbar):
     wl, cfg, hs, _ = ttbar
     res = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)

@@ -104,6 +104,8 @@ def lib():
         "trax_hits_destroy": (None, [P]),
         "trax_hits_add_event": (C.c_int, [P, P, U32, P, P, P, P, P, P, P, C.c_int, C.c_int]),
         "trax_hits_add_pb": (C.c_int, [P, P, C.c_char_p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(U32)]),
+        "trax_hits_add_pb_store": (C.c_int, [P, P, C.c_char_p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.POINTER(U32), C.POINTER(U32)]),
         "trax_hits_count": (U32, [P]),
         "trax_hits_events": (U32, [P]),
         "trax_hits_column": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(C.c_uint64)]),
@@ -210,6 +212,15 @@ class HitSet:
         check(lib().trax_hits_add_pb(self.h, self.cfg.h, data, len(data), max_events, max_tracks, int(only_tracks),
                                      C.byref(n)))
         return int(n.value)
+
+    def add_pb_store(self, data: bytes, skip_events: int = 0, max_events: int = -1, max_tracks: int = -1,
+                     only_tracks: bool = False):
+        """Add the events of an event store (the length-prefixed stream of
+        GenericOutputStore); returns (events added, the store's event count)."""
+        n, stored = U32(), U32()
+        check(lib().trax_hits_add_pb_store(self.h, self.cfg.h, data, len(data), skip_events, max_events, max_tracks,
+                                           int(only_tracks), C.byref(n), C.byref(stored)))
+        return int(n.value), int(stored.value)
 
     @property
     def n_hits(self) -> int:

@@ -284,6 +284,22 @@ extern "C" int trax_hits_add_pb(trax_hits* h, const trax_config* cfg, const void
     });
 }
 
+extern "C" int trax_hits_add_pb_store(trax_hits* h, const trax_config* cfg, const void* data, uint64_t size,
+                                      int skip_events, int max_events, int max_tracks, int only_tracks,
+                                      uint32_t* n_added, uint32_t* n_stored) {
+    return guarded([&] {
+        if (!h || !cfg || (!data && size)) return fail(TRAX_ERR_ARG, "null argument");
+        if (cfg->trip.maxLayer() != h->max_layer)
+            return fail(TRAX_ERR_ARG, "configuration's highest layer differs from the collection's");
+        const std::vector<PEventData> evs =
+            decodeEventStore(static_cast<const uint8_t*>(data), size, skip_events > 0 ? (uint32_t)skip_events : 0u,
+                             max_events >= 0 ? (uint32_t)max_events : 0xffffffffu, n_stored);
+        for (const PEventData& ev : evs) add_event(h, cfg, ev.hits, max_tracks, only_tracks);
+        if (n_added) *n_added = (uint32_t)evs.size();
+        return TRAX_OK;
+    });
+}
+
 extern "C" uint32_t trax_hits_count(const trax_hits* h) { return h->hits.size(); }
 extern "C" uint32_t trax_hits_events(const trax_hits* h) { return h->n_events; }
 

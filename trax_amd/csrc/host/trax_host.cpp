@@ -183,6 +183,25 @@ PHit decodeHit(WireReader r) {
 
 }  // namespace
 
+namespace {
+
+// PEvent: 1 runNumber, 2 lumiSection, 3 eventNumber, 4 hits (PHit)
+PEventData decodeEvent(WireReader e) {
+    PEventData ev;
+    while (!e.done()) {
+        const uint64_t k = e.varint();
+        const uint32_t f = (uint32_t)(k >> 3), w = (uint32_t)(k & 7);
+        if (f == 4 && w == 2) ev.hits.push_back(decodeHit(e.sub()));
+        else if (f == 1 && w == 0) ev.runNumber = e.varint();
+        else if (f == 2 && w == 0) ev.lumiSection = e.varint();
+        else if (f == 3 && w == 0) ev.eventNumber = e.varint();
+        else e.skip(w);
+    }
+    return ev;
+}
+
+}  // namespace
+
 std::vector<PEventData> decodeEventContainer(const uint8_t* data, size_t size) {
     std::vector<PEventData> out;
     WireReader r{data, data + size};
@@ -192,18 +211,35 @@ std::vector<PEventData> decodeEventContainer(const uint8_t* data, size_t size) {
             r.skip((uint32_t)(key & 7));
             continue;
         }
-        WireReader e = r.sub();
-        PEventData ev;
-        while (!e.done()) {
-            const uint64_t k = e.varint();
-            const uint32_t f = (uint32_t)(k >> 3), w = (uint32_t)(k & 7);
-            if (f == 4 && w == 2) ev.hits.push_back(decodeHit(e.sub()));
-            else if (f == 1 && w == 0) ev.runNumber = e.varint();
-            else if (f == 2 && w == 0) ev.lumiSection = e.varint();
-            else if (f == 3 && w == 0) ev.eventNumber = e.varint();
-            else e.skip(w);
+        out.push_back(decodeEvent(r.sub()));
+    }
+    return out;
+}
+
+std::vector<PEventData> decodeEventStore(const uint8_t* data, size_t size, uint32_t skip, uint32_t max_events,
+                                         uint32_t* header_events) {
+    std::vector<PEventData> out;
+    if (header_events) *header_events = 0;
+    if (size < 4) return out;
+    const uint32_t n = (uint32_t)data[0] | (uint32_t)data[1] << 8 | (uint32_t)data[2] << 16 | (uint32_t)data[3] << 24;
+    if (header_events) *header_events = n;
+    WireReader r{data + 4, data + size};
+    for (uint32_t read = 0; read < n && out.size() < max_events; ++read) {
+        // readNextElement: the entry type (1 = an event), its size, its bytes;
+        // a missing or other type, a zero size or a short entry ends the store
+        uint64_t type, len;
+        try {
+            type = r.varint();
+            if (type != 1) break;
+            len = r.varint();
+        } catch (const InputError&) {
+            break;
         }
-        out.push_back(std::move(ev));
+        if (len == 0 || (uint64_t)(r.end - r.p) < len) break;
+        WireReader e{r.p, r.p + len};
+        r.p += len;
+        if (read < skip) continue;
+        out.push_back(decodeEvent(e));
     }
     return out;
 }

@@ -131,6 +131,17 @@ struct PEventData {
 // protocol-buffer wire format, without libprotobuf.  Throws InputError on
 // malformed input.
 std::vector<PEventData> decodeEventContainer(const uint8_t* data, size_t size);
+// Decodes an event store as GenericOutputStore writes it and GenericInputStore
+// reads it (src/datastructures/serialize/PEventStore.h:29-248): the event count
+// as a little-endian u32, then per event its entry type (a varint, 1 =
+// event), its size (a varint) and the serialized PB_Event::PEvent.  Reading
+// stops at the count, at an entry of another type, at a zero size or at an
+// entry the data cuts short (readNextElement's NULL); the first skip events
+// are read and dropped (EventStoreLoader, src/triplet_builder/EventLoader.h:
+// 171-206), at most max_events are returned.  *header_events receives the
+// count.  Throws InputError on a malformed event.
+std::vector<PEventData> decodeEventStore(const uint8_t* data, size_t size, uint32_t skip, uint32_t max_events,
+                                         uint32_t* header_events);
 
 // TripletConfigurations (src/datastructures/TripletConfiguration.h:39-72)
 class TripletConfigurations {
