@@ -225,6 +225,18 @@ def cpu_baseline(wl, events, cfg_cls, hitset_cls, budget_s=20.0):
                       f"oracle (C++ restatement of the reference kernels) on {cores} host threads, {t:.1f} s"}
 
 
+def write_runtime_csv(path, wl, hs, events, step_phase_ms):
+    """The timed steps as one row of the reference's runtime table (mean and
+    deviation over the steps), inputs resident so no transfer columns."""
+    from trax_amd import runtime_record as rr
+    recs = rr.RuntimeRecords()
+    shape = (len(events), wl.n_layers, len(wl.triplets), wl.n_threads, hs.n_hits, sum(e.n_tracks for e in events))
+    for ms in step_phase_ms:
+        recs.add_record(rr.RuntimeRecord(*shape).fill_phases(ms))
+    with open(path, "w") as f:
+        f.write(recs.csv_dump())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +258,8 @@ def main():
     ap.add_argument("--serve-max-batch", type=int, default=256)
     ap.add_argument("--serve-slot-hits", type=int, default=2_000_000,
                     help="hit capacity of each serving slot (its output capacities scale with it)")
+    ap.add_argument("--runtime-csv", type=str, default=None,
+                    help="also write the timed steps as the reference's runtime table (RuntimeRecords::csvDump)")
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="processes generating the serving stream")
     args = ap.parse_args()
@@ -300,14 +314,19 @@ def main():
               "filter_count", "filter_scan", "filter_store", "link_prepare", "link_count", "link_store",
               "circle_fit", "ca_forward", "ca_backward"]
     acc = dict.fromkeys(phases, 0.0)
+    step_phase_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         b.rerun()
         if b.wait() != 0:
             raise RuntimeError("capacity overflow")
-        for p in phases:
-            acc[p] += b.stage_ms(p)
+        step_phase_ms.append({p: b.stage_ms(p) for p in phases})
     t1 = time.perf_counter()
+    for step in step_phase_ms:
+        for p in phases:
+            acc[p] += step[p]
+    if args.runtime_csv:
+        write_runtime_csv(args.runtime_csv, wl, hs, events, step_phase_ms)
     ms_step = (t1 - t0) * 1e3 / args.steps
     counts = b.counts()
     ev_s = N_EVENTS / (ms_step / 1e3)

@@ -183,3 +183,28 @@ def test_batch_with_per_event_groups_equals_per_event_runs(dev):
             assert "ca.states" not in got
         n_conn += single["conn.basis"].size
     assert n_conn > 0
+
+
+def test_runtime_records_from_batch_phases(dev):
+    """Runtime records filled from a batch's phase events (every phase name
+    accepted by trax_batch_stage_ms) read back as the reference's plotting
+    scripts read them; the Kernel columns sum to the run's event span."""
+    import io
+
+    from trax_amd import runtime_record as rr
+    wl, cfg, hs, events = make_case("four-uneven-events")
+    b, _ = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits))
+    recs, totals = rr.RuntimeRecords(), []
+    shape = (hs.n_events, wl.n_layers, len(wl.triplets), wl.n_threads, hs.n_hits, sum(e.n_tracks for e in events))
+    for _ in range(3):
+        b.rerun()
+        assert b.wait() == 0
+        recs.add_record(rr.RuntimeRecord(*shape).fill_runtimes(b))
+        totals.append(b.stage_ms("total"))
+    b.close()
+    tab = np.genfromtxt(io.StringIO(recs.csv_dump()), delimiter=" ", names=True)
+    assert tab["n"] == 3 and tab["events"] == hs.n_events and tab["hits"] == hs.n_hits
+    assert tab["totalKernel"] == pytest.approx(np.mean(totals), rel=1e-3)
+    assert tab["totalWalltime"] == pytest.approx(tab["totalKernel"], rel=1e-5)
+    for s in ("buildGrid", "pairGen", "tripletPredict", "tripletConnectivityTight", "cellularAutomaton"):
+        assert tab[f"{s}Kernel"] > 0.0, s
