@@ -1110,7 +1110,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ uint16_t s_list[kPredBlock * kPredList];
     __shared__ uint32_t s_eex[kPredBlock + 1];  // listed candidates before each pair
     __shared__ uint32_t s_first[kPredBlock];    // first hit of the pair's third layer
-    __shared__ float4 s_pair[kPredBlock];       // x2, y2, x2 - x1, y2 - y1: the prefilter's pair terms
+    __shared__ float4 s_pair[kPredBlock];       // x2, y2, x1, y1: the pair's hits for the prefilter and cuts
     __shared__ float2 s_angles[kPredBlock];     // the cuts' pair angles (phi, theta)
     // blocks up to 256 pairs: the pair of each of the tile's first kOwnerCap
     // listed candidates (one byte each; later ones search the offsets)
@@ -1168,7 +1168,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             s_lt[tid] = lt;
             if (have) {
                 const float2 p1 = hxy[h1], p2 = hxy[h2];
-                s_pair[tid] = make_float4(p2.x, p2.y, p2.x - p1.x, p2.y - p1.y);
+                s_pair[tid] = make_float4(p2.x, p2.y, p1.x, p1.y);
             }
             if (have) {
                 const uint32_t cap = pw.lg3[g.cells - 1] - first <= 0xffffu ? (uint32_t)kPredList : 0u;
@@ -1282,11 +1282,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 uint32_t c;
                 const uint32_t lo = third_of(tag, c);
                 const uint32_t b = pairs[2 * (size_t)(tile * kPredBlock + lo) + 1];
-                const float2 p2 = hxy[b], p3 = hxy[c];
+                const float4 pr = s_pair[lo];
+                const float2 p3 = hxy[c];
                 const float2 ang = s_angles[lo];
                 const uint32_t l = s_lt[lo] & 0xffu;
-                if (filter_phi_cut(tc, l, p2.x, p2.y, p3.x, p3.y, ang.x) &&
-                    filter_theta_cut(tc, l, p2.x, p2.y, hz[b], p3.x, p3.y, hz[c], ang.y)) {
+                if (filter_phi_cut(tc, l, pr.x, pr.y, p3.x, p3.y, ang.x) &&
+                    filter_theta_cut(tc, l, pr.x, pr.y, hz[b], p3.x, p3.y, hz[c], ang.y)) {
                     if (tag & kValidBit) {
                         atomicAdd(&s_pool_valid[lo], 1u);
                         pool[tag & ~kValidBit].x |= kValidBit;
@@ -1304,9 +1305,9 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                     tag = q[k0 + lane];
                     uint32_t c;
                     const uint32_t lo = third_of(tag, c);
-                    const size_t pi = tile * kPredBlock + lo;
-                    const float2 p1 = hxy[pairs[2 * pi]], p2 = hxy[pairs[2 * pi + 1]], p3 = hxy[c];
-                    pass = filter_circle_cuts(tc, s_lt[lo] & 0xffu, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y);
+                    const float4 pr = s_pair[lo];  // the pair's hits, read once per tile from HBM
+                    const float2 p3 = hxy[c];
+                    pass = filter_circle_cuts(tc, s_lt[lo] & 0xffu, pr.z, pr.w, pr.x, pr.y, p3.x, p3.y);
                 }
                 const uint64_t m = __ballot(pass);
                 if (pass) q2[qn2 + (uint32_t)__popcll(m & lanemask_lt())] = tag;
@@ -1346,7 +1347,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                         const uint32_t lo = third_of(tag, c);
                         const float4 pr = s_pair[lo];
                         const float2 p3 = hxy[c];
-                        undecided = !tx_phi_surely_outside(pr.z, pr.w, p3.x - pr.x, p3.y - pr.y,
+                        undecided = !tx_phi_surely_outside(pr.x - pr.z, pr.y - pr.w, p3.x - pr.x, p3.y - pr.y,
                                                            tc.tan_dphi[s_lt[lo] & 0xffu]);
                     }
                 }
