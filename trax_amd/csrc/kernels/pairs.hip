@@ -201,6 +201,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
                              uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
+    uint2* __restrict__ pair2 = reinterpret_cast<uint2*>(pairs);  // (first hit, second hit), one 8-byte store
     uint32_t short_slots = 0;
     bool over = false;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < nslot; slot += gridDim.x * blockDim.x) {
@@ -235,17 +236,13 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
             for (int r = 0; r < 2 && pos < next; ++r)
                 zphi_scan(hzp, lo[r], hi[r], [&](uint32_t index, float2 q) {
                     if (in_window(w, q)) {
-                        pairs[2 * (size_t)pos] = index;
-                        pairs[2 * (size_t)pos + 1] = i;
+                        pair2[pos] = make_uint2(index, i);
                         pos++;
                     }
                     return pos < next;
                 });
         }
-        for (uint32_t q = pos; q < next; ++q) {
-            pairs[2 * (size_t)q] = kSentinel;
-            pairs[2 * (size_t)q + 1] = kSentinel;
-        }
+        for (uint32_t q = pos; q < next; ++q) pair2[q] = make_uint2(kSentinel, kSentinel);
         short_slots += next - pos;
     }
     if (short_slots) atomicAdd(&cnt->diag_pair_short, short_slots);
@@ -1451,6 +1448,7 @@ hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t 
                              const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
                              uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
+    if (reinterpret_cast<uintptr_t>(pairs) & 7u) return hipErrorInvalidValue;  // written as 8-byte pairs
     hipLaunchKernelGGL(k_pair_store, dim3(stride_blocks(n_slot_cap, b, lc)), dim3(b), 0, st, g, tc, grid, hx,
                        hy, hz, hzp, seg_base, slot_counts, pairs, n_pair_cap, cnt);
     hipError_t e = hipGetLastError();
