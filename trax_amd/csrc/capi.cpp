@@ -581,10 +581,11 @@ void trax_batch::enqueue_stage(int s) {
         case 2:  // pair_plan
             ok(launch_pair_plan(g, tc, nseg, grid, seg_base, cnt, st), "pair plan");
             ok(launch_scan(seg_base, &cnt->n_segments, 1u, nseg + 1ull, &cnt->n_pair_slots, scan, st), "scan");
+            ok(launch_slot_segments(seg_base, slot_cap, slot_seg, cnt, st), "slot segments");
             break;
         case 3:  // pair_count
             ok(launch_pair_count(g, tc, nseg, slot_cap, grid, hout.x, hout.y, hout.z, hout.zphi, seg_base, slot_counts,
-                                 cnt, lc, st),
+                                 cnt, lc, st, slot_seg),
                "pair count");
             break;
         case 4:  // pair_scan
@@ -592,7 +593,7 @@ void trax_batch::enqueue_stage(int s) {
             break;
         case 5:  // pair_store
             ok(launch_pair_store(g, tc, nseg, slot_cap, P, grid, hout.x, hout.y, hout.z, hout.zphi, seg_base,
-                                 slot_counts, pairs, pair_off, cnt, lc, st),
+                                 slot_counts, pairs, pair_off, cnt, lc, st, slot_seg),
                "pair store");
             break;
         case 6:  // predict: count, scan and store in one pass (with the filter when fused)
@@ -736,6 +737,7 @@ void trax_batch::layout(Arena& a) {
         B.grid_perm = a.take<uint32_t>(N);
         B.seg_base = a.take<uint32_t>(E * T + 1);
         B.slot_counts = a.take<uint32_t>(slot_cap + 1);
+        B.slot_seg = a.take<uint32_t>(slot_seg_entries(slot_cap));
         B.pairs = a.take<uint32_t>(2 * P);
         B.pair_off = a.take<uint32_t>(E * T + 1);
         B.pred_counts = a.take<uint32_t>(P + 1);

@@ -136,7 +136,8 @@ struct trax_batch {
     trax::DevCounts* h_counts = nullptr;  // pinned copy after a run
     // device buffers
     trax::HitColumns hin, hout;
-    uint32_t *sup = nullptr, *grid = nullptr, *seg_base = nullptr, *slot_counts = nullptr, *pairs = nullptr,
+    uint32_t *sup = nullptr, *grid = nullptr, *seg_base = nullptr, *slot_seg = nullptr, *slot_counts = nullptr,
+             *pairs = nullptr,
              *pair_off = nullptr, *pred_counts = nullptr, *cands = nullptr, *cand_off = nullptr,
              *word_counts = nullptr, *th1 = nullptr, *th2 = nullptr, *th3 = nullptr, *t_off = nullptr;
     uint64_t* bitmap = nullptr;

@@ -183,13 +183,15 @@ __device__ inline uint32_t tile_lookback(uint64_t* status, uint32_t tile, uint32
 }
 
 // upper_bound - 1 over a sorted u32 array: last index s with a[s] <= key
-__device__ inline uint32_t find_segment(const uint32_t* a, uint32_t n_seg, uint32_t key) {
-    uint32_t lo = 0, hi = n_seg;  // search in [0, n_seg)
-    while (hi - lo > 1) {
+__device__ inline uint32_t find_segment_in(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t key) {
+    while (hi - lo > 1) {  // search in [lo, hi), a[lo] <= key
         const uint32_t mid = (lo + hi) >> 1;
         if (a[mid] <= key) lo = mid; else hi = mid;
     }
     return lo;
+}
+__device__ inline uint32_t find_segment(const uint32_t* a, uint32_t n_seg, uint32_t key) {
+    return find_segment_in(a, 0, n_seg, key);
 }
 
 // grid geometry (GridConfig, src/datastructures/Grid.h:5-39)

@@ -154,17 +154,25 @@ hipError_t launch_grid_store(const GridGeom& g, uint32_t n_items_cap, const uint
                              DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
 
 // PairGeneratorBeamspot: plan -> scan(seg_base) -> count -> scan(slot_counts) -> store
+// segment of every 2^kSlotSegShift-th pair slot, so that a slot's segment is
+// searched among the few segments of its block of slots only
+constexpr uint32_t kSlotSegShift = 10;
+inline uint64_t slot_seg_entries(uint64_t n_slot_cap) { return (n_slot_cap >> kSlotSegShift) + 2; }
+hipError_t launch_slot_segments(const uint32_t* seg_base, uint64_t n_slot_cap, uint32_t* slot_seg,
+                                const DevCounts* cnt, hipStream_t st);
 hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                             const uint32_t* grid, uint32_t* seg_base, DevCounts* cnt, hipStream_t st);
 hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
                              const float* hz, const float2* hzp, const uint32_t* seg_base, uint32_t* slot_counts,
-                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             const uint32_t* slot_seg = nullptr);  // null: search every segment
 hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
                              const float* hx, const float* hy, const float* hz, const float2* hzp,
                              const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
-                             uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+                             uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             const uint32_t* slot_seg = nullptr);
 
 // TripletThetaPhiPredictor: count -> scan(counts) -> store (+ offsets, forward fill)
 hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,

@@ -115,10 +115,19 @@ struct SlotInfo {
     uint32_t seg, e, p, t, k;
 };
 
+// slot_seg (may be null) bounds the search: a slot's segment lies between the
+// segments of the first slots of its block and of the next block
 __device__ inline SlotInfo slot_info(const TripletCols& tc, const uint32_t* seg_base, uint32_t nseg,
-                                     uint32_t slot) {
+                                     uint32_t slot, const uint32_t* __restrict__ slot_seg) {
     SlotInfo si;
-    si.seg = find_segment(seg_base, nseg, slot);
+    uint32_t lo = 0, hi = nseg;
+    if (slot_seg) {
+        const uint32_t b = slot >> kSlotSegShift;
+        lo = slot_seg[b];
+        const uint32_t h = slot_seg[b + 1] + 1u;
+        hi = h < nseg ? h : nseg;
+    }
+    si.seg = find_segment_in(seg_base, lo, hi, slot);
     const uint32_t o = slot - seg_base[si.seg];
     const uint32_t span = seg_base[si.seg + 1] - seg_base[si.seg];
     const uint32_t K = span / tc.n_threads;
@@ -155,13 +164,13 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
                              const float* __restrict__ hx, const float* __restrict__ hy,
                              const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, uint32_t* __restrict__ slot_counts,
-                             DevCounts* __restrict__ cnt) {
+                             DevCounts* __restrict__ cnt, const uint32_t* __restrict__ slot_seg) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t overlap = 0;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot <= nslot; slot += gridDim.x * blockDim.x) {
         if (slot == nslot) { slot_counts[slot] = 0; continue; }
-        const SlotInfo si = slot_info(tc, seg_base, nseg, slot);
+        const SlotInfo si = slot_info(tc, seg_base, nseg, slot, slot_seg);
         const uint32_t lay2 = tc.l2[si.p] - 1, lay1 = tc.l1[si.p] - 1;
         const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
         const uint32_t layer2Offset = g2[0], end = g2[g.cells - 1];
@@ -198,7 +207,8 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
                              const float* __restrict__ hx, const float* __restrict__ hy,
                              const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
-                             uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt) {
+                             uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt,
+                             const uint32_t* __restrict__ slot_seg) {
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint2* __restrict__ pair2 = reinterpret_cast<uint2*>(pairs);  // (first hit, second hit), one 8-byte store
@@ -209,7 +219,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
         const uint32_t next = slot_counts[slot + 1];
         if (pos == next) continue;
         if (next > n_pair_cap) { over = true; continue; }
-        const SlotInfo si = slot_info(tc, seg_base, nseg, slot);
+        const SlotInfo si = slot_info(tc, seg_base, nseg, slot, slot_seg);
         const uint32_t lay2 = tc.l2[si.p] - 1, lay1 = tc.l1[si.p] - 1;
         const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
         const uint32_t i = g2[0] + si.k * tc.n_threads + si.t;
@@ -247,6 +257,15 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
     }
     if (short_slots) atomicAdd(&cnt->diag_pair_short, short_slots);
     if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
+// slot_seg[b] = the segment of slot b << kSlotSegShift, for every block of
+// slots up to the capacity (blocks past the last slot get the last segment)
+__global__ void k_slot_segments(const uint32_t* __restrict__ seg_base, uint32_t n_entries,
+                                uint32_t* __restrict__ slot_seg, const DevCounts* __restrict__ cnt) {
+    const uint32_t nseg = cnt->n_segments;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n_entries; b += gridDim.x * blockDim.x)
+        slot_seg[b] = find_segment(seg_base, nseg, b << kSlotSegShift);
 }
 
 __global__ void k_pair_offsets(const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
@@ -1431,14 +1450,23 @@ hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n
     return hipGetLastError();
 }
 
+hipError_t launch_slot_segments(const uint32_t* seg_base, uint64_t n_slot_cap, uint32_t* slot_seg,
+                                const DevCounts* cnt, hipStream_t st) {
+    const uint64_t n = slot_seg_entries(n_slot_cap);
+    if (n >= 0xffffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_slot_segments, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, seg_base, (uint32_t)n,
+                       slot_seg, cnt);
+    return hipGetLastError();
+}
+
 hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
                              uint32_t n_slot_cap, const uint32_t* grid, const float* hx, const float* hy,
                              const float* hz, const float2* hzp, const uint32_t* seg_base, uint32_t* slot_counts,
-                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+                             DevCounts* cnt, const LaunchCfg& lc, hipStream_t st, const uint32_t* slot_seg) {
     (void)n_seg_cap;
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     hipLaunchKernelGGL(k_pair_count, dim3(stride_blocks((uint64_t)n_slot_cap + 1, b, lc)), dim3(b), 0, st, g,
-                       tc, grid, hx, hy, hz, hzp, seg_base, slot_counts, cnt);
+                       tc, grid, hx, hy, hz, hzp, seg_base, slot_counts, cnt, slot_seg);
     return hipGetLastError();
 }
 
@@ -1446,11 +1474,12 @@ hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t 
                              uint32_t n_slot_cap, uint32_t n_pair_cap, const uint32_t* grid,
                              const float* hx, const float* hy, const float* hz, const float2* hzp,
                              const uint32_t* seg_base, const uint32_t* slot_counts, uint32_t* pairs,
-                             uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+                             uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             const uint32_t* slot_seg) {
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     if (reinterpret_cast<uintptr_t>(pairs) & 7u) return hipErrorInvalidValue;  // written as 8-byte pairs
     hipLaunchKernelGGL(k_pair_store, dim3(stride_blocks(n_slot_cap, b, lc)), dim3(b), 0, st, g, tc, grid, hx,
-                       hy, hz, hzp, seg_base, slot_counts, pairs, n_pair_cap, cnt);
+                       hy, hz, hzp, seg_base, slot_counts, pairs, n_pair_cap, cnt, slot_seg);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pair_offsets, dim3((n_seg_cap + 1 + 255) / 256), dim3(256), 0, st, seg_base,
