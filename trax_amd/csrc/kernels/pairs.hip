@@ -1127,7 +1127,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ uint32_t s_eex[kPredBlock + 1];  // listed candidates before each pair
     __shared__ uint32_t s_first[kPredBlock];    // first hit of the pair's third layer
     __shared__ float4 s_pair[kPredBlock];       // x2, y2, x1, y1: the pair's hits for the prefilter and cuts
-    __shared__ float2 s_angles[kPredBlock];     // the cuts' pair angles (phi, theta)
+    __shared__ float2 s_z12[kPredBlock];        // z1, z2: with s_pair, the pair's hits for the angle cuts
     // blocks up to 256 pairs: the pair of each of the tile's first kOwnerCap
     // listed candidates (one byte each; later ones search the offsets)
     constexpr bool kOwner = kPredBlock <= 256;
@@ -1185,6 +1185,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
             if (have) {
                 const float2 p1 = hxy[h1], p2 = hxy[h2];
                 s_pair[tid] = make_float4(p2.x, p2.y, p1.x, p1.y);
+                s_z12[tid] = make_float2(hz[h1], hz[h2]);
             }
             if (have) {
                 const uint32_t cap = pw.lg3[g.cells - 1] - first <= 0xffffu ? (uint32_t)kPredList : 0u;
@@ -1208,13 +1209,6 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         const uint32_t excl_e = block_excl_scan(listed, s_wsum, &total_e);
         if (kOwner)
             for (uint32_t i = excl_e; i < excl_e + listed && i < kOwnerCap; ++i) s_owner[i] = (uint8_t)tid;
-        if (nB) {
-            const size_t pi = tile * kPredBlock + tid;
-            const uint32_t a = pairs[2 * pi], b = pairs[2 * pi + 1];
-            const float2 p1 = hxy[a], p2 = hxy[b];
-            s_angles[tid] = make_float2(filter_pair_phi(p1.x, p1.y, p2.x, p2.y),
-                                        filter_pair_theta(p1.x, p1.y, hz[a], p2.x, p2.y, hz[b]));
-        }
         s_eex[tid] = excl_e;
         s_lt[tid] |= (nB - listed) << 8;
         s_valid[tid] = 0;
@@ -1297,13 +1291,15 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 const uint32_t tag = q2[k0 + lane];
                 uint32_t c;
                 const uint32_t lo = third_of(tag, c);
-                const uint32_t b = pairs[2 * (size_t)(tile * kPredBlock + lo) + 1];
-                const float4 pr = s_pair[lo];
+                const float4 pr = s_pair[lo];  // x2, y2, x1, y1
+                const float2 z12 = s_z12[lo];
                 const float2 p3 = hxy[c];
-                const float2 ang = s_angles[lo];
                 const uint32_t l = s_lt[lo] & 0xffu;
-                if (filter_phi_cut(tc, l, pr.x, pr.y, p3.x, p3.y, ang.x) &&
-                    filter_theta_cut(tc, l, pr.x, pr.y, hz[b], p3.x, p3.y, hz[c], ang.y)) {
+                // the pair's angles, for the few candidates that get this far
+                const float phi12 = filter_pair_phi(pr.z, pr.w, pr.x, pr.y);
+                const float theta12 = filter_pair_theta(pr.z, pr.w, z12.x, pr.x, pr.y, z12.y);
+                if (filter_phi_cut(tc, l, pr.x, pr.y, p3.x, p3.y, phi12) &&
+                    filter_theta_cut(tc, l, pr.x, pr.y, z12.y, p3.x, p3.y, hz[c], theta12)) {
                     if (tag & kValidBit) {
                         atomicAdd(&s_pool_valid[lo], 1u);
                         pool[tag & ~kValidBit].x |= kValidBit;
