@@ -237,6 +237,37 @@ def write_runtime_csv(path, wl, hs, events, step_phase_ms):
         f.write(recs.csv_dump())
 
 
+def runtime_sweep(dev, path, reps=19, max_events=2048):
+    """The shape of the reference's published event sweep
+    (plotting/data/runtime/runtime.bigOne.local.conf.events.100.test.gpu.csv:
+    1, 2, 4, ... 2048 events of 100 tracks, one layer triplet, 19 runs each),
+    here on the one-event-100-tracks geometry (3 layers, 8 z x 32 phi grid),
+    batched with the filter fused: one row per event count in the reference's
+    65-column layout (grid, pair, predict and filter stages)."""
+    from trax_amd import make_events, workload
+    from trax_amd import runtime_record as rr
+    from trax_amd.api import Caps, Config, HitSet, run_batch_fit
+    wl = workload(1)
+    recs = rr.RuntimeRecords()
+    n = 1
+    while n <= max_events:
+        events = make_events(wl, n, seed=SEED + 3)
+        cfg = Config(wl)
+        hs = HitSet(cfg, events)
+        b, _ = run_batch_fit(dev, cfg, hs, Caps.for_hits(n, hs.n_hits, per_event=True, fuse_filter=True))
+        shape = (n, wl.n_layers, len(wl.triplets), wl.n_threads, hs.n_hits, sum(e.n_tracks for e in events))
+        for _ in range(reps):
+            b.rerun()
+            if b.wait() != 0:
+                raise RuntimeError("capacity overflow in the event sweep")
+            recs.add_record(rr.RuntimeRecord(*shape).fill_runtimes(b))
+        b.close()
+        print(f"event sweep: {n} events", file=sys.stderr, flush=True)
+        n *= 2
+    with open(path, "w") as f:
+        f.write(recs.csv_dump("four_stage"))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +291,8 @@ def main():
                     help="hit capacity of each serving slot (its output capacities scale with it)")
     ap.add_argument("--runtime-csv", type=str, default=None,
                     help="also write the timed steps as the reference's runtime table (RuntimeRecords::csvDump)")
+    ap.add_argument("--runtime-sweep", type=str, default=None,
+                    help="also run the reference's published event sweep shape and write its runtime table")
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="processes generating the serving stream")
     args = ap.parse_args()
@@ -442,6 +475,8 @@ def main():
     b.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, events, Config, HitSet)
+    if args.runtime_sweep:
+        runtime_sweep(dev, args.runtime_sweep)
     print(json.dumps(line))
 
 
