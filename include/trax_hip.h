@@ -164,6 +164,14 @@ int trax_config_set_geometry(trax_config* c, uint32_t n, const uint32_t* layer, 
 /* work-group size of the reference (fixes the pair order) and dEtaCut */
 int trax_config_set_exec(trax_config* c, uint32_t n_threads, float d_eta_cut);
 float trax_config_min_radius_curvature(const trax_config* c);
+/* Checks the preconditions every path relies on before it touches a device:
+ * at least one and at most 255 layer triplets, layers numbered from 1, the
+ * first two layers of each triplet not above the largest third layer, and a
+ * geometry row for every layer the triplets use.  The reference assumes them
+ * without checking (the triplet index is hitLayer - 1,
+ * src/algorithms/TripletThetaPhiPredictor.h:88; radii are indexed by layer - 1,
+ * GeometrySupplement.h:21-81).  Every run entry point applies the same check. */
+int trax_config_validate(const trax_config* c);
 
 /* The plain per-event-group path: buildTriplets' call sequence over the host
  * algorithm classes, one blocking count download per stage, exact-size

@@ -155,3 +155,85 @@ def test_filter_phi_prefilter_never_rejects_an_accepted_candidate():
         undecided = (~np.all(np.isfinite(g), 0) | (np.maximum(np.abs(g[0]), np.abs(g[1])) < 1e-30) |
                      (np.maximum(np.abs(g[2]), np.abs(g[3])) < 1e-30))
     assert not np.any(so & undecided)
+
+
+def test_config_validate_rejects_what_the_kernels_cannot_take():
+    """Layer 0 would index grid rows and radii at layer - 1 = 0xffffffff; more
+    than 255 layer triplets do not fit the fused predictor's 8-bit triplet
+    field; a missing geometry row leaves a radius unread."""
+    import ctypes as C
+    from trax_amd.api import P, TraxError, check, lib
+    from trax_amd.config import geometry_arrays
+    L = lib()
+    layer, rmin, rmax = geometry_arrays()
+
+    def make(triplets, geom_rows=None):
+        h = P()
+        check(L.trax_config_create(32, 128, 1.0, C.byref(h)))
+        for t in triplets:
+            rc = L.trax_config_add_triplet(h, *t, 0.1, 1.0, 0.1, 0.0, 0.1, 2, 10.0, 2, 100)
+            if rc != 0:
+                L.trax_config_destroy(h)
+                return None
+        n = layer.size if geom_rows is None else geom_rows
+        l_, a_, b_ = (np.ascontiguousarray(v[:n]) for v in (layer, rmin, rmax))
+        check(L.trax_config_set_geometry(h, n, l_.ctypes.data, a_.ctypes.data, b_.ctypes.data))
+        return h
+
+    good = make([(1, 2, 3), (2, 3, 4)])
+    assert L.trax_config_validate(good) == 0
+    L.trax_config_destroy(good)
+    assert make([(0, 1, 2)]) is None  # layers are numbered from 1
+    many = make([(1, 2, 3)] * 256)
+    assert L.trax_config_validate(many) != 0 and b"255" in L.trax_last_error()
+    L.trax_config_destroy(many)
+    at_limit = make([(1, 2, 3)] * 255)
+    assert L.trax_config_validate(at_limit) == 0
+    L.trax_config_destroy(at_limit)
+    short = make([(1, 2, 3), (2, 3, 4)], geom_rows=3)
+    assert L.trax_config_validate(short) != 0
+    L.trax_config_destroy(short)
+    with pytest.raises(TraxError):
+        check(L.trax_config_validate(None))
+
+
+def _replay_scratch_allocator(chunk_totals_per_wave, chunk_range=256):
+    """The wave-held scratch ranges of the fused predictor + filter and of the
+    single-pass connectivity kernel: a wave whose chunk needs more than its
+    range holds drops the rest and takes max(need, chunk_range) entries with
+    one atomic.  Returns (entries handed out, entries used)."""
+    handed = used = 0
+    for totals in chunk_totals_per_wave:
+        left = 0
+        for t in totals:
+            if t > left:
+                size = max(t, chunk_range)
+                handed += size
+                left = size
+            left -= t
+            used += t
+    return handed, used
+
+
+def test_scratch_allocator_stays_within_twice_the_entries_plus_one_range_per_wave():
+    """The scratch is sized 2 * capacity + one range per resident wave: every
+    refill drops less than the refilling chunk's own entries, so the dropped
+    parts add up to less than all entries.  Replayed on adversarial sequences
+    (each chunk needs one more entry than its wave has left) and random ones."""
+    rng = np.random.default_rng(7)
+    cases = []
+    # adversarial: 1 entry, then a chunk of just over what is left, repeatedly
+    adv = []
+    for _ in range(500):
+        adv += [1, 256]
+    cases.append([adv] * 16)
+    cases.append([[257, 1] * 300] * 8)
+    cases.append([list(rng.integers(0, 300, 400)) for _ in range(64)])
+    cases.append([list(rng.integers(0, 40, 2000)) for _ in range(64)])
+    cases.append([list(rng.geometric(0.01, 300)) for _ in range(32)])
+    for waves in cases:
+        handed, used = _replay_scratch_allocator(waves)
+        assert handed <= 2 * used + 256 * len(waves), (handed, used, len(waves))
+    # the old bound (capacity + one range per wave) is exceeded by the adversarial case
+    handed, used = _replay_scratch_allocator(cases[0])
+    assert handed > used + 256 * len(cases[0])

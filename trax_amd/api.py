@@ -116,6 +116,7 @@ def lib():
         "trax_config_set_geometry": (C.c_int, [P, U32, P, P, P]),
         "trax_config_set_exec": (C.c_int, [P, U32, C.c_float]),
         "trax_config_min_radius_curvature": (C.c_float, [P]),
+        "trax_config_validate": (C.c_int, [P]),
         "trax_run_plain": (C.c_int, [P, P, P, C.c_int, C.POINTER(P)]),
         "trax_autotune": (C.c_int, [P, P, P, C.POINTER(BatchCaps), U32, C.POINTER(Tuning), C.c_char_p, C.c_uint64]),
         "trax_batch_plan": (C.c_int, [P, C.POINTER(BatchCaps), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),

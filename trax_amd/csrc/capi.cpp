@@ -361,12 +361,25 @@ extern "C" int trax_config_set_exec(trax_config* c, uint32_t n_threads, float d_
 
 extern "C" float trax_config_min_radius_curvature(const trax_config* c) { return c->trip.minRadiusCurvature(); }
 
+extern "C" int trax_config_validate(const trax_config* c) {
+    if (!c) return fail(TRAX_ERR_ARG, "null configuration");
+    return guarded([&] {
+        validate_config(c);
+        return TRAX_OK;
+    });
+}
+
 void trax_capi::validate_config(const trax_config* cfg) {
     const uint32_t L = cfg->trip.maxLayer();
     if (cfg->trip.size() == 0) throw InputError("no layer triplet configured");
+    // the fused predictor keeps a pair's layer triplet in 8 bits
+    if (cfg->trip.size() > 255) throw InputError("more than 255 layer triplets");
     if (cfg->geom.size() < L) throw InputError("geometry supplement has fewer layers than the triplets use");
-    for (uint32_t i = 0; i < cfg->trip.size(); ++i)
+    for (uint32_t i = 0; i < cfg->trip.size(); ++i) {
+        if (cfg->trip.layer1[i] == 0 || cfg->trip.layer2[i] == 0 || cfg->trip.layer3[i] == 0)
+            throw InputError("layers are numbered from 1");
         if (cfg->trip.layer1[i] > L || cfg->trip.layer2[i] > L) throw InputError("bad layer triplet");
+    }
 }
 
 // ---- plain per-event-group path ---------------------------------------------------
@@ -539,7 +552,7 @@ int trax_batch::finish() {
     hip_check(hipMemcpyAsync(flags, cb.flags, sizeof(flags), hipMemcpyDeviceToHost, st), "d2h");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     uint32_t k = ca_iter;
-    if (h_counts->n_conn && !(h_counts->err & kErrCapacity) && flags[(k - 1) % 3]) {
+    if (h_counts->n_conn && !(h_counts->err & kErrVoid) && flags[(k - 1) % 3]) {
         while (true) {
             hip_check(launch_ca_iterations(caps.max_conn, lb.basis, lb.foll, cb, cnt, k, 1, lc, st), "ca iteration");
             hip_check(hipMemcpyAsync(flags, cb.flags, sizeof(flags), hipMemcpyDeviceToHost, st), "d2h");
@@ -555,6 +568,7 @@ int trax_batch::finish() {
     const uint32_t err = h_counts->err;
     if (err & kErrCapacity) return fail(TRAX_ERR_CAPACITY, "an output exceeded its capacity");
     if (err & kErrCellRange) return fail(TRAX_ERR_INPUT, "a hit lies outside the z-phi grid");
+    if (err & kErrIndexRange) return fail(TRAX_ERR_INPUT, "an index read from device memory is out of range");
     if (err) return fail(TRAX_ERR_INPUT, "device reported an input error");
     return TRAX_OK;
 }

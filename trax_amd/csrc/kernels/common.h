@@ -62,6 +62,39 @@ __device__ inline uint32_t bounded(uint32_t v, uint32_t cap, DevCounts* cnt) {
     return v;
 }
 
+// Error bits that void a run: an output cut at its capacity, a hit outside
+// the grid or a bad index leaves entries that later stages must not read as
+// addresses.  Every stage kernel that reads an earlier stage's output tests
+// this at entry and does no work once a kernel before it has set one of them
+// (kernels on one stream run in order, so the bits an earlier kernel set are
+// visible).  The host reports the error and discards the run.
+constexpr uint32_t kErrVoid = kErrCellRange | kErrCapacity | kErrIndexRange;
+__device__ inline uint32_t load_err(const DevCounts* cnt) {
+    return __hip_atomic_load(&cnt->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wave-uniform; for kernels without block barriers
+__device__ inline bool run_void(const DevCounts* cnt) {
+    return (__builtin_amdgcn_readfirstlane(load_err(cnt)) & kErrVoid) != 0u;
+}
+// block-uniform (one read by thread 0, shared through LDS); contains __syncthreads()
+__device__ inline bool block_run_void(const DevCounts* cnt) {
+    __shared__ uint32_t s_void;
+    if (threadIdx.x == 0) s_void = load_err(cnt) & kErrVoid;
+    __syncthreads();
+    const bool v = s_void != 0u;
+    __syncthreads();
+    return v;
+}
+
+// An index read from device memory, checked against the size of the array it
+// addresses before it is used as an address: false (and kErrIndexRange) when
+// it is out of range, and the work item is skipped.
+__device__ inline bool index_ok(uint32_t i, uint32_t n, DevCounts* cnt) {
+    if (i < n) return true;
+    atomicOr(&cnt->err, kErrIndexRange);
+    return false;
+}
+
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
 // makes a wave's LDS writes visible to its other lanes (for waves that run

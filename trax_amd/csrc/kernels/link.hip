@@ -25,18 +25,30 @@ namespace {
 __device__ inline uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline uint32_t gstride() { return gridDim.x * blockDim.x; }
 
-// A candidate's hits (false for sentinel slots, which yield nothing)
+// A candidate's hits (false for sentinel slots, which yield nothing, and for
+// a pair or hit index out of range, which is reported)
 __device__ inline bool cand_hits(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* pairs,
-                                 const uint32_t* cands, uint32_t gid, uint32_t& h1, uint32_t& h2, uint32_t& h3,
-                                 uint32_t& lt) {
+                                 const uint32_t* cands, uint32_t gid, uint32_t n_pairs, uint32_t nh, DevCounts* cnt,
+                                 uint32_t& h1, uint32_t& h2, uint32_t& h3, uint32_t& lt) {
     const uint32_t pi = cands[2 * (size_t)gid];
     h3 = cands[2 * (size_t)gid + 1];
     if (pi == kSentinel || h3 == kSentinel) return false;
+    if (!index_ok(pi, n_pairs, cnt) || !index_ok(h3, nh, cnt)) return false;
     h1 = pairs[2 * (size_t)pi];
     h2 = pairs[2 * (size_t)pi + 1];
     if (h1 == kSentinel || h2 == kSentinel) return false;
+    if (!index_ok(h1, nh, cnt) || !index_ok(h2, nh, cnt)) return false;
     lt = hlayer[h1] - 1;
     return lt < tc.n_trip;
+}
+
+// the three hits of tracklet t, checked against the hit count
+__device__ inline bool trk_hits_ok(const uint32_t* th1, const uint32_t* th2, const uint32_t* th3, uint32_t t,
+                                   uint32_t nh, DevCounts* cnt, uint32_t& a, uint32_t& m, uint32_t& b) {
+    a = th1[t];
+    m = th2[t];
+    b = th3[t];
+    return index_ok(a, nh, cnt) && index_ok(m, nh, cnt) && index_ok(b, nh, cnt);
 }
 
 // filterCount (TripletThetaPhiFilter.h:50-162) with the validity bitmap built
@@ -55,7 +67,9 @@ __global__ void k_filter_count(TripletCols tc, const float* __restrict__ hx, con
                                uint64_t* __restrict__ bitmap, uint32_t* __restrict__ word_counts,
                                DevCounts* __restrict__ cnt) {
     extern __shared__ uint32_t lds[];
+    if (run_void(cnt)) return;
     const uint32_t n = bounded(cnt->n_cands, cnt->cap_cands, cnt);
+    const uint32_t n_pairs = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), nh = cnt->n_hits;
     const uint32_t nwords = (n + 63) / 64;
     if (gtid() == 0) {
         cnt->n_filter_words = nwords;
@@ -72,7 +86,7 @@ __global__ void k_filter_count(TripletCols tc, const float* __restrict__ hx, con
             const uint32_t gid = (w0 + w) * 64 + lane;
             bool undecided = false;
             uint32_t h1, h2, h3, lt;
-            if (gid < n && cand_hits(tc, hlayer, pairs, cands, gid, h1, h2, h3, lt))
+            if (gid < n && cand_hits(tc, hlayer, pairs, cands, gid, n_pairs, nh, cnt, h1, h2, h3, lt))
                 undecided = !tx_phi_surely_outside(hx[h2] - hx[h1], hy[h2] - hy[h1], hx[h3] - hx[h2],
                                                    hy[h3] - hy[h2], tc.tan_dphi[lt]);
             const uint64_t m = __ballot(undecided);
@@ -85,7 +99,7 @@ __global__ void k_filter_count(TripletCols tc, const float* __restrict__ hx, con
             if (k + lane < qn) {
                 const uint32_t gid = queue[k + lane];
                 uint32_t h1, h2, h3, lt;
-                if (cand_hits(tc, hlayer, pairs, cands, gid, h1, h2, h3, lt) &&
+                if (cand_hits(tc, hlayer, pairs, cands, gid, n_pairs, nh, cnt, h1, h2, h3, lt) &&
                     filter_valid(tc, hx, hy, hz, h1, h2, h3, lt))
                     atomicOr(reinterpret_cast<unsigned long long*>(&masks[gid / 64 - w0]), 1ull << (gid & 63));
             }
@@ -104,7 +118,9 @@ __global__ void k_filter_store(const uint32_t* __restrict__ pairs, const uint32_
                                const uint64_t* __restrict__ bitmap, const uint32_t* __restrict__ word_counts,
                                uint32_t* __restrict__ th1, uint32_t* __restrict__ th2, uint32_t* __restrict__ th3,
                                uint32_t n_trk_cap, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t nwords = cnt->n_filter_words;
+    const uint32_t n_pairs = bounded(cnt->n_pairs, cnt->cap_pairs, cnt);
     const uint32_t lane = lane_id();
     bool over = false;
     for (uint32_t word = gtid() / kWave; word < nwords; word += gstride() / kWave) {
@@ -113,6 +129,7 @@ __global__ void k_filter_store(const uint32_t* __restrict__ pairs, const uint32_
             const uint32_t pos = word_counts[word] + (uint32_t)__popcll(m & lanemask_lt());
             const uint32_t gid = word * 64 + lane;
             const uint32_t pi = cands[2 * (size_t)gid];
+            if (!index_ok(pi, n_pairs, cnt)) continue;
             if (pos < n_trk_cap) {
                 th1[pos] = pairs[2 * (size_t)pi];
                 th2[pos] = pairs[2 * (size_t)pi + 1];
@@ -130,15 +147,18 @@ __global__ void k_filter_store(const uint32_t* __restrict__ pairs, const uint32_
 __global__ void k_filter_offsets(TripletCols tc, const uint32_t* __restrict__ hlayer,
                                  const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ th1,
                                  uint32_t* __restrict__ t_off, DevCounts* __restrict__ cnt) {
-    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     const uint64_t n_off = (uint64_t)cnt->n_events * tc.n_trip + 1;
     for (uint32_t gid = gtid(); gid < R; gid += gstride()) {
         const uint32_t hitId = th1[gid];
+        if (!index_ok(hitId, nh, cnt)) continue;
         const uint32_t layerTriplet = hlayer[hitId];
         const uint32_t event = hevt[hitId];
         const uint64_t base = (uint64_t)event * tc.n_trip + layerTriplet;
         if (gid > 0) {
             const uint32_t prev = th1[gid - 1];
+            if (!index_ok(prev, nh, cnt)) continue;
             if (layerTriplet != hlayer[prev] || event != hevt[prev])
                 if (base - 1 < n_off) t_off[base - 1] = gid;
             if (gid + 1 == R && base < n_off) t_off[base] = gid;
@@ -153,9 +173,11 @@ __global__ void k_eta(const float* __restrict__ hx, const float* __restrict__ hy
                       const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th3,
                       float* __restrict__ eta, uint32_t* __restrict__ hit_count,
                       DevCounts* __restrict__ cnt) {
-    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     for (uint32_t t = gtid(); t < R; t += gstride()) {  // tripletEtaCalculatorStore (.h:60-75)
         const uint32_t a = th1[t], b = th3[t];
+        if (!(index_ok(a, nh, cnt) && index_ok(b, nh, cnt))) continue;
         const float px = hx[b] - hx[a], py = hy[b] - hy[a], pz = hz[b] - hz[a];
         const float tt = pz / sqrtf(px * px + py * py);
         eta[t] = tx_asinhf(tt);
@@ -171,9 +193,11 @@ __global__ void k_eta_runs(const float* __restrict__ hx, const float* __restrict
                            const uint32_t* __restrict__ th3, float* __restrict__ eta,
                            uint32_t* __restrict__ run_start, uint32_t* __restrict__ run_end,
                            DevCounts* __restrict__ cnt) {
-    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     for (uint32_t t = gtid(); t < R; t += gstride()) {  // tripletEtaCalculatorStore (.h:60-75)
-        const uint32_t a = th1[t], b = th3[t], m = th2[t];
+        uint32_t a, m, b;
+        if (!trk_hits_ok(th1, th2, th3, t, nh, cnt, a, m, b)) continue;
         const float px = hx[b] - hx[a], py = hy[b] - hy[a], pz = hz[b] - hz[a];
         const float tt = pz / sqrtf(px * px + py * py);
         eta[t] = tx_asinhf(tt);
@@ -185,10 +209,13 @@ __global__ void k_eta_runs(const float* __restrict__ hx, const float* __restrict
 __global__ void k_csr_fill(const uint32_t* __restrict__ th1, const uint32_t* __restrict__ hit_start,
                            uint32_t* __restrict__ hit_cursor, uint32_t* __restrict__ csr,
                            DevCounts* __restrict__ cnt) {
-    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     for (uint32_t t = gtid(); t < R; t += gstride()) {
         const uint32_t h = th1[t];
-        csr[hit_start[h] + atomicAdd(&hit_cursor[h], 1u)] = t;
+        if (!index_ok(h, nh, cnt)) continue;
+        const uint32_t q = hit_start[h] + atomicAdd(&hit_cursor[h], 1u);
+        if (index_ok(q, R, cnt)) csr[q] = t;
     }
 }
 
@@ -201,9 +228,12 @@ struct LinkRange {
 // event's T + 1 offsets as a run of that event alone has them.
 __device__ inline bool link_range(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
                                   const uint32_t* th1, const uint32_t* th2, const uint32_t* t_off,
-                                  uint32_t n_off, uint32_t R, uint32_t t, const LinkBufs& b, LinkRange& r) {
-    const uint32_t inner = th2[t];
+                                  uint32_t n_off, uint32_t R, uint32_t t, const LinkBufs& b, LinkRange& r,
+                                  DevCounts* cnt) {
+    const uint32_t inner = th2[t], nh = cnt->n_hits;
+    if (!index_ok(inner, nh, cnt)) return false;
     r.ev = hevt[inner];
+    if (!index_ok(r.ev, cnt->n_events, cnt)) return false;
     const uint32_t* off = t_off;
     uint64_t ib = (uint64_t)r.ev * tc.n_trip + (hlayer[inner] - 1);
     if (b.per_event) {
@@ -217,17 +247,29 @@ __device__ inline bool link_range(const TripletCols& tc, const uint32_t* hlayer,
     r.begin = off[ib];
     r.end = off[ib + 1];
     if (r.begin >= R) return false;
-    return r.ev == hevt[th1[r.begin]];
+    if (!index_ok(r.end, R + 1, cnt)) return false;  // offsets are tracklet positions, at most R
+    const uint32_t h = th1[r.begin];
+    if (!index_ok(h, nh, cnt)) return false;
+    return r.ev == hevt[h];
 }
 
 // first and last tracklet of each event (tracklets are ordered by event)
 __global__ void k_event_bounds(const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ th1, LinkBufs b,
                                DevCounts* __restrict__ cnt) {
-    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits, E = cnt->n_events;
+    // the event of tracklet t (kSentinel for a bad index, which is reported)
+    auto event_of = [&](uint32_t t) {
+        const uint32_t h = th1[t];
+        if (!index_ok(h, nh, cnt)) return kSentinel;
+        const uint32_t e = hevt[h];
+        return index_ok(e, E, cnt) ? e : kSentinel;
+    };
     for (uint32_t t = gtid(); t < R; t += gstride()) {
-        const uint32_t e = hevt[th1[t]];
-        if (t == 0 || hevt[th1[t - 1]] != e) b.ev_first[e] = t;
-        if (t + 1 == R || hevt[th1[t + 1]] != e) b.ev_last[e] = t;
+        const uint32_t e = event_of(t);
+        if (e == kSentinel) continue;
+        if (t == 0 || event_of(t - 1) != e) b.ev_first[e] = t;
+        if (t + 1 == R || event_of(t + 1) != e) b.ev_last[e] = t;
     }
 }
 
@@ -240,7 +282,9 @@ __global__ void k_event_bounds(const uint32_t* __restrict__ hevt, const uint32_t
 __global__ void k_event_offsets(TripletCols tc, const uint32_t* __restrict__ hlayer,
                                 const uint32_t* __restrict__ th1, const uint32_t* __restrict__ t_off, LinkBufs b,
                                 DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t T = tc.n_trip, E = cnt->n_events;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     const uint64_t n = (uint64_t)E * (T + 1);
     for (uint64_t i = gtid(); i < n; i += gstride()) {
         const uint32_t e = (uint32_t)(i / (T + 1)), k = (uint32_t)(i % (T + 1));
@@ -249,6 +293,8 @@ __global__ void k_event_offsets(TripletCols tc, const uint32_t* __restrict__ hla
             b.ev_off[i] = kSentinel;
             continue;
         }
+        if (!(index_ok(f, R, cnt) && index_ok(l, R, cnt)) || !(index_ok(th1[f], nh, cnt) && index_ok(th1[l], nh, cnt)))
+            continue;
         const uint32_t lf = hlayer[th1[f]] - 1, ll = hlayer[th1[l]] - 1;
         b.ev_off[i] = k < lf ? f : (k <= ll ? t_off[(uint64_t)e * T + k] : l);
     }
@@ -258,16 +304,18 @@ __global__ void k_link_count(TripletCols tc, const uint32_t* __restrict__ hlayer
                              const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
                              const uint32_t* __restrict__ th3, const uint32_t* __restrict__ t_off, LinkBufs b,
                              DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     const uint32_t n_off = cnt->n_events * tc.n_trip + 1;
     for (uint32_t t = gtid(); t <= R; t += gstride()) {
         uint32_t n = 0;
         LinkRange r;
-        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
+        if (t < R && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r, cnt)) {
             const uint32_t h = th2[t], h3 = th3[t];
             const float et = b.eta[t];
             for (uint32_t q = b.hit_count[h]; q < b.hit_count[h + 1]; ++q) {
                 const uint32_t i = b.csr[q];
+                if (!index_ok(i, R, cnt)) continue;
                 if (i >= r.begin && i < r.end && th2[i] == h3 && fabsf(et - b.eta[i]) < tc.d_eta_cut) {
                     n++;
                     b.conn[i] = 1u;
@@ -286,21 +334,27 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
                              const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
                              const uint32_t* __restrict__ th3, const uint32_t* __restrict__ t_off, LinkBufs b,
                              uint32_t n_conn_cap, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     const uint32_t n_off = cnt->n_events * tc.n_trip + 1;
     bool over = false;
     for (uint32_t t = gtid(); t < R; t += gstride()) {
         uint32_t so = b.counts[t];
-        const uint32_t sn = b.counts[t + 1];
-        if (so == sn) continue;
-        if (sn > n_conn_cap) { over = true; continue; }
+        const uint32_t sn_all = b.counts[t + 1];
+        if (so >= sn_all) continue;
+        // a base whose connections run past the capacity writes the entries
+        // below it (the run is void, but none below the capacity is unwritten)
+        if (sn_all > n_conn_cap) over = true;
+        const uint32_t sn = sn_all < n_conn_cap ? sn_all : n_conn_cap;
+        if (so >= sn) continue;
         LinkRange r;
-        if (!link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) continue;
+        if (!link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r, cnt)) continue;
         const uint32_t h = th2[t], h3 = th3[t];
         const float et = b.eta[t];
         const uint32_t q0 = b.hit_count[h], q1 = b.hit_count[h + 1];
         auto follows = [&](uint32_t i) {
-            return i >= r.begin && i < r.end && th2[i] == h3 && fabsf(et - b.eta[i]) < tc.d_eta_cut;
+            return index_ok(i, R, cnt) && i >= r.begin && i < r.end && th2[i] == h3 &&
+                   fabsf(et - b.eta[i]) < tc.d_eta_cut;
         };
         // the bucket is in fill (atomic) order: collect the followers, sort
         // them and emit them in ascending index, the reference's order
@@ -324,6 +378,7 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
                 b.foll[so] = f[x];
                 b.pev[so] = r.ev;
             }
+            for (; so < sn; ++so) b.basis[so] = b.foll[so] = b.pev[so] = kSentinel;
         } else {
             uint32_t last = 0;
             bool first = true;
@@ -340,6 +395,7 @@ __global__ void k_link_store(TripletCols tc, const uint32_t* __restrict__ hlayer
                 last = best;
                 first = false;
             }
+            for (; so < sn; ++so) b.basis[so] = b.foll[so] = b.pev[so] = kSentinel;
         }
     }
     if (over) atomicOr(&cnt->err, kErrCapacity);
@@ -361,7 +417,8 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
                              const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
                              const uint32_t* __restrict__ th3, const uint32_t* __restrict__ t_off, LinkBufs b,
                              DevCounts* __restrict__ cnt) {
-    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     const uint32_t n_off = cnt->n_events * tc.n_trip + 1;
     const uint32_t n_chunks = (R + kWave - 1) / kWave;
     if (gtid() == 0) cnt->n_link_chunks = n_chunks;
@@ -375,12 +432,13 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
         uint32_t n = 0, q0 = 0, q1 = 0, h3 = 0;
         float et = 0.0f;
         LinkRange r{0, 0, 0};
-        if (t < R && b.runs && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
+        if (t < R && b.runs && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r, cnt)) {
             // the run of the third hit: tracklets i with th2[i] == h3, in ascending index
             const uint32_t h = th2[t];
             h3 = th3[t];
             et = b.eta[t];
-            const uint32_t i0 = b.hit_count[h3], i1 = b.hit_cursor[h3];
+            const bool h3_ok = index_ok(h3, nh, cnt);
+            const uint32_t i0 = h3_ok ? b.hit_count[h3] : 0u, i1 = h3_ok ? b.hit_cursor[h3] : 0u;
             const uint32_t lo = i0 > r.begin ? i0 : r.begin, hi = i1 < r.end ? i1 : r.end;
             for (uint32_t i = lo; i < hi; ++i) {
                 if (th1[i] == h && fabsf(et - b.eta[i]) < tc.d_eta_cut) {
@@ -397,7 +455,7 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
                 q0 = lo;
                 q1 = hi;
             }
-        } else if (t < R && !b.runs && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r)) {
+        } else if (t < R && !b.runs && link_range(tc, hlayer, hevt, th1, th2, t_off, n_off, R, t, b, r, cnt)) {
             const uint32_t h = th2[t];
             h3 = th3[t];
             et = b.eta[t];
@@ -405,6 +463,7 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
             q1 = b.hit_count[h + 1];
             for (uint32_t q = q0; q < q1; ++q) {
                 const uint32_t i = b.csr[q];
+                if (!index_ok(i, R, cnt)) continue;
                 if (i >= r.begin && i < r.end && th2[i] == h3 && fabsf(et - b.eta[i]) < tc.d_eta_cut) {
                     if (n < kLinkFew) {  // insertion into the sorted slots
                         uint32_t p = n;
@@ -463,7 +522,7 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
                     uint32_t best = kSentinel;
                     for (uint32_t q = q0; q < q1; ++q) {
                         const uint32_t i = b.csr[q];
-                        if ((k == 0 || i > last) && i < best && i >= r.begin && i < r.end && th2[i] == h3 &&
+                        if ((k == 0 || i > last) && i < best && i >= r.begin && i < r.end && i < R && th2[i] == h3 &&
                             fabsf(et - b.eta[i]) < tc.d_eta_cut)
                             best = i;
                     }
@@ -482,12 +541,15 @@ __global__ void k_link_fused(TripletCols tc, const uint32_t* __restrict__ hlayer
 // for the events that have a connection.
 __global__ void k_link_flags(const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ th1, LinkBufs b,
                              DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    const uint32_t nh = cnt->n_hits, E = cnt->n_events;
     for (uint32_t t = gtid(); t <= R; t += gstride()) {
         bool in = K > 0 && t + 1 < R;
         if (b.per_event && t < R) {
-            const uint32_t e = hevt[th1[t]];
-            in = b.ev_has[e] != 0u && t != b.ev_last[e];
+            const uint32_t h = th1[t];
+            const uint32_t e = index_ok(h, nh, cnt) ? hevt[h] : kSentinel;
+            in = e != kSentinel && index_ok(e, E, cnt) && b.ev_has[e] != 0u && t != b.ev_last[e];
         }
         const uint32_t c = (t < R) ? b.conn[t] : 0u;
         b.flag_a[t] = in ? (c != 0u) : 0u;
@@ -496,10 +558,12 @@ __global__ void k_link_flags(const uint32_t* __restrict__ hevt, const uint32_t* 
 }
 
 __global__ void k_link_compact(LinkBufs b, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     for (uint32_t t = gtid(); t + 1 < R; t += gstride()) {
-        if (b.flag_a[t] != b.flag_a[t + 1]) b.connectable[b.flag_a[t]] = t;
-        if (b.flag_b[t] != b.flag_b[t + 1]) b.nonconnectable[b.flag_b[t]] = t;
+        const uint32_t a = b.flag_a[t], c = b.flag_b[t];  // scanned flags: list positions below R
+        if (a != b.flag_a[t + 1] && index_ok(a, R, cnt)) b.connectable[a] = t;
+        if (c != b.flag_b[t + 1] && index_ok(c, R, cnt)) b.nonconnectable[c] = t;
     }
 }
 
@@ -507,10 +571,14 @@ __global__ void k_link_compact(LinkBufs b, DevCounts* __restrict__ cnt) {
 __global__ void k_circle(const float* __restrict__ hx, const float* __restrict__ hy,
                          const uint32_t* __restrict__ th1, const uint32_t* __restrict__ th2,
                          const uint32_t* __restrict__ th3, LinkBufs b, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t n = bounded(cnt->n_connectable, cnt->cap_tracklets, cnt);
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), nh = cnt->n_hits;
     for (uint32_t j = gtid(); j < n; j += gstride()) {
         const uint32_t t = b.connectable[j];
-        const uint32_t h0 = th1[t], h1 = th2[t], h2 = th3[t];
+        if (!index_ok(t, R, cnt)) continue;
+        uint32_t h0, h1, h2;
+        if (!trk_hits_ok(th1, th2, th3, t, nh, cnt, h0, h1, h2)) continue;
         const float x0 = hx[h0], y0 = hy[h0], x1 = hx[h1], y1 = hy[h1], x2 = hx[h2], y2 = hy[h2];
         const float p0x = x0, p0y = y0, p0z = x0 * x0 + y0 * y0;
         const float p1x = x1, p1y = y1, p1z = x1 * x1 + y1 * y1;
@@ -528,6 +596,7 @@ __global__ void k_circle(const float* __restrict__ hx, const float* __restrict__
 
 // ---------------------------------------------------------------------------
 __global__ void k_ca_init(CABufs b, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     for (uint32_t t = gtid(); t < R; t += gstride()) { b.st_a[t] = 1u; b.st_b[t] = 1u; }
     if (gtid() == 0) {
@@ -541,16 +610,18 @@ __global__ void k_ca_iter(const uint32_t* __restrict__ basis, const uint32_t* __
                           uint32_t k, DevCounts* __restrict__ cnt) {
     const uint32_t prev = b.flags[(k + 2) % 3];
     if (gtid() == 0) b.flags[(k + 1) % 3] = 0u;
-    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    if (run_void(cnt)) return;  // no living cell is reported: the iterations stop
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt), R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     if (!prev || K == 0) return;
     if (gtid() == 0) atomicAdd(&cnt->ca_iterations, 1u);
     const uint32_t* cur = (k & 1u) ? b.st_b : b.st_a;
     uint32_t* nxt = (k & 1u) ? b.st_a : b.st_b;
     bool alive = false;
     for (uint32_t p = gtid(); p < K; p += gstride()) {
-        const uint32_t f = foll[p];
+        const uint32_t f = foll[p], bs = basis[p];
+        if (!(index_ok(f, R, cnt) && index_ok(bs, R, cnt))) continue;
         const uint32_t fs = cur[f];
-        const uint32_t same = cur[basis[p]] == fs;
+        const uint32_t same = cur[bs] == fs;
         atomicMax(&nxt[f], fs + same);
         alive |= same != 0u;
     }
@@ -577,15 +648,19 @@ __global__ void k_ca_conn_layer(const uint32_t* __restrict__ basis, const uint32
                                 const uint32_t* __restrict__ hlayer, uint32_t n_trip, uint8_t* __restrict__ clt,
                                 uint32_t* __restrict__ tab, DevCounts* __restrict__ cnt) {
     __shared__ uint32_t hist[256];
+    if (block_run_void(cnt)) return;
     for (uint32_t i = threadIdx.x; i < n_trip; i += blockDim.x) hist[i] = 0;
     __syncthreads();
-    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt), R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
+    const uint32_t nh = cnt->n_hits;
     uint32_t lo, hi;
     ca_block_range(K, lo, hi);
     bool bad = false;
     for (uint32_t p = lo + threadIdx.x; p < hi; p += blockDim.x) {
-        const uint32_t lt = hlayer[th1[basis[p]]] - 1u;
-        clt[p] = (uint8_t)lt;
+        const uint32_t bs = basis[p];
+        const uint32_t h = bs < R ? th1[bs] : kSentinel;
+        const uint32_t lt = h < nh ? hlayer[h] - 1u : kSentinel;
+        clt[p] = (uint8_t)lt;  // 0xff for an entry that is skipped (lt >= n_trip, and n_trip < 256)
         if (lt < n_trip) atomicAdd(&hist[lt], 1u);
         else bad = true;
     }
@@ -600,6 +675,7 @@ __global__ void k_ca_conn_scatter(const uint32_t* __restrict__ basis, const uint
                                   const uint8_t* __restrict__ clt, uint32_t n_trip, const uint32_t* __restrict__ tab,
                                   uint2* __restrict__ lt_conn, DevCounts* __restrict__ cnt) {
     __shared__ uint32_t cur[256];
+    if (block_run_void(cnt)) return;
     for (uint32_t i = threadIdx.x; i < n_trip; i += blockDim.x) cur[i] = tab[i * gridDim.x + blockIdx.x];
     __syncthreads();
     const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
@@ -612,18 +688,22 @@ __global__ void k_ca_conn_scatter(const uint32_t* __restrict__ basis, const uint
 }
 
 __global__ void k_ca_layer(const uint2* __restrict__ lt_conn, const uint32_t* __restrict__ tab, uint32_t tab_blocks,
-                           uint32_t n_trip, uint32_t lt, CABufs b) {
+                           uint32_t n_trip, uint32_t lt, CABufs b, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
+    const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
     // the bucket: from its first table entry to the next bucket's (the total after the last)
-    const uint32_t lo = tab[lt * tab_blocks], hi = tab[lt + 1 < n_trip ? (lt + 1) * tab_blocks : n_trip * tab_blocks];
+    const uint32_t lo = tab[lt * tab_blocks], hi0 = tab[lt + 1 < n_trip ? (lt + 1) * tab_blocks : n_trip * tab_blocks];
+    const uint32_t hi = hi0 < K ? hi0 : K;
     for (uint32_t v = lo + gtid(); v < hi; v += gstride()) {
         const uint2 c = lt_conn[v];
-        atomicMax(&b.st_a[c.y], b.st_a[c.x] + 1u);
+        if (index_ok(c.x, R, cnt) && index_ok(c.y, R, cnt)) atomicMax(&b.st_a[c.y], b.st_a[c.x] + 1u);
     }
 }
 
 // the largest state: per-block maxima (part, gridDim.x entries), combined by k_ca_layers_done
 __global__ void k_ca_state_max(CABufs b, uint32_t* __restrict__ part, DevCounts* __restrict__ cnt) {
     __shared__ uint32_t s_w[1024 / kWave];
+    if (block_run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     uint32_t lo, hi;
     ca_block_range(R, lo, hi);
@@ -645,6 +725,10 @@ __global__ void k_ca_state_max(CABufs b, uint32_t* __restrict__ part, DevCounts*
 __global__ void k_ca_layers_done(CABufs b, const uint32_t* __restrict__ part, uint32_t n_part,
                                  DevCounts* __restrict__ cnt) {
     __shared__ uint32_t s_w[1024 / kWave];
+    if (block_run_void(cnt)) {  // nothing for the host to iterate
+        if (threadIdx.x == 0) b.flags[0] = b.flags[1] = b.flags[2] = 0u;
+        return;
+    }
     uint32_t mx = 0;
     for (uint32_t i = threadIdx.x; i < n_part; i += blockDim.x) mx = part[i] > mx ? part[i] : mx;
     for (int d = 32; d >= 1; d >>= 1) {
@@ -663,20 +747,24 @@ __global__ void k_ca_layers_done(CABufs b, const uint32_t* __restrict__ part, ui
 
 __global__ void k_ca_basis_count(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll, CABufs b,
                                  DevCounts* __restrict__ cnt) {
-    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt), R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     for (uint32_t p = gtid(); p < K; p += gstride()) {  // followerBasisCount (.h:152-168)
-        const uint32_t f = foll[p];
-        if (b.st_a[basis[p]] + 1u == b.st_a[f]) atomicAdd(&b.fbc[f], 1u);
+        const uint32_t f = foll[p], bs = basis[p];
+        if (!(index_ok(f, R, cnt) && index_ok(bs, R, cnt))) continue;
+        if (b.st_a[bs] + 1u == b.st_a[f]) atomicAdd(&b.fbc[f], 1u);
     }
 }
 
 __global__ void k_ca_basis_store(const uint32_t* __restrict__ basis, const uint32_t* __restrict__ foll,
                                  const float* __restrict__ pt, CABufs b, uint32_t n_basis_cap,
                                  DevCounts* __restrict__ cnt) {
-    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
+    if (run_void(cnt)) return;
+    const uint32_t K = bounded(cnt->n_conn, cnt->cap_conn, cnt), R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     bool over = false;
     for (uint32_t p = gtid(); p < K; p += gstride()) {  // followerBasisStore (.h:188-205)
         const uint32_t bs = basis[p], f = foll[p];
+        if (!(index_ok(f, R, cnt) && index_ok(bs, R, cnt))) continue;
         if (b.st_a[bs] + 1u != b.st_a[f]) continue;
         const uint32_t so = b.fbc[f] + atomicAdd(&b.cursor[f], 1u);
         if (so >= n_basis_cap) { over = true; continue; }
@@ -689,6 +777,7 @@ __global__ void k_ca_basis_store(const uint32_t* __restrict__ basis, const uint3
 // followerBestBasisStore (.h:224-263) over a basis list put back into
 // connection-pair (= ascending basis) order
 __global__ void k_ca_best(CABufs b, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     for (uint32_t t = gtid(); t < R; t += gstride()) {
         const uint32_t begin = b.fbc[t], end = b.fbc[t + 1];
@@ -719,12 +808,14 @@ __global__ void k_ca_best(CABufs b, DevCounts* __restrict__ cnt) {
 }
 
 __global__ void k_ca_handler(CABufs b, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt), K = bounded(cnt->n_conn, cnt->cap_conn, cnt);
     for (uint32_t t = gtid(); t <= R; t += gstride())  // handlersStateStore (.h:280-288)
         b.handler[t] = (t < R && K > 0) ? (b.isbest[t] == 0u) * b.st_a[t] : 0u;
 }
 
 __global__ void k_ca_tracks(CABufs b, uint32_t n_track_cap, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     bool over = false;
     for (uint32_t t = gtid(); t < R; t += gstride()) {  // trackCollectionStore (.h:305-323)
@@ -820,8 +911,10 @@ hipError_t launch_link_count(const TripletCols& tc, uint32_t n_trk_cap, const ui
 uint64_t link_chunks(uint64_t n_trk_cap) { return n_trk_cap / kWave + 2; }
 constexpr uint32_t kLinkMaxWaves = 65536;  // waves of the single-pass launch, at most
 uint64_t link_scratch_cap(uint64_t n_conn_cap) {
-    // every wave may leave the rest of its last range unused
-    return n_conn_cap + (uint64_t)kChunkRange * kLinkMaxWaves;
+    // a refill drops less than the refilling chunk's own connections, so the
+    // dropped parts add up to less than all connections; at the end every
+    // wave may leave up to a range unused
+    return 2 * n_conn_cap + (uint64_t)kChunkRange * kLinkMaxWaves;
 }
 
 hipError_t launch_link_fused(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer, const uint32_t* hevt,
@@ -921,7 +1014,7 @@ hipError_t launch_ca_layers(uint32_t n_conn_cap, uint32_t n_trip, const uint32_t
                        cnt);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     for (uint32_t lt = 0; lt < n_trip; ++lt)
-        TRAX_LAUNCH(k_ca_layer, n_conn_cap, 256u, b.lt_conn, b.lt_tab, nb, n_trip, lt, b);
+        TRAX_LAUNCH(k_ca_layer, n_conn_cap, 256u, b.lt_conn, b.lt_tab, nb, n_trip, lt, b, cnt);
     // the table's first nb entries are free now: the per-block maxima
     hipLaunchKernelGGL(k_ca_state_max, dim3(nb), dim3(256), 0, st, b, b.lt_tab, cnt);
     hipLaunchKernelGGL(k_ca_layers_done, dim3(1), dim3(1024), 0, st, b, b.lt_tab, nb, cnt);

@@ -96,9 +96,17 @@ __device__ inline bool in_window(const Window& w, float2 zphi) {
                      (w.wrap && ((w.phiLow <= actPhi && actPhi <= PI) || (-PI <= actPhi && actPhi <= w.phiHigh))));
 }
 
+// the two hits of a pair: false for padding (sentinel) pairs and for a hit
+// index out of range (reported)
+__device__ inline bool pair_hits_ok(uint32_t h1, uint32_t h2, uint32_t nh, DevCounts* cnt) {
+    if (h1 == kSentinel || h2 == kSentinel) return false;
+    return index_ok(h1, nh, cnt) && index_ok(h2, nh, cnt);
+}
+
 // one pair segment (event, layer pair) -> number of count slots
 __global__ void k_pair_plan(GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid,
                             uint32_t* __restrict__ seg_base, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
     const uint32_t nseg = cnt->n_events * tc.n_trip;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += gridDim.x * blockDim.x) {
         if (s == 0) cnt->n_segments = nseg;
@@ -165,6 +173,7 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
                              const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, uint32_t* __restrict__ slot_counts,
                              DevCounts* __restrict__ cnt, const uint32_t* __restrict__ slot_seg) {
+    if (run_void(cnt)) return;
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t overlap = 0;
@@ -209,6 +218,7 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
                              const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
                              uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt,
                              const uint32_t* __restrict__ slot_seg) {
+    if (run_void(cnt)) return;
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint2* __restrict__ pair2 = reinterpret_cast<uint2*>(pairs);  // (first hit, second hit), one 8-byte store
@@ -216,9 +226,13 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
     bool over = false;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < nslot; slot += gridDim.x * blockDim.x) {
         uint32_t pos = slot_counts[slot];
-        const uint32_t next = slot_counts[slot + 1];
-        if (pos == next) continue;
-        if (next > n_pair_cap) { over = true; continue; }
+        const uint32_t next_all = slot_counts[slot + 1];
+        if (pos >= next_all) continue;
+        // a slot that runs past the capacity writes the entries below it (the
+        // run is void, but no entry below the capacity is left unwritten)
+        if (next_all > n_pair_cap) over = true;
+        const uint32_t next = next_all < n_pair_cap ? next_all : n_pair_cap;
+        if (pos >= next) continue;
         const SlotInfo si = slot_info(tc, seg_base, nseg, slot, slot_seg);
         const uint32_t lay2 = tc.l2[si.p] - 1, lay1 = tc.l1[si.p] - 1;
         const uint32_t* g2 = grid + ((size_t)si.e * g.n_layers + lay2) * g.cells;
@@ -375,14 +389,15 @@ __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __re
                                 const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt,
                                 const uint32_t* __restrict__ pairs, uint32_t* __restrict__ counts,
                                 DevCounts* __restrict__ cnt) {
-    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    if (run_void(cnt)) return;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events, nh = cnt->n_hits;
     const uint32_t np1 = g.n_phi + 1;
     for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid <= n; gid += gridDim.x * blockDim.x) {
         uint32_t nFound = 0;
         if (gid < n) {
             const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
             PWin pw;
-            if (h1 != kSentinel && h2 != kSentinel &&
+            if (pair_hits_ok(h1, h2, nh, cnt) &&
                 pred_window(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, false, pw)) {
                 pred_walk(pw.w, pw.lg3, np1, [&](uint32_t index) {
                     nFound += in_window(pw.w, hzp[index]);
@@ -398,8 +413,8 @@ __global__ void k_predict_count(GridGeom g, TripletCols tc, const uint32_t* __re
 // the offsets entry a pair writes as the last of its group, kSentinel when none
 __device__ inline uint32_t pred_group_end_index(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
                                                 const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
-                                                uint32_t h1) {
-    if (h1 == kSentinel) return kSentinel;
+                                                uint32_t h1, uint32_t nh) {
+    if (h1 >= nh) return kSentinel;  // padding, or a bad index (reported by the pair's own check)
     const uint32_t ev = hevt[h1], lt = hlayer[h1] - 1;
     if (ev >= n_events || lt >= tc.n_trip) return kSentinel;
     // against the next pair that is not padding: padding pairs (short slots)
@@ -411,8 +426,9 @@ __device__ inline uint32_t pred_group_end_index(const TripletCols& tc, const uin
     while (j < n && pairs[2 * (size_t)j] == kSentinel) ++j;
     bool last = true;
     if (j < n) {
-        const uint32_t nh = pairs[2 * (size_t)j];
-        last = (lt != hlayer[nh] - 1) || (ev != hevt[nh]);
+        const uint32_t h = pairs[2 * (size_t)j];
+        if (h >= nh) return kSentinel;  // a bad index (reported by that pair's own check)
+        last = (lt != hlayer[h] - 1) || (ev != hevt[h]);
     }
     return last ? ev * tc.n_trip + lt + 1 : kSentinel;
 }
@@ -420,8 +436,8 @@ __device__ inline uint32_t pred_group_end_index(const TripletCols& tc, const uin
 // returns the offsets entry written, kSentinel when none
 __device__ inline uint32_t pred_group_end(const TripletCols& tc, const uint32_t* hlayer, const uint32_t* hevt,
                                           const uint32_t* pairs, uint32_t n, uint32_t n_events, uint32_t gid,
-                                          uint32_t h1, uint32_t next, uint32_t* cand_offsets) {
-    const uint32_t g = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1);
+                                          uint32_t h1, uint32_t nh, uint32_t next, uint32_t* cand_offsets) {
+    const uint32_t g = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1, nh);
     if (g != kSentinel) cand_offsets[g] = next;
     return g;
 }
@@ -433,17 +449,22 @@ __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __re
                                 const uint32_t* __restrict__ pairs, const uint32_t* __restrict__ counts,
                                 uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets,
                                 uint32_t n_cand_cap, DevCounts* __restrict__ cnt) {
-    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    if (run_void(cnt)) return;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events, nh = cnt->n_hits;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t short_slots = 0, trunc = 0;
     bool over = false;
     for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += gridDim.x * blockDim.x) {
         uint32_t pos = counts[gid];
-        const uint32_t next = counts[gid + 1];
-        if (next > n_cand_cap) { over = true; continue; }
+        const uint32_t next_all = counts[gid + 1];
+        // a pair whose candidates run past the capacity writes the entries
+        // below it (the run is void, but none below the capacity is unwritten)
+        if (next_all > n_cand_cap) over = true;
+        const uint32_t next = next_all < n_cand_cap ? next_all : n_cand_cap;
+        if (pos > next) pos = next;
         const uint32_t h1 = pairs[2 * (size_t)gid], h2 = pairs[2 * (size_t)gid + 1];
         PWin pw;
-        const bool ok = h1 != kSentinel && h2 != kSentinel &&
+        const bool ok = pair_hits_ok(h1, h2, nh, cnt) &&
                         pred_window(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, true, pw);
         if (ok && pos < next) {
             bool more = false;
@@ -467,7 +488,7 @@ __global__ void k_predict_store(GridGeom g, TripletCols tc, const uint32_t* __re
             cands[2 * (size_t)q + 1] = kSentinel;
         }
         short_slots += next - pos;
-        pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, next, cand_offsets);
+        pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, nh, next_all, cand_offsets);
     }
     if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
     if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
@@ -920,7 +941,10 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6)))
     __shared__ TileWindows<kSortB> s_win;
     __shared__ uint32_t s_cnt[kSortB], s_perm[kSortB], s_hist[16], s_cur[16];
     const uint32_t tid = threadIdx.x;
-    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    // a block that finds the run void claims no tile (every claimed tile is
+    // processed in full, so no look-back waits on a tile that never publishes)
+    if (block_run_void(cnt)) return;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events, nh = cnt->n_hits;
     const uint32_t n_tiles = (n + kPredBlock - 1) / kPredBlock;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t short_slots = 0, trunc = 0;
@@ -939,7 +963,7 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6)))
         if (gid < n) {
             h1 = pairs[2 * (size_t)gid];
             const uint32_t h2 = pairs[2 * (size_t)gid + 1];
-            have = h1 != kSentinel && h2 != kSentinel &&
+            have = pair_hits_ok(h1, h2, nh, cnt) &&
                    pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw);
         }
         // walk one pair's windows: count window A, list window B (by pair index)
@@ -1075,7 +1099,7 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6)))
         if (gid < n) {
             short_slots += countA > nB ? countA - nB : 0u;
             trunc += (countA > 0 && nB > countA);
-            pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, base + excl + countA, cand_offsets);
+            pred_group_end(tc, hlayer, hevt, pairs, n, n_events, gid, h1, nh, base + excl + countA, cand_offsets);
         }
         __syncthreads();  // the LDS lists are reused by the next tile
     }
@@ -1143,7 +1167,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     __shared__ uint32_t s_wsum[kPredBlock / kWave + 1];
     __shared__ uint32_t s_tile, s_pool_n, s_dead_any;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events, nh = cnt->n_hits;
     const uint32_t n_tiles = (n + kPredBlock - 1) / kPredBlock;
     if (blockIdx.x == 0 && tid == 0) cnt->n_pred_chunks = n_tiles * kWaves;
     uint2* pool = pools + (size_t)blockIdx.x * pool_cap;
@@ -1154,9 +1178,11 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         s_pool_n = 0;
         s_dead_any = 0;
     }
-    // tiles from a work queue: tiles differ in cost, so blocks take the next one when free
+    // tiles from a work queue: tiles differ in cost, so blocks take the next one when free;
+    // once the run is void (an earlier kernel or a tile of this one cut an
+    // output at its capacity) no further tile is taken
     while (true) {
-        if (tid == 0) s_tile = atomicAdd(counter, 1u);
+        if (tid == 0) s_tile = (load_err(cnt) & kErrVoid) ? 0xffffffffu : atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
@@ -1172,8 +1198,8 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
                 h1 = pairs[2 * (size_t)gid];
                 h2 = pairs[2 * (size_t)gid + 1];
                 // whether the pair ends its candidate group (loads issued with the window's)
-                g_end = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1);
-                have = h1 != kSentinel && h2 != kSentinel &&
+                g_end = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1, nh);
+                have = pair_hits_ok(h1, h2, nh, cnt) &&
                        pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw);
             }
             if (have) {
@@ -1510,8 +1536,11 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
 uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap / 128) + 2; }  // every variant
 uint64_t predict_fused_chunks(uint64_t n_pair_cap) { return n_pair_cap / kWave + 8; }  // 64-pair chunks, every variant
 uint64_t fused_scratch_cap(uint64_t n_trk_cap) {
-    // every resident wave may leave the rest of its last range unused
-    return n_trk_cap + (uint64_t)kChunkRange * kFusedPoolBlocks * (512 / kWave);
+    // A wave that needs more than its range holds drops the rest (less than
+    // that chunk's own tracklets) and takes a new one, so the dropped parts
+    // add up to less than all tracklets; at the end every resident wave
+    // (2048 blocks of at most 8 waves) may leave up to a range unused.
+    return 2 * n_trk_cap + (uint64_t)kChunkRange * kFusedPoolBlocks * (512 / kWave);
 }
 
 namespace {
@@ -1554,10 +1583,13 @@ hipError_t predict_fused_variant(const GridGeom& g, const TripletCols& tc, uint3
 // candidate group ends written chunk-local get their chunk's offset (chunk_cands scanned)
 __global__ void k_group_offsets(uint32_t* __restrict__ cand_offsets, const uint32_t* __restrict__ group_chunk,
                                 const uint32_t* __restrict__ chunk_cands, uint32_t n_trip,
-                                const DevCounts* __restrict__ cnt) {
-    const uint32_t n = cnt->n_events * n_trip + 1;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (group_chunk[i] != kSentinel) cand_offsets[i] += chunk_cands[group_chunk[i]];
+                                DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;
+    const uint32_t n = cnt->n_events * n_trip + 1, n_chunks = cnt->n_pred_chunks;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = group_chunk[i];
+        if (c != kSentinel && index_ok(c, n_chunks, cnt)) cand_offsets[i] += chunk_cands[c];
+    }
 }
 
 template <int B, int L>

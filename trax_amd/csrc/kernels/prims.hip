@@ -314,6 +314,7 @@ __global__ void k_chunk_copy3(const uint32_t* __restrict__ d_chunks, const uint3
                               const uint32_t* __restrict__ chunk_at, const uint32_t* __restrict__ scratch,
                               uint32_t scratch_cap, uint32_t* __restrict__ o1, uint32_t* __restrict__ o2,
                               uint32_t* __restrict__ o3, uint32_t out_cap, DevCounts* __restrict__ cnt) {
+    if (run_void(cnt)) return;  // the chunks' ranges may not all be written
     const uint32_t n_chunks = *d_chunks;
     const uint32_t lane = lane_id();
     bool over = false;
@@ -321,11 +322,14 @@ __global__ void k_chunk_copy3(const uint32_t* __restrict__ d_chunks, const uint3
          t += gridDim.x * blockDim.x / kWave) {
         const uint32_t dst = chunk_n[t], m = chunk_n[t + 1] - dst, src = chunk_at[t];
         for (uint32_t k = lane; k < m; k += kWave) {
-            if ((uint64_t)src + k < scratch_cap && (uint64_t)dst + k < out_cap) {
+            if ((uint64_t)dst + k >= out_cap) {
+                over = true;
+            } else if ((uint64_t)src + k < scratch_cap) {
                 o1[dst + k] = scratch[src + k];
                 o2[dst + k] = scratch[(size_t)scratch_cap + src + k];
                 o3[dst + k] = scratch[2 * (size_t)scratch_cap + src + k];
-            } else {
+            } else {  // never left unwritten below the output's capacity
+                o1[dst + k] = o2[dst + k] = o3[dst + k] = kSentinel;
                 over = true;
             }
         }
