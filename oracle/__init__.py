@@ -15,16 +15,20 @@ P = C.c_void_p
 U32 = C.c_uint32
 
 
-def build() -> str:
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    tmp = LIB + ".tmp"
+def build(plant: int = 0) -> str:
+    """Build the oracle (plant = 0) or a variant with planted mistake `plant`
+    (the mutation hooks of trax_oracle.cpp, for showing that the pins catch
+    them); returns the library's path."""
+    out = LIB if plant == 0 else os.path.join(HERE, "_build", f"liboracle_plant{plant}.so")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
-           SRC, "-o", tmp, "-lpthread"]
+           f"-DTRAX_ORACLE_PLANT={int(plant)}", SRC, "-o", tmp, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 class Input(C.Structure):
@@ -39,15 +43,16 @@ class Input(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(path=None):
+    """The oracle library (or the variant at `path`)."""
+    path = path or LIB
+    if path not in _libs:
+        if path == LIB and not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         L.oracle_run.restype = P
         L.oracle_run.argtypes = [C.POINTER(Input)]
         L.oracle_error.restype = C.c_char_p
@@ -58,8 +63,10 @@ def lib():
         L.oracle_timing.argtypes = [P, C.c_char_p]
         L.oracle_free.argtypes = [P]
         L.oracle_prefix_sum.argtypes = [P, U32, U32]
-        _lib = L
-    return _lib
+        L.oracle_run_link.restype = P
+        L.oracle_run_link.argtypes = [C.POINTER(Input), P, P, P, U32]
+        _libs[path] = L
+    return _libs[path]
 
 
 NAMES = ["grid", "hits.x", "hits.y", "hits.z", "hits.layer", "hits.det_id", "hits.hit_id", "hits.event", "pairs",
@@ -69,9 +76,10 @@ NAMES = ["grid", "hits.x", "hits.y", "hits.z", "hits.layer", "hits.det_id", "hit
          "track_offsets", "diag"]
 
 
-def run(cols, wl, min_radius_curvature, stop_after=0, sparse_pairs=True, workers=None):
+def run(cols, wl, min_radius_curvature, stop_after=0, sparse_pairs=True, workers=None, lib_path=None):
     """Run the oracle on HitCollection columns (a dict as produced by
-    trax_amd.api.HitSet.columns()) for a Workload.  Returns named arrays."""
+    trax_amd.api.HitSet.columns()) for a Workload.  Returns named arrays.
+    lib_path selects a variant built by build(plant)."""
     from trax_amd.config import geometry_arrays, triplet_arrays
     keep = []
 
@@ -105,7 +113,7 @@ def run(cols, wl, min_radius_curvature, stop_after=0, sparse_pairs=True, workers
     inp.sparse_pairs = int(sparse_pairs)
     inp.stop_after = stop_after
     inp.n_workers = workers if workers is not None else min(8, os.cpu_count() or 1)
-    L = lib()
+    L = lib(lib_path)
     h = L.oracle_run(C.byref(inp))
     try:
         err = L.oracle_error(h)
@@ -129,7 +137,7 @@ def run(cols, wl, min_radius_curvature, stop_after=0, sparse_pairs=True, workers
         L.oracle_free(h)
 
 
-def run_link(hits, wl, min_radius_curvature, h1, h2, h3, n_events=1):
+def run_link(hits, wl, min_radius_curvature, h1, h2, h3, n_events=1, lib_path=None):
     """Connectivity, circle fit and CA of the oracle on given tracklets.  `hits`
     holds grid-ordered columns x, y, z, layer, event (a hand-built fixture)."""
     keep = []
@@ -150,9 +158,7 @@ def run_link(hits, wl, min_radius_curvature, h1, h2, h3, n_events=1):
     inp.d_eta_cut = wl.d_eta_cut
     inp.min_radius_curvature = min_radius_curvature
     inp.n_workers = 1
-    L = lib()
-    L.oracle_run_link.restype = P
-    L.oracle_run_link.argtypes = [C.POINTER(Input), P, P, P, U32]
+    L = lib(lib_path)
     h = L.oracle_run_link(C.byref(inp), a(h1, np.uint32), a(h2, np.uint32), a(h3, np.uint32), len(h1))
     try:
         out = {}

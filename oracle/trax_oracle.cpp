@@ -24,6 +24,18 @@
 
 #include "../trax_amd/csrc/trax_math.h"
 
+// Mutation hooks: a build with -DTRAX_ORACLE_PLANT=k plants mistake k in the
+// linking half, so that the tests can show that each of its independent pins
+// fails on it (tests/test_oracle_pins.py builds every variant).  The product
+// library has no such hooks; the default build plants nothing.
+//   1: the eta cut taken as <= instead of < (TripletConnectivityTight.h:137)
+//   2: eta from the first to the second hit instead of the third (.h:60-75)
+//   3: the circle plane offset c without the paraboloid lift (TrackletCircleFitter.h:80)
+//   4: the best basis by the largest pT difference (CellularAutomaton.h:254-259)
+#ifndef TRAX_ORACLE_PLANT
+#define TRAX_ORACLE_PLANT 0
+#endif
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -743,7 +755,7 @@ float circle_radius(const Ctx& c, u32 h0, u32 h1, u32 h2) {
     float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
     const float d = nx * nx + ny * ny + nz * nz;
     if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
-    const float cc = -(nx * p0x + ny * p0y + nz * p0z);
+    const float cc = (TRAX_ORACLE_PLANT == 3) ? -(nx * p0x + ny * p0y) : -(nx * p0x + ny * p0y + nz * p0z);
     const float nz2 = nz * nz;
     return sqrtf((1 - nz2 - 4 * cc * nz) / (4 * nz2));
 }
@@ -754,7 +766,7 @@ bool connectivity_ca(Ctx& c) {
     const u32 nOff = (u32)c.t_off.size();
     std::vector<float> eta(R);
     for (u32 t = 0; t < R; ++t) {
-        const u32 a = c.th1[t], b = c.th3[t];
+        const u32 a = c.th1[t], b = (TRAX_ORACLE_PLANT == 2) ? c.th2[t] : c.th3[t];
         const float px = c.hx[b] - c.hx[a], py = c.hy[b] - c.hy[a], pz = c.hz[b] - c.hz[a];
         eta[t] = tx_asinhf(pz / sqrtf(px * px + py * py));
     }
@@ -774,7 +786,9 @@ bool connectivity_ca(Ctx& c) {
         return true;
     };
     auto link = [&](u32 t, u32 i) {
-        return (c.th2[t] == c.th1[i]) && (c.th3[t] == c.th2[i]) && (fabsf(eta[t] - eta[i]) < in.d_eta_cut);
+        const float de = fabsf(eta[t] - eta[i]);
+        const bool close = (TRAX_ORACLE_PLANT == 1) ? de <= in.d_eta_cut : de < in.d_eta_cut;
+        return (c.th2[t] == c.th1[i]) && (c.th3[t] == c.th2[i]) && close;
     };
     for (u32 t = 0; t < R; ++t) {
         u32 begin, end, ev;
@@ -857,7 +871,7 @@ bool connectivity_ca(Ctx& c) {
         u32 minIdx = bidx[begin];
         for (u32 i = begin + 1; i < end; ++i) {  // (.h:254-259)
             const float d = bdiff[i];
-            const u32 test = d < minPt;
+            const u32 test = (TRAX_ORACLE_PLANT == 4) ? d > minPt : d < minPt;
             minPt = minPt * (float)(!test) + d * (float)test;
             minIdx = minIdx * (u32)(!test) + bidx[i] * test;
         }

@@ -1178,11 +1178,12 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
         s_pool_n = 0;
         s_dead_any = 0;
     }
-    // tiles from a work queue: tiles differ in cost, so blocks take the next one when free;
-    // once the run is void (an earlier kernel or a tile of this one cut an
-    // output at its capacity) no further tile is taken
+    // a run voided by an earlier kernel is not walked (this kernel's own pool
+    // and scratch writes are bounded by their capacities)
+    if (block_run_void(cnt)) return;
+    // tiles from a work queue: tiles differ in cost, so blocks take the next one when free
     while (true) {
-        if (tid == 0) s_tile = (load_err(cnt) & kErrVoid) ? 0xffffffffu : atomicAdd(counter, 1u);
+        if (tid == 0) s_tile = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= (n_tiles ? n_tiles : 1u)) break;  // uniform: every thread read the same tile
