@@ -98,6 +98,17 @@ int trax_d2h(trax_ctx* ctx, void* dst, const void* src, uint64_t bytes, trax_str
  * with downloadScalar after scanning count+1 entries). */
 int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint32_t n, uint32_t* d_total, trax_stream* s);
 
+/* The same scan without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The scan's status
+ * words live in a file-scope device array of 32771 u64 (256 KiB) that every
+ * device holds until the process ends, so n is at most 2^28 (TRAX_ERR_ARG
+ * above); no hipMalloc, hipFree or allocating library call is made.  All calls
+ * on one device share that array: calls that overlap on one device (on
+ * different streams, or from several threads) race and must be serialised by
+ * the caller.  Results are those of trax_prefix_sum. */
+int trax_prefix_sum_noalloc(trax_ctx* ctx, uint32_t* d_data, uint32_t n, uint32_t* d_total, trax_stream* s);
+
 /* GridBuilder::run (src/algorithms/GridBuilder.h:42): gridCount + PrefixSum +
  * gridStore.  Hits in the seven SoA columns are scattered into the z-phi grid
  * (stable within a cell).  d_grid receives E*L*(nZ+1)*(nPhi+1) cells; the

@@ -99,6 +99,7 @@ def lib():
         "trax_h2d": (C.c_int, [P, P, P, C.c_uint64, P]),
         "trax_d2h": (C.c_int, [P, P, P, C.c_uint64, P]),
         "trax_prefix_sum": (C.c_int, [P, P, U32, P, P]),
+        "trax_prefix_sum_noalloc": (C.c_int, [P, P, U32, P, P]),
         "trax_grid_build": (C.c_int, [P, U32, U32, U32, U32, U32, P, P, P, HitCols, HitCols, P, P]),
         "trax_hits_create": (C.c_int, [U32, C.POINTER(P)]),
         "trax_hits_destroy": (None, [P]),

@@ -2,6 +2,7 @@
 #include "capi_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <set>
@@ -12,6 +13,34 @@ using namespace trax_capi;
 
 extern "C" const char* trax_last_error(void) { return g_last_error.c_str(); }
 
+// ---- scratch of the no-allocation prefix sum --------------------------------
+// trax_prefix_sum_noalloc keeps the scan's status words (the tile sums of the
+// reduce-then-scan form, or the tile status and tile counter of the look-back
+// form) in this file-scope device array instead of memory it would allocate.
+// Every device holds its own copy until the process ends.  In host code the
+// array's name is not a device address: the address of the current device's
+// copy comes from hipGetSymbolAddress, taken once per device in
+// trax_ctx_create, and only that address is passed to kernels.
+static constexpr uint32_t kNoAllocScanMax = 1u << 28;  // elements per call, at most (1 GiB of u32)
+static constexpr uint32_t kNoAllocScanWords = (uint32_t)(kNoAllocScanMax / kScanTileElements) + 3;  // 256 KiB
+static __device__ __align__(256) uint64_t scratch[kNoAllocScanWords];
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<uint64_t*> g_scratch_addr[kMaxDevices];
+
+// the address of the device's copy of `scratch` (device current); once per device
+int noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(TRAX_ERR_ARG, "device index above the library's table");
+    if (g_scratch_addr[device].load()) return TRAX_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(TRAX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[device].store(static_cast<uint64_t*>(p));
+    return TRAX_OK;
+}
+}  // namespace
+
 // ---- context, streams, memory ---------------------------------------------
 
 extern "C" int trax_ctx_create(int device, trax_ctx** out) {
@@ -20,9 +49,11 @@ extern "C" int trax_ctx_create(int device, trax_ctx** out) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
             return fail(TRAX_ERR_HIP, "no such HIP device");
-        trax_ctx* c = new trax_ctx;
-        c->c.reset(new Context(device));
-        *out = c;
+        std::unique_ptr<trax_ctx> c(new trax_ctx);
+        c->c.reset(new Context(device));  // makes the device current
+        const int rc = noalloc_scratch_setup(device);
+        if (rc != TRAX_OK) return rc;
+        *out = c.release();
         return TRAX_OK;
     });
 }
@@ -170,6 +201,23 @@ extern "C" int trax_prefix_sum(trax_ctx* ctx, uint32_t* d_data, uint32_t n, uint
         hip_check(launch_scan(d_data, nullptr, n, n, d_total, ctx->c->scan_scratch(n), pick(ctx, s)), "scan");
         return TRAX_OK;
     });
+}
+
+extern "C" int trax_prefix_sum_noalloc(trax_ctx* ctx, uint32_t* d_data, uint32_t n, uint32_t* d_total,
+                                       trax_stream* s) {
+    if (!ctx) return fail(TRAX_ERR_ARG, "null context");
+    if (n > kNoAllocScanMax)
+        return fail(TRAX_ERR_ARG, "trax_prefix_sum_noalloc takes at most 2^28 elements (its scratch holds " +
+                                      std::to_string(kNoAllocScanWords) + " status words)");
+    const int device = ctx->c->device();
+    uint64_t* addr = (device >= 0 && device < kMaxDevices) ? g_scratch_addr[device].load() : nullptr;
+    if (!addr) return fail(TRAX_ERR_STATE, "no scratch address for the context's device");
+    ScanScratch ss;
+    ss.status = addr;
+    ss.max_tiles = kNoAllocScanWords - 1;  // one more word past the tiles, as the context's scratch has
+    const hipError_t e = launch_scan(d_data, nullptr, n, n, d_total, ss, pick(ctx, s));
+    if (e != hipSuccess) return fail(TRAX_ERR_HIP, std::string("scan: ") + hipGetErrorString(e));
+    return TRAX_OK;
 }
 
 extern "C" int trax_grid_build(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_t n_hits, uint32_t n_z,

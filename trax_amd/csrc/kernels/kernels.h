@@ -106,6 +106,9 @@ struct TripletCols {
     float d_eta_cut;
 };
 
+// status words (u64) a scan of n_cap elements needs: one per tile of
+// kScanTileElements elements, and two more
+constexpr uint64_t kScanTileElements = 8192;
 uint32_t scan_tiles(uint64_t n_cap);
 uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);

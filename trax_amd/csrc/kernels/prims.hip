@@ -355,6 +355,7 @@ hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_t
 
 // status words (u64) the scan needs at a capacity: tile sums of the
 // reduce-then-scan form, or the look-back form's tile status and counter
+static_assert(kRtsTile == kScanTileElements, "the scan's status words are counted in reduce-then-scan tiles");
 uint32_t scan_tiles(uint64_t n_cap) { return (uint32_t)((n_cap + kRtsTile - 1) / kRtsTile) + 2; }
 
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st) {
