@@ -6,11 +6,14 @@ noise (mean 5 hits per track), 10 barrel layers, 8 consecutive layer triplets,
 32 z x 128 phi grid.  A step is one pass of the whole path (grid build, pairs,
 prediction, filter, connectivity, circle fit, cellular automaton) over the
 256-event batch through the batched, graph-captured pipeline, with the hits
-already resident in device memory.  The plain per-event path (the reference's
-call sequence: one event per launch sequence, blocking count read-backs,
-exact allocations) is timed on a sample of the same events for comparison,
-and the oracle (the CPU restatement of the reference's kernels) on a smaller
-sample as the CPU baseline.
+already resident in device memory (`value`); the rate from the first H2D of
+the hits to the last D2H of the tracklets and tracks is reported beside it.
+The plain per-event path (the reference's call sequence: one event per launch
+sequence, blocking count read-backs, exact allocations, its own transfers) is
+timed on a sample of the same events for comparison, the filter's float32
+decisions on every candidate of the batch are adjudicated against float64, and
+the oracle (the CPU restatement of the reference's kernels) runs on a sample
+of the batch that matches its mix of sizes as the CPU baseline.
 
 Prints one JSON line.
 """
@@ -42,8 +45,8 @@ def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads, fused):
     words = (Q + 63) // 64
     if fused:
         return {
-            "grid_count": 12 * n_hits + 12 * cells,
-            "grid_store": 56 * n_hits + 4 * cells,
+            # the grid build is one kernel (count, scan, scatter), run in the grid_count phase
+            "grid_count": 68 * n_hits + 16 * cells,
             "pair_count": 12 * seg_hits + 4 * slots,
             "pair_scan": 8 * slots,
             "pair_store": 12 * seg_hits + 4 * slots + 8 * P,
@@ -53,8 +56,7 @@ def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads, fused):
             "link_store": 12 * R + 4 * R + 12 * K,
         }
     return {
-        "grid_count": 12 * n_hits + 12 * cells,       # count (12 B/hit, 4 B/cell) + scan (8 B/cell)
-        "grid_store": 56 * n_hits + 4 * cells,        # 28 B/hit read + 28 B/hit write + cells
+        "grid_count": 68 * n_hits + 16 * cells,       # the one-kernel grid build: 12 B/hit count + 56 B/hit scatter
         "pair_count": 12 * seg_hits + 4 * slots,
         "pair_scan": 8 * slots,
         "pair_store": 12 * seg_hits + 4 * slots + 8 * P,
@@ -100,54 +102,31 @@ def scan_hbm(dev, n=1 << 28, reps=10):
     return {"elements": n, "ms": round(ms, 4), "gbs": round(gbs, 1), "frac": round(gbs * 1e9 / HBM_PEAK, 4)}
 
 
-def fp64_check(dev, wl, cfg, events, n_events=4):
-    """The filter's float32 decisions against a float64 recomputation: the
-    first n_events events run through the unfused batched path (which stores
-    the candidates), every candidate's cuts (TripletThetaPhiFilter.h:83-150)
-    evaluated in float64, and the candidates whose float64 decision differs
-    from the GPU's float32 one counted.  Such candidates sit within float32
-    rounding of a cut (the circle fit of near-straight tracks is
-    ill-conditioned); the unit tests explain each one by float32 Monte Carlo
-    arithmetic."""
-    import numpy as np
-    from trax_amd.api import Caps, HitSet, run_batch_fit
-    from trax_amd.config import triplet_arrays
-    hs = HitSet(cfg, events[:n_events])
-    b, res = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, per_event=True))
+def fp64_check(dev, wl, cfg, hs, caps):
+    """The filter's float32 decisions on every candidate of the batch against a
+    float64 recomputation (trax_amd/adjudicate.py): the batch runs once more
+    through the unfused path, which stores the candidates, and every
+    candidate's four cuts (TripletThetaPhiFilter.h:83-150) are evaluated in
+    float64 on the host.  Each disagreement is reported with its cut and its
+    distance to the cut in float32 ulps of the cut value, in float32 ulps of
+    the inputs, and in units of the float32 rounding of the cut's quantity
+    (Monte Carlo arithmetic)."""
+    from trax_amd.adjudicate import adjudicate
+    from trax_amd.api import Batch
+    t0 = time.perf_counter()
+    print("float64 check: the batch through the unfused path", file=sys.stderr, flush=True)
+    b = Batch(dev, cfg, caps)
+    b.submit(hs)
+    if b.wait() != 0:
+        raise RuntimeError("capacity overflow in the float64 check")
+    res = b.result(["cands", "pairs", "hits.x", "hits.y", "hits.z", "hits.layer", "filter.bits"])
     b.close()
-    x, y, z = (res[k].astype(np.float64) for k in ("hits.x", "hits.y", "hits.z"))
-    pairs = res["pairs"].reshape(-1, 2).astype(np.int64)
-    cands = res["cands"].reshape(-1, 2).astype(np.int64)
-    cands = cands[(cands[:, 0] != 0xFFFFFFFF) & (cands[:, 1] != 0xFFFFFFFF)]
-    h1, h2, h3 = pairs[cands[:, 0], 0], pairs[cands[:, 0], 1], cands[:, 1]
-    lt = res["hits.layer"][h1].astype(np.int64) - 1
-    t = triplet_arrays(wl.triplets)
-    dphi, dth, tip = (t[k][lt].astype(np.float64) for k in ("dPhiCut", "dThetaCut", "tipCut"))
-    th1 = np.arctan2(np.hypot(x[h2] - x[h1], y[h2] - y[h1]), z[h2] - z[h1])
-    th2 = np.arctan2(np.hypot(x[h3] - x[h2], y[h3] - y[h2]), z[h3] - z[h2])
-    r = np.abs(th2 / th1)
-    ok = (1 - dth <= r) & (r <= 1 + dth)
-    d = np.arctan2(y[h3] - y[h2], x[h3] - x[h2]) - np.arctan2(y[h2] - y[h1], x[h2] - x[h1])
-    d = np.where(d > np.pi, d - 2 * np.pi, np.where(d < -np.pi, d + 2 * np.pi, d))
-    ok &= np.abs(d) <= dphi
-    p1 = np.stack([x[h1], y[h1], x[h1] ** 2 + y[h1] ** 2], 1)
-    p2 = np.stack([x[h2], y[h2], x[h2] ** 2 + y[h2] ** 2], 1)
-    p3 = np.stack([x[h3], y[h3], x[h3] ** 2 + y[h3] ** 2], 1)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        n = np.cross(p2 - p1, p3 - p1)
-        n /= np.linalg.norm(n, axis=1, keepdims=True)
-        cx, cy = -n[:, 0] / (2 * n[:, 2]), -n[:, 1] / (2 * n[:, 2])
-        c = -(n * p1).sum(1)
-        radius = np.sqrt((1 - n[:, 2] ** 2 - 4 * c * n[:, 2]) / (4 * n[:, 2] ** 2))
-        v = np.stack([-cx, -cy], 1)
-        v /= np.linalg.norm(v, axis=1, keepdims=True)
-        pca = np.stack([cx, cy], 1) + radius[:, None] * v
-        ok &= ((pca ** 2).sum(1) <= tip ** 2) & (radius >= cfg.min_radius_curvature)
-    key = lambda a, b_, c_: (a << 42) | (b_ << 21) | c_
-    kept = np.isin(key(h1, h2, h3), key(res["tracklets.h1"].astype(np.int64), res["tracklets.h2"].astype(np.int64),
-                                        res["tracklets.h3"].astype(np.int64)))
-    return {"events": hs.n_events, "candidates": int(h3.size), "tracklets": int(kept.sum()),
-            "float32_float64_disagreements": int((kept != ok).sum())}
+    a = adjudicate(res, cfg, workers=min(16, os.cpu_count() or 1),
+                   progress=lambda m: print(f"float64 check: {m}", file=sys.stderr, flush=True))
+    a["events"] = hs.n_events
+    a["details"] = a["details"][:4]
+    a["host_s"] = round(time.perf_counter() - t0, 1)
+    return a
 
 
 def serving_stream(wl, n_events, seed, workers):
@@ -163,30 +142,39 @@ def serving_stream(wl, n_events, seed, workers):
     return cfg, hs
 
 
-def serve(dev, cfg, hs, bounds, rate_hz, seed, n_slots, max_batch_events, slot_hits, fused):
-    """Replay the stream with Poisson arrivals at rate_hz through the
-    continuous-batching server; latency = arrival to results on the host."""
+def serve(dev, cfg, hs, bounds, rates_hz, seed, n_slots, max_batch_events, slot_hits, fused, group):
+    """Replay the stream with Poisson arrivals at each rate of rates_hz through
+    one continuous-batching server (events in arrival order, or grouped by hit
+    count); latency = arrival to results on the host.  One record per rate."""
     import numpy as np
     from trax_amd.api import Server
-    rng = np.random.default_rng(seed)
-    arrival = np.cumsum(rng.exponential(1.0 / rate_hz, hs.n_events))
-    arrival -= arrival[0]
     sv = Server(dev, cfg, bounds, n_slots=n_slots, max_batch_events=max_batch_events, max_batch_hits=slot_hits,
-                device_budget=64 << 30, fuse_filter=fused)
+                device_budget=64 << 30, fuse_filter=fused, group_by_size=group)
     caps = sv.caps()
-    _, st = sv.replay(hs, arrival)
+    out = []
+    for rate_hz in rates_hz:
+        rng = np.random.default_rng(seed)
+        arrival = np.cumsum(rng.exponential(1.0 / rate_hz, hs.n_events))
+        arrival -= arrival[0]
+        _, st = sv.replay(hs, arrival)
+        out.append({"events": st["events"], "arrival": "poisson", "rate_hz": rate_hz, "seed": seed,
+                    "admission": "grouped-by-hit-count" if group else "arrival-order",
+                    "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                    "max_ms": round(st["max_ms"], 3), "events_per_s": round(st["events_per_s"], 1),
+                    "tracklets_per_s": round(st["tracklets_per_s"], 1), "batches": st["batches"],
+                    "mean_batch_events": round(st["mean_batch_events"], 2),
+                    "max_batch_events": st["max_batch_events"], "resubmits": st["resubmits"],
+                    "failed": st["failed"], "slots": n_slots, "slot_hit_capacity": caps["max_hits"],
+                    "device_bytes": st["device_bytes"],
+                    "mean_ms": {k: round(st[k + "_ms"], 3) for k in ("queue", "pipeline", "copy", "split")}})
+        print(f"serving {out[-1]['admission']} at {rate_hz:.0f}/s: p50 {out[-1]['p50_ms']} ms, "
+              f"p99 {out[-1]['p99_ms']} ms, {out[-1]['events_per_s']} events/s", file=sys.stderr, flush=True)
     sv.close()
-    return {"events": st["events"], "arrival": "poisson", "rate_hz": rate_hz, "seed": seed,
-            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
-            "events_per_s": round(st["events_per_s"], 1), "tracklets_per_s": round(st["tracklets_per_s"], 1),
-            "batches": st["batches"], "mean_batch_events": round(st["mean_batch_events"], 2),
-            "max_batch_events": st["max_batch_events"], "resubmits": st["resubmits"], "failed": st["failed"],
-            "slots": n_slots, "slot_hit_capacity": caps["max_hits"], "device_bytes": st["device_bytes"],
-            "mean_ms": {k: round(st[k + "_ms"], 3) for k in ("queue", "pipeline", "copy", "split")}}
+    return out
 
 
 # the kernel each phase's time belongs to (the phase also holds small fills)
-PHASE_KERNEL = {"grid_count": "k_grid_count", "grid_store": "k_grid_store", "pair_count": "k_pair_count",
+PHASE_KERNEL = {"grid_count": "k_grid_build", "grid_store": "k_grid_build", "pair_count": "k_pair_count",
                 "pair_store": "k_pair_store", "predict": "k_predict_fused", "filter_count": "k_filter_count",
                 "filter_store": "k_filter_store", "link_count": "k_link_count", "link_store": "k_link_store"}
 PHASE_KERNEL_FUSED = dict(PHASE_KERNEL, predict="k_predict_filter", filter_store="k_filter_offsets")
@@ -206,23 +194,27 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(wl, events, cfg_cls, hitset_cls, budget_s=20.0):
+def cpu_baseline(wl, events, cfg_cls, hitset_cls, n_sample=32):
+    """The oracle (the CPU restatement of the reference's kernels) on a sample
+    of the step's events that matches its mix of sizes: every eighth event in
+    order of hit count, run as one event group on the host's threads."""
     import oracle
     cores = min(16, os.cpu_count() or 1)
-    n, t = 0, 0.0
-    for k in range(4, len(events) + 1, 4):
-        sample = events[:k]
-        cfg = cfg_cls(wl)
-        hs = hitset_cls(cfg, sample)
-        t0 = time.perf_counter()
-        oracle.run(hs.columns(), wl, cfg.min_radius_curvature, workers=cores)
-        t = time.perf_counter() - t0
-        n = k
-        if t > budget_s / 2 or k >= 32:
-            break
-    return {"value": n / t, "unit": "events/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} of the {N_EVENTS} batch events as one event group, "
-                      f"oracle (C++ restatement of the reference kernels) on {cores} host threads, {t:.1f} s"}
+    order = sorted(range(len(events)), key=lambda e: events[e].n_hits)
+    step = max(1, len(events) // n_sample)
+    sample = [events[e] for e in order[step // 2::step]][:n_sample]
+    cfg = cfg_cls(wl)
+    hs = hitset_cls(cfg, sample)
+    t0 = time.perf_counter()
+    oracle.run(hs.columns(), wl, cfg.min_radius_curvature, workers=cores)
+    t = time.perf_counter() - t0
+    mean_all = sum(ev.n_hits for ev in events) / len(events)
+    return {"value": len(sample) / t, "unit": "events/s", "cores": cores, "kind": "port",
+            "hits_per_s": round(hs.n_hits / t, 1),
+            "sample": f"{len(sample)} of the {len(events)} batch events (every {step}th by hit count: "
+                      f"{hs.n_hits / len(sample):.0f} hits per event against {mean_all:.0f} in the batch) as one "
+                      f"event group, oracle (C++ restatement of the reference kernels) on {cores} host threads, "
+                      f"{t:.1f} s"}
 
 
 def write_runtime_csv(path, wl, hs, events, step_phase_ms):
@@ -285,6 +277,8 @@ def main():
                     help="also run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--serve-events", type=int, default=10000, help="events of the serving stream (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=4000.0, help="Poisson arrival rate of the stream (events/s)")
+    ap.add_argument("--serve-sweep", type=str, default="8000,16000,32000,64000",
+                    help="further arrival rates (events/s) of the load sweep, comma-separated ('' = none)")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-max-batch", type=int, default=256)
     ap.add_argument("--serve-slot-hits", type=int, default=2_000_000,
@@ -366,22 +360,31 @@ def main():
     trip_s = counts["tracklets"] / (ms_step / 1e3)
     phase_ms = {p: acc[p] / args.steps for p in phases}
 
-    serving = None
+    serving, serving_sweep = None, []
     if stream is not None:
         b.close()  # the server holds its own workspaces
         print("serving stream", file=sys.stderr, flush=True)
-        serving = serve(dev, stream[0], stream[1], bounds, args.serve_rate, SEED + 2, args.serve_slots,
-                        args.serve_max_batch, args.serve_slot_hits, not args.no_fuse)
+        rates = [args.serve_rate] + [float(r) for r in args.serve_sweep.split(",") if r.strip()]
+        for group in (True, False):
+            serving_sweep += serve(dev, stream[0], stream[1], bounds, rates, SEED + 2, args.serve_slots,
+                                   args.serve_max_batch, args.serve_slot_hits, not args.no_fuse, group)
+        serving = serving_sweep[0]  # grouped by hit count, at the stated rate
         b = Batch(dev, cfg, caps, tuning)
         b.submit(hs)
         b.wait()
 
-    # PCIe-inclusive rate (hits uploaded every step)
-    t2 = time.perf_counter()
+    # from the first H2D to the last D2H: the hits uploaded, the pipeline, and
+    # the tracklets and CA tracks downloaded, every step
+    out_names = ["tracklets.h1", "tracklets.h2", "tracklets.h3", "tracklet_offsets", "tracks", "track_offsets"]
+    e2e = []
     for _ in range(5):
+        t2 = time.perf_counter()
         b.submit(hs)
-        b.wait()
-    pcie_ev_s = 5 * N_EVENTS / (time.perf_counter() - t2)
+        if b.wait() != 0:
+            raise RuntimeError("capacity overflow")
+        b.result(out_names)
+        e2e.append(time.perf_counter() - t2)
+    e2e_ev_s = N_EVENTS / (sum(e2e) / len(e2e))
 
     # plain per-event path on a sample of the same events
     n_plain = min(args.plain_events, N_EVENTS)
@@ -392,8 +395,12 @@ def main():
         plain_ms = sum(dev.run_plain_bench(cfg, s) for s in singles)
         sample_hits = sum(s.n_hits for s in singles)
         plain_ev_s = n_plain / (plain_ms / 1e3)
-        # the batch rate on the same mix of event sizes
-        batch_ev_s_same_mix = ev_s * (hs.n_hits / N_EVENTS) / (sample_hits / n_plain)
+        # the batch rates on the same mix of event sizes (the plain path's time
+        # includes its uploads and downloads, so the end-to-end rate is the
+        # like-for-like one)
+        mix = (hs.n_hits / N_EVENTS) / (sample_hits / n_plain)
+        batch_ev_s_same_mix = ev_s * mix
+        batch_e2e_ev_s_same_mix = e2e_ev_s * mix
 
     if args.copy_calibration:
         # a plain copy of known size: HBM traffic counters are scaled by what
@@ -413,7 +420,9 @@ def main():
 
     scan = scan_hbm(dev)
     print(f"scan alone: {scan}", file=sys.stderr, flush=True)
-    fp64 = fp64_check(dev, wl, cfg, events)
+    caps64 = Caps(N_EVENTS, hs.n_hits, caps.max_pairs, head(c["cands"]), caps.max_tracklets, caps.max_conn,
+                  caps.max_track_entries, use_graph=False, per_event=True, fuse_filter=False)
+    fp64 = fp64_check(dev, wl, cfg, hs, caps64)
     print(f"float64 check: {fp64}", file=sys.stderr, flush=True)
 
     # roofline of the dominant phase, HBM-bound
@@ -430,7 +439,7 @@ def main():
     dom = max(pb, key=lambda p: phase_ms[p])
     achieved = pb[dom] / (phase_ms[dom] * 1e-3)
     grid_ms = phase_ms["grid_count"] + phase_ms["grid_store"]
-    grid_gbs = (pb["grid_count"] + pb["grid_store"]) / (grid_ms * 1e-3) / 1e9
+    grid_gbs = pb["grid_count"] / (grid_ms * 1e-3) / 1e9
 
     line = {
         "metric": "events/s & triplets/s, batched vs plain per-event HIP; p50/p99 event latency",
@@ -457,7 +466,8 @@ def main():
         "plain_events_per_s": round(plain_ev_s, 1) if plain_ev_s else None,
         "plain_sample_events": n_plain,
         "batched_vs_plain_same_mix": round(batch_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s else None,
-        "pcie_inclusive_events_per_s": round(pcie_ev_s, 1),
+        "events_per_s_h2d_to_d2h": round(e2e_ev_s, 1),
+        "batched_h2d_to_d2h_vs_plain_same_mix": round(batch_e2e_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s else None,
         "grid_build_hbm_gbs": round(grid_gbs, 1),
         "grid_build_hbm_frac": round(grid_gbs * 1e9 / HBM_PEAK, 4),
         "scan_hbm": scan,
@@ -471,6 +481,8 @@ def main():
         "latency_ms": {"p50": serving["p50_ms"] if serving else None,
                        "p99": serving["p99_ms"] if serving else None},
         "serving": serving,
+        "serving_sweep": [{k: r[k] for k in ("admission", "rate_hz", "events_per_s", "p50_ms", "p99_ms",
+                                             "mean_batch_events", "batches")} for r in serving_sweep],
     }
     b.close()
     if not args.no_cpu_baseline:

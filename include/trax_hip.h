@@ -127,6 +127,203 @@ int trax_grid_build(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_
                     const uint32_t* d_layer_offsets, trax_hit_cols in, trax_hit_cols_out out, uint32_t* d_grid,
                     trax_stream* s);
 
+/* launch shapes (0 = default): block sizes per kernel family, blocks per CU
+ * of the grid-stride kernels, and the fused predictor's variant: (block,
+ * LDS list entries per pair, walk order) one of (128, 32|64), (256,
+ * 24|32|48|64), (512, 24|32) in pair order or (256, 32) longest walk first;
+ * 0 = (256, 24, pair order); the z sub-rows per grid row of the fused
+ * predictor's walk structure (1, 2, 4, 8 or 16; 0 = 8); and the batched grid
+ * build's hits per (event, layer) kept in LDS (0 = as many as fit), its item
+ * order (1 = largest items first), its in-cell ordering (0 = each cell
+ * sorted in LDS, 1 = each hit ranks itself among its cell's) and its gather
+ * (0 = seven columns, 1 = one 32-byte record per hit written by the count
+ * pass).  Results do not depend on them. */
+typedef struct {
+    uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
+    uint32_t blocks_per_cu;
+    uint32_t predict_list;
+    uint32_t predict_sort;
+    uint32_t predict_fine;
+    uint32_t grid_lds_hits, grid_order, grid_rank, grid_gather;
+} trax_tuning;
+
+/* ---- 2b. stage launchers ------------------------------------------------
+ *
+ * One launcher per kernel group of the reference's algorithm classes, on
+ * device buffers the caller allocates, so that a driver can keep the
+ * reference's call sequence (TripletBuilder.cpp:297-510) stage by stage.  As
+ * there, sizes come from the host: every count a stage needs is the total of
+ * a trax_prefix_sum over the previous stage's counts, downloaded (the
+ * reference's downloadScalar) and passed on.  Launchers enqueue on the stream
+ * and return; the stages of one chain share the context's device-side record
+ * of sizes and error bits, so they run on one stream, between
+ * trax_stage_begin and trax_stage_check.  A device-side error (a hit outside
+ * the grid, an index out of range, an output past its size) stops the later
+ * stages' kernels and is returned by trax_stage_check.  tuning may be NULL
+ * (default launch shapes; results do not depend on them).
+ *
+ * The layer triplets (TripletConfigurations) and the geometry supplement go
+ * to the device once, as a trax_device_config; the work-group size of the
+ * configuration (trax_config_set_exec) fixes the pair order as the
+ * reference's nThreads does, and its dEtaCut is the connectivity cut. */
+typedef struct trax_device_config trax_device_config;
+int trax_device_config_create(trax_ctx* ctx, const trax_config* cfg, trax_device_config** out);
+void trax_device_config_destroy(trax_device_config* dc);
+
+/* the hit columns after the grid build (grid order), as every later stage
+ * reads them: the HitCollection columns the kernels use and the (z, phi)
+ * column the window walks read (2 floats per hit, (z, atan2(y, x)), and room
+ * for 8 more floats past the last hit) */
+typedef struct {
+    const float* x; const float* y; const float* z;
+    const uint32_t* layer; const uint32_t* event;
+    const float* zphi;
+    uint32_t n_hits;
+} trax_grid_hits;
+
+/* clears the context's size record and error bits and sets the shape of the
+ * event group (n_events events, n_hits hits) */
+int trax_stage_begin(trax_ctx* ctx, uint32_t n_events, uint32_t n_hits, trax_stream* s);
+/* waits for the stream and returns the chain's device-side errors as status */
+int trax_stage_check(trax_ctx* ctx, trax_stream* s);
+
+/* GridBuilder::run (GridBuilder.h:42) writing the (z, phi) column as well:
+ * d_zphi holds 2 * n_hits + 8 floats */
+int trax_grid_build_zphi(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_t n_hits, uint32_t n_z,
+                         uint32_t n_phi, const uint32_t* d_event_offsets, const uint32_t* d_layer_nhits,
+                         const uint32_t* d_layer_offsets, trax_hit_cols in, trax_hit_cols_out out, float* d_zphi,
+                         uint32_t* d_grid, trax_stream* s);
+
+/* PairGeneratorBeamspot (PairGeneratorBeamspot.cpp:5-183; pairCount .h:43-246,
+ * pairStore .h:448-525).  plan: d_seg_base[n_events * T + 1] receives the
+ * count slots of every (event, layer pair), nThreads * ceil(N_outer /
+ * nThreads), and 0 last; its scan gives n_slots.  count: d_slot_counts[n_slots
+ * + 1] receives the pairs of every slot in the reference's work-item order;
+ * its scan gives n_pairs.  store: d_pairs[2 * n_pairs] (inner, outer) and
+ * d_pair_offsets[n_events * T + 1]. */
+int trax_pair_plan(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const uint32_t* d_grid,
+                   uint32_t* d_seg_base, trax_stream* s);
+int trax_pair_count(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const uint32_t* d_grid,
+                    const trax_grid_hits* hits, const uint32_t* d_seg_base, uint32_t n_slots, uint32_t* d_slot_counts,
+                    const trax_tuning* tuning, trax_stream* s);
+int trax_pair_store(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const uint32_t* d_grid,
+                    const trax_grid_hits* hits, const uint32_t* d_seg_base, uint32_t n_slots,
+                    const uint32_t* d_slot_counts, uint32_t n_pairs, uint32_t* d_pairs, uint32_t* d_pair_offsets,
+                    const trax_tuning* tuning, trax_stream* s);
+
+/* TripletThetaPhiPredictor (TripletThetaPhiPredictor.cpp:10-170; predictCount
+ * .h:49-230, predictStore .h:232-430).  count: d_counts[n_pairs + 1] receives
+ * the candidates of every pair; its scan gives n_cands.  store: d_cands[2 *
+ * n_cands] (pair, third hit) and the group ends in d_cand_offsets[n_events *
+ * T + 1] (zeroed first); trax_offsets_fill then fills the empty groups
+ * (predictOffsetMonotonizeStore, .h:432-447). */
+int trax_predict_count(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const uint32_t* d_grid,
+                       const trax_grid_hits* hits, uint32_t n_pairs, const uint32_t* d_pairs, uint32_t* d_counts,
+                       const trax_tuning* tuning, trax_stream* s);
+int trax_predict_store(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const uint32_t* d_grid,
+                       const trax_grid_hits* hits, uint32_t n_pairs, const uint32_t* d_pairs, const uint32_t* d_counts,
+                       uint32_t n_cands, uint32_t* d_cands, uint32_t* d_cand_offsets, const trax_tuning* tuning,
+                       trax_stream* s);
+/* the offsets "monotonize" (TripletThetaPhiPredictor.h:432-447,
+ * TripletThetaPhiFilter.h:267-280) as the sequential forward fill of the zero
+ * entries of d_offsets[n_events * n_triplets + 1] */
+int trax_offsets_fill(trax_ctx* ctx, uint32_t* d_offsets, uint32_t n_events, uint32_t n_triplets, trax_stream* s);
+
+/* TripletThetaPhiFilter (TripletThetaPhiFilter.cpp:10-208).  count
+ * (filterCount + filterPopCount, .h:50-179): one validity bit per candidate in
+ * d_bitmap[ceil(n_cands / 64)] (u64 words) and their counts in
+ * d_word_counts[words + 1]; its scan gives n_tracklets.  store (filterStore +
+ * filterOffsetStore, .h:181-261): the tracklets' hits d_h1, d_h2, d_h3
+ * [n_tracklets] and d_tracklet_offsets[n_events * T + 1] (zeroed first, the
+ * last tracklet's entry R - 1 as in the reference); trax_offsets_fill then
+ * fills the empty groups. */
+int trax_filter_count(trax_ctx* ctx, const trax_device_config* dc, const trax_grid_hits* hits, uint32_t n_pairs,
+                      const uint32_t* d_pairs, uint32_t n_cands, const uint32_t* d_cands, uint64_t* d_bitmap,
+                      uint32_t* d_word_counts, const trax_tuning* tuning, trax_stream* s);
+int trax_filter_store(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const trax_grid_hits* hits,
+                      uint32_t n_pairs, const uint32_t* d_pairs, uint32_t n_cands, const uint32_t* d_cands,
+                      const uint64_t* d_bitmap, const uint32_t* d_word_counts, uint32_t n_tracklets, uint32_t* d_h1,
+                      uint32_t* d_h2, uint32_t* d_h3, uint32_t* d_tracklet_offsets, const trax_tuning* tuning,
+                      trax_stream* s);
+
+/* TripletConnectivityTight (TripletConnectivityTight.cpp:5-474) and
+ * TrackletCircleFitter (TrackletCircleFitter.cpp:4-48) buffers, sized by the
+ * caller as noted */
+typedef struct {
+    float* eta;              /* n_tracklets */
+    uint32_t* hit_count;     /* n_hits + 1: the tracklets of every first hit, scanned (CSR starts) */
+    uint32_t* hit_cursor;    /* n_hits */
+    uint32_t* csr;           /* n_tracklets */
+    uint32_t* conn;          /* n_tracklets + 1: takes part in a connection */
+    uint32_t* counts;        /* n_tracklets + 1: followers per base; its scan gives n_conn */
+    uint32_t* basis;         /* n_conn */
+    uint32_t* followers;     /* n_conn */
+    uint32_t* event;         /* n_conn */
+    uint32_t* flag_a;        /* n_tracklets + 1: connectable predicate; its scan gives n_connectable */
+    uint32_t* flag_b;        /* n_tracklets + 1: inverted predicate; its scan gives n_nonconnectable */
+    uint32_t* connectable;   /* n_connectable */
+    uint32_t* nonconnectable;/* n_nonconnectable */
+} trax_link_bufs;
+/* tripletEtaCalculatorStore (.h:47-79) and the per-first-hit buckets of the tracklets */
+int trax_link_prepare(trax_ctx* ctx, const trax_device_config* dc, const trax_grid_hits* hits, uint32_t n_tracklets,
+                      const uint32_t* d_h1, const uint32_t* d_h2, const uint32_t* d_h3, const trax_link_bufs* b,
+                      const trax_tuning* tuning, trax_stream* s);
+/* tripletConnectivityTightCount (.h:83-167) */
+int trax_link_count(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const trax_grid_hits* hits,
+                    uint32_t n_tracklets, const uint32_t* d_h1, const uint32_t* d_h2, const uint32_t* d_h3,
+                    const uint32_t* d_tracklet_offsets, const trax_link_bufs* b, const trax_tuning* tuning,
+                    trax_stream* s);
+/* tripletConnectivityTightStore (.h:169-243), counts scanned */
+int trax_link_store(trax_ctx* ctx, const trax_device_config* dc, uint32_t n_events, const trax_grid_hits* hits,
+                    uint32_t n_tracklets, const uint32_t* d_h1, const uint32_t* d_h2, const uint32_t* d_h3,
+                    const uint32_t* d_tracklet_offsets, uint32_t n_conn, const trax_link_bufs* b,
+                    const trax_tuning* tuning, trax_stream* s);
+/* predicateInversionStore (.h:327-346) over the first n_tracklets - 1
+ * tracklets (.cpp:277): flag_a and flag_b */
+int trax_link_flags(trax_ctx* ctx, const trax_grid_hits* hits, uint32_t n_tracklets, uint32_t n_conn,
+                    const uint32_t* d_h1, const trax_link_bufs* b, trax_stream* s);
+/* streamCompactionGetValidIndexesStore (.h:251-273), flags scanned */
+int trax_link_compact(trax_ctx* ctx, uint32_t n_tracklets, uint32_t n_connectable, uint32_t n_nonconnectable,
+                      const trax_link_bufs* b, trax_stream* s);
+/* trackletCircleFitterStore (TrackletCircleFitter.h:32-101): d_pt[n_tracklets]
+ * receives the radius of every listed tracklet, 0 for the others */
+int trax_circle_fit(trax_ctx* ctx, const trax_grid_hits* hits, uint32_t n_tracklets, const uint32_t* d_h1,
+                    const uint32_t* d_h2, const uint32_t* d_h3, const uint32_t* d_connectable, uint32_t n_connectable,
+                    float* d_pt, trax_stream* s);
+
+/* CellularAutomaton (CellularAutomaton.cpp:7-464) buffers */
+typedef struct {
+    uint32_t* states;        /* n_tracklets: the states once the forward phase ends */
+    uint32_t* states_next;   /* n_tracklets */
+    uint32_t* flags;         /* 3 */
+    uint32_t* basis_count;   /* n_tracklets + 1: bases per follower; its scan gives n_basis */
+    uint32_t* cursor;        /* n_tracklets */
+    uint32_t* basis_index;   /* n_basis */
+    float* basis_diff;       /* n_basis */
+    uint32_t* best_basis;    /* n_tracklets */
+    uint32_t* is_best;       /* n_tracklets */
+    uint32_t* track_offsets; /* n_tracklets + 1: track lengths; its scan gives n_track_entries */
+    uint32_t* tracks;        /* n_track_entries */
+} trax_ca_bufs;
+/* states and next states 1, flags cleared */
+int trax_ca_init(trax_ctx* ctx, uint32_t n_tracklets, const trax_ca_bufs* b, trax_stream* s);
+/* forward iteration number `iteration` (0, 1, ...) of the CA (iteration,
+ * .h:52-100); afterwards flags[iteration % 3] is non-zero while cells live
+ * (the reference's living-cell count, CellularAutomaton.cpp:40-76) */
+int trax_ca_iterate(trax_ctx* ctx, uint32_t n_tracklets, uint32_t n_conn, const uint32_t* d_basis,
+                    const uint32_t* d_followers, const trax_ca_bufs* b, uint32_t iteration, trax_stream* s);
+/* followerBasisCount (.h:142-168) */
+int trax_ca_basis_count(trax_ctx* ctx, uint32_t n_tracklets, uint32_t n_conn, const uint32_t* d_basis,
+                        const uint32_t* d_followers, const trax_ca_bufs* b, trax_stream* s);
+/* followerBasisStore, followerBestBasisStore and handlersStateStore
+ * (.h:170-288), basis counts scanned */
+int trax_ca_basis_store(trax_ctx* ctx, uint32_t n_tracklets, uint32_t n_conn, const uint32_t* d_basis,
+                        const uint32_t* d_followers, const float* d_pt, uint32_t n_basis, const trax_ca_bufs* b,
+                        trax_stream* s);
+/* trackCollectionStore (.h:290-323), track lengths scanned */
+int trax_ca_tracks(trax_ctx* ctx, uint32_t n_tracklets, uint32_t n_track_entries, const trax_ca_bufs* b,
+                   trax_stream* s);
+
 /* ---- 3. whole-path entry points ---------------------------------------- */
 
 /* HitCollection / EventSupplement / LayerSupplement (src/datastructures/
@@ -224,20 +421,6 @@ typedef struct {
      *    is absent; every other result is the same */
     uint32_t fuse_filter;
 } trax_batch_caps;
-/* launch shapes (0 = default): block sizes per kernel family, blocks per CU
- * of the grid-stride kernels, and the fused predictor's variant: (block,
- * LDS list entries per pair, walk order) one of (128, 32|64), (256,
- * 24|32|48|64), (512, 24|32) in pair order or (256, 32) longest walk first;
- * 0 = (256, 24, pair order); and the z sub-rows per grid row of the fused
- * predictor's walk structure (1, 2, 4, 8 or 16; 0 = 8).  Results do not
- * depend on them. */
-typedef struct {
-    uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
-    uint32_t blocks_per_cu;
-    uint32_t predict_list;
-    uint32_t predict_sort;
-    uint32_t predict_fine;
-} trax_tuning;
 /* device and pinned host bytes a batch of these capacities would hold, without
  * allocating (the planning step of buildTriplets' per-group buffer sizing,
  * TripletBuilder.cpp:297-510, done once for all groups) */
@@ -300,6 +483,11 @@ typedef struct {
     uint32_t keep_results; /* keep each event's tracklets and tracks (trax_server_event_result) */
     uint32_t use_graph;
     uint32_t fuse_filter;  /* the slots' batches run the predictor and the filter as one pass */
+    /* 0: a batch takes consecutive waiting events in arrival order; 1: events
+     * are grouped by hit count: the oldest waiting event's size class (hit
+     * count by powers of four) first, in arrival order, then any other
+     * waiting events that fit */
+    uint32_t group_by_size;
 } trax_serve_opts;
 typedef struct {
     uint32_t events, failed, batches, resubmits, max_batch_events, slot_hits;
@@ -331,7 +519,9 @@ int trax_server_event_result(const trax_server* s, uint32_t event, const char* n
  * "cands", "cand_offsets", "tracklets.h1/h2/h3", "tracklet_offsets", "eta",
  * "conn.basis", "conn.followers", "conn.event", "conn.connectable",
  * "conn.nonconnectable", "pt", "ca.states", "ca.best_basis", "tracks",
- * "track_offsets", "ca.iterations", "diag" */
+ * "track_offsets", "ca.iterations", "diag"; batches that store the
+ * candidates also give "filter.bits", the filter's decision per candidate
+ * (bit i % 64 of the u64 word i / 64, as two u32 halves per word) */
 int trax_result_get(const trax_result* r, const char* name, const void** data, uint64_t* count, int* is_float);
 double trax_result_timing_ms(const trax_result* r, const char* stage);
 void trax_result_destroy(trax_result* r);
@@ -350,6 +540,72 @@ int trax_physics_eval(const trax_config* cfg, uint32_t n_events, uint32_t n_hits
                       const uint32_t* hit_layer, const uint32_t* hit_event, uint32_t n_tracklets, const uint32_t* h1,
                       const uint32_t* h2, const uint32_t* h3, const uint32_t* tracklet_offsets, int only_tracks,
                       trax_physics_row* rows);
+
+/* PhysicsRecords (src/triplet_builder/PhysicsRecord.h:205-214): one merged
+ * PhysicsRecord per layer triplet, with its eta and pT histograms (bins of
+ * 0.1, counts valid / fake / clones / missed / misc and the per-bin rates). */
+typedef struct trax_physics trax_physics;
+int trax_physics_create(trax_physics** out);
+void trax_physics_destroy(trax_physics* p);
+/* PhysicsRecord(e, events, p, triplets).fillData + PhysicsRecords::addRecord
+ * for every event e of the collection and layer triplet p, in that order
+ * (TripletBuilder.cpp:515-524; PhysicsRecord.cpp:86-255, 349-360): the
+ * result's grid-ordered hit coordinates and simulated track ids, its
+ * tracklets and (reference) offsets, against the findable tracks
+ * HitCollection::addEvent returned for each event of `hits`.  On error the
+ * records are unchanged. */
+int trax_physics_add(trax_physics* p, const trax_config* cfg, const trax_hits* hits, uint32_t n_hits, const float* x,
+                     const float* y, const float* z, const uint32_t* hit_id, uint32_t n_tracklets, const uint32_t* h1,
+                     const uint32_t* h2, const uint32_t* h3, const uint32_t* tracklet_offsets);
+/* PhysicsRecords::merge (PhysicsRecord.cpp:362-367) */
+int trax_physics_merge(trax_physics* into, const trax_physics* from);
+int trax_physics_count(const trax_physics* p, uint32_t* n);
+/* PhysicsRecords::csvDump (PhysicsRecord.cpp:369-392) as text: table 0 the
+ * records' table ("layerTriplet n eff effVar fr frVar cr crVar", one row per
+ * record, then the total row numbered records + 1); table 1 / 2 the eta / pT
+ * histogram file of the record of `layer_triplet` (the total's for records
+ * + 1), layout 0 as tHistogram::csvDump writes it (12 columns), layout 1 the
+ * five columns "bin valid fake clones missed" of the published histogram
+ * files.  Writes at most `size` bytes with a terminating NUL; *needed is the
+ * full length + 1 (TRAX_ERR_CAPACITY when it exceeds size). */
+int trax_physics_csv(const trax_physics* p, int table, uint32_t layer_triplet, int layout, char* buf, uint64_t size,
+                     uint64_t* needed);
+/* the findable tracks HitCollection::addEvent returned for `event`
+ * (UINT32_MAX: all events in order), track ids ascending, each track's hits
+ * in layer order.  With every output null, *n_tracks / *n_hits receive the
+ * counts; otherwise they give the arrays' capacities and receive the counts;
+ * hit_offsets has n_tracks + 1 entries. */
+int trax_hits_truth(const trax_hits* h, uint32_t event, uint32_t* n_tracks, uint32_t* n_hits, uint32_t* track_id,
+                    uint32_t* hit_offsets, float* x, float* y, float* z, uint32_t* layer);
+
+/* Track-level evaluation of the CA's chains, printTracks
+ * (TripletBuilder.cpp:1008-1066): extractTracks (:955-988, chains of at
+ * least three tracklets as hit lists), cloneRemoval (:880-913, a track whose
+ * hits are shared at 0.19 of the shorter's length or more with a kept track
+ * replaces it when longer, is kept beside it when as long, is dropped when
+ * shorter), splitRealFakeTracks (:915-929) and findUniqueRealTracks
+ * (:990-1006).  simulated: the findable tracks of the events (the
+ * reference's SIMULATED).  kept (optional, one per chain): bit 0 the chain's
+ * track survives clone removal, bit 1 it is real. */
+typedef struct {
+    uint64_t total, filtered, valids, valid_unique, fakes, simulated;
+    float fake_rate, clone_rate, efficiency;
+} trax_track_summary;
+int trax_track_eval(uint32_t n_hits, const uint32_t* hit_id, uint32_t n_tracklets, const uint32_t* h1,
+                    const uint32_t* h2, const uint32_t* h3, uint32_t n_chains, const uint32_t* track_offsets,
+                    const uint32_t* tracks, uint64_t simulated, trax_track_summary* out, uint8_t* kept);
+
+/* Float64 audit of the filter's decisions (TripletThetaPhiFilter.h:83-150)
+ * on the host's threads: for each of n_cands stored candidates ((pair,
+ * third hit) interleaved, pairs (h1, h2) interleaved, hit columns
+ * grid-ordered), pass[i] gets bit 0 theta window, 1 phi, 2 closest
+ * approach, 3 radius when that cut passes in float64 from the float32
+ * coordinates, against the float32 thresholds the kernels compare with;
+ * 0x80 for an empty slot.  No counterpart in the reference: it measures how
+ * the float32 decisions differ from float64 ones. */
+int trax_filter_audit_f64(const trax_config* cfg, uint64_t n_cands, const uint32_t* cands, uint64_t n_pairs,
+                          const uint32_t* pairs, uint32_t n_hits, const float* x, const float* y, const float* z,
+                          const uint32_t* layer, float min_radius_curvature, int n_threads, uint8_t* pass);
 
 /* host and device evaluation of the shared float built-ins (atan2, acos,
  * atan, asinh), for accuracy and host/device agreement tests.

@@ -1,5 +1,5 @@
 """ORACLE -- test infrastructure only (CPU restatement of the reference's
-kernels, trax_oracle.cpp).  Loaded by tests/, __graft_entry__.smoke() and the
+kernels, trax_oracle.cpp, and of its physics evaluation, physics.cpp).  Loaded by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py as the checker; the product never imports it."""
 import ctypes as C
 import os
@@ -9,6 +9,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "trax_oracle.cpp")
+SRC_PHYSICS = os.path.join(HERE, "physics.cpp")
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 
 P = C.c_void_p
@@ -23,7 +24,7 @@ def build(plant: int = 0) -> str:
     os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-unused-function",
-           f"-DTRAX_ORACLE_PLANT={int(plant)}", SRC, "-o", tmp, "-lpthread"]
+           f"-DTRAX_ORACLE_PLANT={int(plant)}", SRC, SRC_PHYSICS, "-o", tmp, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
@@ -65,6 +66,15 @@ def lib(path=None):
         L.oracle_prefix_sum.argtypes = [P, U32, U32]
         L.oracle_run_link.restype = P
         L.oracle_run_link.argtypes = [C.POINTER(Input), P, P, P, U32]
+        L.oracle_physics.restype = P
+        L.oracle_physics.argtypes = [U32, U32] + [P] * 6 + [U32] + [P] * 4 + [U32] + [P] * 4
+        L.oracle_physics_text.restype = C.c_char_p
+        L.oracle_physics_text.argtypes = [P, C.c_int, U32, C.c_int]
+        L.oracle_physics_fill.argtypes = [P, U32, P, P]
+        L.oracle_physics_fill_text.restype = C.c_char_p
+        L.oracle_physics_fill_text.argtypes = [P, U32, C.c_int, C.c_int]
+        L.oracle_physics_free.argtypes = [P]
+        L.oracle_tracks.argtypes = [U32, P, P, P, P, U32, P, P, C.c_ulonglong, P, P, P]
         _libs[path] = L
     return _libs[path]
 
@@ -179,3 +189,112 @@ def prefix_sum(values, wg):
     v = np.ascontiguousarray(values, dtype=np.uint32).copy()
     lib().oracle_prefix_sum(C.c_void_p(v.ctypes.data), v.size, wg)
     return v
+
+
+def findable_tracks(event, wl, max_tracks=-1, only_tracks=False):
+    """HitCollection::addEvent's findable tracks (HitCollection.cpp:41-96)
+    of one event (an Event of trax_amd.generator): hits of at least minPt
+    grouped by simulated track id (id 0 skipped with only_tracks), each
+    track's hits ordered by layer (stably), kept when they cover every layer
+    the triplets use; at most max_tracks findable tracks (-1: all).  Returns
+    {track id: indices of its hits in the event}, ids ascending."""
+    used = sorted({l for t in wl.triplets for l in (t.layer1, t.layer2, t.layer3)})
+    min_pt = float(np.float32(wl.min_pt))
+    tid, layer, pt = (np.asarray(event.track_id), np.asarray(event.layer), np.asarray(event.track_pt))
+    groups = {}
+    for i in range(tid.size):
+        if only_tracks and tid[i] == 0:
+            continue
+        if float(pt[i]) < min_pt:
+            continue
+        groups.setdefault(int(tid[i]), []).append(i)
+    out, found = {}, 0
+    for t in sorted(groups):
+        if found == max_tracks:
+            break
+        if only_tracks and t == 0:
+            continue
+        hits = sorted(groups[t], key=lambda i: int(layer[i]))
+        cov = {int(layer[i]) for i in hits}
+        if all(l in cov for l in used):
+            out[t] = hits
+            found += 1
+    return out
+
+
+def _flat_truth(events, truths):
+    ev_off, tids, hoff, xs, ys, zs = [0], [], [0], [], [], []
+    for ev, tr in zip(events, truths):
+        for t, idx in tr.items():
+            tids.append(t)
+            xs.extend(np.asarray(ev.x)[idx]), ys.extend(np.asarray(ev.y)[idx]), zs.extend(np.asarray(ev.z)[idx])
+            hoff.append(hoff[-1] + len(idx))
+        ev_off.append(len(tids))
+    return ([np.asarray(v, np.uint32) for v in (ev_off, tids, hoff)] +
+            [np.asarray(v, np.float32) for v in (xs, ys, zs)])
+
+
+def physics(events, truths, res, n_lt, lib_path=None):
+    """PhysicsRecords of a result against the findable tracks `truths` (one
+    findable_tracks dict per event of `events`).  Returns {"table": csvDump's
+    table, ("eta" | "pt", layer triplet, layout): histogram text, "fills":
+    [(event, layer triplet, efficiency, fake rate, clone rate)] as filled,
+    "fill_hist": [{"eta": text, "pt": text}] of each record as filled}."""
+    keep = []
+
+    def a(v, dt):
+        v = np.ascontiguousarray(v, dtype=dt)
+        keep.append(v)
+        return C.c_void_p(v.ctypes.data)
+
+    tr = _flat_truth(events, truths)
+    L = lib(lib_path)
+    x, y, z = (a(res[k], np.float32) for k in ("hits.x", "hits.y", "hits.z"))
+    h = L.oracle_physics(len(events), n_lt, *[a(v, v.dtype) for v in tr], res["hits.x"].size, x, y, z,
+                         a(res["hits.hit_id"], np.uint32), res["tracklets.h1"].size,
+                         a(res["tracklets.h1"], np.uint32), a(res["tracklets.h2"], np.uint32),
+                         a(res["tracklets.h3"], np.uint32), a(res["tracklet_offsets"], np.uint32))
+    try:
+        out = {"table": L.oracle_physics_text(h, 0, 0, 0).decode()}
+        lts = [int(l.split()[0]) for l in out["table"].splitlines()[1:] if l.strip()]
+        for lt in lts:
+            for which, name in ((1, "eta"), (2, "pt")):
+                for layout in (0, 1):
+                    out[(name, lt, layout)] = L.oracle_physics_text(h, which, lt, layout).decode()
+        fills, ids, rates, i = [], (U32 * 2)(), (C.c_double * 3)(), 0
+        hists = []
+        while L.oracle_physics_fill(h, i, ids, rates) == 0:
+            fills.append((ids[0], ids[1], rates[0], rates[1], rates[2]))
+            hists.append({"eta": L.oracle_physics_fill_text(h, i, 1, 0).decode(),
+                          "pt": L.oracle_physics_fill_text(h, i, 2, 0).decode()})
+            i += 1
+        out["fills"] = fills
+        out["fill_hist"] = hists
+        return out
+    finally:
+        L.oracle_physics_free(h)
+
+
+def tracks(res, simulated, lib_path=None):
+    """printTracks' counts (TripletBuilder.cpp:1008-1066) over a result's CA
+    chains: dict as trax_amd.physics.track_eval returns it, with "kept"."""
+    keep = []
+
+    def a(v, dt):
+        v = np.ascontiguousarray(v, dtype=dt)
+        keep.append(v)
+        return C.c_void_p(v.ctypes.data)
+
+    off = np.asarray(res["track_offsets"], np.uint32)
+    n_chains = max(off.size - 1, 0)
+    counts, rates = np.zeros(6, np.uint64), np.zeros(3, np.float32)
+    kept = np.zeros(max(n_chains, 1), np.uint8)
+    lib(lib_path).oracle_tracks(res["hits.hit_id"].size, a(res["hits.hit_id"], np.uint32),
+                                *[a(res[f"tracklets.h{i}"], np.uint32) for i in (1, 2, 3)], n_chains,
+                                a(off if off.size else np.zeros(1), np.uint32), a(res["tracks"], np.uint32),
+                                int(simulated), a(counts, np.uint64), a(rates, np.float32), a(kept, np.uint8))
+    names = ("total", "filtered", "valids", "valid_unique", "fakes", "simulated")
+    out = {n: int(v) for n, v in zip(names, counts)}
+    out.update({n: float(v) for n, v in zip(("fake_rate", "clone_rate", "efficiency"), rates)})
+    out["kept"] = kept[:n_chains].copy()
+    return out

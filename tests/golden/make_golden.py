@@ -28,14 +28,22 @@ CASES = {
     # one event at the full density of the CA workload (5000 tracks + 50,000
     # noise hits): about 7 minutes of oracle time on 8 threads
     "ca-scale-one-event": (5, dict(events=1, tracks=5000, noise=50000), 11),
+    # single events of the 256-event batch of the throughput measurement
+    # (log-uniform 50-5000 tracks, seed 2024), each run alone: its largest
+    # (4927 tracks), a median one (392) and its smallest (51)
+    "batch-256-event-157": (3, dict(first=157, of=256), 2024),
+    "batch-256-event-166": (3, dict(first=166, of=256), 2024),
+    "batch-256-event-183": (3, dict(first=183, of=256), 2024),
 }
 FULL = {"one-event-100-tracks"}
-SLOW = {"ca-scale-one-event"}  # not re-run by the CPU tests
+SLOW = {"ca-scale-one-event", "batch-256-event-157"}  # not re-run by the CPU tests
 
 
 def case_workload(name):
     cfg_no, over, seed = CASES[name]
     over = dict(over)
+    if "first" in over:
+        return small_workload(cfg_no), seed
     if "tracks" in over:
         wl = small_workload(cfg_no, events=over["events"], tracks=over["tracks"], noise=over["noise"])
     elif "events" in over:
@@ -59,7 +67,9 @@ def main(names=None):
             digests = json.load(f)
     for name in names or CASES:
         wl, seed = case_workload(name)
-        cfg, hs, _ = hitset(wl, seed=seed)
+        first = CASES[name][1].get("first")
+        cfg, hs, _ = (hitset(wl, seed=seed) if first is None else
+                     hitset(wl, n_events=1, seed=seed, first=first, of=CASES[name][1]["of"]))
         res = oracle.run(hs.columns(), wl, cfg.min_radius_curvature, workers=os.cpu_count() or 1)
         keys = [k for k in PIPELINE_KEYS if k in res]
         digests[name] = {k: digest(res[k]) for k in keys}

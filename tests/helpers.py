@@ -21,9 +21,9 @@ def small_workload(cfg_no, events=None, tracks=None, noise=None, **extra):
     return wl
 
 
-def hitset(wl, n_events=None, seed=11, first=0):
+def hitset(wl, n_events=None, seed=11, first=0, of=None):
     cfg = Config(wl)
-    evs = make_events(wl, n_events, seed=seed, first=first)
+    evs = make_events(wl, n_events, seed=seed, first=first, of=of)
     return cfg, HitSet(cfg, evs), evs
 
 

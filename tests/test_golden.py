@@ -35,7 +35,9 @@ def _check(res, name):
 
 def _inputs(name):
     wl, seed = case_workload(name)
-    cfg, hs, _ = hitset(wl, seed=seed)
+    first = CASES[name][1].get("first")
+    cfg, hs, _ = (hitset(wl, seed=seed) if first is None else
+                 hitset(wl, n_events=1, seed=seed, first=first, of=CASES[name][1]["of"]))
     return wl, cfg, hs
 
 
@@ -47,7 +49,8 @@ def test_full_fixture_arrays():
             assert np.array_equal(res[k], z[k]), k
 
 
-@pytest.mark.parametrize("name", ["one-event-100-tracks", "four-uneven-events", "two-events-ca-density"])
+@pytest.mark.parametrize("name", ["one-event-100-tracks", "four-uneven-events", "two-events-ca-density",
+                                  "batch-256-event-166", "batch-256-event-183"])
 def test_oracle_reproduces_fixtures(name):
     wl, cfg, hs = _inputs(name)
     res = oracle.run(hs.columns(), wl, cfg.min_radius_curvature, workers=4)

@@ -41,8 +41,9 @@ def _check_events(sv, alone):
         assert got_tracks == extract_tracks(a), f"event {e} tracks"
 
 
+@pytest.mark.parametrize("group", [False, True], ids=["arrival-order", "grouped-by-size"])
 @pytest.mark.parametrize("pattern", ["all-at-once", "poisson", "one-slot"])
-def test_served_events_match_alone_runs(dev, pattern):
+def test_served_events_match_alone_runs(dev, pattern, group):
     wl, cfg, hs, evs = _stream(16, seed=21, tracks_min=20, tracks_max=600)
     print(f"stream {pattern}: {hs.n_events} events, {hs.n_hits} hits", flush=True)
     alone = _alone(dev, cfg, evs)
@@ -53,7 +54,7 @@ def test_served_events_match_alone_runs(dev, pattern):
     else:
         arrival = np.zeros(hs.n_events)
     sv = Server(dev, cfg, bounds, n_slots=1 if pattern == "one-slot" else 2, max_batch_events=8,
-                max_batch_hits=hs.n_hits, keep_results=True)
+                max_batch_hits=hs.n_hits, keep_results=True, group_by_size=group)
     lat, st = sv.replay(hs, arrival)
     assert st["failed"] == 0 and st["events"] == hs.n_events
     assert (lat > 0).all()
@@ -67,7 +68,8 @@ def test_served_events_match_alone_runs(dev, pattern):
     sv.close()
 
 
-def test_overflowing_batches_are_split(dev):
+@pytest.mark.parametrize("group", [False, True], ids=["arrival-order", "grouped-by-size"])
+def test_overflowing_batches_are_split(dev, group):
     # tiny events: every output of one event fits the capacities' fixed slack,
     # while bounds far below the real ratios make every multi-event batch overflow
     wl, cfg, hs, evs = _stream(12, seed=4, tracks_min=40, tracks_max=120)
@@ -75,7 +77,8 @@ def test_overflowing_batches_are_split(dev):
     pairs = [a["pairs"].size // 2 for a in alone]
     assert max(pairs) < 4096 < sum(pairs)
     tiny = OutputBounds(1e-3, 1e-3, 1e-3, 1e-3, 1e-3)
-    sv = Server(dev, cfg, tiny, n_slots=2, max_batch_events=12, max_batch_hits=hs.n_hits, keep_results=True)
+    sv = Server(dev, cfg, tiny, n_slots=2, max_batch_events=12, max_batch_hits=hs.n_hits, keep_results=True,
+                group_by_size=group)
     _, st = sv.replay(hs, np.zeros(hs.n_events))
     assert st["failed"] == 0 and st["resubmits"] > 0
     _check_events(sv, alone)

@@ -77,6 +77,28 @@ def test_block_shapes_match_oracle(dev, shape):
     assert not bad, "\n".join(bad)
 
 
+@pytest.mark.parametrize("lds_hits", [64, 2048, 0])
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("rank,gather", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_grid_build_variants_match_oracle(dev, lds_hits, order, rank, gather):
+    """The batched grid build's variants: few hits in LDS (the larger
+    (event, layer) items take the global-memory scatter), the largest items
+    first, cells sorted in LDS or each hit ranking itself in its cell, the
+    seven columns or one record per hit gathered; on the uneven events and on
+    a dense one."""
+    shape = dict(grid_lds_hits=lds_hits, grid_order=order, grid_rank=rank, grid_gather=gather)
+    for wl, (cfg, hs, _) in ((w, hitset(w, seed=s)) for w, s in
+                             ((small_workload(3, events=4, tracks_min=20, tracks_max=400), 11),
+                              (small_workload(2, tracks=1500, noise=20000, events=1), 5))):
+        print(f"grid variant {shape}: {wl.name}, {hs.n_hits} hits", flush=True)
+        want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature, stop_after=1)
+        b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits), tuning=shape)
+        b.close()
+        keys = ["grid", "hits.x", "hits.y", "hits.z", "hits.layer", "hits.det_id", "hits.hit_id", "hits.event"]
+        bad = compare(got, want, keys=keys)
+        assert not bad, "\n".join(bad)
+
+
 def test_autotune_picks_supported_shapes_with_same_results(dev):
     wl, cfg, hs, _ = _case()
     caps = Caps.for_hits(hs.n_events, hs.n_hits, per_event=True)
@@ -84,7 +106,7 @@ def test_autotune_picks_supported_shapes_with_same_results(dev):
     caps = b.caps
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
-    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 4 + 4
+    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 4 + 3 + 2 + 2 + 2 + 4
     assert all(t["same_results"] for t in trials)
     assert (tuning["predict_block"], tuning["predict_list"], tuning["predict_sort"]) in PREDICT_VARIANTS
     assert tuning["blocks_per_cu"] in (2, 4, 8, 16)

@@ -328,6 +328,23 @@ def test_filter_cuts_agree_with_float64_up_to_cut_edges(dense):
     assert disagree <= max(10, total // 100000)
 
 
+def test_filter_disagreements_are_the_float32_expressions_own(dense):
+    """The float64 adjudicator (trax_amd/adjudicate.py) on the oracle's filter
+    decisions over a dense event: every candidate whose float64 decision
+    differs is decided the same way by a bit-exact float32 host replica of the
+    reference's cut expressions (numpy float32 with the library's atan2), so
+    the difference is the float32 arithmetic's, and each lies within one unit
+    of float32 rounding per operation of its cut's quantity."""
+    from trax_amd.adjudicate import adjudicate
+    wl, cfg, cols, res, _ = dense
+    a = adjudicate(res, cfg, workers=4)
+    n = a["float32_float64_disagreements"]
+    assert a["candidates"] > 1_000_000 and n <= a["candidates"] // 100_000
+    assert a["float32_replica_agrees"] == n
+    assert a["within_float32_rounding"] == n
+    assert n == sum(a["by_cut"].values())
+
+
 def test_true_helix_triplets_are_found(dense):
     """Physics: consecutive-layer hit triplets of simulated helix tracks pass the
     reference's cuts (the published efficiencies are 0.79-0.84,

@@ -44,14 +44,15 @@ class BatchCaps(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [(n, U32) for n in ("grid_count_block", "grid_store_block", "pair_block", "predict_block",
                                    "filter_block", "link_block", "blocks_per_cu", "predict_list", "predict_sort",
-                                   "predict_fine")]
+                                   "predict_fine", "grid_lds_hits", "grid_order", "grid_rank",
+                                   "grid_gather")]
 
 
 class ServeOpts(C.Structure):
     _fields_ = [("n_slots", U32), ("max_batch_events", U32), ("max_batch_hits", U32), ("device_budget", C.c_uint64),
                 ("pairs_per_hit", C.c_float), ("cands_per_hit", C.c_float), ("tracklets_per_hit", C.c_float),
                 ("conn_per_hit", C.c_float), ("track_entries_per_hit", C.c_float), ("keep_results", U32),
-                ("use_graph", U32), ("fuse_filter", U32)]
+                ("use_graph", U32), ("fuse_filter", U32), ("group_by_size", U32)]
 
 
 class ServeStats(C.Structure):
@@ -66,6 +67,25 @@ class PhysicsRow(C.Structure):
     _fields_ = [(n, U32) for n in ("event", "layer_triplet", "n_tracklets", "n_findable", "found", "fakes",
                                    "clones", "misc")] + [(n, C.c_double) for n in ("efficiency", "fake_rate",
                                                                                    "clone_rate")]
+
+
+class TrackSummary(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("total", "filtered", "valids", "valid_unique", "fakes", "simulated")] + \
+               [(n, C.c_float) for n in ("fake_rate", "clone_rate", "efficiency")]
+
+
+class GridHits(C.Structure):
+    _fields_ = [(n, P) for n in ("x", "y", "z", "layer", "event", "zphi")] + [("n_hits", U32)]
+
+
+class LinkBufs(C.Structure):
+    _fields_ = [(n, P) for n in ("eta", "hit_count", "hit_cursor", "csr", "conn", "counts", "basis", "followers",
+                                 "event", "flag_a", "flag_b", "connectable", "nonconnectable")]
+
+
+class CABufs(C.Structure):
+    _fields_ = [(n, P) for n in ("states", "states_next", "flags", "basis_count", "cursor", "basis_index",
+                                 "basis_diff", "best_basis", "is_best", "track_offsets", "tracks")]
 
 
 class DeviceInfo(C.Structure):
@@ -143,7 +163,48 @@ def lib():
         "trax_math_eval_host": (C.c_int, [C.c_int, P, P, P, C.c_uint64]),
         "trax_filter_phi_decide_host": (C.c_int, [C.c_uint64, P, P, P, P, P, P, P]),
         "trax_physics_eval": (C.c_int, [P, U32, U32, P, P, P, U32, P, P, P, P, C.c_int, C.POINTER(PhysicsRow)]),
+        "trax_physics_create": (C.c_int, [C.POINTER(P)]),
+        "trax_physics_destroy": (None, [P]),
+        "trax_physics_add": (C.c_int, [P, P, P, U32, P, P, P, P, U32, P, P, P, P]),
+        "trax_physics_merge": (C.c_int, [P, P]),
+        "trax_physics_count": (C.c_int, [P, C.POINTER(U32)]),
+        "trax_physics_csv": (C.c_int, [P, C.c_int, U32, C.c_int, P, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "trax_hits_truth": (C.c_int, [P, U32, C.POINTER(U32), C.POINTER(U32), P, P, P, P, P, P]),
+        "trax_filter_audit_f64": (C.c_int, [P, C.c_uint64, P, C.c_uint64, P, U32, P, P, P, P, C.c_float, C.c_int,
+                                            P]),
+        "trax_track_eval": (C.c_int, [U32, P, U32, P, P, P, U32, P, P, C.c_uint64, C.POINTER(TrackSummary), P]),
         "trax_math_eval_device": (C.c_int, [P, C.c_int, P, P, P, C.c_uint64]),
+        "trax_buf_copy_u32": (C.c_int, [P, P, P, C.c_uint64, P]),
+        # stage launchers
+        "trax_device_config_create": (C.c_int, [P, P, C.POINTER(P)]),
+        "trax_device_config_destroy": (None, [P]),
+        "trax_stage_begin": (C.c_int, [P, U32, U32, P]),
+        "trax_stage_check": (C.c_int, [P, P]),
+        "trax_grid_build_zphi": (C.c_int, [P, U32, U32, U32, U32, U32, P, P, P, HitCols, HitCols, P, P, P]),
+        "trax_pair_plan": (C.c_int, [P, P, U32, P, P, P]),
+        "trax_pair_count": (C.c_int, [P, P, U32, P, C.POINTER(GridHits), P, U32, P, C.POINTER(Tuning), P]),
+        "trax_pair_store": (C.c_int, [P, P, U32, P, C.POINTER(GridHits), P, U32, P, U32, P, P, C.POINTER(Tuning), P]),
+        "trax_predict_count": (C.c_int, [P, P, U32, P, C.POINTER(GridHits), U32, P, P, C.POINTER(Tuning), P]),
+        "trax_predict_store": (C.c_int, [P, P, U32, P, C.POINTER(GridHits), U32, P, P, U32, P, P,
+                                         C.POINTER(Tuning), P]),
+        "trax_offsets_fill": (C.c_int, [P, P, U32, U32, P]),
+        "trax_filter_count": (C.c_int, [P, P, C.POINTER(GridHits), U32, P, U32, P, P, P, C.POINTER(Tuning), P]),
+        "trax_filter_store": (C.c_int, [P, P, U32, C.POINTER(GridHits), U32, P, U32, P, P, P, U32, P, P, P, P,
+                                        C.POINTER(Tuning), P]),
+        "trax_link_prepare": (C.c_int, [P, P, C.POINTER(GridHits), U32, P, P, P, C.POINTER(LinkBufs),
+                                        C.POINTER(Tuning), P]),
+        "trax_link_count": (C.c_int, [P, P, U32, C.POINTER(GridHits), U32, P, P, P, P, C.POINTER(LinkBufs),
+                                      C.POINTER(Tuning), P]),
+        "trax_link_store": (C.c_int, [P, P, U32, C.POINTER(GridHits), U32, P, P, P, P, U32, C.POINTER(LinkBufs),
+                                      C.POINTER(Tuning), P]),
+        "trax_link_flags": (C.c_int, [P, C.POINTER(GridHits), U32, U32, P, C.POINTER(LinkBufs), P]),
+        "trax_link_compact": (C.c_int, [P, U32, U32, U32, C.POINTER(LinkBufs), P]),
+        "trax_circle_fit": (C.c_int, [P, C.POINTER(GridHits), U32, P, P, P, P, U32, P, P]),
+        "trax_ca_init": (C.c_int, [P, U32, C.POINTER(CABufs), P]),
+        "trax_ca_iterate": (C.c_int, [P, U32, U32, P, P, C.POINTER(CABufs), U32, P]),
+        "trax_ca_basis_count": (C.c_int, [P, U32, U32, P, P, C.POINTER(CABufs), P]),
+        "trax_ca_basis_store": (C.c_int, [P, U32, U32, P, P, P, U32, C.POINTER(CABufs), P]),
+        "trax_ca_tracks": (C.c_int, [P, U32, U32, C.POINTER(CABufs), P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -244,6 +305,27 @@ class HitSet:
         return {k: self.column(k) for k in ("x", "y", "z", "layer", "det_id", "hit_id", "event", "event_nhits",
                                             "event_offset", "layer_nhits", "layer_offset")}
 
+    def truth(self, event: Optional[int] = None) -> Dict[str, np.ndarray]:
+        """The findable tracks addEvent returned for `event` (None: all events
+        in order): track ids, hit offsets per track and the hits' x, y, z,
+        layer, each track's hits in layer order."""
+        ev = 0xFFFFFFFF if event is None else int(event)
+        nt, nh = U32(), U32()
+        check(lib().trax_hits_truth(self.h, ev, C.byref(nt), C.byref(nh), None, None, None, None, None, None))
+        out = {"track_id": np.zeros(nt.value, np.uint32), "hit_offsets": np.zeros(nt.value + 1, np.uint32),
+               "x": np.zeros(nh.value, np.float32), "y": np.zeros(nh.value, np.float32),
+               "z": np.zeros(nh.value, np.float32), "layer": np.zeros(nh.value, np.uint32)}
+        check(lib().trax_hits_truth(self.h, ev, C.byref(nt), C.byref(nh), *[ptr(out[k]) for k in
+                                    ("track_id", "hit_offsets", "x", "y", "z", "layer")]))
+        return out
+
+    @property
+    def n_findable(self) -> int:
+        """Findable tracks of all events: the reference's SIMULATED count."""
+        nt, nh = U32(), U32()
+        check(lib().trax_hits_truth(self.h, 0xFFFFFFFF, C.byref(nt), C.byref(nh), None, None, None, None, None, None))
+        return int(nt.value)
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.trax_hits_destroy(self.h)
@@ -257,7 +339,7 @@ def _result_dict(rh) -> Dict[str, np.ndarray]:
              "pairs", "pair_offsets", "cands", "cand_offsets", "tracklets.h1", "tracklets.h2", "tracklets.h3",
              "tracklet_offsets", "eta", "conn.basis", "conn.followers", "conn.event", "conn.connectable",
              "conn.nonconnectable", "pt", "ca.states", "ca.best_basis", "tracks", "track_offsets",
-             "ca.iterations", "diag"]
+             "ca.iterations", "diag", "filter.bits"]
     for k in names:
         d, n, f = P(), C.c_uint64(), C.c_int()
         if L.trax_result_get(rh, k.encode(), C.byref(d), C.byref(n), C.byref(f)) != 0:
@@ -508,10 +590,10 @@ class Server:
 
     def __init__(self, dev: Device, cfg: Config, bounds: OutputBounds, n_slots: int = 2, max_batch_events: int = 256,
                  max_batch_hits: int = 0, device_budget: int = 0, keep_results: bool = False,
-                 use_graph: bool = True, fuse_filter: bool = False):
+                 use_graph: bool = True, fuse_filter: bool = False, group_by_size: bool = False):
         o = ServeOpts(n_slots, max_batch_events, max_batch_hits, device_budget, bounds.pairs, bounds.cands,
                       bounds.tracklets, bounds.conn, bounds.track_entries, int(keep_results), int(use_graph),
-                      int(fuse_filter))
+                      int(fuse_filter), int(group_by_size))
         h = P()
         check(lib().trax_server_create(dev.h, cfg.h, C.byref(o), C.byref(h)))
         self.h = h

@@ -23,9 +23,11 @@ SOURCES = [
     "kernels/link.hip",
     "host/trax_host.cpp",
     "capi.cpp",
+    "stages.cpp",
     "serve.cpp",
     "tune.cpp",
     "physics.cpp",
+    "audit.cpp",
 ]
 
 FLAGS = [

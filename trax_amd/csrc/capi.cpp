@@ -300,7 +300,7 @@ static void add_event(trax_hits* h, const trax_config* cfg, const std::vector<PH
         ls.nHits.resize((size_t)(e + 1) * L, 0);
         ls.offset.resize((size_t)(e + 1) * L, 0);
         try {
-            h->hits.addEvent(ev, es, e, ls, cfg->trip, max_tracks, only_tracks != 0);
+            h->truth.push_back(h->hits.addEvent(ev, es, e, ls, cfg->trip, max_tracks, only_tracks != 0));
         } catch (...) {
             es.nHits.resize(e); es.offset.resize(e);
             ls.nHits.resize((size_t)e * L); ls.offset.resize((size_t)e * L);
@@ -462,9 +462,11 @@ static int run_plain_impl(trax_ctx* ctx, const trax_config* cfg, const trax_hits
         es.nHits = h->evt_nhits; es.offset = h->evt_offset;
         LayerSupplement ls(L, E);
         ls.nHits = h->lay_nhits; ls.offset = h->lay_offset;
-        HitMirror dh(c, N);
-        dh.upload(c, hits);  // hits.transfer.toDevice (TripletBuilder.cpp:297-298)
-        GridBuilder(c).run(hits, dh, nt, es, ls, grid, true);
+        hits.transfer.initBuffers(c, hits);  // hits.transfer.initBuffers / toDevice (TripletBuilder.cpp:297-298)
+        hits.transfer.toDevice(c, hits);
+        const GeometrySupplement& geom = cfg->geom;
+        const TripletConfigurations& trip = cfg->trip;
+        GridBuilder(c).run(hits, nt, es, ls, grid);
         lap("grid");
         r->put("grid", grid.host);
         r->putf("hits.x", hits.x); r->putf("hits.y", hits.y); r->putf("hits.z", hits.z);
@@ -479,20 +481,18 @@ static int run_plain_impl(trax_ctx* ctx, const trax_config* cfg, const trax_hits
             r->put("diag", d);
         };
         if (!want(2)) { diag(); *out = guard.release(); return TRAX_OK; }
-        ConfigMirror cm(c, cfg->trip, cfg->geom, nt, cfg->d_eta_cut);
-        std::unique_ptr<Pairing> pairs(PairGeneratorBeamspot(c).run(dh, cm, nt, cfg->trip, grid));
+        std::unique_ptr<Pairing> pairs(PairGeneratorBeamspot(c).run(hits, geom, nt, trip, grid));
         r->put("pairs", D(pairs->pairing));
         r->put("pair_offsets", D(pairs->pairingOffsets));
         lap("pairs");
         if (!want(3)) { diag(); *out = guard.release(); return TRAX_OK; }
-        std::unique_ptr<Pairing> cands(TripletThetaPhiPredictor(c).run(dh, DetectorGeometry(), cm, Dictionary(), (int)nt,
-                                                                      cfg->trip, grid, *pairs));
+        std::unique_ptr<Pairing> cands(TripletThetaPhiPredictor(c).run(hits, DetectorGeometry(), geom, Dictionary(),
+                                                                      (int)nt, trip, grid, *pairs));
         r->put("cands", D(cands->pairing));
         r->put("cand_offsets", D(cands->pairingOffsets));
         lap("predict");
         if (!want(4)) { diag(); *out = guard.release(); return TRAX_OK; }
-        std::unique_ptr<TrackletCollection> tr(
-            TripletThetaPhiFilter(c).run(dh, grid, *pairs, *cands, (int)nt, cm, cfg->trip, true));
+        std::unique_ptr<TrackletCollection> tr(TripletThetaPhiFilter(c).run(hits, grid, *pairs, *cands, (int)nt, trip));
         pairs.reset();  // TripletBuilder.cpp:357-358
         cands.reset();
         r->put("tracklets.h1", tr->h1); r->put("tracklets.h2", tr->h2); r->put("tracklets.h3", tr->h3);
@@ -500,30 +500,37 @@ static int run_plain_impl(trax_ctx* ctx, const trax_config* cfg, const trax_hits
         lap("filter");
         if (collect) diag();
         if (!want(5)) { *out = guard.release(); return TRAX_OK; }
-        ConnectivityOut co = TripletConnectivityTight(c).run(dh, *tr, cm, cfg->trip, cfg->d_eta_cut, nt);
-        if (co.eta && collect) r->putf("eta", co.eta->download(c));
+        // TripletBuilder.cpp:395-469: connectivity, circle fit, CA; the caller owns what run returns
+        ConnectivityDiag cdiag;
+        UVec *basis_p, *foll_p, *connectable_p, *nonconnectable_p;
+        std::tie(basis_p, foll_p, connectable_p, nonconnectable_p) =
+            TripletConnectivityTight(c).run(hits, *tr, trip, cfg->d_eta_cut, nt, false, &cdiag);
+        std::unique_ptr<UVec> basis(basis_p), foll(foll_p), connectable(connectable_p),
+            nonconnectable(nonconnectable_p);
+        if (cdiag.eta && collect) r->putf("eta", cdiag.eta->download(c));
         else r->putf("eta", {});
-        auto dl = [&](const std::unique_ptr<UVec>& v, uint32_t n, bool always = false) {
-            std::vector<uint32_t> a = (v && (collect || always)) ? v->download(c) : std::vector<uint32_t>();
-            a.resize(std::min<size_t>(a.size(), n));
-            return a;
+        auto dl = [&](const UVec* v, bool always = false) {
+            return (v && (collect || always)) ? v->download(c) : std::vector<uint32_t>();
         };
-        r->put("conn.basis", dl(co.basis, co.nConn));
-        r->put("conn.followers", dl(co.followers, co.nConn));
-        r->put("conn.event", dl(co.event, co.nConn));
-        r->put("conn.connectable", dl(co.connectable, co.nConnectable));
-        r->put("conn.nonconnectable", dl(co.nonconnectable, co.nNonConnectable));
+        r->put("conn.basis", dl(basis.get()));
+        r->put("conn.followers", dl(foll.get()));
+        r->put("conn.event", dl(cdiag.event.get()));
+        r->put("conn.connectable", dl(connectable.get()));
+        r->put("conn.nonconnectable", dl(nonconnectable.get()));
         lap("connectivity");
-        if (co.nConn == 0) { *out = guard.release(); return TRAX_OK; }  // TripletBuilder.cpp:414-422
-        std::unique_ptr<FVec> pt(TrackletCircleFitter(c).run(dh, *tr, *co.connectable, co.nConnectable, nt));
+        if (!basis) { *out = guard.release(); return TRAX_OK; }  // TripletBuilder.cpp:414-422
+        std::unique_ptr<FVec> pt(TrackletCircleFitter(c).run(hits, *tr, *connectable, nt));
         lap("circle");
-        CAOut ca = CellularAutomaton(c).run(*co.basis, *co.followers, co.nConn, *pt, tr->size(), nt);
+        CADiag adiag;
+        const UVec *tracks_p, *track_off_p;
+        std::tie(tracks_p, track_off_p) = CellularAutomaton(c).run(*basis, *foll, *pt, *connectable, nt, false, &adiag);
+        std::unique_ptr<const UVec> tracks(tracks_p), track_off(track_off_p);
         r->putf("pt", pt->download(c));  // TripletBuilder.cpp:474-481
-        if (collect) r->put("ca.states", ca.states->download(c));
-        if (collect) r->put("ca.best_basis", ca.bestBasis->download(c));
-        r->put("tracks", dl(ca.trackCollection, ca.nTrackEntries, true));
-        r->put("track_offsets", ca.trackOffsets->download(c));
-        r->put("ca.iterations", {ca.iterations});
+        if (collect) r->put("ca.states", adiag.states->download(c));
+        if (collect) r->put("ca.best_basis", adiag.bestBasis->download(c));
+        r->put("tracks", tracks->download(c));
+        r->put("track_offsets", track_off->download(c));
+        r->put("ca.iterations", {adiag.iterations});
         lap("ca");
         *out = guard.release();
         return TRAX_OK;
@@ -635,7 +642,7 @@ void trax_batch::enqueue_stage(int s) {
         case 0:  // grid_count: the whole grid build (count, scan, stable scatter) in one kernel
             ok(launch_fill(reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords, 0u,
                            sizeof(DevCounts) / 4 - kCountsShapeWords, st), "reset");
-            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st),
+            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st, grid_rec),
                "grid build");
             break;
         case 1:  // grid_store: part of the grid build above
@@ -684,10 +691,12 @@ void trax_batch::enqueue_stage(int s) {
         case 9:  // filter_store
             if (caps.fuse_filter)
                 ok(launch_filter_offsets(tc, R, hout.layer, hout.evt, th1, t_off, cnt, lc, st), "filter offsets");
-            else
+            else {
                 ok(launch_filter_store(tc, Q, R, hout.layer, hout.evt, pairs, cands, bitmap, word_counts, th1, th2,
                                        th3, t_off, cnt, lc, st),
                    "filter store");
+                ok(launch_filter_offsets(tc, R, hout.layer, hout.evt, th1, t_off, cnt, lc, st), "filter offsets");
+            }
             break;
         case 10:  // link_prepare
             ok(launch_link_prepare(tc, N, R, hout.x, hout.y, hout.z, th1, th2, th3, lb, cnt, scan, lc, st),
@@ -796,7 +805,8 @@ void trax_batch::layout(Arena& a) {
         }
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
-        B.grid_perm = a.take<uint32_t>(N);
+        B.grid_perm = a.take<uint32_t>(N + (uint64_t)E * L);  // scatter scratch, then the item order
+        B.grid_rec = reinterpret_cast<uint4*>(a.take<uint32_t>(8ull * N));  // the hits as 32-byte records
         B.seg_base = a.take<uint32_t>(E * T + 1);
         B.slot_counts = a.take<uint32_t>(slot_cap + 1);
         B.slot_seg = a.take<uint32_t>(slot_seg_entries(slot_cap));
@@ -1082,6 +1092,9 @@ static int batch_result_impl(trax_batch* b, const char* names, trax_result** out
         if (want("pair_offsets")) r->put("pair_offsets", dl(b->pair_off, (uint64_t)E * T + 1));
         if (want("cands") && !b->caps.fuse_filter) r->put("cands", dl(b->cands, 2ull * Q));
         if (want("cand_offsets")) r->put("cand_offsets", dl(b->cand_off, (uint64_t)E * T + 1));
+        // the filter's decision per stored candidate: bit i % 64 of u64 word i / 64, as u32 halves
+        if (want("filter.bits") && !b->caps.fuse_filter)
+            r->put("filter.bits", dl(reinterpret_cast<const uint32_t*>(b->bitmap), 2ull * ((Q + 63) / 64)));
         if (want("tracklets.h1")) r->put("tracklets.h1", dl(b->th1, R)); if (want("tracklets.h2")) r->put("tracklets.h2", dl(b->th2, R)); if (want("tracklets.h3")) r->put("tracklets.h3", dl(b->th3, R));
         if (want("tracklet_offsets")) r->put("tracklet_offsets", dl(b->t_off, (uint64_t)E * T + 1));
         if (want("eta")) r->putf("eta", dlf(b->lb.eta, R));

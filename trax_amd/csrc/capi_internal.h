@@ -75,6 +75,27 @@ extern const StageRange kStageRanges[6];
 
 void validate_config(const struct ::trax_config* cfg);
 
+// launch shapes of a trax_tuning (NULL: the defaults)
+inline trax::LaunchCfg launch_cfg(const trax_tuning* t) {
+    trax::LaunchCfg lc;
+    if (!t) return lc;
+    lc.grid_count_block = t->grid_count_block;
+    lc.grid_store_block = t->grid_store_block;
+    lc.pair_block = t->pair_block;
+    lc.predict_block = t->predict_block;
+    lc.filter_block = t->filter_block;
+    lc.link_block = t->link_block;
+    lc.blocks_per_cu = t->blocks_per_cu;
+    lc.predict_list = t->predict_list;
+    lc.predict_sort = t->predict_sort;
+    lc.predict_fine = t->predict_fine;
+    lc.grid_lds_hits = t->grid_lds_hits;
+    lc.grid_order = t->grid_order;
+    lc.grid_rank = t->grid_rank;
+    lc.grid_gather = t->grid_gather;
+    return lc;
+}
+
 }  // namespace trax_capi
 
 struct trax_result {
@@ -111,6 +132,9 @@ struct trax_hits {
     uint32_t n_events = 0;
     trax::HitCollection hits;
     std::vector<uint32_t> evt_nhits, evt_offset, lay_nhits, lay_offset;
+    // the findable tracks of every event with their hits in layer order, as
+    // HitCollection::addEvent returns them (the mcTruth of PhysicsRecord::fillData)
+    std::vector<trax::HitCollection::tTrackList> truth;
 };
 
 inline hipStream_t pick_stream(trax_ctx* ctx, trax_stream* s) { return s ? s->s : ctx->c->stream(); }
@@ -144,7 +168,8 @@ struct trax_batch {
     trax::LinkBufs lb;
     trax::CABufs cb;
     trax::FineBufs fine;  // the fused predictor's walk structure
-    uint32_t* grid_perm = nullptr;  // the grid build's scatter scratch (N)
+    uint32_t* grid_perm = nullptr;  // the grid build's scatter scratch (N), then its item order (E * L)
+    uint4* grid_rec = nullptr;      // the grid build's hit records (2 N)
     hipGraph_t graphs[trax_capi::kNumStages] = {};
     hipGraphExec_t execs[trax_capi::kNumStages] = {};
     bool captured = false;

@@ -37,8 +37,8 @@ Context::~Context() {
 const ScanScratch& Context::scan_scratch(uint64_t n_cap) {
     const uint32_t tiles = scan_tiles(n_cap);
     if (tiles > scan_.max_tiles) {
-        if (scan_.status) {
-            finish_default_queue();
+        if (scan_.status) {  // a scan on any stream may still use it
+            hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
             hip_check(hipFree(scan_.status), "hipFree(scan)");
         }
         const uint32_t t = std::max<uint32_t>(tiles, 1024);
@@ -405,9 +405,33 @@ void PrefixSum::run(uint32_t* input, uint size, uint nThreads, uint32_t* d_total
     hip_check(launch_scan(input, nullptr, size, size, d_total, ctx.scan_scratch(size), ctx.stream()), "scan");
 }
 
-uint64_t GridBuilder::run(HitCollection& hits, HitMirror& dhits, uint nThreads, const EventSupplement& es,
-                          const LayerSupplement& ls, Grid& grid, bool download) {
+void HitTransfer::initBuffers(Context& ctx, const HitCollection& hits) { m.reset(new HitMirror(ctx, hits.size())); }
+
+void HitTransfer::toDevice(Context& ctx, const HitCollection& hits) {
+    if (!m || m->n_ != hits.size()) initBuffers(ctx, hits);
+    m->upload(ctx, hits);
+}
+
+void HitTransfer::fromDevice(Context& ctx, HitCollection& hits) const { mirror().download(ctx, hits); }
+
+HitMirror& HitTransfer::mirror() const {
+    if (!m) throw InputError("the hit collection has no device buffers (transfer.initBuffers / toDevice first)");
+    return *m;
+}
+
+uint64_t GridBuilder::run(HitCollection& hits, uint nThreads, const EventSupplement& es, const LayerSupplement& ls,
+                          Grid& grid, bool printProlix) {
     (void)nThreads;
+    (void)printProlix;
+    build(hits, hits.transfer.mirror(), es, ls, grid);
+    grid.host = grid.boundary.download(ctx);   // GridBuilder.cpp:64
+    hits.transfer.fromDevice(ctx, hits);       // GridBuilder.cpp:151
+    return 0;
+}
+
+void GridBuilder::build(HitCollection& hits, HitMirror& dhits, const EventSupplement& es, const LayerSupplement& ls,
+                        Grid& grid) {
+
     const GridConfig& gc = grid.config;
     const uint E = gc.nEvents, L = gc.nLayers, N = hits.size();
     if (es.nEvents != E || ls.nLayers != L) throw InputError("supplement shape does not match the grid");
@@ -446,16 +470,13 @@ uint64_t GridBuilder::run(HitCollection& hits, HitMirror& dhits, uint nThreads, 
     ctx.finish_default_queue();  // GridBuilder.cpp:146
     ctx.check_errors();
     dhits.swap(out);  // the old buffers are released with `out`
-    if (download) {
-        grid.host = grid.boundary.download(ctx);  // GridBuilder.cpp:64
-        dhits.download(ctx, hits);                // GridBuilder.cpp:151
-    }
-    return 0;
 }
 
-Pairing* PairGeneratorBeamspot::run(const HitMirror& hits, const ConfigMirror& cfg, uint nThreads,
-                                    const TripletConfigurations& layerTriplets, const Grid& grid) {
-    (void)nThreads;
+
+Pairing* PairGeneratorBeamspot::run(HitCollection& hits, const GeometrySupplement& geomSupplement, uint nThreads,
+                                    const TripletConfigurations& layerTriplets, const Grid& grid, bool printProlix) {
+    (void)printProlix;
+    const ConfigMirror cfg(ctx, layerTriplets, geomSupplement, nThreads, 0.0f);
     const uint E = grid.config.nEvents, T = layerTriplets.size();
     const GridGeom g = grid.config.geom();
     const TripletCols tc = cfg.cols();
@@ -466,7 +487,7 @@ Pairing* PairGeneratorBeamspot::run(const HitMirror& hits, const ConfigMirror& c
     PrefixSum(ctx).run(seg_base.get_mem(), nseg + 1, nThreads, &cnt->n_pair_slots);
     const uint nslot = ctx.download_count(&cnt->n_pair_slots);
     UVec slot_counts(ctx, nslot + 1ull);
-    const HitColumns hc = hits.cols();
+    const HitColumns hc = hits.transfer.mirror().cols();
     hip_check(launch_pair_count(g, tc, nseg, nslot, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.zphi, seg_base.get_mem(),
                                 slot_counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "pair count");
@@ -481,9 +502,12 @@ Pairing* PairGeneratorBeamspot::run(const HitMirror& hits, const ConfigMirror& c
     return p;
 }
 
-Pairing* TripletThetaPhiPredictor::run(const HitMirror& hits, const DetectorGeometry&, const ConfigMirror& cfg,
-                                       const Dictionary&, int nThreads, const TripletConfigurations& layerTriplets,
-                                       const Grid& grid, const Pairing& pairs) {
+Pairing* TripletThetaPhiPredictor::run(HitCollection& hits, const DetectorGeometry&,
+                                       const GeometrySupplement& geomSupplement, const Dictionary&, int nThreads,
+                                       const TripletConfigurations& layerTriplets, const Grid& grid,
+                                       const Pairing& pairs, bool printProlix) {
+    (void)printProlix;
+    const ConfigMirror cfg(ctx, layerTriplets, geomSupplement, (uint)nThreads, 0.0f);
     const uint E = grid.config.nEvents, T = layerTriplets.size();
     const GridGeom g = grid.config.geom();
     const TripletCols tc = cfg.cols();
@@ -491,7 +515,7 @@ Pairing* TripletThetaPhiPredictor::run(const HitMirror& hits, const DetectorGeom
     const uint nPairs = pairs.n;
     ctx.set_count(&cnt->n_pairs, pairs.n);
     UVec counts(ctx, nPairs + 1ull);
-    const HitColumns hc = hits.cols();
+    const HitColumns hc = hits.transfer.mirror().cols();
     hip_check(launch_predict_count(g, tc, nPairs, grid.boundary.get_mem(), hc.x, hc.y, hc.z, hc.zphi, hc.layer, hc.evt,
                                    pairs.pairing.get_mem(), counts.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "predict count");
@@ -503,13 +527,17 @@ Pairing* TripletThetaPhiPredictor::run(const HitMirror& hits, const DetectorGeom
                                    hc.evt, pairs.pairing.get_mem(), counts.get_mem(), c->pairing.get_mem(),
                                    c->pairingOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "predict store");
+    // predictOffsetMonotonizeStore (TripletThetaPhiPredictor.h:432-447)
+    hip_check(launch_forward_fill(c->pairingOffsets.get_mem(), cnt, T, ctx.stream()), "offsets fill");
     ctx.finish_default_queue();
     return c;
 }
 
-TrackletCollection* TripletThetaPhiFilter::run(const HitMirror& hits, const Grid& grid, const Pairing& pairs,
-                                               const Pairing& cands, int nThreads, const ConfigMirror& cfg,
-                                               const TripletConfigurations& layerTriplets, bool download) {
+TrackletCollection* TripletThetaPhiFilter::run(HitCollection& hits, const Grid& grid, const Pairing& pairs,
+                                               const Pairing& cands, int nThreads,
+                                               const TripletConfigurations& layerTriplets, bool printProlix) {
+    (void)printProlix;
+    const ConfigMirror cfg(ctx, layerTriplets, GeometrySupplement(), (uint)nThreads, 0.0f);
     const uint E = grid.config.nEvents, T = layerTriplets.size();
     const TripletCols tc = cfg.cols();
     DevCounts* cnt = ctx.counts();
@@ -518,7 +546,7 @@ TrackletCollection* TripletThetaPhiFilter::run(const HitMirror& hits, const Grid
     ctx.set_count(&cnt->n_cands, cands.n);
     DeviceVector<uint64_t> bitmap(ctx, nWords ? nWords : 1);
     UVec word_counts(ctx, nWords + 1ull);
-    const HitColumns hc = hits.cols();
+    const HitColumns hc = hits.transfer.mirror().cols();
     hip_check(launch_filter_count(tc, nCand, hc.x, hc.y, hc.z, hc.layer, pairs.pairing.get_mem(),
                                   cands.pairing.get_mem(), bitmap.get_mem(), word_counts.get_mem(), cnt,
                                   ctx.launch(), ctx.stream()),
@@ -531,98 +559,107 @@ TrackletCollection* TripletThetaPhiFilter::run(const HitMirror& hits, const Grid
                                   bitmap.get_mem(), word_counts.get_mem(), t->hit1.get_mem(), t->hit2.get_mem(),
                                   t->hit3.get_mem(), t->trackletOffsets.get_mem(), cnt, ctx.launch(), ctx.stream()),
               "filter store");
+    // filterOffsetStore + filterOffsetMonotonizeStore (TripletThetaPhiFilter.h:221-280)
+    hip_check(launch_filter_offsets(tc, R, hc.layer, hc.evt, t->hit1.get_mem(), t->trackletOffsets.get_mem(), cnt,
+                                    ctx.launch(), ctx.stream()),
+              "filter offsets");
     ctx.finish_default_queue();
-    if (download) {  // TripletThetaPhiFilter.cpp:184
-        t->h1 = t->hit1.download(ctx);
-        t->h2 = t->hit2.download(ctx);
-        t->h3 = t->hit3.download(ctx);
-    }
+    t->h1 = t->hit1.download(ctx);  // TripletThetaPhiFilter.cpp:184
+    t->h2 = t->hit2.download(ctx);
+    t->h3 = t->hit3.download(ctx);
     return t;
 }
 
-ConnectivityOut TripletConnectivityTight::run(const HitMirror& hits, TrackletCollection& tr, const ConfigMirror& cfg,
-                                              const TripletConfigurations& layerTriplets, float dEtaCut,
-                                              uint nThreads) {
-    (void)layerTriplets;
-    ConnectivityOut out;
-    TripletCols tc = cfg.cols();
+std::tuple<UVec*, UVec*, UVec*, UVec*> TripletConnectivityTight::run(const HitCollection& hits, TrackletCollection& tr,
+                                                                    const TripletConfigurations& layerTriplets,
+                                                                    float dEtaCut, uint nThreads, bool printPROLIX,
+                                                                    ConnectivityDiag* diag) const {
+    (void)printPROLIX;
+    TripletCols tc{};  // the connectivity kernels read only the triplet count and dEtaCut
+    tc.n_trip = layerTriplets.size();
     tc.d_eta_cut = dEtaCut;
+    tc.n_threads = nThreads;
     DevCounts* cnt = ctx.counts();
     const uint R = tr.size();
-    const uint N = hits.n_;
+    const HitMirror& dh = hits.transfer.mirror();
+    const uint N = dh.n_;
     ctx.set_count(&cnt->n_tracklets, tr.n);
-    if (R == 0) return out;
-    out.eta.reset(new FVec(ctx, R));
+    std::tuple<UVec*, UVec*, UVec*, UVec*> none(nullptr, nullptr, nullptr, nullptr);
+    if (R == 0) return none;
+    std::unique_ptr<FVec> eta(new FVec(ctx, R));
     UVec hit_count(ctx, N + 1ull), hit_cursor(ctx, N ? N : 1), csr(ctx, R), conn(ctx, R + 1ull),
         counts(ctx, R + 1ull), flag_a(ctx, R + 1ull), flag_b(ctx, R + 1ull);
     LinkBufs b;
-    b.eta = out.eta->get_mem();
+    b.eta = eta->get_mem();
     b.hit_count = hit_count.get_mem(); b.hit_cursor = hit_cursor.get_mem(); b.csr = csr.get_mem();
     b.conn = conn.get_mem(); b.counts = counts.get_mem(); b.flag_a = flag_a.get_mem(); b.flag_b = flag_b.get_mem();
-    const HitColumns hc = hits.cols();
+    const HitColumns hc = dh.cols();
+    // tripletEtaCalculatorStore (.h:47-79) and the per-hit buckets of the tracklets
     hip_check(launch_link_prepare(tc, N, R, hc.x, hc.y, hc.z, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(),
-                                  b, cnt,
-                                  ctx.scan_scratch(N + 1ull), ctx.launch(), ctx.stream()),
+                                  b, cnt, ctx.scan_scratch(N + 1ull), ctx.launch(), ctx.stream()),
               "link prepare");
+    if (diag) diag->eta = std::move(eta);
     hip_check(launch_link_count(tc, R, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(),
                                 tr.trackletOffsets.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
               "link count");
     PrefixSum(ctx).run(counts.get_mem(), R + 1, nThreads, &cnt->n_conn);
-    out.nConn = ctx.download_count(&cnt->n_conn);  // TripletConnectivityTight.cpp:89
-    if (out.nConn == 0) return out;                // .cpp:93-95: null tuple
-    out.basis.reset(new UVec(ctx, out.nConn));
-    out.followers.reset(new UVec(ctx, out.nConn));
-    out.event.reset(new UVec(ctx, out.nConn));
-    b.basis = out.basis->get_mem(); b.foll = out.followers->get_mem(); b.pev = out.event->get_mem();
-    hip_check(launch_link_store(tc, R, out.nConn, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(),
+    const uint nConn = ctx.download_count(&cnt->n_conn);  // TripletConnectivityTight.cpp:89
+    if (nConn == 0) return none;                           // .cpp:93-95: null tuple
+    std::unique_ptr<UVec> basis(new UVec(ctx, nConn)), foll(new UVec(ctx, nConn)), pev(new UVec(ctx, nConn));
+    b.basis = basis->get_mem(); b.foll = foll->get_mem(); b.pev = pev->get_mem();
+    hip_check(launch_link_store(tc, R, nConn, hc.layer, hc.evt, tr.hit1.get_mem(), tr.hit2.get_mem(),
                                 tr.hit3.get_mem(), tr.trackletOffsets.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
               "link store");
+    // predicateInversionStore + streamCompactionGetValidIndexesStore (.h:251-346, .cpp:184-256)
     hip_check(launch_link_flags(R, hc.evt, tr.hit1.get_mem(), b, cnt, ctx.launch(), ctx.stream()), "link flags");
     PrefixSum(ctx).run(flag_a.get_mem(), R + 1, nThreads, &cnt->n_connectable);
-    out.nConnectable = ctx.download_count(&cnt->n_connectable);  // .cpp:200
+    const uint nConnectable = ctx.download_count(&cnt->n_connectable);  // .cpp:200
     PrefixSum(ctx).run(flag_b.get_mem(), R + 1, nThreads, &cnt->n_nonconnectable);
-    out.nNonConnectable = ctx.download_count(&cnt->n_nonconnectable);  // .cpp:211
-    out.connectable.reset(new UVec(ctx, std::max<uint>(out.nConnectable, 1)));
-    out.nonconnectable.reset(new UVec(ctx, std::max<uint>(out.nNonConnectable, 1)));
-    b.connectable = out.connectable->get_mem();
-    b.nonconnectable = out.nonconnectable->get_mem();
+    const uint nNonConnectable = ctx.download_count(&cnt->n_nonconnectable);  // .cpp:211
+    std::unique_ptr<UVec> connectable(new UVec(ctx, nConnectable)), nonconnectable(new UVec(ctx, nNonConnectable));
+    b.connectable = connectable->get_mem();
+    b.nonconnectable = nonconnectable->get_mem();
     hip_check(launch_link_compact(R, b, cnt, ctx.launch(), ctx.stream()), "link compact");
     ctx.finish_default_queue();
     ctx.check_errors();
-    return out;
+    if (diag) diag->event = std::move(pev);
+    return std::make_tuple(basis.release(), foll.release(), connectable.release(), nonconnectable.release());
 }
 
-FVec* TrackletCircleFitter::run(const HitMirror& hits, const TrackletCollection& tr, const UVec& valid, uint nValid,
-                                uint nThreads) {
+FVec* TrackletCircleFitter::run(const HitCollection& hits, const TrackletCollection& tr, const UVec& valid,
+                                uint nThreads, bool printPROLIX) const {
     (void)nThreads;
+    (void)printPROLIX;
     DevCounts* cnt = ctx.counts();
-    FVec* pt = new FVec(ctx, tr.size() ? tr.size() : 1);
-    ctx.set_count(&cnt->n_connectable, nValid);
+    std::unique_ptr<FVec> pt(new FVec(ctx, tr.size()));
+    ctx.set_count(&cnt->n_tracklets, tr.n);
+    ctx.set_count(&cnt->n_connectable, (uint)valid.get_count());
     LinkBufs b;
     b.connectable = valid.get_mem();
     b.pt = pt->get_mem();
-    const HitColumns hc = hits.cols();
+    const HitColumns hc = hits.transfer.mirror().cols();
     hip_check(launch_circle_fit(tr.size(), hc.x, hc.y, tr.hit1.get_mem(), tr.hit2.get_mem(), tr.hit3.get_mem(), b,
                                 cnt, ctx.launch(), ctx.stream()),
               "circle fit");
     ctx.finish_default_queue();
-    return pt;
+    return pt.release();
 }
 
-CAOut CellularAutomaton::run(const UVec& basis, const UVec& followers, uint nConn, const FVec& pt, uint R,
-                             uint nThreads) {
-    CAOut out;
+std::tuple<const UVec*, const UVec*> CellularAutomaton::run(const UVec& basis, const UVec& followers, const FVec& pt,
+                                                           const UVec& connectableTriplets, uint nThreads,
+                                                           bool printPROLIX, CADiag* diag) const {
+    (void)connectableTriplets;  // the reference uses it only for debugging output (CellularAutomaton.cpp:11)
+    (void)printPROLIX;
+    const uint nConn = (uint)basis.get_count(), R = (uint)pt.get_count();  // CellularAutomaton.cpp:29-30
     DevCounts* cnt = ctx.counts();
     ctx.set_count(&cnt->n_tracklets, R);
     ctx.set_count(&cnt->n_conn, nConn);
-    out.states.reset(new UVec(ctx, R));
+    std::unique_ptr<UVec> states(new UVec(ctx, R)), bestBasis(new UVec(ctx, R)), trackOffsets(new UVec(ctx, R + 1ull));
     UVec st_b(ctx, R), flags(ctx, 3), fbc(ctx, R + 1ull), cursor(ctx, R), isbest(ctx, R);
-    out.trackOffsets.reset(new UVec(ctx, R + 1ull));
-    out.bestBasis.reset(new UVec(ctx, R));
     CABufs b;
-    b.st_a = out.states->get_mem(); b.st_b = st_b.get_mem(); b.flags = flags.get_mem();
-    b.fbc = fbc.get_mem(); b.cursor = cursor.get_mem(); b.best = out.bestBasis->get_mem();
-    b.isbest = isbest.get_mem(); b.handler = out.trackOffsets->get_mem();
+    b.st_a = states->get_mem(); b.st_b = st_b.get_mem(); b.flags = flags.get_mem();
+    b.fbc = fbc.get_mem(); b.cursor = cursor.get_mem(); b.best = bestBasis->get_mem();
+    b.isbest = isbest.get_mem(); b.handler = trackOffsets->get_mem();
     hip_check(launch_ca_init(R, b, cnt, ctx.stream()), "ca init");
     // forward phase: one iteration per launch, living cells read back each time
     // (CellularAutomaton.cpp:40-76)
@@ -635,7 +672,7 @@ CAOut CellularAutomaton::run(const UVec& basis, const UVec& followers, uint nCon
         ++k;
         if (!alive) break;
     }
-    out.iterations = ctx.download_count(&cnt->ca_iterations);
+    const uint iterations = ctx.download_count(&cnt->ca_iterations);
     hip_check(launch_ca_basis_count(R, nConn, basis.get_mem(), followers.get_mem(), b, cnt, ctx.launch(), ctx.stream()),
               "ca basis count");
     PrefixSum(ctx).run(fbc.get_mem(), R + 1, nThreads, &cnt->n_basis);
@@ -648,14 +685,19 @@ CAOut CellularAutomaton::run(const UVec& basis, const UVec& followers, uint nCon
                                     ctx.launch(), ctx.stream()),
               "ca basis store");
     PrefixSum(ctx).run(b.handler, R + 1, nThreads, &cnt->n_track_entries);
-    out.nTrackEntries = ctx.download_count(&cnt->n_track_entries);  // CellularAutomaton.cpp:284
-    out.trackCollection.reset(new UVec(ctx, out.nTrackEntries ? out.nTrackEntries : 1));
-    b.tracks = out.trackCollection->get_mem();
-    hip_check(launch_ca_tracks(R, out.nTrackEntries, b, cnt, ctx.scan_scratch(R + 1ull), ctx.launch(), ctx.stream(), 0),
+    const uint nTrackEntries = ctx.download_count(&cnt->n_track_entries);  // CellularAutomaton.cpp:284
+    std::unique_ptr<UVec> tracks(new UVec(ctx, nTrackEntries));
+    b.tracks = tracks->get_mem();
+    hip_check(launch_ca_tracks(R, nTrackEntries, b, cnt, ctx.scan_scratch(R + 1ull), ctx.launch(), ctx.stream(), 0),
               "ca tracks");
     ctx.finish_default_queue();
     ctx.check_errors();
-    return out;
+    if (diag) {
+        diag->states = std::move(states);
+        diag->bestBasis = std::move(bestBasis);
+        diag->iterations = iterations;
+    }
+    return std::make_tuple(tracks.release(), trackOffsets.release());
 }
 
 }  // namespace trax

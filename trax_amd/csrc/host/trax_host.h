@@ -190,6 +190,20 @@ struct LayerSupplement {
     std::vector<uint> nHits, offset;
 };
 
+struct HitMirror;
+class HitCollection;
+// clever::OpenCLTransfer<...> of a HitCollection (HitCollection.h:58): the
+// device buffers of its columns.  initBuffers allocates them for the
+// collection's size, toDevice / fromDevice copy; the GridBuilder replaces
+// them with the grid-ordered columns (GridBuilder.cpp:70,157).
+struct HitTransfer {
+    void initBuffers(Context& ctx, const HitCollection& hits);
+    void toDevice(Context& ctx, const HitCollection& hits);
+    void fromDevice(Context& ctx, HitCollection& hits) const;
+    HitMirror& mirror() const;  // InputError before initBuffers
+    std::shared_ptr<HitMirror> m;
+};
+
 // HitCollection (src/datastructures/HitCollection.h:28-62)
 class HitCollection {
 public:
@@ -201,6 +215,7 @@ public:
                         int numTracks = -1, bool onlyTracks = false);
     std::vector<float> x, y, z;
     std::vector<uint> layer, detId, hitId, event;
+    HitTransfer transfer;
 };
 
 // GridConfig / Grid (src/datastructures/Grid.h:5-131)
@@ -284,69 +299,93 @@ public:
 class GridBuilder {  // src/algorithms/GridBuilder.h:25-46
 public:
     explicit GridBuilder(Context& c) : ctx(c) {}
-    uint64_t run(HitCollection& hits, HitMirror& dhits, uint nThreads, const EventSupplement& eventSupplement,
-                 const LayerSupplement& layerSupplement, Grid& grid, bool download = true);
+    // GridBuilder.h:42: count, scan and stable scatter of hits.transfer's
+    // columns into the grid; the transfer then holds the grid-ordered
+    // columns, and the grid and the hits are downloaded (GridBuilder.cpp:64,151)
+    uint64_t run(HitCollection& hits, uint nThreads, const EventSupplement& eventSupplement,
+                 const LayerSupplement& layerSupplement, Grid& grid, bool printProlix = false);
     Context& ctx;
+
+private:
+    void build(HitCollection& hits, HitMirror& dhits, const EventSupplement& es, const LayerSupplement& ls,
+               Grid& grid);
 };
 
 class PairGeneratorBeamspot {  // src/algorithms/PairGeneratorBeamspot.h:24-41
 public:
     explicit PairGeneratorBeamspot(Context& c) : ctx(c) {}
-    Pairing* run(const HitMirror& hits, const ConfigMirror& cfg, uint nThreads, const TripletConfigurations& layerTriplets,
-                 const Grid& grid);
+    // PairGeneratorBeamspot.h:40; nThreads fixes the pair order (the
+    // reference's work-item order)
+    Pairing* run(HitCollection& hits, const GeometrySupplement& geomSupplement, uint nThreads,
+                 const TripletConfigurations& layerTriplets, const Grid& grid, bool printProlix = false);
     Context& ctx;
 };
 
 class TripletThetaPhiPredictor {  // src/algorithms/TripletThetaPhiPredictor.h:44-47
 public:
     explicit TripletThetaPhiPredictor(Context& c) : ctx(c) {}
-    Pairing* run(const HitMirror& hits, const DetectorGeometry& geom, const ConfigMirror& cfg, const Dictionary& dict,
-                 int nThreads, const TripletConfigurations& layerTriplets, const Grid& grid, const Pairing& pairs);
+    Pairing* run(HitCollection& hits, const DetectorGeometry& geom, const GeometrySupplement& geomSupplement,
+                 const Dictionary& dict, int nThreads, const TripletConfigurations& layerTriplets, const Grid& grid,
+                 const Pairing& pairs, bool printProlix = false);
     Context& ctx;
 };
 
 class TripletThetaPhiFilter {  // src/algorithms/TripletThetaPhiFilter.h:46-48
 public:
     explicit TripletThetaPhiFilter(Context& c) : ctx(c) {}
-    TrackletCollection* run(const HitMirror& hits, const Grid& grid, const Pairing& pairs,
-                            const Pairing& tripletCandidates, int nThreads, const ConfigMirror& cfg,
-                            const TripletConfigurations& layerTriplets, bool download = true);
+    // the tracklets' hit columns are downloaded into h1..h3 (TripletThetaPhiFilter.cpp:184)
+    TrackletCollection* run(HitCollection& hits, const Grid& grid, const Pairing& pairs,
+                            const Pairing& tripletCandidates, int nThreads, const TripletConfigurations& layerTriplets,
+                            bool printProlix = false);
     Context& ctx;
 };
 
-struct ConnectivityOut {
-    std::unique_ptr<UVec> basis, followers, event, connectable, nonconnectable;
+// what the reference computes inside TripletConnectivityTight::run and
+// CellularAutomaton::run but does not return (the eta of every tracklet, the
+// event of every connection; the CA states, best bases and iteration count),
+// kept when a caller asks for it
+struct ConnectivityDiag {
     std::unique_ptr<FVec> eta;
-    uint nConn = 0, nConnectable = 0, nNonConnectable = 0;
+    std::unique_ptr<UVec> event;
+};
+struct CADiag {
+    std::unique_ptr<UVec> states, bestBasis;
+    uint iterations = 0;
 };
 
-class TripletConnectivityTight {  // src/algorithms/TripletConnectivityTight.h:42-44
+class TripletConnectivityTight {  // src/algorithms/TripletConnectivityTight.h:25-44
 public:
     explicit TripletConnectivityTight(Context& c) : ctx(c) {}
-    // empty basis (null) when no connection exists, as the reference's nullptr tuple
-    ConnectivityOut run(const HitMirror& hits, TrackletCollection& tracklets, const ConfigMirror& cfg,
-                        const TripletConfigurations& layerTriplets, float dEtaCut, uint nThreads);
+    // TripletConnectivityTight.h:42: (basis, followers, connectable indices,
+    // non-connectable indices), owned by the caller, each sized exactly
+    // (get_count()); all null when no connection exists (.cpp:93-95)
+    std::tuple<UVec*, UVec*, UVec*, UVec*> run(const HitCollection& hits, TrackletCollection& tracklets,
+                                               const TripletConfigurations& layerTriplets, float dEtaCut,
+                                               uint nThreads, bool printPROLIX = false,
+                                               ConnectivityDiag* diag = nullptr) const;
     Context& ctx;
 };
 
-class TrackletCircleFitter {  // src/algorithms/TrackletCircleFitter.h:25-29
+class TrackletCircleFitter {  // src/algorithms/TrackletCircleFitter.h:17-29
 public:
     explicit TrackletCircleFitter(Context& c) : ctx(c) {}
-    FVec* run(const HitMirror& hits, const TrackletCollection& tracklets, const UVec& validTrackletsIndices,
-              uint nValid, uint nThreads);
+    // TrackletCircleFitter.h:25: the radius of every tracklet listed in
+    // validTrackletsIndices (all of its get_count() entries); one entry per
+    // tracklet, the others 0
+    FVec* run(const HitCollection& hits, const TrackletCollection& tracklets, const UVec& validTrackletsIndices,
+              uint nThreads, bool printPROLIX = false) const;
     Context& ctx;
 };
 
-struct CAOut {
-    std::unique_ptr<UVec> trackCollection, trackOffsets, states, bestBasis;
-    uint nTrackEntries = 0, iterations = 0;
-};
-
-class CellularAutomaton {  // src/algorithms/CellularAutomaton.h:43-49
+class CellularAutomaton {  // src/algorithms/CellularAutomaton.h:22-49
 public:
     explicit CellularAutomaton(Context& c) : ctx(c) {}
-    CAOut run(const UVec& tripletsBasis, const UVec& tripletsFollowers, uint nConn, const FVec& tripletsPt,
-              uint nTriplets, uint nThreads);
+    // CellularAutomaton.h:43-49: (tracks, track offsets), owned by the
+    // caller; the connections are tripletsBasis / tripletsFollowers (all of
+    // their entries), the tracklets those of tripletsPt
+    std::tuple<const UVec*, const UVec*> run(const UVec& tripletsBasis, const UVec& tripletsFollowers,
+                                             const FVec& tripletsPt, const UVec& connectableTriplets, uint nThreads,
+                                             bool printPROLIX = false, CADiag* diag = nullptr) const;
     Context& ctx;
 };
 

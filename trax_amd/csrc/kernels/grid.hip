@@ -166,6 +166,44 @@ __device__ inline void sort_cells(T* perm, const uint32_t* ends, uint32_t cells)
     }
 }
 
+// The stable position of the hit at unsorted position p of its cell [lo, hi)
+// (placed there by an atomic): lo + the number of the cell's hits with a
+// smaller input index.
+template <typename T>
+__device__ inline uint32_t stable_pos(const T* perm, uint32_t p, uint32_t lo, uint32_t hi) {
+    if (hi - lo < 2) return p;
+    const T v = perm[p];
+    uint32_t r = lo;
+    for (uint32_t k = lo; k < hi; ++k) r += perm[k] < v ? 1u : 0u;
+    return r;
+}
+
+// Items (event, layer) ordered by size, largest first (by the bit length of
+// the hit count: a coarse order, enough to start the long items first); one
+// block.  order[t] is the item the build's t-th block takes.
+__global__ __launch_bounds__(1024) void k_grid_order(const uint32_t* __restrict__ d_nevents, uint32_t n_layers,
+                                                     const uint32_t* __restrict__ lay_nhits,
+                                                     uint32_t* __restrict__ order, uint32_t n_items_cap) {
+    __shared__ uint32_t bucket[33];
+    const uint32_t n_items = min(*d_nevents * n_layers, n_items_cap);
+    if (threadIdx.x < 33) bucket[threadIdx.x] = 0;
+    __syncthreads();
+    // leading zeros of the hit count: 0 for the largest items, 32 for empty ones
+    auto key = [&](uint32_t item) { return (uint32_t)__clz((int)lay_nhits[item]); };
+    for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) atomicAdd(&bucket[key(i)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < 33; ++b) {
+            const uint32_t c = bucket[b];
+            bucket[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n_items; i += blockDim.x) order[atomicAdd(&bucket[key(i)], 1u)] = i;
+}
+
 constexpr uint32_t kGridUnroll = 4;       // hits per thread whose loads are issued together
 constexpr uint16_t kNoCell = 0xffffu;     // a hit outside the grid, in the LDS cell list
 
@@ -174,8 +212,11 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                                                    const uint32_t* __restrict__ lay_nhits,
                                                    const uint32_t* __restrict__ lay_offset, HitCols in,
                                                    HitColsOut out, uint32_t* __restrict__ grid,
-                                                   uint32_t* __restrict__ perm, uint32_t n_hits_cap,
-                                                   uint32_t lds_hits, DevCounts* __restrict__ cnt) {
+                                                   uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ order, uint4* __restrict__ rec,
+                                                   uint32_t n_hits_cap,
+                                                   uint32_t lds_hits, uint32_t rank_mode,
+                                                   DevCounts* __restrict__ cnt) {
     extern __shared__ uint32_t lds[];
     uint32_t* hist = lds;
     uint32_t* wsum = lds + g.cells;
@@ -183,7 +224,8 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
     uint16_t* lperm = lcell + lds_hits;
     const uint32_t n_items = *d_nevents * g.n_layers;
     const uint32_t bs = blockDim.x;
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    for (uint32_t t = blockIdx.x; t < n_items; t += gridDim.x) {
+        const uint32_t item = order ? order[t] : t;
         const uint32_t e = item / g.n_layers;
         for (uint32_t i = threadIdx.x; i < g.cells; i += bs) hist[i] = 0;
         __syncthreads();
@@ -198,6 +240,20 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
             for (uint32_t k = 0; k < kGridUnroll; ++k) {
                 const uint32_t i = i0 + k * bs;
                 if (i < n) { x[k] = in.x[off + i]; y[k] = in.y[off + i]; z[k] = in.z[off + i]; }
+            }
+            if (rec && fits) {
+                // each hit's seven columns as one 32-byte record, written in
+                // input order (coalesced) for the gather below
+#pragma unroll
+                for (uint32_t k = 0; k < kGridUnroll; ++k) {
+                    const uint32_t i = i0 + k * bs;
+                    if (i < n) {
+                        const size_t r = 2 * ((size_t)off + i);
+                        rec[r] = make_uint4(__float_as_uint(x[k]), __float_as_uint(y[k]), __float_as_uint(z[k]),
+                                            in.layer[off + i]);
+                        rec[r + 1] = make_uint4(in.det[off + i], in.id[off + i], in.evt[off + i], 0u);
+                    }
+                }
             }
 #pragma unroll
             for (uint32_t k = 0; k < kGridUnroll; ++k) {
@@ -224,39 +280,56 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                     if (c != kNoCell) lperm[atomicAdd(&hist[c], 1u)] = (uint16_t)i;
                 }
                 __syncthreads();
-                sort_cells(lperm, hist, g.cells);
+                if (rank_mode == 0) sort_cells(lperm, hist, g.cells);
             } else {
                 for (uint32_t i = threadIdx.x; i < n; i += bs) {
                     const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
                     if (c < g.cells) perm[off + atomicAdd(&hist[c], 1u)] = i;
                 }
                 __syncthreads();
-                sort_cells(perm + off, hist, g.cells);
+                if (rank_mode == 0) sort_cells(perm + off, hist, g.cells);
             }
-            __syncthreads();
+            if (rank_mode == 0) __syncthreads();
             const uint32_t placed = g.cells ? hist[g.cells - 1] : 0u;  // hits with a cell
             for (uint32_t p0 = threadIdx.x; p0 < placed; p0 += kGridUnroll * bs) {
-                uint32_t src[kGridUnroll];
+                uint32_t src[kGridUnroll], dst[kGridUnroll];
                 float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll];
                 uint32_t la[kGridUnroll], de[kGridUnroll], id[kGridUnroll], ev[kGridUnroll];
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
                     const uint32_t p = p0 + k * bs;
-                    src[k] = p < placed ? off + (local ? (uint32_t)lperm[p] : perm[off + p]) : off;
+                    src[k] = off;
+                    dst[k] = p;
+                    if (p < placed) {
+                        const uint32_t i = local ? (uint32_t)lperm[p] : perm[off + p];
+                        src[k] = off + i;
+                        if (rank_mode != 0) {
+                            // the cell of position p: its hit's cell (local) or recomputed
+                            const uint32_t c = local ? (uint32_t)lcell[i]
+                                                     : hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
+                            const uint32_t lo = c ? hist[c - 1] : 0u, hi = hist[c];
+                            dst[k] = local ? stable_pos(lperm, p, lo, hi) : stable_pos(perm + off, p, lo, hi);
+                        }
+                    }
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
                     if (p0 + k * bs < placed) {
                         const uint32_t i = src[k];
-                        x[k] = in.x[i]; y[k] = in.y[i]; z[k] = in.z[i];
-                        la[k] = in.layer[i]; de[k] = in.det[i]; id[k] = in.id[i]; ev[k] = in.evt[i];
+                        if (rec) {
+                            const uint4 a = rec[2 * (size_t)i], b = rec[2 * (size_t)i + 1];
+                            x[k] = __uint_as_float(a.x); y[k] = __uint_as_float(a.y); z[k] = __uint_as_float(a.z);
+                            la[k] = a.w; de[k] = b.x; id[k] = b.y; ev[k] = b.z;
+                        } else {
+                            x[k] = in.x[i]; y[k] = in.y[i]; z[k] = in.z[i];
+                            la[k] = in.layer[i]; de[k] = in.det[i]; id[k] = in.id[i]; ev[k] = in.evt[i];
+                        }
                     }
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
-                    const uint32_t p = p0 + k * bs;
-                    if (p < placed) {
-                        const uint32_t pos = off + p;
+                    if (p0 + k * bs < placed) {
+                        const uint32_t pos = off + dst[k];
                         out.x[pos] = x[k]; out.y[pos] = y[k]; out.z[pos] = z[k];
                         out.layer[pos] = la[k];
                         out.det[pos] = de[k];
@@ -277,17 +350,27 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
-                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             uint4* rec) {
     (void)grid_unused;
     const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 256;
     const uint32_t blocks = n_items_cap ? n_items_cap : 1;
     const size_t base = (size_t)(g.cells + bs / kWave + 1) * sizeof(uint32_t);
-    const uint32_t lds_hits = grid_lds_hits(g, bs);
+    uint32_t lds_hits = grid_lds_hits(g, bs);
+    if (lc.grid_lds_hits) lds_hits = std::min(lds_hits, lc.grid_lds_hits & ~63u);
     const size_t lds = base + (size_t)lds_hits * 2 * sizeof(uint16_t);
+    // perm holds n_hits_cap scatter entries, then the item order (n_items_cap)
+    uint32_t* order = nullptr;
+    if (lc.grid_order) {
+        order = perm + n_hits_cap;
+        hipLaunchKernelGGL(k_grid_order, dim3(1), dim3(1024), 0, st, &cnt->n_events, g.n_layers, lay_nhits, order,
+                           n_items_cap);
+    }
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
     HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};
     hipLaunchKernelGGL(k_grid_build, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset, lay_nhits,
-                       lay_offset, ic, oc, grid, perm, n_hits_cap, lds_hits, cnt);
+                       lay_offset, ic, oc, grid, perm, order, lc.grid_gather ? rec : nullptr, n_hits_cap, lds_hits,
+                       lc.grid_rank, cnt);
     return hipGetLastError();
 }
 

@@ -43,6 +43,10 @@ struct LaunchCfg {
     uint32_t predict_list = 0;   // fused predictor: LDS list entries per pair (0 = 24)
     uint32_t predict_sort = 0;   // fused predictor: walk a tile's pairs longest first
     uint32_t predict_fine = 0;   // fused predictor: z sub-rows per grid row (power of two; 0 = 8)
+    uint32_t grid_lds_hits = 0;  // grid build: hits of an (event, layer) kept in LDS (0 = as many as fit)
+    uint32_t grid_order = 0;     // grid build: 1 = the largest (event, layer) items first
+    uint32_t grid_rank = 0;      // grid build: 0 = cells sorted in LDS, 1 = each hit ranks itself in its cell
+    uint32_t grid_gather = 0;    // grid build: 1 = gather each hit as one 32-byte record written by the count pass
 };
 
 // The fused predictor's own walk structure: every z row of every (event,
@@ -112,6 +116,17 @@ constexpr uint64_t kScanTileElements = 8192;
 uint32_t scan_tiles(uint64_t n_cap);
 uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
+// writes of DevCounts fields (word index, value) by a kernel on the stream, in order
+struct CountWrites {
+    uint32_t n = 0;
+    uint32_t idx[12];
+    uint32_t val[12];
+    void put(const uint32_t* field, const DevCounts* base, uint32_t v) {
+        idx[n] = (uint32_t)(field - reinterpret_cast<const uint32_t*>(base));
+        val[n++] = v;
+    }
+};
+hipError_t launch_set_counts(DevCounts* cnt, const CountWrites& w, hipStream_t st);
 hipError_t launch_copy(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t st);
 // event column of a batch of whole events from its event offsets (sizes on device)
 hipError_t launch_event_ids(uint32_t* evt, const uint32_t* evt_offset, const DevCounts* cnt, uint32_t n_events_cap,
@@ -146,7 +161,8 @@ uint32_t grid_lds_hits(const GridGeom& g, uint32_t block);
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
-                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             uint4* rec = nullptr);
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const float* hx,
                              const float* hy, const float* hz, uint32_t* grid, DevCounts* cnt,
@@ -177,7 +193,7 @@ hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t 
                              uint32_t* pair_offsets, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
                              const uint32_t* slot_seg = nullptr);
 
-// TripletThetaPhiPredictor: count -> scan(counts) -> store (+ offsets, forward fill)
+// TripletThetaPhiPredictor: count -> scan(counts) -> store (+ group offsets) -> launch_forward_fill
 hipError_t launch_predict_count(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap,
                                 const uint32_t* grid, const float* hx, const float* hy, const float* hz,
                                 const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
@@ -221,7 +237,7 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
                                  uint32_t* th2, uint32_t* th3, DevCounts* cnt, const FineBufs& fb,
                                  const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
 
-// TripletThetaPhiFilter: count (bitmap + per-word counts) -> scan -> store (+ offsets)
+// TripletThetaPhiFilter: count (bitmap + per-word counts) -> scan -> store -> launch_filter_offsets
 hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const float* hx,
                                const float* hy, const float* hz, const uint32_t* hlayer,
                                const uint32_t* pairs, const uint32_t* cands, uint64_t* bitmap,
@@ -231,7 +247,12 @@ hipError_t launch_filter_store(const TripletCols& tc, uint32_t n_cand_cap, uint3
                                const uint32_t* cands, const uint64_t* bitmap, const uint32_t* word_counts,
                                uint32_t* th1, uint32_t* th2, uint32_t* th3, uint32_t* t_off,
                                DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
-// filterOffsetStore alone, over tracklets already stored (the offsets array zeroed before)
+// filterOffsetStore and its monotonize (the forward fill), over tracklets already stored
+// (the offsets array zeroed before)
+// filterOffsetStore alone (the monotonize is launch_forward_fill)
+hipError_t launch_filter_offset_store(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
+                                      const uint32_t* hevt, const uint32_t* th1, uint32_t* t_off, DevCounts* cnt,
+                                      hipStream_t st);
 hipError_t launch_filter_offsets(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
                                  const uint32_t* hevt, const uint32_t* th1, uint32_t* t_off, DevCounts* cnt,
                                  const LaunchCfg& lc, hipStream_t st);

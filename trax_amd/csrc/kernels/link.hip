@@ -862,14 +862,23 @@ hipError_t launch_filter_store(const TripletCols& tc, uint32_t n_cand_cap, uint3
     const uint32_t b = lc.filter_block ? lc.filter_block : 256;
     TRAX_LAUNCH(k_filter_store, (uint64_t)n_cand_cap + 64, b, pairs, cands, bitmap, word_counts, th1, th2, th3,
                 n_trk_cap, cnt);
+    return hipSuccess;
+}
+
+hipError_t launch_filter_offset_store(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
+                                      const uint32_t* hevt, const uint32_t* th1, uint32_t* t_off, DevCounts* cnt,
+                                      hipStream_t st) {
+    LaunchCfg lc;
     TRAX_LAUNCH(k_filter_offsets, n_trk_cap, 256u, tc, hlayer, hevt, th1, t_off, cnt);
-    return launch_forward_fill(t_off, cnt, tc.n_trip, st);
+    return hipSuccess;
 }
 
 hipError_t launch_filter_offsets(const TripletCols& tc, uint32_t n_trk_cap, const uint32_t* hlayer,
                                  const uint32_t* hevt, const uint32_t* th1, uint32_t* t_off, DevCounts* cnt,
                                  const LaunchCfg& lc, hipStream_t st) {
-    TRAX_LAUNCH(k_filter_offsets, n_trk_cap, 256u, tc, hlayer, hevt, th1, t_off, cnt);
+    (void)lc;
+    const hipError_t e = launch_filter_offset_store(tc, n_trk_cap, hlayer, hevt, th1, t_off, cnt, st);
+    if (e != hipSuccess) return e;
     return launch_forward_fill(t_off, cnt, tc.n_trip, st);
 }
 

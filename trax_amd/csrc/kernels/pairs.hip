@@ -1529,9 +1529,7 @@ hipError_t launch_predict_store(const GridGeom& g, const TripletCols& tc, uint32
     const uint32_t b = lc.predict_block ? lc.predict_block : 256;
     hipLaunchKernelGGL(k_predict_store, dim3(stride_blocks(n_pair_cap, b, lc)), dim3(b), 0, st, g, tc, grid,
                        hx, hy, hz, hzp, hlayer, hevt, pairs, counts, cands, cand_offsets, n_cand_cap, cnt);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_forward_fill(cand_offsets, cnt, tc.n_trip, st);
+    return hipGetLastError();
 }
 
 uint32_t predict_fused_tiles(uint64_t n_pair_cap) { return (uint32_t)(n_pair_cap / 128) + 2; }  // every variant

@@ -285,7 +285,21 @@ __global__ __launch_bounds__(256) void k_forward_fill(uint32_t* __restrict__ off
     }
 }
 
+__global__ void k_set_counts(DevCounts* __restrict__ cnt, CountWrites w) {
+    uint32_t* c = reinterpret_cast<uint32_t*>(cnt);
+    if (threadIdx.x == 0)
+        for (uint32_t i = 0; i < w.n; ++i) c[w.idx[i]] = w.val[i];
+}
+
 }  // namespace
+
+hipError_t launch_set_counts(DevCounts* cnt, const CountWrites& w, hipStream_t st) {
+    if (w.n > 12) return hipErrorInvalidValue;
+    for (uint32_t i = 0; i < w.n; ++i)
+        if (w.idx[i] >= sizeof(DevCounts) / 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_set_counts, dim3(1), dim3(64), 0, st, cnt, w);
+    return hipGetLastError();
+}
 
 uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc) {
     static uint32_t n_cu = 0;

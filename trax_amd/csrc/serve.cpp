@@ -7,9 +7,13 @@
 // queue (src/triplet_builder/TripletBuilder.cpp:297-552); this loop keeps the
 // reference's per-event results (tracklets of TripletThetaPhiFilter, tracks of
 // CellularAutomaton) and adds continuous batching in front of it:
-//   - admission: a batch takes waiting events in arrival order while its hit
-//     count fits the slot's hit capacity, which the device-memory budget sets;
-//     output capacities are bounds per input hit measured on the workload;
+//   - admission: a batch takes waiting events while its hit count fits the
+//     slot's hit capacity, which the device-memory budget sets; output
+//     capacities are bounds per input hit measured on the workload.  In
+//     arrival order (consecutive events of the stream), or grouped by hit
+//     count: the oldest waiting event's size class (hit count by powers of
+//     four) first, in arrival order, then the other waiting events to fill
+//     the batch, so that events of like size share batches;
 //   - an output that still overflows re-queues the batch's events at the
 //     front, and the next batches take half as many events;
 //   - completion: the counts are read when the slot's stream has drained, the
@@ -123,6 +127,13 @@ struct trax_server {
         lay_n = h->lay_nhits; lay_off = h->lay_offset;
     }
 
+    // size class of an event for grouping: its hit count by powers of four
+    uint32_t size_class(uint32_t e) const {
+        uint32_t c = 0;
+        for (uint64_t n = evt_n[e]; n >= 4; n >>= 2) ++c;
+        return c;
+    }
+
     // stage the events into the slot's workspace and enqueue the pipeline
     void submit(Slot& s, std::vector<uint32_t> evs, double now) {
         trax_batch* b = s.b;
@@ -143,17 +154,30 @@ struct trax_server {
         uint32_t* shape = sp + Ecap + 2ull * Ecap * L;
         shape[0] = E;
         shape[1] = N;
-        // events are taken in stream order, so their hits are one contiguous range
-        const uint64_t first = evt_off[s.events[0]];
-        for (uint32_t k = 1; k < E; ++k)
-            if (s.events[k] != s.events[k - 1] + 1) throw InputError("batch events must be consecutive");
-        float* fcols[3] = {x, y, z};
-        float* dfc[3] = {b->hin.x, b->hin.y, b->hin.z};
-        uint32_t* ucols[3] = {layer, det, hid};
-        uint32_t* duc[3] = {b->hin.layer, b->hin.det, b->hin.id};
-        for (int k = 0; k < 3; ++k) {
-            hip_check(hipMemcpyAsync(dfc[k], fcols[k] + first, N * 4ull, hipMemcpyHostToDevice, s.st), "h2d");
-            hip_check(hipMemcpyAsync(duc[k], ucols[k] + first, N * 4ull, hipMemcpyHostToDevice, s.st), "h2d");
+        // consecutive events of the stream are one contiguous range of the
+        // pinned store and go up directly; others are gathered into the
+        // slot's pinned staging area first (the slot's stream has drained, so
+        // the area is free), one copy per run of consecutive events
+        bool consecutive = true;
+        for (uint32_t k = 1; k < E; ++k) consecutive = consecutive && s.events[k] == s.events[k - 1] + 1;
+        const uint32_t* src[6] = {reinterpret_cast<const uint32_t*>(x), reinterpret_cast<const uint32_t*>(y),
+                                  reinterpret_cast<const uint32_t*>(z), layer, det, hid};
+        uint32_t* dst[6] = {reinterpret_cast<uint32_t*>(b->hin.x), reinterpret_cast<uint32_t*>(b->hin.y),
+                            reinterpret_cast<uint32_t*>(b->hin.z), b->hin.layer, b->hin.det, b->hin.id};
+        for (int c = 0; c < 6; ++c) {
+            const uint32_t* from = src[c] + evt_off[s.events[0]];
+            if (!consecutive) {
+                uint32_t* stg = b->staging + (size_t)c * b->caps.max_hits;
+                for (uint32_t k = 0; k < E;) {
+                    uint32_t k1 = k + 1;
+                    while (k1 < E && s.events[k1] == s.events[k1 - 1] + 1) ++k1;
+                    const uint64_t n_run = s.hit_base[k1] - s.hit_base[k];
+                    std::memcpy(stg + s.hit_base[k], src[c] + evt_off[s.events[k]], n_run * 4);
+                    k = k1;
+                }
+                from = stg;
+            }
+            hip_check(hipMemcpyAsync(dst[c], from, N * 4ull, hipMemcpyHostToDevice, s.st), "h2d");
         }
         hip_check(hipMemcpyAsync(b->sup, sp, (Ecap + 2ull * Ecap * L) * 4, hipMemcpyHostToDevice, s.st), "h2d");
         hip_check(hipMemcpyAsync(b->cnt, shape, 8, hipMemcpyHostToDevice, s.st), "h2d");
@@ -402,15 +426,38 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
             }
             for (Slot& s : sv->slots) {
                 if (s.state != 0 || waiting.empty()) continue;
-                // events in arrival order, all consecutive, within the hit capacity
                 std::vector<uint32_t> evs;
                 uint64_t hits = 0;
-                while (!waiting.empty() && evs.size() < limit) {
-                    const uint32_t e = waiting.front();
-                    if (!evs.empty() && (e != evs.back() + 1 || hits + sv->evt_n[e] > sv->caps.max_hits)) break;
-                    hits += sv->evt_n[e];
-                    evs.push_back(e);
-                    waiting.pop_front();
+                if (!sv->opts.group_by_size) {
+                    // events in arrival order, all consecutive, within the hit capacity
+                    while (!waiting.empty() && evs.size() < limit) {
+                        const uint32_t e = waiting.front();
+                        if (!evs.empty() && (e != evs.back() + 1 || hits + sv->evt_n[e] > sv->caps.max_hits)) break;
+                        hits += sv->evt_n[e];
+                        evs.push_back(e);
+                        waiting.pop_front();
+                    }
+                } else {
+                    // the oldest event's size class first, then the rest, each in arrival order
+                    const uint32_t cls = sv->size_class(waiting.front());
+                    std::vector<char> taken(waiting.size(), 0);
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (size_t i = 0; i < waiting.size() && evs.size() < limit; ++i) {
+                            const uint32_t e = waiting[i];
+                            if (taken[i] || (pass == 0 && sv->size_class(e) != cls)) continue;
+                            if (hits + sv->evt_n[e] > sv->caps.max_hits) {
+                                if (evs.empty()) continue;  // cannot happen: every event fits a batch alone
+                                break;
+                            }
+                            hits += sv->evt_n[e];
+                            evs.push_back(e);
+                            taken[i] = 1;
+                        }
+                    std::deque<uint32_t> rest;
+                    for (size_t i = 0; i < waiting.size(); ++i)
+                        if (!taken[i]) rest.push_back(waiting[i]);
+                    waiting.swap(rest);
+                    std::sort(evs.begin(), evs.end());  // stream order inside the batch: runs copy in one piece
                 }
                 sv->submit(s, std::move(evs), now_s());
                 progressed = true;

@@ -42,22 +42,13 @@ void set_param(trax_tuning& t, const char* name, uint32_t v) {
     else if (n == "link_block") t.link_block = v;
     else if (n == "blocks_per_cu") t.blocks_per_cu = v;
     else if (n == "predict_fine") t.predict_fine = v;
+    else if (n == "grid_lds_hits") t.grid_lds_hits = v;
+    else if (n == "grid_order") t.grid_order = v;
+    else if (n == "grid_rank") t.grid_rank = v;
+    else if (n == "grid_gather") t.grid_gather = v;
 }
 
-LaunchCfg to_launch(const trax_tuning& t) {
-    LaunchCfg lc;
-    lc.grid_count_block = t.grid_count_block;
-    lc.grid_store_block = t.grid_store_block;
-    lc.pair_block = t.pair_block;
-    lc.predict_block = t.predict_block;
-    lc.filter_block = t.filter_block;
-    lc.link_block = t.link_block;
-    lc.blocks_per_cu = t.blocks_per_cu;
-    lc.predict_list = t.predict_list;
-    lc.predict_sort = t.predict_sort;
-    lc.predict_fine = t.predict_fine;
-    return lc;
-}
+LaunchCfg to_launch(const trax_tuning& t) { return launch_cfg(&t); }
 
 uint64_t mix(uint64_t h, uint64_t v) {
     h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
@@ -123,6 +114,10 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
             {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
             {"grid_count_block", {128, 256, 512, 1024}, {"grid_count"}},
+            {"grid_lds_hits", {2048, 4096, 0}, {"grid_count"}},
+            {"grid_order", {0, 1}, {"grid_count"}},
+            {"grid_rank", {0, 1}, {"grid_count"}},
+            {"grid_gather", {0, 1}, {"grid_count"}},
             {"blocks_per_cu", {2, 4, 8, 16}, {}},
         };
         std::string log;

@@ -112,9 +112,15 @@ def event_sizes(wl: Workload, n_events: int, seed: int):
     return [(int(t), int(n)) for t, n in zip(tracks, noise)]
 
 
-def make_events(wl: Workload, n_events: int = None, seed: int = 2024, first: int = 0):
+def make_events(wl: Workload, n_events: int = None, seed: int = 2024, first: int = 0, of: int = None):
+    """Events first .. first + n_events - 1 of a list of `of` events (default:
+    the first + n_events events); the sizes of a list depend on its length,
+    its events on their index and the seed."""
     n_events = wl.extra.get("events", 1) if n_events is None else n_events
-    sizes = event_sizes(wl, first + n_events, seed)[first:]
+    total = first + n_events if of is None else of
+    if total < first + n_events:
+        raise ValueError("the events asked for lie past the end of the list")
+    sizes = event_sizes(wl, total, seed)[first:first + n_events]
     return [make_event(wl, t, n, seed, first + i) for i, (t, n) in enumerate(sizes)]
 
 
