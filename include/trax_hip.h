@@ -133,18 +133,17 @@ int trax_grid_build(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_
  * 24|32|48|64), (512, 24|32) in pair order or (256, 32) longest walk first;
  * 0 = (256, 24, pair order); the z sub-rows per grid row of the fused
  * predictor's walk structure (1, 2, 4, 8 or 16; 0 = 8); and the batched grid
- * build's hits per (event, layer) kept in LDS (0 = as many as fit), its item
- * order (1 = largest items first), its in-cell ordering (0 = each cell
- * sorted in LDS, 1 = each hit ranks itself among its cell's) and its gather
- * (0 = seven columns, 1 = one 32-byte record per hit written by the count
- * pass).  Results do not depend on them. */
+ * build's block (grid_count_block; 0 = 1024), item order (0 or 1 = largest
+ * (event, layer) items first, 2 = item order) and LDS per block in KiB (up to
+ * 160; above 64 the items that fit are staged whole in LDS; 0 = 160).
+ * Results do not depend on them. */
 typedef struct {
     uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
     uint32_t blocks_per_cu;
     uint32_t predict_list;
     uint32_t predict_sort;
     uint32_t predict_fine;
-    uint32_t grid_lds_hits, grid_order, grid_rank, grid_gather;
+    uint32_t grid_order, grid_stage_kib;
 } trax_tuning;
 
 /* ---- 2b. stage launchers ------------------------------------------------

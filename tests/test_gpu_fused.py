@@ -67,7 +67,7 @@ def test_fused_autotune_keeps_results(dev):
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
     # no sorted-walk predictor variant and no filter block size when fused
-    assert len(trials) == 8 + 3 + 4 + 4 + 4 + 4
+    assert len(trials) == 8 + 3 + 4 + 4 + 6 + 4
     assert all(t["same_results"] for t in trials)
     assert tuning["predict_sort"] == 0
     tb = Batch(dev, cfg, caps, tuning)

@@ -77,16 +77,15 @@ def test_block_shapes_match_oracle(dev, shape):
     assert not bad, "\n".join(bad)
 
 
-@pytest.mark.parametrize("lds_hits", [64, 2048, 0])
-@pytest.mark.parametrize("order", [0, 1])
-@pytest.mark.parametrize("rank,gather", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_grid_build_variants_match_oracle(dev, lds_hits, order, rank, gather):
-    """The batched grid build's variants: few hits in LDS (the larger
-    (event, layer) items take the global-memory scatter), the largest items
-    first, cells sorted in LDS or each hit ranking itself in its cell, the
-    seven columns or one record per hit gathered; on the uneven events and on
-    a dense one."""
-    shape = dict(grid_lds_hits=lds_hits, grid_order=order, grid_rank=rank, grid_gather=gather)
+@pytest.mark.parametrize("stage_kib", [64, 80, 0])
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("block", [256, 1024])
+def test_grid_build_variants_match_oracle(dev, stage_kib, order, block):
+    """The batched grid build's variants: items staged whole in LDS (those
+    that fit; larger ones take the global-memory scatter) or the column
+    gather, the largest items first or in order, small and large blocks; on
+    the uneven events and on a dense one (layers of up to 3,600 hits)."""
+    shape = dict(grid_stage_kib=stage_kib, grid_order=order, grid_count_block=block)
     for wl, (cfg, hs, _) in ((w, hitset(w, seed=s)) for w, s in
                              ((small_workload(3, events=4, tracks_min=20, tracks_max=400), 11),
                               (small_workload(2, tracks=1500, noise=20000, events=1), 5))):
@@ -106,7 +105,7 @@ def test_autotune_picks_supported_shapes_with_same_results(dev):
     caps = b.caps
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
-    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 4 + 3 + 2 + 2 + 2 + 4
+    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 6 + 4
     assert all(t["same_results"] for t in trials)
     assert (tuning["predict_block"], tuning["predict_list"], tuning["predict_sort"]) in PREDICT_VARIANTS
     assert tuning["blocks_per_cu"] in (2, 4, 8, 16)

@@ -642,7 +642,7 @@ void trax_batch::enqueue_stage(int s) {
         case 0:  // grid_count: the whole grid build (count, scan, stable scatter) in one kernel
             ok(launch_fill(reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords, 0u,
                            sizeof(DevCounts) / 4 - kCountsShapeWords, st), "reset");
-            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st, grid_rec),
+            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st),
                "grid build");
             break;
         case 1:  // grid_store: part of the grid build above
@@ -806,7 +806,6 @@ void trax_batch::layout(Arena& a) {
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
         B.grid_perm = a.take<uint32_t>(N + (uint64_t)E * L);  // scatter scratch, then the item order
-        B.grid_rec = reinterpret_cast<uint4*>(a.take<uint32_t>(8ull * N));  // the hits as 32-byte records
         B.seg_base = a.take<uint32_t>(E * T + 1);
         B.slot_counts = a.take<uint32_t>(slot_cap + 1);
         B.slot_seg = a.take<uint32_t>(slot_seg_entries(slot_cap));
@@ -882,24 +881,16 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         std::unique_ptr<trax_batch> b(new trax_batch);
         b->owner = ctx;
         b->st = pick(ctx, s);
-        if (tuning) {
-            b->lc.grid_count_block = tuning->grid_count_block;
-            b->lc.grid_store_block = tuning->grid_store_block;
-            b->lc.pair_block = tuning->pair_block;
-            b->lc.predict_block = tuning->predict_block;
-            b->lc.filter_block = tuning->filter_block;
-            b->lc.link_block = tuning->link_block;
-            b->lc.blocks_per_cu = tuning->blocks_per_cu;
-            b->lc.predict_list = tuning->predict_list;
-            b->lc.predict_sort = tuning->predict_sort;
-            b->lc.predict_fine = tuning->predict_fine;
-        }
+        b->lc = launch_cfg(tuning);
         if (!predict_variant_supported(b->lc.predict_block, b->lc.predict_list, b->lc.predict_sort))
             throw InputError("unsupported fused predictor variant (block, list, sort)");
+        if (b->lc.grid_stage_kib > kGridLdsMax / 1024 || b->lc.grid_order > 2)
+            throw InputError("grid build: LDS per block at most 160 KiB, item order 0, 1 or 2");
         for (uint32_t v : {b->lc.grid_count_block, b->lc.grid_store_block, b->lc.pair_block, b->lc.filter_block,
                            b->lc.link_block})
             if (v % 64 != 0 || v > 1024) throw InputError("block sizes must be multiples of 64 up to 1024");
         predict_fused_prepare(b->lc);  // before any capture
+        grid_build_prepare();
         auto& B = *b;
         batch_shape(B, cfg, caps);
         {

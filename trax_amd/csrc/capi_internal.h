@@ -89,10 +89,8 @@ inline trax::LaunchCfg launch_cfg(const trax_tuning* t) {
     lc.predict_list = t->predict_list;
     lc.predict_sort = t->predict_sort;
     lc.predict_fine = t->predict_fine;
-    lc.grid_lds_hits = t->grid_lds_hits;
     lc.grid_order = t->grid_order;
-    lc.grid_rank = t->grid_rank;
-    lc.grid_gather = t->grid_gather;
+    lc.grid_stage_kib = t->grid_stage_kib;
     return lc;
 }
 
@@ -169,7 +167,6 @@ struct trax_batch {
     trax::CABufs cb;
     trax::FineBufs fine;  // the fused predictor's walk structure
     uint32_t* grid_perm = nullptr;  // the grid build's scatter scratch (N), then its item order (E * L)
-    uint4* grid_rec = nullptr;      // the grid build's hit records (2 N)
     hipGraph_t graphs[trax_capi::kNumStages] = {};
     hipGraphExec_t execs[trax_capi::kNumStages] = {};
     bool captured = false;

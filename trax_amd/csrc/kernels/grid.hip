@@ -141,13 +141,21 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
 // wave-ordered cursors: every hit takes a position in its cell with an LDS
 // atomic and leaves its local index there (perm), each cell's few indices are
 // sorted ascending (so a cell holds its hits in input order: the same stable
-// outcome), and the outputs are gathered in output order, coalesced.  A layer
-// of at most lds_hits hits keeps its cells and perm in LDS (2 B per hit each):
-// x, y, z are read from HBM once and the sort never leaves LDS.  A larger
-// layer re-reads x, y, z for the scatter and keeps perm in global memory at
-// the output positions.  Reads 12 B per hit for the count, 28 B per hit for
-// the gather; writes 4 B per cell and 44 B per hit of output (52 B with the
-// (x, y) column).
+// outcome), and the outputs are written in output order, coalesced.
+//
+// Where the outputs' sources come from depends on the item's size:
+//  - staged (n <= stage_cap): the count pass reads the item's x, y, z, detId
+//    and hitId once, coalesced, into LDS (20 B per hit), and layer and event,
+//    which are one value each for the whole item; the outputs are read from
+//    LDS in output order.  Every input byte is read from HBM once, coalesced,
+//    and every output byte written once.  An item whose layer or event column
+//    is not uniform reads those two columns at the sources instead.
+//  - local (n <= lds_hits): cells and perm in LDS (2 B per hit each); the
+//    seven columns are gathered at the sources in output order.
+//  - global: perm in global memory at the output positions, x, y, z re-read
+//    for the scatter.
+// Reads 28 B per hit (staged) or 12 + 28 B per hit; writes 4 B per cell and
+// 44 B per hit of output (52 B with the (x, y) column).
 
 // insertion sort of each cell's entries of perm; ends[c] is the end of cell c
 template <typename T>
@@ -164,18 +172,6 @@ __device__ inline void sort_cells(T* perm, const uint32_t* ends, uint32_t cells)
             perm[j] = v;
         }
     }
-}
-
-// The stable position of the hit at unsorted position p of its cell [lo, hi)
-// (placed there by an atomic): lo + the number of the cell's hits with a
-// smaller input index.
-template <typename T>
-__device__ inline uint32_t stable_pos(const T* perm, uint32_t p, uint32_t lo, uint32_t hi) {
-    if (hi - lo < 2) return p;
-    const T v = perm[p];
-    uint32_t r = lo;
-    for (uint32_t k = lo; k < hi; ++k) r += perm[k] < v ? 1u : 0u;
-    return r;
 }
 
 // Items (event, layer) ordered by size, largest first (by the bit length of
@@ -207,51 +203,77 @@ __global__ __launch_bounds__(1024) void k_grid_order(const uint32_t* __restrict_
 constexpr uint32_t kGridUnroll = 4;       // hits per thread whose loads are issued together
 constexpr uint16_t kNoCell = 0xffffu;     // a hit outside the grid, in the LDS cell list
 
+// LDS of one block: hist (cells) | wsum (waves + 1) | mixed flag | the staged
+// columns x, y, z, detId, hitId (stage_cap each) | cell and perm lists
+// (lds_hits u16 each)
+struct GridLds {
+    uint32_t *hist, *wsum, *mixed, *sx, *sy, *sz, *sdet, *sid;
+    uint16_t *lcell, *lperm;
+};
+__device__ inline GridLds grid_lds(uint32_t* lds, uint32_t cells, uint32_t stage_cap, uint32_t lds_hits) {
+    GridLds L;
+    L.hist = lds;
+    L.wsum = L.hist + cells;
+    L.mixed = L.wsum + blockDim.x / kWave + 1;
+    L.sx = L.mixed + 1;
+    L.sy = L.sx + stage_cap;
+    L.sz = L.sy + stage_cap;
+    L.sdet = L.sz + stage_cap;
+    L.sid = L.sdet + stage_cap;
+    L.lcell = reinterpret_cast<uint16_t*>(L.sid + stage_cap);
+    L.lperm = L.lcell + lds_hits;
+    return L;
+}
+size_t grid_lds_bytes(uint32_t cells, uint32_t block, uint32_t stage_cap, uint32_t lds_hits) {
+    return (size_t)(cells + block / kWave + 2 + 5ull * stage_cap) * 4 + (size_t)lds_hits * 4;
+}
+
 __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t* __restrict__ d_nevents,
                                                    const uint32_t* __restrict__ evt_offset,
                                                    const uint32_t* __restrict__ lay_nhits,
                                                    const uint32_t* __restrict__ lay_offset, HitCols in,
                                                    HitColsOut out, uint32_t* __restrict__ grid,
                                                    uint32_t* __restrict__ perm,
-                                                   const uint32_t* __restrict__ order, uint4* __restrict__ rec,
-                                                   uint32_t n_hits_cap,
-                                                   uint32_t lds_hits, uint32_t rank_mode,
+                                                   const uint32_t* __restrict__ order, uint32_t n_hits_cap,
+                                                   uint32_t lds_hits, uint32_t stage_cap,
                                                    DevCounts* __restrict__ cnt) {
     extern __shared__ uint32_t lds[];
-    uint32_t* hist = lds;
-    uint32_t* wsum = lds + g.cells;
-    uint16_t* lcell = reinterpret_cast<uint16_t*>(wsum + blockDim.x / kWave + 1);
-    uint16_t* lperm = lcell + lds_hits;
+    const GridLds L = grid_lds(lds, g.cells, stage_cap, lds_hits);
+    uint32_t* hist = L.hist;
     const uint32_t n_items = *d_nevents * g.n_layers;
     const uint32_t bs = blockDim.x;
+    // the block's next item (its index, size and first hit), loaded one item
+    // ahead so that those loads overlap the current item's work
+    uint32_t nx_item = 0, nx_n = 0, nx_off = 0;
+    auto fetch = [&](uint32_t tt) {
+        nx_item = order ? order[tt] : tt;
+        nx_n = lay_nhits[nx_item];
+        nx_off = evt_offset[nx_item / g.n_layers] + lay_offset[nx_item];
+    };
+    if (blockIdx.x < n_items) fetch(blockIdx.x);
     for (uint32_t t = blockIdx.x; t < n_items; t += gridDim.x) {
-        const uint32_t item = order ? order[t] : t;
-        const uint32_t e = item / g.n_layers;
+        const uint32_t item = nx_item, n = nx_n, off = nx_off;
+        if (t + gridDim.x < n_items) fetch(t + gridDim.x);
         for (uint32_t i = threadIdx.x; i < g.cells; i += bs) hist[i] = 0;
+        if (threadIdx.x == 0) *L.mixed = 0;
         __syncthreads();
-        const uint32_t n = lay_nhits[item];
-        const uint32_t off = evt_offset[e] + lay_offset[item];
         const bool fits = (uint64_t)off + n <= n_hits_cap;
-        const bool local = n <= lds_hits;  // block-uniform
-        bool bad = false;
+        const bool staged = fits && n <= stage_cap;  // block-uniform
+        const bool local = n <= lds_hits;            // block-uniform
+        // the item's layer and event: one value each for a uniform item
+        const uint32_t la0 = (staged && n) ? in.layer[off] : 0u, ev0 = (staged && n) ? in.evt[off] : 0u;
+        bool bad = false, mixed = false;
         for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kGridUnroll * bs) {
             float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll];
+            uint32_t de[kGridUnroll], id[kGridUnroll], la[kGridUnroll], ev[kGridUnroll];
 #pragma unroll
             for (uint32_t k = 0; k < kGridUnroll; ++k) {
                 const uint32_t i = i0 + k * bs;
-                if (i < n) { x[k] = in.x[off + i]; y[k] = in.y[off + i]; z[k] = in.z[off + i]; }
-            }
-            if (rec && fits) {
-                // each hit's seven columns as one 32-byte record, written in
-                // input order (coalesced) for the gather below
-#pragma unroll
-                for (uint32_t k = 0; k < kGridUnroll; ++k) {
-                    const uint32_t i = i0 + k * bs;
-                    if (i < n) {
-                        const size_t r = 2 * ((size_t)off + i);
-                        rec[r] = make_uint4(__float_as_uint(x[k]), __float_as_uint(y[k]), __float_as_uint(z[k]),
-                                            in.layer[off + i]);
-                        rec[r + 1] = make_uint4(in.det[off + i], in.id[off + i], in.evt[off + i], 0u);
+                if (i < n) {
+                    x[k] = in.x[off + i]; y[k] = in.y[off + i]; z[k] = in.z[off + i];
+                    if (staged) {
+                        de[k] = in.det[off + i]; id[k] = in.id[off + i];
+                        la[k] = in.layer[off + i]; ev[k] = in.evt[off + i];
                     }
                 }
             }
@@ -259,16 +281,22 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
             for (uint32_t k = 0; k < kGridUnroll; ++k) {
                 const uint32_t i = i0 + k * bs;
                 if (i < n) {
+                    if (staged) {
+                        L.sx[i] = __float_as_uint(x[k]); L.sy[i] = __float_as_uint(y[k]);
+                        L.sz[i] = __float_as_uint(z[k]); L.sdet[i] = de[k]; L.sid[i] = id[k];
+                        mixed |= la[k] != la0 || ev[k] != ev0;
+                    }
                     const uint32_t c = hit_cell(g, x[k], y[k], z[k]);
                     if (c < g.cells) atomicAdd(&hist[c], 1u);
                     else bad = true;
-                    if (local) lcell[i] = c < g.cells ? (uint16_t)c : kNoCell;
+                    if (local) L.lcell[i] = c < g.cells ? (uint16_t)c : kNoCell;
                 }
             }
         }
         if (bad) atomicOr(&cnt->err, kErrCellRange);
+        if (mixed) atomicOr(L.mixed, 1u);
         __syncthreads();
-        lds_excl_scan(hist, g.cells, wsum);
+        lds_excl_scan(hist, g.cells, L.wsum);
         uint32_t* gout = grid + (size_t)item * g.cells;
         for (uint32_t i = threadIdx.x; i < g.cells; i += bs) gout[i] = off + hist[i];
         __syncthreads();
@@ -276,60 +304,52 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
             // positions: hist becomes each cell's end
             if (local) {
                 for (uint32_t i = threadIdx.x; i < n; i += bs) {
-                    const uint16_t c = lcell[i];
-                    if (c != kNoCell) lperm[atomicAdd(&hist[c], 1u)] = (uint16_t)i;
+                    const uint16_t c = L.lcell[i];
+                    if (c != kNoCell) L.lperm[atomicAdd(&hist[c], 1u)] = (uint16_t)i;
                 }
                 __syncthreads();
-                if (rank_mode == 0) sort_cells(lperm, hist, g.cells);
+                sort_cells(L.lperm, hist, g.cells);
             } else {
                 for (uint32_t i = threadIdx.x; i < n; i += bs) {
                     const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
                     if (c < g.cells) perm[off + atomicAdd(&hist[c], 1u)] = i;
                 }
                 __syncthreads();
-                if (rank_mode == 0) sort_cells(perm + off, hist, g.cells);
+                sort_cells(perm + off, hist, g.cells);
             }
-            if (rank_mode == 0) __syncthreads();
+            __syncthreads();
             const uint32_t placed = g.cells ? hist[g.cells - 1] : 0u;  // hits with a cell
+            const bool from_lds = staged, cols_mixed = *L.mixed != 0u;
             for (uint32_t p0 = threadIdx.x; p0 < placed; p0 += kGridUnroll * bs) {
-                uint32_t src[kGridUnroll], dst[kGridUnroll];
+                uint32_t src[kGridUnroll];
                 float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll];
                 uint32_t la[kGridUnroll], de[kGridUnroll], id[kGridUnroll], ev[kGridUnroll];
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
                     const uint32_t p = p0 + k * bs;
-                    src[k] = off;
-                    dst[k] = p;
-                    if (p < placed) {
-                        const uint32_t i = local ? (uint32_t)lperm[p] : perm[off + p];
-                        src[k] = off + i;
-                        if (rank_mode != 0) {
-                            // the cell of position p: its hit's cell (local) or recomputed
-                            const uint32_t c = local ? (uint32_t)lcell[i]
-                                                     : hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
-                            const uint32_t lo = c ? hist[c - 1] : 0u, hi = hist[c];
-                            dst[k] = local ? stable_pos(lperm, p, lo, hi) : stable_pos(perm + off, p, lo, hi);
-                        }
-                    }
+                    src[k] = p < placed ? (local ? (uint32_t)L.lperm[p] : perm[off + p]) : 0u;
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
                     if (p0 + k * bs < placed) {
                         const uint32_t i = src[k];
-                        if (rec) {
-                            const uint4 a = rec[2 * (size_t)i], b = rec[2 * (size_t)i + 1];
-                            x[k] = __uint_as_float(a.x); y[k] = __uint_as_float(a.y); z[k] = __uint_as_float(a.z);
-                            la[k] = a.w; de[k] = b.x; id[k] = b.y; ev[k] = b.z;
+                        if (from_lds) {
+                            x[k] = __uint_as_float(L.sx[i]); y[k] = __uint_as_float(L.sy[i]);
+                            z[k] = __uint_as_float(L.sz[i]); de[k] = L.sdet[i]; id[k] = L.sid[i];
+                            la[k] = cols_mixed ? in.layer[off + i] : la0;
+                            ev[k] = cols_mixed ? in.evt[off + i] : ev0;
                         } else {
-                            x[k] = in.x[i]; y[k] = in.y[i]; z[k] = in.z[i];
-                            la[k] = in.layer[i]; de[k] = in.det[i]; id[k] = in.id[i]; ev[k] = in.evt[i];
+                            const uint32_t j = off + i;
+                            x[k] = in.x[j]; y[k] = in.y[j]; z[k] = in.z[j];
+                            la[k] = in.layer[j]; de[k] = in.det[j]; id[k] = in.id[j]; ev[k] = in.evt[j];
                         }
                     }
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
-                    if (p0 + k * bs < placed) {
-                        const uint32_t pos = off + dst[k];
+                    const uint32_t p = p0 + k * bs;
+                    if (p < placed) {
+                        const uint32_t pos = off + p;
                         out.x[pos] = x[k]; out.y[pos] = y[k]; out.z[pos] = z[k];
                         out.layer[pos] = la[k];
                         out.det[pos] = de[k];
@@ -345,23 +365,40 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
     }
 }
 
+// hits per item the LDS lists hold, and the items staged whole, for a block
+// of `block` threads within an LDS budget of `budget` bytes
+void grid_lds_plan(const GridGeom& g, uint32_t block, size_t budget, uint32_t& lds_hits, uint32_t& stage_cap) {
+    lds_hits = stage_cap = 0;
+    const size_t base = grid_lds_bytes(g.cells, block, 0, 0);
+    if (g.cells >= kNoCell || base >= budget) return;
+    if (budget > kGridLdsBase) {
+        // staged: 20 B of columns + 4 B of lists per hit
+        const size_t fit = (budget - base) / 24;
+        stage_cap = lds_hits = (uint32_t)std::min<size_t>(fit & ~(size_t)63, 0xfff0u);
+        return;
+    }
+    const size_t fit = (budget - base) / (2 * sizeof(uint16_t));
+    lds_hits = (uint32_t)std::min<size_t>(fit & ~(size_t)63, kGridLdsHitsMax);
+}
+
 }  // namespace
 
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
-                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
-                             uint4* rec) {
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
     (void)grid_unused;
-    const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 256;
+    // defaults: 1024 threads, 160 KiB of LDS (items staged whole), largest items first
+    const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 1024;
+    uint32_t lds_hits = 0, stage_cap = 0;
+    const size_t budget = lc.grid_stage_kib ? (size_t)lc.grid_stage_kib * 1024 : kGridLdsMax;
+    grid_lds_plan(g, bs, std::min<size_t>(budget, kGridLdsMax), lds_hits, stage_cap);
+    const size_t lds = grid_lds_bytes(g.cells, bs, stage_cap, lds_hits);
+    // one block per item: the dispatcher hands the next item to the first block slot that frees up
     const uint32_t blocks = n_items_cap ? n_items_cap : 1;
-    const size_t base = (size_t)(g.cells + bs / kWave + 1) * sizeof(uint32_t);
-    uint32_t lds_hits = grid_lds_hits(g, bs);
-    if (lc.grid_lds_hits) lds_hits = std::min(lds_hits, lc.grid_lds_hits & ~63u);
-    const size_t lds = base + (size_t)lds_hits * 2 * sizeof(uint16_t);
     // perm holds n_hits_cap scatter entries, then the item order (n_items_cap)
     uint32_t* order = nullptr;
-    if (lc.grid_order) {
+    if (lc.grid_order != 2) {
         order = perm + n_hits_cap;
         hipLaunchKernelGGL(k_grid_order, dim3(1), dim3(1024), 0, st, &cnt->n_events, g.n_layers, lay_nhits, order,
                            n_items_cap);
@@ -369,18 +406,19 @@ hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
     HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};
     hipLaunchKernelGGL(k_grid_build, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset, lay_nhits,
-                       lay_offset, ic, oc, grid, perm, order, lc.grid_gather ? rec : nullptr, n_hits_cap, lds_hits,
-                       lc.grid_rank, cnt);
+                       lay_offset, ic, oc, grid, perm, order, n_hits_cap, lds_hits, stage_cap, cnt);
     return hipGetLastError();
 }
 
-uint32_t grid_lds_hits(const GridGeom& g, uint32_t block) {
-    // LDS per block stays within 64 KiB; cells are kept as u16 with 0xffff free
-    constexpr size_t kBudget = 64 * 1024;
-    const size_t base = (size_t)(g.cells + block / kWave + 1) * sizeof(uint32_t);
-    if (g.cells >= kNoCell || base >= kBudget) return 0;
-    const size_t fit = (kBudget - base) / (2 * sizeof(uint16_t));
-    return (uint32_t)std::min<size_t>(fit & ~(size_t)63, kGridLdsHitsMax);
+// the build may take up to kGridLdsMax of LDS per block: set once, outside
+// any graph capture
+void grid_build_prepare() {
+    static bool done = false;
+    if (!done) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grid_build),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGridLdsMax);
+        done = true;
+    }
 }
 
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,

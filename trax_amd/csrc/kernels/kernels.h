@@ -43,10 +43,8 @@ struct LaunchCfg {
     uint32_t predict_list = 0;   // fused predictor: LDS list entries per pair (0 = 24)
     uint32_t predict_sort = 0;   // fused predictor: walk a tile's pairs longest first
     uint32_t predict_fine = 0;   // fused predictor: z sub-rows per grid row (power of two; 0 = 8)
-    uint32_t grid_lds_hits = 0;  // grid build: hits of an (event, layer) kept in LDS (0 = as many as fit)
-    uint32_t grid_order = 0;     // grid build: 1 = the largest (event, layer) items first
-    uint32_t grid_rank = 0;      // grid build: 0 = cells sorted in LDS, 1 = each hit ranks itself in its cell
-    uint32_t grid_gather = 0;    // grid build: 1 = gather each hit as one 32-byte record written by the count pass
+    uint32_t grid_order = 0;     // grid build: 0 or 1 = the largest (event, layer) items first, 2 = item order
+    uint32_t grid_stage_kib = 0; // grid build: LDS per block in KiB (above 64: items staged whole; 0 = 160)
 };
 
 // The fused predictor's own walk structure: every z row of every (event,
@@ -157,12 +155,13 @@ hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_t
 // block size (0 when the grid's cells do not fit a u16 or LDS); larger layers
 // take its global-memory path
 constexpr uint32_t kGridLdsHitsMax = 8192;
-uint32_t grid_lds_hits(const GridGeom& g, uint32_t block);
+constexpr size_t kGridLdsBase = 64 * 1024;   // LDS per block of the unstaged build
+constexpr size_t kGridLdsMax = 160 * 1024;   // gfx950's LDS per workgroup
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
-                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
-                             uint4* rec = nullptr);
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+void grid_build_prepare();
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const float* hx,
                              const float* hy, const float* hz, uint32_t* grid, DevCounts* cnt,

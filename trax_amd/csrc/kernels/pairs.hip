@@ -786,60 +786,72 @@ __device__ inline void pred_walk_fine(const PredWin2& w, const GridGeom& g, cons
     const uint32_t np1 = g.n_phi + 1;
     const float PI = TX_PI_F;
     const uint32_t wr = (uint32_t)w.wrap;
-    uint32_t j = 0, end2 = 0, fEnd = 0, fLen = 0;
-    bool sa = false, sb = false;
     uint32_t row = 0xffffffffu, ord = 0;  // grid row of the last match, grid rows with matches so far
-    auto next_row = [&]() {
-        for (; f < f1; ++f) {
-            sa = ea && f >= aLo && f < aHi;
-            sb = eb && f >= bLo && f < bHi;
-            if (!(sa || sb)) continue;
-            const uint32_t* r = fl + (size_t)f * np1;
-            const uint32_t fStart = r[0];
-            fEnd = r[np1];
-            fLen = fEnd - fStart;
-            end2 = wr * fEnd + r[w.pHi] - wr * fStart;
-            j = r[w.pLo];
-            if (j < end2) return true;
-        }
-        return false;
+    // A sub-row's four prefix entries (its start, its end, the window's first
+    // and last cell) are loaded one active sub-row ahead: the next sub-row's
+    // loads are in flight while the current one's hits are tested.
+    auto active_from = [&](uint32_t ff) {
+        for (; ff < f1; ++ff)
+            if ((ea && ff >= aLo && ff < aHi) || (eb && ff >= bLo && ff < bHi)) break;
+        return ff;
     };
-    bool active = next_row();
-    while (active) {
-        float4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t jj = j + u < end2 ? j + u : j;  // j < end2: always a hit of the row
-            q[u] = fh[jj - (uint32_t)(jj >= fEnd) * fLen];
+    struct RowPre {
+        uint32_t f, start, end, lo, hi;
+    };
+    auto load_row = [&](uint32_t ff) {
+        RowPre rp{ff, 0u, 0u, 0u, 0u};
+        if (ff < f1) {
+            const uint32_t* r = fl + (size_t)ff * np1;
+            rp.start = r[0];
+            rp.end = r[np1];
+            rp.lo = r[w.pLo];
+            rp.hi = r[w.pHi];
         }
+        return rp;
+    };
+    RowPre cur = load_row(active_from(f));
+    RowPre nxt = load_row(active_from(cur.f + 1));
+    while (cur.f < f1) {
+        const uint32_t fr = cur.f;
+        const bool sa = ea && fr >= aLo && fr < aHi;
+        const bool sb = eb && fr >= bLo && fr < bHi;
+        const uint32_t fEnd = cur.end, fLen = fEnd - cur.start;
+        const uint32_t end2 = wr * fEnd + cur.hi - wr * cur.start;
+        uint32_t j = cur.lo;
+        while (j < end2) {
+            float4 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const float z = q[u].x, ph = q[u].y;
-            const bool inr = j + u < end2;
-            const bool p1 = (w.phiLow <= ph) & (ph <= w.phiHigh);
-            const bool p2 = ((w.phiLow <= ph) & (ph <= PI)) | ((-PI <= ph) & (ph <= w.phiHigh));
-            const bool pin = inr & ((w.wrap & p2) | (!w.wrap & p1));
-            countA += (uint32_t)(pin & sa & (w.zLowA <= z) & (z <= w.zHighA));
-            if (pin & sb & (w.zLowB <= z) & (z <= w.zHighB)) {
-                const uint32_t rf = f >> fg.shift;
-                const bool new_row = rf != row, seg2 = j + u >= fEnd;
-                row = rf;
-                ord += (uint32_t)new_row;
-                if (nB < cap) {
-                    rmask |= new_row ? fine_bit<M>(nB) : (M)0;
-                    smask |= seg2 ? fine_bit<M>(nB) : (M)0;
-                    my[nB] = (uint16_t)(__float_as_uint(q[u].z) - first);
-                } else {
-                    over(ord, seg2, __float_as_uint(q[u].z));
-                }
-                nB++;
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t jj = j + u < end2 ? j + u : j;  // j < end2: always a hit of the row
+                q[u] = fh[jj - (uint32_t)(jj >= fEnd) * fLen];
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float z = q[u].x, ph = q[u].y;
+                const bool inr = j + u < end2;
+                const bool p1 = (w.phiLow <= ph) & (ph <= w.phiHigh);
+                const bool p2 = ((w.phiLow <= ph) & (ph <= PI)) | ((-PI <= ph) & (ph <= w.phiHigh));
+                const bool pin = inr & ((w.wrap & p2) | (!w.wrap & p1));
+                countA += (uint32_t)(pin & sa & (w.zLowA <= z) & (z <= w.zHighA));
+                if (pin & sb & (w.zLowB <= z) & (z <= w.zHighB)) {
+                    const uint32_t rf = fr >> fg.shift;
+                    const bool new_row = rf != row, seg2 = j + u >= fEnd;
+                    row = rf;
+                    ord += (uint32_t)new_row;
+                    if (nB < cap) {
+                        rmask |= new_row ? fine_bit<M>(nB) : (M)0;
+                        smask |= seg2 ? fine_bit<M>(nB) : (M)0;
+                        my[nB] = (uint16_t)(__float_as_uint(q[u].z) - first);
+                    } else {
+                        over(ord, seg2, __float_as_uint(q[u].z));
+                    }
+                    nB++;
+                }
+            }
+            j += 4;
         }
-        j += 4;
-        if (j >= end2) {
-            ++f;
-            active = next_row();
-        }
+        cur = nxt;
+        if (cur.f < f1) nxt = load_row(active_from(cur.f + 1));
     }
 }
 

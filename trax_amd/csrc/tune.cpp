@@ -30,7 +30,12 @@ struct Param {
 
 void set_param(trax_tuning& t, const char* name, uint32_t v) {
     const std::string n(name);
-    if (n == "grid_count_block") t.grid_count_block = v;
+    if (n == "grid") {
+        t.grid_count_block = v >> 16;
+        t.grid_stage_kib = (v >> 2) & 0x3fffu;
+        t.grid_order = v & 3u;
+    }
+    else if (n == "grid_count_block") t.grid_count_block = v;
     else if (n == "grid_store_block") t.grid_store_block = v;
     else if (n == "pair_block") t.pair_block = v;
     else if (n == "predict") {
@@ -42,10 +47,7 @@ void set_param(trax_tuning& t, const char* name, uint32_t v) {
     else if (n == "link_block") t.link_block = v;
     else if (n == "blocks_per_cu") t.blocks_per_cu = v;
     else if (n == "predict_fine") t.predict_fine = v;
-    else if (n == "grid_lds_hits") t.grid_lds_hits = v;
-    else if (n == "grid_order") t.grid_order = v;
-    else if (n == "grid_rank") t.grid_rank = v;
-    else if (n == "grid_gather") t.grid_gather = v;
+
 }
 
 LaunchCfg to_launch(const trax_tuning& t) { return launch_cfg(&t); }
@@ -113,11 +115,10 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
             {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
             {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
-            {"grid_count_block", {128, 256, 512, 1024}, {"grid_count"}},
-            {"grid_lds_hits", {2048, 4096, 0}, {"grid_count"}},
-            {"grid_order", {0, 1}, {"grid_count"}},
-            {"grid_rank", {0, 1}, {"grid_count"}},
-            {"grid_gather", {0, 1}, {"grid_count"}},
+            // the grid build: (block, LDS KiB, item order) together
+            {"grid", {1024u << 16 | 160u << 2 | 1u, 512u << 16 | 160u << 2 | 1u, 512u << 16 | 80u << 2 | 1u,
+                      1024u << 16 | 64u << 2 | 1u, 512u << 16 | 64u << 2 | 1u, 256u << 16 | 64u << 2 | 2u},
+             {"grid_count"}},
             {"blocks_per_cu", {2, 4, 8, 16}, {}},
         };
         std::string log;
@@ -135,6 +136,7 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
                 set_param(t, p.name, v);
                 b->lc = to_launch(t);
                 predict_fused_prepare(b->lc);
+                grid_build_prepare();
                 float ms = 0.0f;
                 bool same = true;
                 for (uint32_t r = 0; r < repeats; ++r) {
