@@ -177,7 +177,7 @@ def serve(dev, cfg, hs, bounds, rates_hz, seed, n_slots, max_batch_events, slot_
 PHASE_KERNEL = {"grid_count": "k_grid_build", "grid_store": "k_grid_build", "pair_count": "k_pair_count",
                 "pair_store": "k_pair_store", "predict": "k_predict_fused", "filter_count": "k_filter_count",
                 "filter_store": "k_filter_store", "link_count": "k_link_count", "link_store": "k_link_store"}
-PHASE_KERNEL_FUSED = dict(PHASE_KERNEL, predict="k_predict_filter", filter_store="k_filter_offsets")
+PHASE_KERNEL_FUSED = dict(PHASE_KERNEL, predict="k_predict_filter_wave", filter_store="k_filter_offsets")
 
 
 def measured_traffic(kernel):

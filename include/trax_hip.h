@@ -135,8 +135,13 @@ int trax_grid_build(trax_ctx* ctx, uint32_t n_events, uint32_t n_layers, uint32_
  * predictor's walk structure (1, 2, 4, 8 or 16; 0 = 8); and the batched grid
  * build's block (grid_count_block; 0 = 1024), item order (0 or 1 = largest
  * (event, layer) items first, 2 = item order) and LDS per block in KiB (up to
- * 160; above 64 the items that fit are staged whole in LDS; 0 = 160).
- * Results do not depend on them. */
+ * 160; above 64 the items that fit are staged whole in LDS; 0 = 160); and
+ * the fused predictor + filter's scheduling (every wave on its own 64-pair
+ * chunks, k = 1..64 consecutive chunks a claim, from the region of the pairs
+ * of its XCD first; 0 = 2; 128 = tiles of the block's pairs); and the pair
+ * kernels' block order (pair_xcd XCDs given contiguous slices of every
+ * stride; 0 = 8, 1 = the plain grid-stride order).  Results do not depend on
+ * them. */
 typedef struct {
     uint32_t grid_count_block, grid_store_block, pair_block, predict_block, filter_block, link_block;
     uint32_t blocks_per_cu;
@@ -144,6 +149,8 @@ typedef struct {
     uint32_t predict_sort;
     uint32_t predict_fine;
     uint32_t grid_order, grid_stage_kib;
+    uint32_t predict_wave;
+    uint32_t pair_xcd;
 } trax_tuning;
 
 /* ---- 2b. stage launchers ------------------------------------------------

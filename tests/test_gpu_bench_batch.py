@@ -5,7 +5,8 @@ fused, the stage graphs captured and replayed.
 
 - Two replays of the captured graphs give the same arrays.
 - The fused batch equals the unfused batch (the candidates stored, no graphs)
-  on every array but the candidates.
+  on every array but the candidates, scheduled by every wave on its own
+  chunks or by the block's tiles.
 - Eight events of it (its three largest, a median one, its smallest and three
   more) equal their runs alone through the plain path, every per-event array
   bit for bit; the largest, the median and the smallest also equal the
@@ -54,7 +55,8 @@ def bench_batch():
 @pytest.fixture(scope="module")
 def fused(dev, bench_batch):
     wl, cfg, hs, events = bench_batch
-    b, res = run_batch_fit(dev, cfg, hs, _caps(hs, N_EVENTS, use_graph=True, fuse_filter=True))
+    b, res = run_batch_fit(dev, cfg, hs, _caps(hs, N_EVENTS, use_graph=True, fuse_filter=True),
+                           tuning=dict(predict_wave=4))
     b.rerun()
     assert b.wait() == 0
     again = b.result(FUSED_KEYS)
@@ -80,6 +82,17 @@ def test_graph_replays_agree(fused):
 
 def test_fused_equals_unfused(fused, unfused):
     bad = compare(fused[0], unfused, FUSED_KEYS)
+    assert not bad, "\n".join(bad)
+
+
+def test_fused_tiles_equal_wave_chunks(dev, bench_batch, fused):
+    """The fused predictor + filter scheduled by the block's tiles gives the
+    arrays it gives with every wave on its own chunks, at full scale."""
+    wl, cfg, hs, events = bench_batch
+    b, res = run_batch_fit(dev, cfg, hs, _caps(hs, N_EVENTS, use_graph=True, fuse_filter=True),
+                           tuning=dict(predict_wave=128))
+    b.close()
+    bad = compare(res, fused[0], FUSED_KEYS)
     assert not bad, "\n".join(bad)
 
 

@@ -65,9 +65,10 @@ def test_fine_sub_rows_must_be_a_supported_power_of_two(dev, fine):
 
 
 @pytest.mark.parametrize("shape", [dict(filter_block=64, pair_block=64, link_block=64, grid_store_block=128,
-                                        grid_count_block=128, blocks_per_cu=2),
+                                        grid_count_block=128, blocks_per_cu=2, pair_xcd=1),
                                    dict(filter_block=1024, pair_block=512, link_block=512, grid_store_block=1024,
-                                        grid_count_block=1024, blocks_per_cu=16)])
+                                        grid_count_block=1024, blocks_per_cu=16, pair_xcd=8),
+                                   dict(pair_block=192, blocks_per_cu=3, pair_xcd=5)])
 def test_block_shapes_match_oracle(dev, shape):
     wl, cfg, hs, _ = _case()
     want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
@@ -105,7 +106,7 @@ def test_autotune_picks_supported_shapes_with_same_results(dev):
     caps = b.caps
     b.close()
     tuning, trials = autotune(dev, cfg, hs, caps, repeats=1)
-    assert len(trials) == 9 + 3 + 5 + 4 + 4 + 6 + 4
+    assert len(trials) == 9 + 3 + 4 + 5 + 4 + 2 + 4 + 6 + 4
     assert all(t["same_results"] for t in trials)
     assert (tuning["predict_block"], tuning["predict_list"], tuning["predict_sort"]) in PREDICT_VARIANTS
     assert tuning["blocks_per_cu"] in (2, 4, 8, 16)

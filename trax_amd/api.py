@@ -44,7 +44,7 @@ class BatchCaps(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [(n, U32) for n in ("grid_count_block", "grid_store_block", "pair_block", "predict_block",
                                    "filter_block", "link_block", "blocks_per_cu", "predict_list", "predict_sort",
-                                   "predict_fine", "grid_order", "grid_stage_kib")]
+                                   "predict_fine", "grid_order", "grid_stage_kib", "predict_wave", "pair_xcd")]
 
 
 class ServeOpts(C.Structure):

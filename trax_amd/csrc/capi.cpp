@@ -768,7 +768,8 @@ void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* c
 }
 
 uint32_t batch_scan_tiles(const trax_batch& B) {
-    return std::max(scan_tiles(B.scan_cap), predict_fused_tiles(B.caps.max_pairs));
+    // the status words also hold the fused predictor's work counters
+    return std::max({scan_tiles(B.scan_cap), predict_fused_tiles(B.caps.max_pairs), kPredCounterWords / 2});
 }
 
 }  // namespace
@@ -884,8 +885,10 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
         b->lc = launch_cfg(tuning);
         if (!predict_variant_supported(b->lc.predict_block, b->lc.predict_list, b->lc.predict_sort))
             throw InputError("unsupported fused predictor variant (block, list, sort)");
-        if (b->lc.grid_stage_kib > kGridLdsMax / 1024 || b->lc.grid_order > 2)
-            throw InputError("grid build: LDS per block at most 160 KiB, item order 0, 1 or 2");
+        if (b->lc.grid_stage_kib > kGridLdsMax / 1024 || b->lc.grid_order > 2 ||
+            (b->lc.predict_wave > 64 && b->lc.predict_wave != kPredTiles))
+            throw InputError("grid build: LDS per block at most 160 KiB, item order 0, 1 or 2; predictor "
+                             "chunks per claim 0 to 64, or 128 for tiles");
         for (uint32_t v : {b->lc.grid_count_block, b->lc.grid_store_block, b->lc.pair_block, b->lc.filter_block,
                            b->lc.link_block})
             if (v % 64 != 0 || v > 1024) throw InputError("block sizes must be multiples of 64 up to 1024");

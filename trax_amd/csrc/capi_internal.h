@@ -91,6 +91,8 @@ inline trax::LaunchCfg launch_cfg(const trax_tuning* t) {
     lc.predict_fine = t->predict_fine;
     lc.grid_order = t->grid_order;
     lc.grid_stage_kib = t->grid_stage_kib;
+    lc.predict_wave = t->predict_wave;
+    lc.pair_xcd = t->pair_xcd;
     return lc;
 }
 

@@ -105,6 +105,21 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A block index for grid-stride loops that gives each XCD a contiguous slice
+// of every stride: blocks are dispatched to the xcds XCDs round-robin
+// (block b to XCD b % xcds), and block b takes the slice position
+// (b % xcds) * (G / xcds) + b / xcds of the G blocks, so work items next to
+// each other, which read the same data, run on one XCD and share its L2.  A
+// bijection of [0, G) (the last G % xcds blocks keep their index); xcds <= 1
+// keeps every index.
+__device__ inline uint32_t xcd_block(uint32_t xcds) {
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    if (xcds <= 1) return b;
+    const uint32_t full = G - G % xcds;
+    if (b >= full) return b;
+    return (b % xcds) * (full / xcds) + b / xcds;
+}
+
 __device__ inline uint64_t lanemask_lt() {
     const uint32_t l = lane_id();
     return (l == 0) ? 0ull : ((~0ull) >> (64 - l));

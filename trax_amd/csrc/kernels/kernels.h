@@ -43,6 +43,9 @@ struct LaunchCfg {
     uint32_t predict_list = 0;   // fused predictor: LDS list entries per pair (0 = 24)
     uint32_t predict_sort = 0;   // fused predictor: walk a tile's pairs longest first
     uint32_t predict_fine = 0;   // fused predictor: z sub-rows per grid row (power of two; 0 = 8)
+    uint32_t pair_xcd = 0;       // pair kernels: stride slices per XCD (0 = 8, one per XCD; 1 = plain grid-stride)
+    uint32_t predict_wave = 0;   // fused predictor + filter: every wave on its own 64-pair chunks, k (1..64; 0 = 2)
+                                 // a claim; kPredTiles: tiles of the block's pairs
     uint32_t grid_order = 0;     // grid build: 0 or 1 = the largest (event, layer) items first, 2 = item order
     uint32_t grid_stage_kib = 0; // grid build: LDS per block in KiB (above 64: items staged whole; 0 = 160)
 };
@@ -84,6 +87,10 @@ struct FineBufs {
     bool prebuilt = false;           // grid and hits built by launch_fine_grid before the predictor's launch
 };
 constexpr uint32_t kFusedPoolCap = 8192, kFusedPoolBlocks = 2048;
+// the fused predictor's work counters: kPredRegions of them, a cache line
+// (kPredCounterStride u32) apart, at the start of the scan status words
+constexpr uint32_t kPredRegions = 8, kPredCounterStride = 16, kPredCounterWords = kPredRegions * kPredCounterStride;
+constexpr uint32_t kPredTiles = 128;  // LaunchCfg::predict_wave: the block-tiled pass
 // tracklet scratch of the fused predictor + filter for n_trk_cap tracklets
 uint64_t fused_scratch_cap(uint64_t n_trk_cap);
 

@@ -172,12 +172,12 @@ __global__ void k_pair_count(GridGeom g, TripletCols tc, const uint32_t* __restr
                              const float* __restrict__ hx, const float* __restrict__ hy,
                              const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, uint32_t* __restrict__ slot_counts,
-                             DevCounts* __restrict__ cnt, const uint32_t* __restrict__ slot_seg) {
+                             DevCounts* __restrict__ cnt, const uint32_t* __restrict__ slot_seg, uint32_t xcds) {
     if (run_void(cnt)) return;
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint32_t overlap = 0;
-    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot <= nslot; slot += gridDim.x * blockDim.x) {
+    for (uint32_t slot = xcd_block(xcds) * blockDim.x + threadIdx.x; slot <= nslot; slot += gridDim.x * blockDim.x) {
         if (slot == nslot) { slot_counts[slot] = 0; continue; }
         const SlotInfo si = slot_info(tc, seg_base, nseg, slot, slot_seg);
         const uint32_t lay2 = tc.l2[si.p] - 1, lay1 = tc.l1[si.p] - 1;
@@ -217,14 +217,14 @@ __global__ void k_pair_store(GridGeom g, TripletCols tc, const uint32_t* __restr
                              const float* __restrict__ hz, const float2* __restrict__ hzp,
                              const uint32_t* __restrict__ seg_base, const uint32_t* __restrict__ slot_counts,
                              uint32_t* __restrict__ pairs, uint32_t n_pair_cap, DevCounts* __restrict__ cnt,
-                             const uint32_t* __restrict__ slot_seg) {
+                             const uint32_t* __restrict__ slot_seg, uint32_t xcds) {
     if (run_void(cnt)) return;
     const uint32_t nslot = bounded(cnt->n_pair_slots, cnt->cap_slots, cnt), nseg = cnt->n_segments;
     const uint32_t np1 = g.n_phi + 1;
     uint2* __restrict__ pair2 = reinterpret_cast<uint2*>(pairs);  // (first hit, second hit), one 8-byte store
     uint32_t short_slots = 0;
     bool over = false;
-    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < nslot; slot += gridDim.x * blockDim.x) {
+    for (uint32_t slot = xcd_block(xcds) * blockDim.x + threadIdx.x; slot < nslot; slot += gridDim.x * blockDim.x) {
         uint32_t pos = slot_counts[slot];
         const uint32_t next_all = slot_counts[slot + 1];
         if (pos >= next_all) continue;
@@ -1476,6 +1476,371 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     if (over) atomicOr(&cnt->err, kErrCapacity);
 }
 
+// The same pass with every wave on its own: a wave claims a chunk of 64
+// pairs, walks them, tests their candidates and writes their tracklets with
+// wave-level synchronization only (its own LDS lists, candidate offsets and a
+// quarter of the block's pool), then claims the next chunk.  No wave waits at
+// a block barrier for the block's slowest walk.  Results are those of
+// k_predict_filter: the chunk is the same 64 pairs, its candidates the same
+// in the same order.
+template <int kPredBlock, int kPredList>
+__global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
+    GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
+    const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
+    const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
+    uint32_t* __restrict__ cand_offsets, uint32_t* __restrict__ th1, uint32_t* __restrict__ th2,
+    uint32_t* __restrict__ th3, uint32_t scratch_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
+    FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
+    uint2* __restrict__ pools,
+    uint32_t pool_cap, uint32_t* __restrict__ chunk_cands, uint32_t* __restrict__ chunk_trk,
+    uint32_t* __restrict__ chunk_at, uint32_t* __restrict__ group_chunk, uint32_t per_claim) {
+    using Mask = typename std::conditional<(kPredList <= 32), uint32_t, uint64_t>::type;
+    static_assert(kPredList <= 64, "list masks hold 64 entries");
+    constexpr uint32_t kWaves = kPredBlock / kWave, kQueue = 2 * kWave;
+    constexpr uint32_t kValidBit = 0x80000000u, kDeadBit = 0x40000000u;  // pool entry flags (x word)
+    __shared__ uint16_t s_list[kPredBlock * kPredList];
+    __shared__ uint32_t s_eex[kPredBlock + kWaves];  // per wave: listed candidates before each pair, then the total
+    __shared__ uint32_t s_first[kPredBlock];    // first hit of the pair's third layer
+    __shared__ float4 s_pair[kPredBlock];       // x2, y2, x1, y1: the pair's hits for the prefilter and cuts
+    __shared__ float2 s_z12[kPredBlock];        // z1, z2: with s_pair, the pair's hits for the angle cuts
+    // per wave: the pair (lane) of each of the chunk's first kOwnerCap listed
+    // candidates (one byte each; later ones search the offsets)
+    constexpr uint32_t kOwnerCap = kWave * 16;
+    __shared__ uint8_t s_owner[kWaves * kOwnerCap];
+    __shared__ uint32_t s_lt[kPredBlock];       // layer triplet | pool entries << 8
+    __shared__ Mask s_valid[kPredBlock];        // listed candidates that pass the cuts
+    __shared__ Mask s_dead[kPredBlock];         // listed matches that are not candidates
+    __shared__ Mask s_rm[kPredBlock], s_sm[kPredBlock];
+    __shared__ uint32_t s_pool_valid[kPredBlock];
+    __shared__ uint32_t s_queue[kWaves * kQueue];
+    __shared__ uint32_t s_queue2[kWaves * kQueue];
+    __shared__ uint32_t s_pool_n_w[kWaves], s_dead_any_w[kWaves];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    const uint32_t n = bounded(cnt->n_pairs, cnt->cap_pairs, cnt), n_events = cnt->n_events, nh = cnt->n_hits;
+    const uint32_t n_chunks = (n + kWave - 1) / kWave;
+    if (blockIdx.x == 0 && tid == 0) cnt->n_pred_chunks = n_chunks;
+    // the wave's pool: its share of the block's
+    const uint32_t pw_cap = pool_cap / kWaves;
+    uint2* pool = pools + (size_t)blockIdx.x * pool_cap + (size_t)wave * pw_cap;
+    uint32_t* eex = s_eex + wave * (kWave + 1);
+    uint8_t* owner = s_owner + wave * kOwnerCap;
+    uint32_t& s_pool_n = s_pool_n_w[wave];
+    uint32_t& s_dead_any = s_dead_any_w[wave];
+    uint32_t short_slots = 0, trunc = 0;
+    uint32_t range_at = 0, range_left = 0;  // the wave's scratch range (wave-uniform)
+    bool over = false;
+    if (lane == 0) {
+        s_pool_n = 0;
+        s_dead_any = 0;
+    }
+    // a run voided by an earlier kernel is not walked (this kernel's own pool
+    // and scratch writes are bounded by their capacities)
+    if (block_run_void(cnt)) return;
+    // chunks of 64 pairs from work queues: chunks differ in cost, so a wave
+    // takes the next ones when free and never waits for the block's other
+    // waves.  The chunks are split into kPredRegions contiguous regions, one
+    // counter each (kPredCounterStride words apart); a block starts on the
+    // region of its XCD (blocks go to the XCDs round-robin), so the waves of
+    // one XCD work near one front of the pairs and its L2 holds their hits,
+    // and a wave whose region is done moves on to the next.  A claim takes
+    // per_claim consecutive chunks.
+    const uint32_t n_work = n_chunks ? n_chunks : 1u;
+    const uint32_t home = blockIdx.x % kPredRegions;
+    uint32_t chunk = 0, left = 0, region = 0;
+    while (true) {
+        if (left == 0) {
+            for (; region < kPredRegions; ++region) {
+                const uint32_t r = (home + region) % kPredRegions;
+                const uint32_t r0 = (uint32_t)((uint64_t)n_work * r / kPredRegions);
+                const uint32_t r1 = (uint32_t)((uint64_t)n_work * (r + 1) / kPredRegions);
+                uint32_t claimed = 0;
+                if (lane == 0) claimed = atomicAdd(&counter[r * kPredCounterStride], per_claim);
+                claimed = __shfl(claimed, 0, kWave);
+                if (claimed < r1 - r0) {
+                    chunk = r0 + claimed;
+                    left = per_claim < r1 - chunk ? per_claim : r1 - chunk;
+                    break;
+                }
+            }
+            if (region >= kPredRegions) break;  // wave-uniform: every region is done
+        }
+        uint32_t countA = 0, nB = 0, listed = 0, g_end = kSentinel;
+        Mask rm = 0, sm = 0;
+        {
+            // the pair's hits and layer triplet go to LDS before the walk (fewer registers held across it)
+            const uint32_t gid = chunk * kWave + lane;
+            PredWin2 pw;
+            bool have = false;
+            uint32_t h1 = kSentinel, h2 = kSentinel, first = 0, lt = 0;
+            if (gid < n) {
+                h1 = pairs[2 * (size_t)gid];
+                h2 = pairs[2 * (size_t)gid + 1];
+                // whether the pair ends its candidate group (loads issued with the window's)
+                g_end = pred_group_end_index(tc, hlayer, hevt, pairs, n, n_events, gid, h1, nh);
+                have = pair_hits_ok(h1, h2, nh, cnt) &&
+                       pred_windows(g, tc, grid, hx, hy, hz, hzp, hlayer, hevt, n_events, h1, h2, pw);
+            }
+            if (have) {
+                lt = pw.lt;
+                first = pw.lg3[0];
+            }
+            s_first[tid] = first;
+            s_lt[tid] = lt;
+            if (have) {
+                const float2 p1 = hxy[h1], p2 = hxy[h2];
+                s_pair[tid] = make_float4(p2.x, p2.y, p1.x, p1.y);
+                s_z12[tid] = make_float2(hz[h1], hz[h2]);
+            }
+            if (have) {
+                const uint32_t cap = pw.lg3[g.cells - 1] - first <= 0xffffu ? (uint32_t)kPredList : 0u;
+                pred_walk_fine<Mask>(pw, g, fg, fgrid + (size_t)pw.el * fg.cells, fh, cap, first,
+                                     s_list + tid * kPredList, countA, nB, rm, sm,
+                                     [&](uint32_t ord, bool seg2, uint32_t h3) {
+                                         const uint32_t slot = atomicAdd(&s_pool_n, 1u);
+                                         if (slot < pw_cap)
+                                             pool[slot] = make_uint2(tid | (uint32_t)seg2 << 16 | (ord & 0x1fffu) << 17,
+                                                                     h3);
+                                     });
+                listed = nB < cap ? nB : cap;
+            }
+        }
+        // the wave's candidates: chunk-local offsets and the chunk count
+        const uint32_t incl_c = wave_incl_scan(countA);
+        const uint32_t excl_c = incl_c - countA;
+        if (lane == kWave - 1) chunk_cands[chunk] = incl_c;
+        const uint32_t incl_e = wave_incl_scan(listed);
+        const uint32_t excl_e = incl_e - listed, total_e = __shfl(incl_e, kWave - 1, kWave);
+        for (uint32_t i = excl_e; i < excl_e + listed && i < kOwnerCap; ++i) owner[i] = (uint8_t)lane;
+        eex[lane] = excl_e;
+        s_lt[tid] |= (nB - listed) << 8;
+        s_valid[tid] = 0;
+        s_dead[tid] = 0;
+        s_rm[tid] = rm;
+        s_sm[tid] = sm;
+        s_pool_valid[tid] = 0;
+        if (lane == 0) eex[kWave] = total_e;
+        if (nB > countA && nB > 0) s_dead_any = 1;
+        wave_sync();
+        const uint32_t n_pool = s_pool_n < pw_cap ? s_pool_n : pw_cap;
+        if (lane == 0 && s_pool_n > pw_cap) atomicOr(&cnt->err, kErrCapacity);
+        if (chunk * kWave + lane < n) {
+            short_slots += countA > nB ? countA - nB : 0u;
+            trunc += (countA > 0 && nB > countA);
+            // the group's end, chunk-local; k_group_offsets adds the chunk's offset
+            if (g_end != kSentinel) {
+                cand_offsets[g_end] = excl_c + countA;
+                group_chunk[g_end] = chunk;
+            }
+        }
+        // entry keys in the reference's order: (grid-row ordinal, phi segment, grid index)
+        auto list_key = [&](uint32_t lo, uint32_t i) {
+            const Mask r = s_rm[lo];
+            const uint64_t ord = (uint64_t)__builtin_popcountll((uint64_t)(r & (((Mask)2 << i) - 1)));
+            return ord << 33 | (uint64_t)((s_sm[lo] >> i) & 1u) << 32 | (s_first[lo] + s_list[lo * kPredList + i]);
+        };
+        auto pool_key = [](uint2 e) { return (uint64_t)((e.x >> 17) & 0x1fffu) << 33 | (uint64_t)((e.x >> 16) & 1u) << 32 | e.y; };
+        // 2. only the first countA matches (in the reference's order) are candidates
+        const bool prune = s_dead_any != 0;  // uniform: rarely any pair of a chunk needs it
+        if (prune && nB > countA && nB > 0) {
+            const uint32_t n_own_pool = nB - listed;
+            auto rank = [&](uint64_t k) {
+                uint32_t r = 0;
+                for (uint32_t i = 0; i < listed; ++i) r += list_key(tid, i) < k;
+                if (n_own_pool)
+                    for (uint32_t p = 0; p < n_pool; ++p) {
+                        const uint2 e = pool[p];
+                        r += (e.x & 0xffffu) == tid && pool_key(e) < k;
+                    }
+                return r;
+            };
+            Mask dead = 0;
+            for (uint32_t i = 0; i < listed; ++i) dead |= rank(list_key(tid, i)) >= countA ? (Mask)1 << i : (Mask)0;
+            s_dead[tid] = dead;
+            if (n_own_pool)
+                for (uint32_t p = 0; p < n_pool; ++p) {
+                    const uint2 e = pool[p];
+                    if ((e.x & 0xffffu) == tid && rank(pool_key(e)) >= countA) pool[p].x = e.x | kDeadBit;
+                }
+        }
+        if (prune) wave_sync();
+        // 3. the chunk's candidates against the filter's cuts: listed entries
+        //    [0, total_e), then the pool [total_e, total_e + n_pool)
+        {
+            uint32_t* q = s_queue + wave * kQueue;
+            uint32_t qn = 0;
+            const uint32_t n_all = total_e + n_pool;
+            const uint32_t e0 = 0, e1 = n_all;
+            // a candidate's pair (in the block) and third hit: tag = lo << 16 | i
+            // (listed) or kValidBit | p (pool)
+            auto third_of = [&](uint32_t tag, uint32_t& c) {
+                if (tag & kValidBit) {
+                    const uint2 e = pool[tag & ~kValidBit];
+                    c = e.y;
+                    return e.x & 0xffffu;
+                }
+                const uint32_t lo = tag >> 16;
+                c = s_first[lo] + s_list[lo * kPredList + (tag & 0xffffu)];
+                return lo;
+            };
+            // The cuts are a conjunction of pure expressions, tested in two
+            // compacted stages: the circle cuts first (they reject most of
+            // what the phi prefilter keeps), then the phi and theta cuts (two
+            // atan2) on the candidates left, 64 at a time from a second queue.
+            uint32_t* q2 = s_queue2 + wave * kQueue;
+            uint32_t qn2 = 0;
+            auto test_angles = [&](uint32_t k0) {
+                const uint32_t tag = q2[k0 + lane];
+                uint32_t c;
+                const uint32_t lo = third_of(tag, c);
+                const float4 pr = s_pair[lo];  // x2, y2, x1, y1
+                const float2 z12 = s_z12[lo];
+                const float2 p3 = hxy[c];
+                const uint32_t l = s_lt[lo] & 0xffu;
+                // the pair's angles, for the few candidates that get this far
+                const float phi12 = filter_pair_phi(pr.z, pr.w, pr.x, pr.y);
+                const float theta12 = filter_pair_theta(pr.z, pr.w, z12.x, pr.x, pr.y, z12.y);
+                if (filter_phi_cut(tc, l, pr.x, pr.y, p3.x, p3.y, phi12) &&
+                    filter_theta_cut(tc, l, pr.x, pr.y, z12.y, p3.x, p3.y, hz[c], theta12)) {
+                    if (tag & kValidBit) {
+                        atomicAdd(&s_pool_valid[lo], 1u);
+                        pool[tag & ~kValidBit].x |= kValidBit;
+                    } else {
+                        atomicOr(&s_valid[lo], (Mask)1 << (tag & 0xffffu));
+                    }
+                }
+            };
+            // called by the whole wave: the circle cuts on q[k0 + lane] for
+            // the active lanes, those passing appended to the second queue
+            auto test_queue = [&](uint32_t k0, bool active) {
+                bool pass = false;
+                uint32_t tag = 0;
+                if (active) {
+                    tag = q[k0 + lane];
+                    uint32_t c;
+                    const uint32_t lo = third_of(tag, c);
+                    const float4 pr = s_pair[lo];  // the pair's hits, read once per chunk from HBM
+                    const float2 p3 = hxy[c];
+                    pass = filter_circle_cuts(tc, s_lt[lo] & 0xffu, pr.z, pr.w, pr.x, pr.y, p3.x, p3.y);
+                }
+                const uint64_t m = __ballot(pass);
+                if (pass) q2[qn2 + (uint32_t)__popcll(m & lanemask_lt())] = tag;
+                qn2 += (uint32_t)__popcll(m);
+                wave_sync();
+                if (qn2 >= kWave) {
+                    qn2 -= kWave;
+                    test_angles(qn2);
+                    wave_sync();
+                }
+            };
+            for (uint32_t eb = e0; eb < e1; eb += kWave) {
+                const uint32_t e = eb + lane;
+                bool undecided = false;
+                uint32_t tag = 0;
+                if (e < e1) {
+                    if (e < total_e) {
+                        uint32_t ol = 0;
+                        if (e < kOwnerCap) {
+                            ol = owner[e];
+                        } else {
+                            uint32_t hi = kWave;
+                            while (hi - ol > 1) {
+                                const uint32_t mid = (ol + hi) >> 1;
+                                if (eex[mid] <= e) ol = mid; else hi = mid;
+                            }
+                        }
+                        const uint32_t i = e - eex[ol];
+                        const uint32_t lo = wave * kWave + ol;
+                        tag = lo << 16 | i;
+                        undecided = !((s_dead[lo] >> i) & 1u);
+                    } else {
+                        tag = kValidBit | (e - total_e);
+                        undecided = !(pool[e - total_e].x & kDeadBit);
+                    }
+                    if (undecided) {
+                        uint32_t c;
+                        const uint32_t lo = third_of(tag, c);
+                        const float4 pr = s_pair[lo];
+                        const float2 p3 = hxy[c];
+                        undecided = !tx_phi_surely_outside(pr.x - pr.z, pr.y - pr.w, p3.x - pr.x, p3.y - pr.y,
+                                                           tc.tan_dphi[s_lt[lo] & 0xffu]);
+                    }
+                }
+                const uint64_t m = __ballot(undecided);
+                if (undecided) q[qn + (uint32_t)__popcll(m & lanemask_lt())] = tag;
+                qn += (uint32_t)__popcll(m);
+                wave_sync();
+                if (qn >= kWave) {
+                    qn -= kWave;
+                    test_queue(qn, true);
+                    wave_sync();
+                }
+            }
+            test_queue(0, lane < qn);
+            if (lane < qn2) test_angles(0);
+        }
+        wave_sync();
+        // 4. the pair's valid triplets: the wave's room in the scratch, then
+        //    every pair's triplets in the reference's order
+        const Mask vm = s_valid[tid];
+        const uint32_t nv = (uint32_t)__builtin_popcountll((uint64_t)vm) + s_pool_valid[tid];
+        const uint32_t incl_t = wave_incl_scan(nv);
+        const uint32_t total_t = __shfl(incl_t, kWave - 1, kWave);
+        if (total_t > range_left) {  // uniform
+            const uint32_t size = total_t > kChunkRange ? total_t : kChunkRange;
+            uint32_t a = 0;
+            if (lane == 0) a = atomicAdd(&cnt->trk_alloc, size);
+            range_at = __shfl(a, 0, kWave);
+            range_left = size;
+        }
+        const uint32_t at = range_at;
+        range_at += total_t;
+        range_left -= total_t;
+        if (lane == kWave - 1) {
+            chunk_trk[chunk] = total_t;
+            chunk_at[chunk] = at;
+        }
+        if (nv) {
+            const uint32_t o = at + incl_t - nv, pv = s_pool_valid[tid];
+            auto write = [&](uint64_t k, uint32_t h3) {
+                uint32_t r = 0;
+                for (Mask m = vm; m; m &= m - 1) r += list_key(tid, (uint32_t)__builtin_ctzll((uint64_t)m)) < k;
+                if (pv)
+                    for (uint32_t p = 0; p < n_pool; ++p) {
+                        const uint2 e = pool[p];
+                        r += (e.x & 0xffffu) == tid && (e.x & kValidBit) && pool_key(e) < k;
+                    }
+                const uint64_t pos = (uint64_t)o + r;
+                if (pos < scratch_cap) {
+                    th1[pos] = pairs[2 * (size_t)(chunk * kWave + lane)];
+                    th2[pos] = pairs[2 * (size_t)(chunk * kWave + lane) + 1];
+                    th3[pos] = h3;
+                } else {
+                    over = true;
+                }
+            };
+            for (Mask m = vm; m; m &= m - 1) {
+                const uint32_t i = (uint32_t)__builtin_ctzll((uint64_t)m);
+                write(list_key(tid, i), s_first[tid] + s_list[tid * kPredList + i]);
+            }
+            if (pv)
+                for (uint32_t p = 0; p < n_pool; ++p) {
+                    const uint2 e = pool[p];
+                    if ((e.x & 0xffffu) == tid && (e.x & kValidBit)) write(pool_key(e), e.y);
+                }
+        }
+        if (lane == 0) {  // read into registers above; cleared for the next chunk
+            s_pool_n = 0;
+            s_dead_any = 0;
+        }
+        wave_sync();  // the wave's LDS lists and pool are reused by its next chunk
+        ++chunk;
+        --left;
+    }
+    if (short_slots) atomicAdd(&cnt->diag_pred_short, short_slots);
+    if (trunc) atomicAdd(&cnt->diag_pred_trunc, trunc);
+    if (over) atomicOr(&cnt->err, kErrCapacity);
+}
+
 }  // namespace
 
 hipError_t launch_pair_plan(const GridGeom& g, const TripletCols& tc, uint32_t n_seg_cap,
@@ -1501,7 +1866,7 @@ hipError_t launch_pair_count(const GridGeom& g, const TripletCols& tc, uint32_t 
     (void)n_seg_cap;
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     hipLaunchKernelGGL(k_pair_count, dim3(stride_blocks((uint64_t)n_slot_cap + 1, b, lc)), dim3(b), 0, st, g,
-                       tc, grid, hx, hy, hz, hzp, seg_base, slot_counts, cnt, slot_seg);
+                       tc, grid, hx, hy, hz, hzp, seg_base, slot_counts, cnt, slot_seg, lc.pair_xcd ? lc.pair_xcd : 8u);
     return hipGetLastError();
 }
 
@@ -1514,7 +1879,8 @@ hipError_t launch_pair_store(const GridGeom& g, const TripletCols& tc, uint32_t 
     const uint32_t b = lc.pair_block ? lc.pair_block : 256;
     if (reinterpret_cast<uintptr_t>(pairs) & 7u) return hipErrorInvalidValue;  // written as 8-byte pairs
     hipLaunchKernelGGL(k_pair_store, dim3(stride_blocks(n_slot_cap, b, lc)), dim3(b), 0, st, g, tc, grid, hx,
-                       hy, hz, hzp, seg_base, slot_counts, pairs, n_pair_cap, cnt, slot_seg);
+                       hy, hz, hzp, seg_base, slot_counts, pairs, n_pair_cap, cnt, slot_seg,
+                       lc.pair_xcd ? lc.pair_xcd : 8u);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pair_offsets, dim3((n_seg_cap + 1 + 255) / 256), dim3(256), 0, st, seg_base,
@@ -1620,19 +1986,42 @@ uint32_t resident_blocks_filter() {
 }
 
 template <int B, int L>
+uint32_t resident_blocks_filter_wave() {
+    static uint32_t resident = 0;
+    if (resident == 0) {
+        int per_cu = 0, dev = 0, n_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_predict_filter_wave<B, L>, B, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu < 1)
+            n_cu = 256;
+        resident = (uint32_t)(per_cu * n_cu);
+    }
+    return resident;
+}
+
+template <int B, int L>
 hipError_t predict_filter_variant(const GridGeom& g, const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_trk_cap,
                                   const uint32_t* grid, const float* hx, const float* hy, const float* hz,
                                   const float2* hzp, const uint32_t* hlayer, const uint32_t* hevt,
                                   const uint32_t* pairs, uint32_t* cand_offsets, uint32_t* th1, uint32_t* th2,
                                   uint32_t* th3, DevCounts* cnt, uint32_t* counter, const FineGeom& fg,
-                                  const FineBufs& fb, hipStream_t st) {
-    uint32_t resident = resident_blocks_filter<B, L>();
+                                  const FineBufs& fb, uint32_t per_wave, hipStream_t st) {
+    uint32_t resident = per_wave ? resident_blocks_filter_wave<B, L>() : resident_blocks_filter<B, L>();
     resident = resident < fb.pool_blocks ? resident : fb.pool_blocks;  // one pool per block
     const uint64_t tiles = (uint64_t)n_pair_cap / B + 1;
     const uint32_t blocks = (uint32_t)(tiles < resident ? tiles : resident);
-    hipLaunchKernelGGL((k_predict_filter<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp, hlayer,
-                       hevt, pairs, cand_offsets, th1, th2, th3, n_trk_cap, cnt, counter, fg, fb.grid, fb.hits,
-                       fb.xy, fb.pool, fb.pool_cap, fb.chunk_cands, fb.chunk_trk, fb.chunk_at, fb.group_chunk);
+    if (per_wave)
+        hipLaunchKernelGGL((k_predict_filter_wave<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp,
+                           hlayer, hevt, pairs, cand_offsets, th1, th2, th3, n_trk_cap, cnt, counter, fg, fb.grid,
+                           fb.hits, fb.xy, fb.pool, fb.pool_cap, fb.chunk_cands, fb.chunk_trk, fb.chunk_at,
+                           fb.group_chunk, per_wave);
+    else
+        hipLaunchKernelGGL((k_predict_filter<B, L>), dim3(blocks), dim3(B), 0, st, g, tc, grid, hx, hy, hz, hzp,
+                           hlayer, hevt, pairs, cand_offsets, th1, th2, th3, n_trk_cap, cnt, counter, fg, fb.grid,
+                           fb.hits, fb.xy, fb.pool, fb.pool_cap, fb.chunk_cands, fb.chunk_trk, fb.chunk_at,
+                           fb.group_chunk);
     return hipGetLastError();
 }
 
@@ -1659,6 +2048,7 @@ void predict_fused_prepare(const LaunchCfg& lc) {
     if (b == B && l == L && s == S) {                        \
         (void)resident_blocks<B, L, S>();                    \
         if (!S) (void)resident_blocks_filter<B, L>();        \
+        if (!S) (void)resident_blocks_filter_wave<B, L>();   \
     }
     TRAX_PRED_VARIANTS(TRAX_PRED_PREP)
 #undef TRAX_PRED_PREP
@@ -1711,6 +2101,12 @@ hipError_t launch_fine_grid(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit
     return hipGetLastError();
 }
 
+// chunks per claim of the wave-scheduled pass (0: the block's tiles)
+uint32_t predict_wave_claim(const LaunchCfg& lc) {
+    if (lc.predict_wave == kPredTiles) return 0;
+    return lc.predict_wave ? lc.predict_wave : 2u;
+}
+
 hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint32_t n_el_cap, uint32_t n_hit_cap,
                                  uint32_t n_pair_cap, uint32_t n_trk_cap, const uint32_t* grid, const float* hx,
                                  const float* hy, const float* hz, const float2* hzp, const uint32_t* hlayer,
@@ -1730,8 +2126,10 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
     if ((e = launch_fill(fb.group_chunk, kSentinel, (uint64_t)n_el_cap / g.n_layers * tc.n_trip + 1, st)) !=
         hipSuccess)
         return e;
+    // the tile counter, or the wave chunks' region counters
     uint32_t* counter = reinterpret_cast<uint32_t*>(ss.status);
-    if ((e = launch_fill(counter, 0u, 1, st)) != hipSuccess) return e;
+    if ((uint64_t)ss.max_tiles + 1 < kPredCounterWords / 2) return hipErrorInvalidValue;
+    if ((e = launch_fill(counter, 0u, kPredCounterWords, st)) != hipSuccess) return e;
     const uint32_t b = lc.predict_block ? lc.predict_block : 256, l = lc.predict_list ? lc.predict_list : 24;
     if (lc.predict_sort) return hipErrorInvalidConfiguration;
     e = hipErrorInvalidConfiguration;
@@ -1739,7 +2137,8 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
     if (!S && b == B && l == L)                                                                                   \
         e = predict_filter_variant<B, L>(g, tc, n_pair_cap, fb.scratch_cap, grid, hx, hy, hz, hzp, hlayer, hevt,    \
                                          pairs, cand_offsets, fb.scratch, fb.scratch + fb.scratch_cap,               \
-                                         fb.scratch + 2 * (size_t)fb.scratch_cap, cnt, counter, fg, fb, st);
+                                         fb.scratch + 2 * (size_t)fb.scratch_cap, cnt, counter, fg, fb,        \
+                                         predict_wave_claim(lc), st);
     TRAX_PRED_VARIANTS(TRAX_PF_RUN)
 #undef TRAX_PF_RUN
     if (e != hipSuccess) return e;

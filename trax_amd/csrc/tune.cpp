@@ -36,6 +36,8 @@ void set_param(trax_tuning& t, const char* name, uint32_t v) {
         t.grid_order = v & 3u;
     }
     else if (n == "grid_count_block") t.grid_count_block = v;
+    else if (n == "predict_wave") t.predict_wave = v;
+    else if (n == "pair_xcd") t.pair_xcd = v;
     else if (n == "grid_store_block") t.grid_store_block = v;
     else if (n == "pair_block") t.pair_block = v;
     else if (n == "predict") {
@@ -112,8 +114,10 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
                          (256u << 16) | 48u, (256u << 16) | 64u, (512u << 16) | 24u, (512u << 16) | 32u,
                          1u << 31 | (256u << 16) | 32u}, {"predict"}},
             {"predict_fine", {4, 8, 16}, {"predict"}},
+            {"predict_wave", {1, 2, 4, 128}, {"predict"}},
             {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
+            {"pair_xcd", {1, 8}, {"pair_count", "pair_store"}},
             {"link_block", {64, 128, 256, 512}, {"link_prepare", "link_count", "link_store"}},
             // the grid build: (block, LDS KiB, item order) together
             {"grid", {1024u << 16 | 160u << 2 | 1u, 512u << 16 | 160u << 2 | 1u, 512u << 16 | 80u << 2 | 1u,
