@@ -51,6 +51,8 @@ def phase_bytes(n_hits, seg_hits, trip_hits, counts, cells, n_threads, fused):
             "pair_scan": 8 * slots,
             "pair_store": 12 * seg_hits + 4 * slots + 8 * P,
             "predict": 8 * P + 16 * trip_hits + 12 * R,
+            # the pass's tracklets copied from the waves' ranges into pair order, chunk counts scanned
+            "filter_count": 24 * R + 16 * ((P + 63) // 64),
             "filter_store": 12 * R,
             "link_count": 12 * R + 4 * R + 8 * R,
             "link_store": 12 * R + 4 * R + 12 * K,
@@ -177,7 +179,8 @@ def serve(dev, cfg, hs, bounds, rates_hz, seed, n_slots, max_batch_events, slot_
 PHASE_KERNEL = {"grid_count": "k_grid_build", "grid_store": "k_grid_build", "pair_count": "k_pair_count",
                 "pair_store": "k_pair_store", "predict": "k_predict_fused", "filter_count": "k_filter_count",
                 "filter_store": "k_filter_store", "link_count": "k_link_count", "link_store": "k_link_store"}
-PHASE_KERNEL_FUSED = dict(PHASE_KERNEL, predict="k_predict_filter_wave", filter_store="k_filter_offsets")
+PHASE_KERNEL_FUSED = dict(PHASE_KERNEL, predict="k_predict_filter_wave", filter_count="k_chunk_copy3",
+                          filter_store="k_filter_offsets")
 
 
 def measured_traffic(kernel):
@@ -192,6 +195,24 @@ def measured_traffic(kernel):
             if p and not line.startswith("#") and p[0] == kernel and len(p) == 4:
                 return (int(p[2]) + int(p[3])) // max(int(p[1]), 1)
     return None
+
+
+def end_to_end(dev, cfg, events, bounds, batch_hits, fused, reps=4):
+    """Events/s from the first H2D to the last D2H: `reps` copies of the
+    step's events, all arriving at once, through the serving loop with two
+    slots that each hold a whole step (256 events); every event's hits go up
+    from pinned host memory and its tracklets and CA tracks come back."""
+    import numpy as np
+    from trax_amd.api import HitSet, Server
+    hs = HitSet(cfg, [ev for _ in range(reps) for ev in events])
+    sv = Server(dev, cfg, bounds, n_slots=2, max_batch_events=len(events), max_batch_hits=batch_hits + 4096,
+                device_budget=64 << 30, fuse_filter=fused, group_by_size=False)
+    sv.replay(hs, np.zeros(hs.n_events))  # warm-up: graph capture of both slots
+    _, st = sv.replay(hs, np.zeros(hs.n_events))
+    sv.close()
+    return {"events": hs.n_events, "events_per_s": round(st["events_per_s"], 1), "batches": st["batches"],
+            "mean_batch_events": round(st["mean_batch_events"], 2), "slots": 2,
+            "mean_ms": {k: round(st[k + "_ms"], 3) for k in ("pipeline", "copy", "split")}}
 
 
 def cpu_baseline(wl, events, cfg_cls, hitset_cls, n_sample=32):
@@ -273,6 +294,10 @@ def main():
                     help="predictor and filter as separate passes (the candidates stored) instead of one")
     ap.add_argument("--tuning", type=str, default=None,
                     help="fixed launch shapes as JSON (e.g. the autotuner's choice), skipping the autotuner")
+    ap.add_argument("--e2e-reps", type=int, default=4,
+                    help="copies of the step's events in the end-to-end (first H2D to last D2H) run (0 = skip)")
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the float64 adjudication of the filter's decisions (a run for the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (traffic-counter calibration)")
     ap.add_argument("--serve-events", type=int, default=10000, help="events of the serving stream (0 = skip)")
@@ -360,31 +385,25 @@ def main():
     trip_s = counts["tracklets"] / (ms_step / 1e3)
     phase_ms = {p: acc[p] / args.steps for p in phases}
 
+    b.close()  # the servers below hold their own workspaces
     serving, serving_sweep = None, []
     if stream is not None:
-        b.close()  # the server holds its own workspaces
         print("serving stream", file=sys.stderr, flush=True)
         rates = [args.serve_rate] + [float(r) for r in args.serve_sweep.split(",") if r.strip()]
         for group in (True, False):
             serving_sweep += serve(dev, stream[0], stream[1], bounds, rates, SEED + 2, args.serve_slots,
                                    args.serve_max_batch, args.serve_slot_hits, not args.no_fuse, group)
         serving = serving_sweep[0]  # grouped by hit count, at the stated rate
-        b = Batch(dev, cfg, caps, tuning)
-        b.submit(hs)
-        b.wait()
 
-    # from the first H2D to the last D2H: the hits uploaded, the pipeline, and
-    # the tracklets and CA tracks downloaded, every step
-    out_names = ["tracklets.h1", "tracklets.h2", "tracklets.h3", "tracklet_offsets", "tracks", "track_offsets"]
-    e2e = []
-    for _ in range(5):
-        t2 = time.perf_counter()
-        b.submit(hs)
-        if b.wait() != 0:
-            raise RuntimeError("capacity overflow")
-        b.result(out_names)
-        e2e.append(time.perf_counter() - t2)
-    e2e_ev_s = N_EVENTS / (sum(e2e) / len(e2e))
+    # from the first H2D to the last D2H: the step's events, four times over,
+    # all waiting at the start, through the continuous-batching server (hits
+    # uploaded from pinned host memory, tracklets and CA tracks downloaded and
+    # split per event, two batches in flight so transfers overlap compute)
+    e2e, e2e_ev_s = None, None
+    if args.e2e_reps > 0:
+        e2e = end_to_end(dev, cfg, events, bounds, hs.n_hits, not args.no_fuse, reps=args.e2e_reps)
+        e2e_ev_s = e2e["events_per_s"]
+        print(f"end to end: {e2e}", file=sys.stderr, flush=True)
 
     # plain per-event path on a sample of the same events
     n_plain = min(args.plain_events, N_EVENTS)
@@ -400,7 +419,7 @@ def main():
         # like-for-like one)
         mix = (hs.n_hits / N_EVENTS) / (sample_hits / n_plain)
         batch_ev_s_same_mix = ev_s * mix
-        batch_e2e_ev_s_same_mix = e2e_ev_s * mix
+        batch_e2e_ev_s_same_mix = e2e_ev_s * mix if e2e_ev_s else None
 
     if args.copy_calibration:
         # a plain copy of known size: HBM traffic counters are scaled by what
@@ -422,7 +441,7 @@ def main():
     print(f"scan alone: {scan}", file=sys.stderr, flush=True)
     caps64 = Caps(N_EVENTS, hs.n_hits, caps.max_pairs, head(c["cands"]), caps.max_tracklets, caps.max_conn,
                   caps.max_track_entries, use_graph=False, per_event=True, fuse_filter=False)
-    fp64 = fp64_check(dev, wl, cfg, hs, caps64)
+    fp64 = None if args.no_fp64_check else fp64_check(dev, wl, cfg, hs, caps64)
     print(f"float64 check: {fp64}", file=sys.stderr, flush=True)
 
     # roofline of the dominant phase, HBM-bound
@@ -466,8 +485,10 @@ def main():
         "plain_events_per_s": round(plain_ev_s, 1) if plain_ev_s else None,
         "plain_sample_events": n_plain,
         "batched_vs_plain_same_mix": round(batch_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s else None,
-        "events_per_s_h2d_to_d2h": round(e2e_ev_s, 1),
-        "batched_h2d_to_d2h_vs_plain_same_mix": round(batch_e2e_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s else None,
+        "events_per_s_h2d_to_d2h": e2e_ev_s,
+        "h2d_to_d2h": e2e,
+        "batched_h2d_to_d2h_vs_plain_same_mix":
+            round(batch_e2e_ev_s_same_mix / plain_ev_s, 2) if plain_ev_s and e2e_ev_s else None,
         "grid_build_hbm_gbs": round(grid_gbs, 1),
         "grid_build_hbm_frac": round(grid_gbs * 1e9 / HBM_PEAK, 4),
         "scan_hbm": scan,
@@ -484,7 +505,6 @@ def main():
         "serving_sweep": [{k: r[k] for k in ("admission", "rate_hz", "events_per_s", "p50_ms", "p99_ms",
                                              "mean_batch_events", "batches")} for r in serving_sweep],
     }
-    b.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(wl, events, Config, HitSet)
     if args.runtime_sweep:

@@ -665,7 +665,8 @@ void trax_batch::enqueue_stage(int s) {
                                  slot_counts, pairs, pair_off, cnt, lc, st, slot_seg),
                "pair store");
             break;
-        case 6:  // predict: count, scan and store in one pass (with the filter when fused)
+        case 6:  // predict: count, scan and store in one pass (with the filter when fused, its outputs placed in
+                 // the filter_count phase)
             ok(launch_fill(cand_off, 0u, nseg + 1ull, st), "fill");
             if (caps.fuse_filter)
                 ok(launch_predict_filter(g, tc, E * L, N, P, R, grid, hout.x, hout.y, hout.z, hout.zphi, hout.layer,
@@ -676,8 +677,12 @@ void trax_batch::enqueue_stage(int s) {
                                         hout.evt, pairs, cands, cand_off, cnt, fine, scan, lc, st),
                    "predict");
             break;
-        case 7:  // filter_count
-            if (caps.fuse_filter) break;
+        case 7:  // filter_count (fused: the predict pass's tracklets and candidate groups placed in pair order)
+            if (caps.fuse_filter) {
+                ok(launch_predict_filter_place(tc, P, R, cand_off, th1, th2, th3, cnt, fine, scan, lc, st),
+                   "predict + filter placement");
+                break;
+            }
             ok(launch_filter_count(tc, Q, hout.x, hout.y, hout.z, hout.layer, pairs, cands, bitmap, word_counts, cnt,
                                    lc, st),
                "filter count");

@@ -242,6 +242,13 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
                                  const uint32_t* hevt, const uint32_t* pairs, uint32_t* cand_offsets, uint32_t* th1,
                                  uint32_t* th2, uint32_t* th3, DevCounts* cnt, const FineBufs& fb,
                                  const ScanScratch& ss, const LaunchCfg& lc, hipStream_t st);
+// the fused pass's outputs placed in pair order (after launch_predict_filter on
+// the same buffers): the chunk counts scanned, the tracklets copied from the
+// waves' scratch ranges, the candidate-group offsets filled in
+hipError_t launch_predict_filter_place(const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_trk_cap,
+                                       uint32_t* cand_offsets, uint32_t* th1, uint32_t* th2, uint32_t* th3,
+                                       DevCounts* cnt, const FineBufs& fb, const ScanScratch& ss, const LaunchCfg& lc,
+                                       hipStream_t st);
 
 // TripletThetaPhiFilter: count (bitmap + per-word counts) -> scan -> store -> launch_filter_offsets
 hipError_t launch_filter_count(const TripletCols& tc, uint32_t n_cand_cap, const float* hx,

@@ -2141,7 +2141,18 @@ hipError_t launch_predict_filter(const GridGeom& g, const TripletCols& tc, uint3
                                          predict_wave_claim(lc), st);
     TRAX_PRED_VARIANTS(TRAX_PF_RUN)
 #undef TRAX_PF_RUN
-    if (e != hipSuccess) return e;
+    return e;
+}
+
+hipError_t launch_predict_filter_place(const TripletCols& tc, uint32_t n_pair_cap, uint32_t n_trk_cap,
+                                       uint32_t* cand_offsets, uint32_t* th1, uint32_t* th2, uint32_t* th3,
+                                       DevCounts* cnt, const FineBufs& fb, const ScanScratch& ss, const LaunchCfg& lc,
+                                       hipStream_t st) {
+    const uint64_t chunks = predict_fused_chunks(n_pair_cap);
+    if (!fb.chunk_cands || !fb.chunk_trk || !fb.chunk_at || !fb.group_chunk || !fb.scratch ||
+        fb.scratch_cap < fused_scratch_cap(n_trk_cap))
+        return hipErrorInvalidValue;
+    hipError_t e;
     // the chunks' counts in pair order: candidate and tracklet offsets per chunk, and the totals
     if ((e = launch_scan(fb.chunk_cands, &cnt->n_pred_chunks, 1u, chunks + 1, &cnt->n_cands, ss, st)) != hipSuccess)
         return e;

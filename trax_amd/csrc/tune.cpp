@@ -112,9 +112,9 @@ extern "C" int trax_autotune(trax_ctx* ctx, const trax_config* cfg, const trax_h
         const std::vector<Param> params = {
             {"predict", {(128u << 16) | 32u, (128u << 16) | 64u, (256u << 16) | 24u, (256u << 16) | 32u,
                          (256u << 16) | 48u, (256u << 16) | 64u, (512u << 16) | 24u, (512u << 16) | 32u,
-                         1u << 31 | (256u << 16) | 32u}, {"predict"}},
-            {"predict_fine", {4, 8, 16}, {"predict"}},
-            {"predict_wave", {1, 2, 4, 128}, {"predict"}},
+                         1u << 31 | (256u << 16) | 32u}, {"predict", "filter_count"}},
+            {"predict_fine", {4, 8, 16}, {"predict", "filter_count"}},
+            {"predict_wave", {1, 2, 4, 128}, {"predict", "filter_count"}},
             {"filter_block", {64, 128, 256, 512, 1024}, {"filter_count", "filter_store"}},
             {"pair_block", {64, 128, 256, 512}, {"pair_count", "pair_store"}},
             {"pair_xcd", {1, 8}, {"pair_count", "pair_store"}},
