@@ -145,6 +145,19 @@ def test_filter_phi_prefilter_never_rejects_an_accepted_candidate():
     # it decides almost everything clearly outside (by more than 1e-4 rad)
     clear = np.abs(off) > cuts + 1e-4
     assert so[clear].mean() > 0.999
+    # a decision implies the test on the cross and dot products in float64
+    # (which holds the products of floats exactly): d <= 0 or
+    # |c| > tan(cut + 1e-5) * d; near-orthogonal pairs probe d around 0
+    perp = rng.random(n) < 0.25
+    off2 = np.where(perp, np.sign(rng.random(n) - 0.5) * (np.pi / 2 + rng.uniform(-1e-7, 1e-7, n)), off)
+    vx2, vy2 = (r2 * np.cos(a + off2)).astype(np.float32), (r2 * np.sin(a + off2)).astype(np.float32)
+    so2, ps2 = filter_phi_decide_host(ux, uy, vx2, vy2, cuts)
+    assert not np.any(so2 & ps2)
+    u_x, u_y, v_x, v_y = (np.float64(t) for t in (ux, uy, vx2, vy2))
+    c64 = u_x * v_y - u_y * v_x
+    d64 = u_x * v_x + u_y * v_y
+    t64 = np.tan(cuts.astype(np.float64) + 1e-5)
+    assert np.all(~so2 | (d64 <= 0) | (np.abs(c64) > t64 * d64))
     # degenerate and extreme vectors go to the reference expression
     special = np.array([0.0, -0.0, 1e-38, 1e-30, 1.0, 3e38, np.inf, -np.inf, np.nan], np.float32)
     g = np.array(np.meshgrid(special, special, special, special)).reshape(4, -1)

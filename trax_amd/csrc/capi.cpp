@@ -794,7 +794,7 @@ void trax_batch::layout(Arena& a) {
         cols(B.hout);
         B.hout.zphi = a.take<float2>((uint64_t)N + kZphiSlack);
         B.fine.grid = a.take<uint32_t>(E * L * fine_geom(B.cfg.grid.geom(), fine_max(B.cfg.grid.geom())).cells);
-        B.fine.hits = a.take<float4>(N);
+        B.fine.hits = a.take<float4>((uint64_t)N + kFineSlack);
         if (caps.fuse_filter) {
             B.hout.xy = a.take<float2>(N);
             B.fine.xy = B.hout.xy;

@@ -65,12 +65,15 @@ FineGeom fine_geom(const GridGeom& g, uint32_t F);
 // entries of room past the last hit of every (z, phi) column: the pair walks
 // read it in aligned 16-byte blocks of four entries
 constexpr uint32_t kZphiSlack = 4;
+// entries of room past the last fine hit: the fine walk reads four entries
+// at a time from a segment and masks the ones past its end
+constexpr uint32_t kFineSlack = 4;
 constexpr uint32_t kFineMax = 16;
 // largest sub-row count the launch supports for this grid (power of two <= kFineMax)
 uint32_t fine_max(const GridGeom& g);
 struct FineBufs {
     uint32_t* grid = nullptr;  // E * L * fine_geom(g, kFineMax).cells prefix entries
-    float4* hits = nullptr;    // (z, phi, grid index as bits, 0) at fine positions; N entries
+    float4* hits = nullptr;    // (z, phi, grid index as bits, 0) at fine positions; N + kFineSlack entries
     // the fused predictor + filter: per resident block, a pool of pool_cap
     // store-window matches beyond a pair's LDS list
     uint2* pool = nullptr;

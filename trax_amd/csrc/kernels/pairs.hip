@@ -855,6 +855,131 @@ __device__ inline void pred_walk_fine(const PredWin2& w, const GridGeom& g, cons
     }
 }
 
+// Bits lo .. hi-1 of a list mask (lo <= hi <= the mask's width).
+template <class M>
+__device__ inline M fine_bits(uint32_t lo, uint32_t hi) {
+    constexpr uint32_t W = sizeof(M) * 8;
+    const M upto = hi >= W ? ~(M)0 : (((M)1 << hi) - (M)1);
+    const M below = lo >= W ? ~(M)0 : (((M)1 << lo) - (M)1);
+    return upto & ~below;
+}
+
+// pred_walk_fine with the same results (countA, nB, the listed entries in the
+// same order, rmask, smask and the overflow calls), in fewer instructions per
+// visited hit:
+//  - a row of a wrapped window is walked as its two index segments, [lo, row
+//    end) and then [row start, hi), so a hit's index needs no wrap arithmetic;
+//  - the phi test of a wrapped window is phiLow <= phi || phi <= phiHigh: every
+//    hit's phi is an atan2 value in [-pi, pi] (as a float) or NaN, so the
+//    reference's extra -pi <= phi and phi <= pi terms always hold;
+//  - a match only appends its entry: rmask gets the bit of the first entry of
+//    each grid row when that grid row is left (grid rows arrive in order),
+//    smask the bits of the wrapped segment's entries when that segment ends,
+//    and the grid-row ordinal that an overflowing entry carries is counted per
+//    grid row.
+template <class M, class OV>
+__device__ inline void pred_walk_fine2(const PredWin2& w, const GridGeom& g, const FineGeom& fg,
+                                       const uint32_t* __restrict__ fl, const float4* __restrict__ fh, uint32_t cap,
+                                       uint32_t first, uint16_t* my, uint32_t& countA, uint32_t& nB, M& rmask,
+                                       M& smask, OV&& over) {
+    uint32_t aLo, aHi, bLo, bHi;
+    fine_range(g, fg, w.zLoA, w.zHiA, w.zLowA, w.zHighA, aLo, aHi);
+    fine_range(g, fg, w.zLoB, w.zHiB, w.zLowB, w.zHighB, bLo, bHi);
+    const bool ea = aLo < aHi, eb = bLo < bHi;
+    uint32_t f1 = 0, f = 0;
+    if (ea || eb) {
+        f = !ea ? bLo : (!eb ? aLo : (aLo < bLo ? aLo : bLo));
+        f1 = !ea ? bHi : (!eb ? aHi : (aHi > bHi ? aHi : bHi));
+    }
+    const uint32_t np1 = g.n_phi + 1;
+    const bool wrap = w.wrap;
+    // the phi test as two intervals, (phiLow <= phi <= phiHigh) for an
+    // unwrapped window and (phiLow <= phi) or (phi <= phiHigh) for a wrapped
+    // one: [phiLow, hi1] or [lo2, phiHigh] (a NaN phi fails both)
+    const float phiLow = w.phiLow, phiHigh = w.phiHigh;
+    const float hi1 = wrap ? INFINITY : phiHigh, lo2 = wrap ? -INFINITY : INFINITY;
+    const float zLowA = w.zLowA, zHighA = w.zHighA, zLowB = w.zLowB, zHighB = w.zHighB;
+    // the grid row being walked, the entry count at its start, grid rows with matches before it
+    uint32_t grow = 0xffffffffu, gstart = 0, ord_before = 0;
+    auto close_grid_row = [&]() {
+        if (nB > gstart) {
+            if (gstart < cap) rmask |= fine_bit<M>(gstart);
+            ++ord_before;
+        }
+    };
+    auto active_from = [&](uint32_t ff) {
+        for (; ff < f1; ++ff)
+            if ((ea && ff >= aLo && ff < aHi) || (eb && ff >= bLo && ff < bHi)) break;
+        return ff;
+    };
+    struct RowPre {
+        uint32_t f, start, end, lo, hi;
+    };
+    auto load_row = [&](uint32_t ff) {
+        RowPre rp{ff, 0u, 0u, 0u, 0u};
+        if (ff < f1) {
+            const uint32_t* r = fl + (size_t)ff * np1;
+            rp.start = r[0];
+            rp.end = r[np1];
+            rp.lo = r[w.pLo];
+            rp.hi = r[w.pHi];
+        }
+        return rp;
+    };
+    RowPre cur = load_row(active_from(f));
+    RowPre nxt = load_row(active_from(cur.f + 1));
+    while (cur.f < f1) {
+        const uint32_t fr = cur.f;
+        const bool sa = ea && fr >= aLo && fr < aHi;
+        const bool sb = eb && fr >= bLo && fr < bHi;
+        const uint32_t rf = fr >> fg.shift;
+        if (rf != grow) {
+            close_grid_row();
+            grow = rf;
+            gstart = nB;
+        }
+        const uint32_t ordc = ord_before + 1;
+        // the row's first segment, then (wrapped window) its second
+        uint32_t a = cur.lo, b = wrap ? cur.end : cur.hi;
+        bool seg2 = false;
+        while (true) {
+            const uint32_t s0 = nB;
+            for (uint32_t j = a; j < b; j += 4) {
+                // entries j .. j + 3: those past b (at most three, within the
+                // buffer's kFineSlack entries past its last hit) are masked below
+                const float4* p = fh + j;
+                float4 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) q[u] = p[u];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float z = q[u].x, ph = q[u].y;
+                    const bool inr = j + u < b;
+                    const bool pin = inr & (((phiLow <= ph) & (ph <= hi1)) | ((lo2 <= ph) & (ph <= phiHigh)));
+                    countA += (uint32_t)(pin & sa & (zLowA <= z) & (z <= zHighA));
+                    if (pin & sb & (zLowB <= z) & (z <= zHighB)) {
+                        if (nB < cap) my[nB] = (uint16_t)(__float_as_uint(q[u].z) - first);
+                        else over(ordc, seg2, __float_as_uint(q[u].z));
+                        nB++;
+                    }
+                }
+            }
+            if (seg2) {
+                const uint32_t s1 = nB < cap ? nB : cap;
+                if (s0 < s1) smask |= fine_bits<M>(s0, s1);
+                break;
+            }
+            if (!wrap) break;
+            a = cur.start;
+            b = cur.hi;
+            seg2 = true;
+        }
+        cur = nxt;
+        if (cur.f < f1) nxt = load_row(active_from(cur.f + 1));
+    }
+    close_grid_row();
+}
+
 // Count, scan and store of the predictor in one pass over tiles of
 // kPredBlock pairs (batched path).  One merged walk counts the count-window
 // matches and collects the store-window matches of every pair in LDS
@@ -1484,7 +1609,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
 // k_predict_filter: the chunk is the same 64 pairs, its candidates the same
 // in the same order.
 template <int kPredBlock, int kPredList>
-__global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
+__global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_predict_filter_wave(
     GridGeom g, TripletCols tc, const uint32_t* __restrict__ grid, const float* __restrict__ hx,
     const float* __restrict__ hy, const float* __restrict__ hz, const float2* __restrict__ hzp,
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
@@ -1505,7 +1630,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
     __shared__ float2 s_z12[kPredBlock];        // z1, z2: with s_pair, the pair's hits for the angle cuts
     // per wave: the pair (lane) of each of the chunk's first kOwnerCap listed
     // candidates (one byte each; later ones search the offsets)
-    constexpr uint32_t kOwnerCap = kWave * 16;
+    constexpr uint32_t kOwnerCap = kWave * 7;
     __shared__ uint8_t s_owner[kWaves * kOwnerCap];
     __shared__ uint32_t s_lt[kPredBlock];       // layer triplet | pool entries << 8
     __shared__ Mask s_valid[kPredBlock];        // listed candidates that pass the cuts
@@ -1513,6 +1638,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
     __shared__ Mask s_rm[kPredBlock], s_sm[kPredBlock];
     __shared__ uint32_t s_pool_valid[kPredBlock];
     __shared__ uint32_t s_queue[kWaves * kQueue];
+    __shared__ float2 s_qxy[kWaves * kQueue];  // the third hit (x, y) of each queued candidate
     __shared__ uint32_t s_queue2[kWaves * kQueue];
     __shared__ uint32_t s_pool_n_w[kWaves], s_dead_any_w[kWaves];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
@@ -1593,7 +1719,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
             }
             if (have) {
                 const uint32_t cap = pw.lg3[g.cells - 1] - first <= 0xffffu ? (uint32_t)kPredList : 0u;
-                pred_walk_fine<Mask>(pw, g, fg, fgrid + (size_t)pw.el * fg.cells, fh, cap, first,
+                pred_walk_fine2<Mask>(pw, g, fg, fgrid + (size_t)pw.el * fg.cells, fh, cap, first,
                                      s_list + tid * kPredList, countA, nB, rm, sm,
                                      [&](uint32_t ord, bool seg2, uint32_t h3) {
                                          const uint32_t slot = atomicAdd(&s_pool_n, 1u);
@@ -1667,6 +1793,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
         //    [0, total_e), then the pool [total_e, total_e + n_pool)
         {
             uint32_t* q = s_queue + wave * kQueue;
+            float2* qxy = s_qxy + wave * kQueue;
             uint32_t qn = 0;
             const uint32_t n_all = total_e + n_pool;
             const uint32_t e0 = 0, e1 = n_all;
@@ -1716,10 +1843,9 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
                 uint32_t tag = 0;
                 if (active) {
                     tag = q[k0 + lane];
-                    uint32_t c;
-                    const uint32_t lo = third_of(tag, c);
+                    const uint32_t lo = (tag & kValidBit) ? pool[tag & ~kValidBit].x & 0xffffu : tag >> 16;
                     const float4 pr = s_pair[lo];  // the pair's hits, read once per chunk from HBM
-                    const float2 p3 = hxy[c];
+                    const float2 p3 = qxy[k0 + lane];  // loaded by the prefilter
                     pass = filter_circle_cuts(tc, s_lt[lo] & 0xffu, pr.z, pr.w, pr.x, pr.y, p3.x, p3.y);
                 }
                 const uint64_t m = __ballot(pass);
@@ -1732,10 +1858,13 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
                     wave_sync();
                 }
             };
-            for (uint32_t eb = e0; eb < e1; eb += kWave) {
-                const uint32_t e = eb + lane;
+            // the candidate at e: its tag, whether it is still undecided
+            // (not pruned) and its pair (in the block) and third hit
+            auto pick = [&](uint32_t e, uint32_t& tag, uint32_t& lo, uint32_t& c) {
                 bool undecided = false;
-                uint32_t tag = 0;
+                tag = 0;
+                lo = 0;
+                c = 0;
                 if (e < e1) {
                     if (e < total_e) {
                         uint32_t ol = 0;
@@ -1749,24 +1878,31 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
                             }
                         }
                         const uint32_t i = e - eex[ol];
-                        const uint32_t lo = wave * kWave + ol;
+                        lo = wave * kWave + ol;
                         tag = lo << 16 | i;
                         undecided = !((s_dead[lo] >> i) & 1u);
                     } else {
                         tag = kValidBit | (e - total_e);
                         undecided = !(pool[e - total_e].x & kDeadBit);
                     }
-                    if (undecided) {
-                        uint32_t c;
-                        const uint32_t lo = third_of(tag, c);
-                        const float4 pr = s_pair[lo];
-                        const float2 p3 = hxy[c];
-                        undecided = !tx_phi_surely_outside(pr.x - pr.z, pr.y - pr.w, p3.x - pr.x, p3.y - pr.y,
-                                                           tc.tan_dphi[s_lt[lo] & 0xffu]);
-                    }
+                    if (undecided) lo = third_of(tag, c);
                 }
+                return undecided;
+            };
+            // whether the prefilter leaves the candidate undecided, given its third hit
+            auto prefilter = [&](bool undecided, uint32_t lo, float2 p3) {
+                if (!undecided) return false;
+                const float4 pr = s_pair[lo];
+                return !tx_phi_surely_outside(pr.x - pr.z, pr.y - pr.w, p3.x - pr.x, p3.y - pr.y,
+                                              tc.tan_dphi[s_lt[lo] & 0xffu]);
+            };
+            auto push = [&](bool undecided, uint32_t tag, float2 p3) {
                 const uint64_t m = __ballot(undecided);
-                if (undecided) q[qn + (uint32_t)__popcll(m & lanemask_lt())] = tag;
+                if (undecided) {
+                    const uint32_t slot = qn + (uint32_t)__popcll(m & lanemask_lt());
+                    q[slot] = tag;
+                    qxy[slot] = p3;
+                }
                 qn += (uint32_t)__popcll(m);
                 wave_sync();
                 if (qn >= kWave) {
@@ -1774,6 +1910,18 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter_wave(
                     test_queue(qn, true);
                     wave_sync();
                 }
+            };
+            // two groups of 64 candidates at a time: both groups' third hits
+            // are loaded before either is tested
+            for (uint32_t eb = e0; eb < e1; eb += 2 * kWave) {
+                uint32_t tagA, loA, cA, tagB, loB, cB;
+                const bool uA = pick(eb + lane, tagA, loA, cA);
+                const bool uB = pick(eb + kWave + lane, tagB, loB, cB);
+                const float2 p3A = hxy[cA], p3B = hxy[cB];
+                const bool vA = prefilter(uA, loA, p3A);
+                const bool vB = prefilter(uB, loB, p3B);
+                push(vA, tagA, p3A);
+                if (eb + kWave < e1) push(vB, tagB, p3B);  // uniform
             }
             test_queue(0, lane < qn);
             if (lane < qn2) test_angles(0);
