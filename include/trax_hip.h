@@ -410,7 +410,11 @@ typedef struct {
     uint32_t max_tracklets;
     uint32_t max_conn;
     uint32_t max_track_entries;
-    uint32_t use_graph;    /* capture the sequence once, replay per batch */
+    /* 0: launch the sequence eagerly every run; 1: capture it once, one
+     * graph per phase, replayed per run (every phase timed by
+     * trax_batch_stage_ms); 2: capture it once as one graph (one launch per
+     * run, no gaps between phases; only "total" is timed) */
+    uint32_t use_graph;
     uint32_t ca_iterations; /* CA iterations in the sequence; 0 = one pass per layer triplet when every
                                triplet's second layer is above its first (same states and iteration
                                count), else layer triplets + 2 iterations */
@@ -487,7 +491,7 @@ typedef struct {
      * the workload; a batch that still overflows is re-run as smaller batches */
     float pairs_per_hit, cands_per_hit, tracklets_per_hit, conn_per_hit, track_entries_per_hit;
     uint32_t keep_results; /* keep each event's tracklets and tracks (trax_server_event_result) */
-    uint32_t use_graph;
+    uint32_t use_graph;    /* the slots' batches: 0 eager, 1 a graph per phase, 2 one graph (as trax_batch_caps) */
     uint32_t fuse_filter;  /* the slots' batches run the predictor and the filter as one pass */
     /* 0: a batch takes consecutive waiting events in arrival order; 1: events
      * are grouped by hit count: the oldest waiting event's size class (hit

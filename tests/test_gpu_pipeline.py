@@ -6,7 +6,7 @@ import pytest
 
 import oracle
 from helpers import PIPELINE_KEYS, compare, hitset, small_workload
-from trax_amd.api import Batch, Caps, HitSet, run_batch_fit
+from trax_amd.api import Batch, Caps, HitSet, TraxError, run_batch_fit
 
 pytestmark = pytest.mark.gpu
 
@@ -67,6 +67,33 @@ def test_batched_graph_matches_oracle_on_every_replay(dev, name):
         bad = compare(again, want)
         assert not bad, "\n".join(bad)
     b.close()
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_one_graph_batch_matches_oracle_on_every_replay(dev, fused):
+    """The whole sequence captured as one graph (use_graph 2): every replay,
+    with the inputs uploaded again and eager runs of another batch between
+    the replays, equals the oracle; only the total is timed."""
+    wl, cfg, hs, _ = make_case("four-uneven-events")
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    keys = [k for k in PIPELINE_KEYS if k != "cands"] if fused else PIPELINE_KEYS
+    caps = Caps.for_hits(hs.n_events, hs.n_hits, use_graph=2, fuse_filter=fused)
+    b, got = run_batch_fit(dev, cfg, hs, caps)
+    e, _ = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, use_graph=0, fuse_filter=fused))
+    bad = compare(got, want, keys)
+    assert not bad, "\n".join(bad)
+    for _ in range(3):
+        e.rerun()
+        assert e.wait() == 0
+        b.submit(hs)
+        assert b.wait() == 0
+        bad = compare(b.result(), want, keys)
+        assert not bad, "\n".join(bad)
+        assert b.stage_ms("total") > 0.0
+    with pytest.raises(TraxError):
+        b.stage_ms("predict")
+    b.close()
+    e.close()
 
 
 def test_batched_eager_matches_graph(dev):

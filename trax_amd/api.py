@@ -456,7 +456,7 @@ class Caps:
     max_tracklets: int
     max_conn: int
     max_track_entries: int
-    use_graph: bool = True
+    use_graph: int = 1  # 0 eager, 1 a graph per phase (phases timed), 2 one graph (only the total timed)
     ca_iterations: int = 0
     per_event: bool = False  # each event its own group (results equal per-event runs)
     fuse_filter: bool = False  # predictor and filter in one pass; the candidates are not stored
@@ -589,7 +589,7 @@ class Server:
 
     def __init__(self, dev: Device, cfg: Config, bounds: OutputBounds, n_slots: int = 2, max_batch_events: int = 256,
                  max_batch_hits: int = 0, device_budget: int = 0, keep_results: bool = False,
-                 use_graph: bool = True, fuse_filter: bool = False, group_by_size: bool = False):
+                 use_graph: int = 1, fuse_filter: bool = False, group_by_size: bool = False):
         o = ServeOpts(n_slots, max_batch_events, max_batch_hits, device_budget, bounds.pairs, bounds.cands,
                       bounds.tracklets, bounds.conn, bounds.track_entries, int(keep_results), int(use_graph),
                       int(fuse_filter), int(group_by_size))

@@ -567,34 +567,80 @@ void trax_batch::enqueue_fine(hipStream_t s) {
               "fine grid");
 }
 
-void trax_batch::launch() {
-    if (caps.use_graph && !captured) {
-        for (int k = 0; k < kNumStages; ++k) {
-            hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-            enqueue_stage(k);
-            hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
-            hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
-        }
-        hip_check(hipStreamBeginCapture(st2, hipStreamCaptureModeThreadLocal), "begin capture");
-        enqueue_fine(st2);
-        hip_check(hipStreamEndCapture(st2, &fine_graph), "end capture");
-        hip_check(hipGraphInstantiate(&fine_exec, fine_graph, nullptr, nullptr, 0), "instantiate");
-        captured = true;
-    }
+void trax_batch::enqueue_all(bool phase_events) {
     for (int k = 0; k < kNumStages; ++k) {
-        hip_check(hipEventRecord(ev[k], st), "event");
+        if (phase_events) hip_check(hipEventRecord(ev[k], st), "event");
         if (k == kPredictStage) hip_check(hipStreamWaitEvent(st, ev_fine, 0), "wait fine grid");
-        if (caps.use_graph) hip_check(hipGraphLaunch(execs[k], st), "graph launch");
-        else enqueue_stage(k);
+        enqueue_stage(k);
         if (k == kGridStage) {  // fork: the fine walk structure beside the pair stages
             hip_check(hipEventRecord(ev_grid, st), "event");
             hip_check(hipStreamWaitEvent(st2, ev_grid, 0), "wait grid");
-            if (caps.use_graph) hip_check(hipGraphLaunch(fine_exec, st2), "graph launch");
-            else enqueue_fine(st2);
+            enqueue_fine(st2);
             hip_check(hipEventRecord(ev_fine, st2), "event");
         }
     }
-    hip_check(hipEventRecord(ev[kNumStages], st), "event");
+    if (phase_events) hip_check(hipEventRecord(ev[kNumStages], st), "event");
+}
+
+bool trax_batch::capture_whole() {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+    bool ok = true;
+    try {
+        enqueue_all(false);  // the runtime does not time events recorded inside a graph
+    } catch (const std::exception&) {
+        ok = false;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (ok && e == hipSuccess && g && hipGraphInstantiate(&whole_exec, g, nullptr, nullptr, 0) == hipSuccess) {
+        whole_graph = g;
+        return true;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    whole_exec = nullptr;
+    (void)hipGetLastError();  // a failed capture leaves no error behind for later calls
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(st2);
+    return false;
+}
+
+void trax_batch::launch() {
+    if (caps.use_graph && !captured) {
+        whole = caps.use_graph == 2 && capture_whole();
+        if (!whole) {
+            for (int k = 0; k < kNumStages; ++k) {
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+                enqueue_stage(k);
+                hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
+                hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
+            }
+            hip_check(hipStreamBeginCapture(st2, hipStreamCaptureModeThreadLocal), "begin capture");
+            enqueue_fine(st2);
+            hip_check(hipStreamEndCapture(st2, &fine_graph), "end capture");
+            hip_check(hipGraphInstantiate(&fine_exec, fine_graph, nullptr, nullptr, 0), "instantiate");
+        }
+        captured = true;
+    }
+    if (!caps.use_graph) {
+        enqueue_all(true);
+    } else if (whole) {
+        hip_check(hipEventRecord(ev[0], st), "event");
+        hip_check(hipGraphLaunch(whole_exec, st), "graph launch");
+        hip_check(hipEventRecord(ev[kNumStages], st), "event");
+    } else {
+        for (int k = 0; k < kNumStages; ++k) {
+            hip_check(hipEventRecord(ev[k], st), "event");
+            if (k == kPredictStage) hip_check(hipStreamWaitEvent(st, ev_fine, 0), "wait fine grid");
+            hip_check(hipGraphLaunch(execs[k], st), "graph launch");
+            if (k == kGridStage) {  // fork: the fine walk structure beside the pair stages
+                hip_check(hipEventRecord(ev_grid, st), "event");
+                hip_check(hipStreamWaitEvent(st2, ev_grid, 0), "wait grid");
+                hip_check(hipGraphLaunch(fine_exec, st2), "graph launch");
+                hip_check(hipEventRecord(ev_fine, st2), "event");
+            }
+        }
+        hip_check(hipEventRecord(ev[kNumStages], st), "event");
+    }
     hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
     submitted = true;
     have_counts = false;
@@ -749,6 +795,7 @@ namespace {
 void batch_shape(trax_batch& B, const trax_config* cfg, const trax_batch_caps* caps) {
     validate_config(cfg);
     if (!caps->max_events || !caps->max_hits) throw InputError("capacities need at least one event and hit");
+    if (caps->use_graph > 2) throw InputError("use_graph is 0 (eager), 1 (a graph per phase) or 2 (one graph)");
     B.cfg = *cfg;
     B.caps = *caps;
     B.E = caps->max_events;
@@ -951,6 +998,8 @@ extern "C" void trax_batch_destroy(trax_batch* b) {
         if (b->graphs[i]) (void)hipGraphDestroy(b->graphs[i]);
     }
     if (b->fine_exec) (void)hipGraphExecDestroy(b->fine_exec);
+    if (b->whole_exec) (void)hipGraphExecDestroy(b->whole_exec);
+    if (b->whole_graph) (void)hipGraphDestroy(b->whole_graph);
     if (b->fine_graph) (void)hipGraphDestroy(b->fine_graph);
     if (b->ev_grid) (void)hipEventDestroy(b->ev_grid);
     if (b->ev_fine) (void)hipEventDestroy(b->ev_fine);
@@ -1043,6 +1092,8 @@ extern "C" int trax_batch_counts(trax_batch* b, uint32_t out[8]) {
 
 extern "C" int trax_batch_stage_ms(trax_batch* b, const char* stage, float* ms) {
     return guarded([&] {
+        if (b->whole && std::strcmp(stage, "total") != 0)
+            return fail(TRAX_ERR_STATE, "phase times need per-phase graphs (use_graph = 1) or eager launches");
         for (int k = 0; k < kNumStages; ++k)
             if (std::strcmp(stage, kPhases[k]) == 0) {
                 hip_check(hipEventElapsedTime(ms, b->ev[k], b->ev[k + 1]), "hipEventElapsedTime");

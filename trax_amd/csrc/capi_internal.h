@@ -141,7 +141,7 @@ inline hipStream_t pick_stream(trax_ctx* ctx, trax_stream* s) { return s ? s->s 
 
 // The batched pipeline's workspace: every buffer of the path for up to
 // caps.max_events events in one arena, sizes kept on the device (DevCounts),
-// the launch sequence captured per phase into graphs.
+// the launch sequence captured into one graph (or one per phase).
 struct trax_batch {
     trax_ctx* owner = nullptr;
     hipStream_t st = nullptr;
@@ -180,6 +180,13 @@ struct trax_batch {
     hipEvent_t ev_grid = nullptr, ev_fine = nullptr;
     hipGraph_t fine_graph = nullptr;
     hipGraphExec_t fine_exec = nullptr;
+    // the whole sequence (every phase, the fork to st2 and the join) as one
+    // graph, launched with one call between the first and last phase events
+    // (caps.use_graph == 2); the per-phase graphs above are the fallback when
+    // the runtime does not capture that form
+    hipGraph_t whole_graph = nullptr;
+    hipGraphExec_t whole_exec = nullptr;
+    bool whole = false;
     uint32_t n_events = 0, n_hits = 0;
     bool submitted = false, have_counts = false;
 
@@ -190,6 +197,11 @@ struct trax_batch {
     void layout(trax_capi::Arena& a);
     // enqueue the whole sequence on the resident inputs (graph replay or eager)
     void launch();
+    // the sequence as launch() enqueues it eagerly: phases, fork and join, and
+    // (phase_events) an event before every phase and after the last
+    void enqueue_all(bool phase_events);
+    // capture enqueue_all() into whole_exec; false (and nothing kept) when the capture fails
+    bool capture_whole();
     // after the stream has drained: continue the CA forward phase on the host
     // while the fixed number of iterations left living cells, then rerun the
     // backward phase; returns the status of the run from the device counts
