@@ -686,10 +686,8 @@ void trax_batch::enqueue_stage(int s) {
     auto ok = [](hipError_t e, const char* w) { hip_check(e, w); };
     switch (s) {
         case 0:  // grid_count: the whole grid build (count, scan, stable scatter) in one kernel
-            ok(launch_fill(reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords, 0u,
-                           sizeof(DevCounts) / 4 - kCountsShapeWords, st), "reset");
-            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st),
-               "grid build");
+            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st, true),
+               "grid build");  // the run's counts are cleared first, in the same launches
             break;
         case 1:  // grid_store: part of the grid build above
             break;

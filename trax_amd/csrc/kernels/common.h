@@ -150,14 +150,11 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t*
     const uint32_t inc = wave_incl_scan(v);
     if (lane_id() == kWave - 1) wsum[w] = inc;
     __syncthreads();
-    if (tid == 0) {
-        uint32_t run = 0;
-        for (uint32_t i = 0; i < nw; ++i) {
-            const uint32_t t = wsum[i];
-            wsum[i] = run;
-            run += t;
-        }
-        wsum[nw] = run;
+    if (tid < kWave) {  // the first wave scans the block's wave sums (at most 16)
+        const uint32_t x = tid < nw ? wsum[tid] : 0u;
+        const uint32_t xi = wave_incl_scan(x);
+        if (tid < nw) wsum[tid] = xi - x;
+        if (tid + 1 == nw) wsum[nw] = xi;
     }
     __syncthreads();
     const uint32_t ex = wsum[w] + inc - v;

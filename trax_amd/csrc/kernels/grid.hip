@@ -179,8 +179,12 @@ __device__ inline void sort_cells(T* perm, const uint32_t* ends, uint32_t cells)
 // block.  order[t] is the item the build's t-th block takes.
 __global__ __launch_bounds__(1024) void k_grid_order(const uint32_t* __restrict__ d_nevents, uint32_t n_layers,
                                                      const uint32_t* __restrict__ lay_nhits,
-                                                     uint32_t* __restrict__ order, uint32_t n_items_cap) {
+                                                     uint32_t* __restrict__ order, uint32_t n_items_cap,
+                                                     uint32_t* __restrict__ reset, uint32_t reset_words) {
     __shared__ uint32_t bucket[33];
+    // words to clear before the build (the run's counts after the batch's
+    // shape), in the same launch rather than a fill of their own
+    for (uint32_t i = threadIdx.x; i < reset_words; i += blockDim.x) reset[i] = 0u;
     const uint32_t n_items = min(*d_nevents * n_layers, n_items_cap);
     if (threadIdx.x < 33) bucket[threadIdx.x] = 0;
     __syncthreads();
@@ -386,7 +390,8 @@ void grid_lds_plan(const GridGeom& g, uint32_t block, size_t budget, uint32_t& l
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
-                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st) {
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             bool reset_counts) {
     (void)grid_unused;
     // defaults: 1024 threads, 160 KiB of LDS (items staged whole), largest items first
     const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 1024;
@@ -398,10 +403,15 @@ hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint
     const uint32_t blocks = n_items_cap ? n_items_cap : 1;
     // perm holds n_hits_cap scatter entries, then the item order (n_items_cap)
     uint32_t* order = nullptr;
+    uint32_t* reset = reinterpret_cast<uint32_t*>(cnt) + kCountsShapeWords;
+    const uint32_t reset_words = reset_counts ? (uint32_t)(sizeof(DevCounts) / 4 - kCountsShapeWords) : 0u;
     if (lc.grid_order != 2) {
         order = perm + n_hits_cap;
         hipLaunchKernelGGL(k_grid_order, dim3(1), dim3(1024), 0, st, &cnt->n_events, g.n_layers, lay_nhits, order,
-                           n_items_cap);
+                           n_items_cap, reset, reset_words);
+    } else if (reset_words) {
+        const hipError_t e = launch_fill(reset, 0u, reset_words, st);
+        if (e != hipSuccess) return e;
     }
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
     HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};

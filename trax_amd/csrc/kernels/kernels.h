@@ -170,7 +170,8 @@ constexpr size_t kGridLdsMax = 160 * 1024;   // gfx950's LDS per workgroup
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
-                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st);
+                             uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
+                             bool reset_counts = false);  // also clear the run's counts (after the shape) first
 void grid_build_prepare();
 hipError_t launch_grid_count(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
                              const uint32_t* lay_nhits, const uint32_t* lay_offset, const float* hx,
