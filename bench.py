@@ -199,6 +199,21 @@ def measured_traffic(kernel):
     return None
 
 
+def measured_valu_busy(kernel):
+    """Fraction of the SIMDs' VALU issue the kernel used (SQ_ACTIVE_INST_VALU
+    over the kernel's quad-cycles) from the rocprofv3 summary committed under
+    profiles/, or None: the bound of a kernel that HBM does not bound."""
+    path = os.path.join(ROOT, "profiles", "valu-issue-256-events.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if p and not line.startswith("#") and p[0] == kernel and len(p) == 6:
+                return float(p[5])
+    return None
+
+
 def end_to_end(dev, cfg, events, bounds, batch_hits, fused, reps=4):
     """Events/s from the first H2D to the last D2H: `reps` copies of the
     step's events, all arriving at once, through the serving loop with two
@@ -521,7 +536,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": measured_traffic(kernel_of.get(dom, dom)),
-                     "kernel": kernel_of.get(dom, dom), "algorithmic_bytes": int(pb[dom])},
+                     "kernel": kernel_of.get(dom, dom), "algorithmic_bytes": int(pb[dom]),
+                     "valu_busy": measured_valu_busy(kernel_of.get(dom, dom))},
         "latency_ms": {"p50": serving["p50_ms"] if serving else None,
                        "p99": serving["p99_ms"] if serving else None},
         "serving": serving,
