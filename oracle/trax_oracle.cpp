@@ -653,18 +653,12 @@ bool filter_valid(const Ctx& c, u32 h1, u32 h2, u32 h3, u32 lt) {
     const float ax = p2x - p1x, ay = p2y - p1y, az = p2z - p1z;
     const float bx = p3x - p1x, by = p3y - p1y, bz = p3z - p1z;
     float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-    {  // normalize
-        const float d = nx * nx + ny * ny + nz * nz;
-        if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
-    }
+    tx_normalize3(nx, ny, nz);
     const float cox = (-nx) / (2 * nz), coy = (-ny) / (2 * nz);
     const float cc = -(nx * p1x + ny * p1y + nz * p1z);
     const float cR = sqrtf((1 - nz * nz - 4 * cc * nz) / (4 * nz * nz));
     float vx = -cox, vy = -coy;
-    {
-        const float d = vx * vx + vy * vy;
-        if (d != 0.0f) { const float l = sqrtf(d); vx = vx / l; vy = vy / l; }
-    }
+    tx_normalize2(vx, vy);
     const float pcax = cox + cR * vx, pcay = coy + cR * vy;
     const float tip_squared = pcax * pcax + pcay * pcay;
     valid = valid && (tip_squared <= (tipCut * tipCut));
@@ -753,8 +747,7 @@ float circle_radius(const Ctx& c, u32 h0, u32 h1, u32 h2) {
     const float ax = p1x - p0x, ay = p1y - p0y, az = p1z - p0z;
     const float bx = p2x - p0x, by = p2y - p0y, bz = p2z - p0z;
     float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-    const float d = nx * nx + ny * ny + nz * nz;
-    if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
+    tx_normalize3(nx, ny, nz);
     const float cc = (TRAX_ORACLE_PLANT == 3) ? -(nx * p0x + ny * p0y) : -(nx * p0x + ny * p0y + nz * p0z);
     const float nz2 = nz * nz;
     return sqrtf((1 - nz2 - 4 * cc * nz) / (4 * nz2));

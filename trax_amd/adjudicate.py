@@ -97,13 +97,13 @@ def _mca_quantities(p1, p2, p3, rng, n=256):
         B = [sub(P[2][i], P[0][i]) for i in range(3)]
         nv = [sub(mul(A[1], B[2]), mul(A[2], B[1])), sub(mul(A[2], B[0]), mul(A[0], B[2])),
               sub(mul(A[0], B[1]), mul(A[1], B[0]))]
-        ln = sqrt(add(add(mul(nv[0], nv[0]), mul(nv[1], nv[1])), mul(nv[2], nv[2])))
-        nv = [div(v, ln) for v in nv]
+        rn = div(1.0, sqrt(add(add(mul(nv[0], nv[0]), mul(nv[1], nv[1])), mul(nv[2], nv[2]))))
+        nv = [mul(v, rn) for v in nv]
         cx, cy = div(-nv[0], mul(2.0, nv[2])), div(-nv[1], mul(2.0, nv[2]))
         cc = -add(add(mul(nv[0], P[0][0]), mul(nv[1], P[0][1])), mul(nv[2], P[0][2]))
         rad = sqrt(div(sub(sub(1.0, mul(nv[2], nv[2])), mul(mul(4.0, cc), nv[2])), mul(mul(4.0, nv[2]), nv[2])))
-        lv = sqrt(add(mul(cx, cx), mul(cy, cy)))
-        vx, vy = div(-cx, lv), div(-cy, lv)
+        rv = div(1.0, sqrt(add(mul(cx, cx), mul(cy, cy))))
+        vx, vy = mul(-cx, rv), mul(-cy, rv)
         px, py = add(cx, mul(rad, vx)), add(cy, mul(rad, vy))
         tip = np.sqrt(add(mul(px, px), mul(py, py)))
     return ratio, np.abs(d), tip, rad
@@ -134,15 +134,15 @@ def _float32_quantities(p1, p2, p3):
         bx, by, bz = x3 - x1, y3 - y1, p3z - p1z
         nx, ny, nz = ay * bz - az * by, az * bx - ax * bz, ax * by - ay * bx
         dd = nx * nx + ny * ny + nz * nz
-        ln = np.where(dd != 0, np.sqrt(dd), f(1))
-        nx, ny, nz = (np.where(dd != 0, v / ln, v) for v in (nx, ny, nz))
+        rn = f(1) / np.where(dd != 0, np.sqrt(dd), f(1))  # normalize: one reciprocal, three multiplies
+        nx, ny, nz = (np.where(dd != 0, v * rn, v) for v in (nx, ny, nz))
         cox, coy = (-nx) / (f(2) * nz), (-ny) / (f(2) * nz)
         cc = -(nx * x1 + ny * y1 + nz * p1z)
         rad = np.sqrt((f(1) - nz * nz - f(4) * cc * nz) / (f(4) * nz * nz))
         vx, vy = -cox, -coy
         dv = vx * vx + vy * vy
-        lv = np.where(dv != 0, np.sqrt(dv), f(1))
-        vx, vy = np.where(dv != 0, vx / lv, vx), np.where(dv != 0, vy / lv, vy)
+        rv = f(1) / np.where(dv != 0, np.sqrt(dv), f(1))
+        vx, vy = np.where(dv != 0, vx * rv, vx), np.where(dv != 0, vy * rv, vy)
         px, py = cox + rad * vx, coy + rad * vy
         tip2 = px * px + py * py
     return ratio, np.abs(d), tip2, rad

@@ -49,18 +49,12 @@ __device__ inline bool filter_circle_cuts(const TripletCols& tc, uint32_t lt, fl
     const float ax = p2x - p1x, ay = p2y - p1y, az = p2z - p1z;
     const float bx = p3x - p1x, by = p3y - p1y, bz = p3z - p1z;
     float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-    {
-        const float d = nx * nx + ny * ny + nz * nz;
-        if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
-    }
+    tx_normalize3(nx, ny, nz);
     const float cox = (-nx) / (2 * nz), coy = (-ny) / (2 * nz);
     const float cc = -(nx * p1x + ny * p1y + nz * p1z);
     const float cR = sqrtf((1 - nz * nz - 4 * cc * nz) / (4 * nz * nz));
     float vx = -cox, vy = -coy;
-    {
-        const float d = vx * vx + vy * vy;
-        if (d != 0.0f) { const float l = sqrtf(d); vx = vx / l; vy = vy / l; }
-    }
+    tx_normalize2(vx, vy);
     const float pcax = cox + cR * vx, pcay = coy + cR * vy;
     const float tip_squared = pcax * pcax + pcay * pcay;
     valid = valid && (tip_squared <= (tipCut * tipCut));

@@ -586,8 +586,7 @@ __global__ void k_circle(const float* __restrict__ hx, const float* __restrict__
         const float ax = p1x - p0x, ay = p1y - p0y, az = p1z - p0z;
         const float bx = p2x - p0x, by = p2y - p0y, bz = p2z - p0z;
         float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-        const float d = nx * nx + ny * ny + nz * nz;
-        if (d != 0.0f) { const float l = sqrtf(d); nx = nx / l; ny = ny / l; nz = nz / l; }
+        tx_normalize3(nx, ny, nz);
         const float cc = -(nx * p0x + ny * p0y + nz * p0z);
         const float nz2 = nz * nz;
         b.pt[t] = sqrtf((1 - nz2 - 4 * cc * nz) / (4 * nz2));

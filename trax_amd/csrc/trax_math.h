@@ -8,13 +8,15 @@
 // ulp).  This header is one such implementation, written to be evaluated
 // bit-identically by the device compiler (hipcc, gfx950) and by a host C++
 // compiler: every operation is a single IEEE-754 binary32 add, multiply,
-// divide or square root (all correctly rounded on both sides), with FMA
-// contraction disabled (-ffp-contract=off on both compilers, and the pragma
-// below for clang).  sqrt and floor are the correctly rounded / exact C
+// fused multiply-add, divide or square root (all correctly rounded on both
+// sides: fmaf is the C99 function, one rounding of x*y+z, and v_fma_f32 on the
+// device), with implicit FMA contraction disabled (-ffp-contract=off on both
+// compilers, and the pragma below for clang), so that only the fmaf calls
+// written here fuse.  sqrt and floor are the correctly rounded / exact C
 // functions.
 //
 // Each function reduces its argument to a small interval and evaluates
-// s + s*t*P(t), t = s*s.  The coefficients of P are weighted-minimax fits
+// s + s*t*P(t), t = s*s, P by Horner's rule with one fmaf per step.  The coefficients of P are weighted-minimax fits
 // derived by fit_math_coeffs.py in this directory (Lawson iteration on a dense
 // Chebyshev grid, relative-error weight).  Measured accuracy over dense sweeps
 // against float64: atan <= 1.2 ulp, atan2 <= 2 ulp, acos <= 1.5 ulp,
@@ -74,34 +76,56 @@ TX_HD int32_t tx_f2i_sat(float f) {
 TX_HD float tx_atan_core(float a) {
     const float t = a * a;
     float p = -1.793621108e-03f;
-    p = p * t + 1.091460325e-02f;
-    p = p * t + -3.117784299e-02f;
-    p = p * t + 5.795758963e-02f;
-    p = p * t + -8.403450251e-02f;
-    p = p * t + 1.095218584e-01f;
-    p = p * t + -1.426424235e-01f;
-    p = p * t + 1.999854892e-01f;
-    p = p * t + -3.333329856e-01f;
-    return a + (a * t) * p;
+    p = fmaf(p, t, 1.091460325e-02f);
+    p = fmaf(p, t, -3.117784299e-02f);
+    p = fmaf(p, t, 5.795758963e-02f);
+    p = fmaf(p, t, -8.403450251e-02f);
+    p = fmaf(p, t, 1.095218584e-01f);
+    p = fmaf(p, t, -1.426424235e-01f);
+    p = fmaf(p, t, 1.999854892e-01f);
+    p = fmaf(p, t, -3.333329856e-01f);
+    return fmaf(a * t, p, a);
 }
 
 // asin(s) for s in [0, 0.5], with t = s*s supplied: s + s*t*P(t), degree 4
 TX_HD float tx_asin_core(float s, float t) {
     float p = 4.216632247e-02f;
-    p = p * t + 2.417950705e-02f;
-    p = p * t + 4.547037557e-02f;
-    p = p * t + 7.495297492e-02f;
-    p = p * t + 1.666675210e-01f;
-    return s + (s * t) * p;
+    p = fmaf(p, t, 2.417950705e-02f);
+    p = fmaf(p, t, 4.547037557e-02f);
+    p = fmaf(p, t, 7.495297492e-02f);
+    p = fmaf(p, t, 1.666675210e-01f);
+    return fmaf(s * t, p, s);
 }
 
 // atanh(s) for |s| <= 3 - 2*sqrt(2): s + s*t*P(t), degree 2
 TX_HD float tx_atanh_core(float s) {
     const float t = s * s;
     float p = 1.493553668e-01f;
-    p = p * t + 1.998878419e-01f;
-    p = p * t + 3.333338797e-01f;
-    return s + (s * t) * p;
+    p = fmaf(p, t, 1.998878419e-01f);
+    p = fmaf(p, t, 3.333338797e-01f);
+    return fmaf(s * t, p, s);
+}
+
+// OpenCL normalize (its accuracy is implementation-defined): the squared
+// length, one correctly rounded square root and one correctly rounded
+// reciprocal of it, then one multiply per component.  A zero vector is left
+// as it is.
+TX_HD void tx_normalize3(float& x, float& y, float& z) {
+    const float d = x * x + y * y + z * z;
+    if (d != 0.0f) {
+        const float r = 1.0f / sqrtf(d);
+        x = x * r;
+        y = y * r;
+        z = z * r;
+    }
+}
+TX_HD void tx_normalize2(float& x, float& y) {
+    const float d = x * x + y * y;
+    if (d != 0.0f) {
+        const float r = 1.0f / sqrtf(d);
+        x = x * r;
+        y = y * r;
+    }
 }
 
 TX_HD float tx_atan2f(float y, float x) {
