@@ -2,12 +2,13 @@
 every result except the (unstored) candidates is bit-identical to the
 oracle's and to the unfused batched path's, on every graph replay, for the
 list sizes and sub-row counts of the launch shapes, and per event."""
+import numpy as np
 import pytest
 
 import oracle
 from helpers import PIPELINE_KEYS, compare, hitset, small_workload
 from test_gpu_pipeline import make_case
-from trax_amd.api import Batch, Caps, autotune, run_batch_fit
+from trax_amd.api import Batch, Caps, HitSet, TraxError, autotune, run_batch_fit
 
 pytestmark = pytest.mark.gpu
 
@@ -82,3 +83,61 @@ def test_fused_autotune_keeps_results(dev):
     bad = compare(tb.result(), want, FUSED_KEYS)
     tb.close()
     assert not bad, "\n".join(bad)
+
+
+def _one_big_grid_row(n_noise, seed=3):
+    """One event on three layers with a 1 z x 8 phi grid: clean tracks whose
+    hits all lie at phi in [-2.8, 0.3], and n_noise hits on the third layer at
+    phi in [1.2, 2.9], so the third layer's single grid row holds about
+    n_noise hits while every window stays clear of the noise."""
+    import dataclasses
+    from trax_amd.api import Config
+    from trax_amd.generator import NOISE_PT, Event, make_event
+    wl = dataclasses.replace(small_workload(1), n_z=1, n_phi=8, extra=dict(tracks=0, noise=0, events=1))
+    ev = make_event(wl, 300, 0, seed, 0)
+    phi = np.arctan2(ev.y, ev.x)
+    bad = np.unique(ev.track_id[(phi < -2.8) | (phi > 0.3)])
+    keep = ~np.isin(ev.track_id, bad)
+    keep &= np.isin(ev.track_id, np.unique(ev.track_id[keep])[:40])
+    rng = np.random.default_rng(seed)
+    r3, h3 = wl.radii[2], wl.half_lengths[2]
+    nphi = rng.uniform(1.2, 2.9, n_noise)
+    cols = dict(x=r3 * np.cos(nphi), y=r3 * np.sin(nphi), z=rng.uniform(-0.99, 0.99, n_noise) * h3)
+    big = Event(
+        x=np.concatenate([ev.x[keep], cols["x"]]).astype(np.float32),
+        y=np.concatenate([ev.y[keep], cols["y"]]).astype(np.float32),
+        z=np.concatenate([ev.z[keep], cols["z"]]).astype(np.float32),
+        layer=np.concatenate([ev.layer[keep], np.full(n_noise, 3)]).astype(np.uint32),
+        det_id=np.concatenate([ev.det_id[keep], np.full(n_noise, 300_300_000)]).astype(np.uint32),
+        track_id=np.concatenate([ev.track_id[keep], np.zeros(n_noise)]).astype(np.uint32),
+        track_pt=np.concatenate([ev.track_pt[keep], np.full(n_noise, NOISE_PT)]).astype(np.float32),
+        n_tracks=40)
+    cfg = Config(wl)
+    hs = HitSet(cfg, [big])
+    return wl, cfg, hs
+
+
+def test_grid_row_at_the_fine_walk_offset_limit(dev):
+    """The fine walk addresses a grid row's hits by 16-bit offsets from the
+    row's start: a row of about 65,000 hits (offsets above 32767 in every
+    late sub-row) gives the oracle's results."""
+    wl, cfg, hs = _one_big_grid_row(65_000)
+    layer = hs.column("layer")
+    assert 65_000 <= int((layer == 3).sum()) <= 65_535
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    assert want["tracklets.h1"].size > 0
+    b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, fuse_filter=True))
+    b.close()
+    bad = compare(got, want, FUSED_KEYS)
+    assert not bad, "\n".join(bad)
+
+
+def test_grid_row_beyond_the_fine_walk_offsets_is_an_input_error(dev):
+    """A grid row of more than 65535 hits of one event and layer fails the run
+    as an input error (larger buffers would not help), with nothing walked."""
+    wl, cfg, hs = _one_big_grid_row(66_000)
+    b = Batch(dev, cfg, Caps.for_hits(hs.n_events, hs.n_hits, fuse_filter=True))
+    b.submit(hs)
+    with pytest.raises(TraxError, match="65535"):
+        b.wait()
+    b.close()
