@@ -8,7 +8,7 @@ import pytest
 import oracle
 from helpers import PIPELINE_KEYS, compare, hitset, small_workload
 from test_gpu_pipeline import make_case
-from trax_amd.api import Batch, Caps, HitSet, TraxError, autotune, run_batch_fit
+from trax_amd.api import Batch, Caps, HitSet, autotune, run_batch_fit
 
 pytestmark = pytest.mark.gpu
 
@@ -87,21 +87,22 @@ def test_fused_autotune_keeps_results(dev):
 
 def _one_big_grid_row(n_noise, seed=3):
     """One event on three layers with a 1 z x 8 phi grid: clean tracks whose
-    hits all lie at phi in [-2.8, 0.3], and n_noise hits on the third layer at
-    phi in [1.2, 2.9], so the third layer's single grid row holds about
-    n_noise hits while every window stays clear of the noise."""
+    hits all lie at phi in [-0.2, 2.8], and n_noise hits on the third layer at
+    phi in [-3.1, -1.2], so the third layer's single grid row holds about
+    n_noise hits, all of them before the tracks' hits in grid order, while
+    every window stays clear of the noise."""
     import dataclasses
     from trax_amd.api import Config
     from trax_amd.generator import NOISE_PT, Event, make_event
     wl = dataclasses.replace(small_workload(1), n_z=1, n_phi=8, extra=dict(tracks=0, noise=0, events=1))
     ev = make_event(wl, 300, 0, seed, 0)
     phi = np.arctan2(ev.y, ev.x)
-    bad = np.unique(ev.track_id[(phi < -2.8) | (phi > 0.3)])
+    bad = np.unique(ev.track_id[(phi < -0.2) | (phi > 2.8)])
     keep = ~np.isin(ev.track_id, bad)
     keep &= np.isin(ev.track_id, np.unique(ev.track_id[keep])[:40])
     rng = np.random.default_rng(seed)
     r3, h3 = wl.radii[2], wl.half_lengths[2]
-    nphi = rng.uniform(1.2, 2.9, n_noise)
+    nphi = rng.uniform(-3.1, -1.2, n_noise)
     cols = dict(x=r3 * np.cos(nphi), y=r3 * np.sin(nphi), z=rng.uniform(-0.99, 0.99, n_noise) * h3)
     big = Event(
         x=np.concatenate([ev.x[keep], cols["x"]]).astype(np.float32),
@@ -117,27 +118,19 @@ def _one_big_grid_row(n_noise, seed=3):
     return wl, cfg, hs
 
 
-def test_grid_row_at_the_fine_walk_offset_limit(dev):
-    """The fine walk addresses a grid row's hits by 16-bit offsets from the
-    row's start: a row of about 65,000 hits (offsets above 32767 in every
-    late sub-row) gives the oracle's results."""
-    wl, cfg, hs = _one_big_grid_row(65_000)
-    layer = hs.column("layer")
-    assert 65_000 <= int((layer == 3).sum()) <= 65_535
+@pytest.mark.parametrize("n_noise", [65_000, 66_000])
+def test_third_layer_at_and_beyond_the_list_offset_limit(dev, n_noise):
+    """The fused predictor lists a pair's matches as 16-bit offsets from the
+    first hit of the third layer's (event, layer) grid: with about 65,000 hits
+    there (one grid row) the listed offsets run up to the limit; with 66,000
+    the lists are off and every match goes through the overflow pool.  Both
+    give the oracle's results."""
+    wl, cfg, hs = _one_big_grid_row(n_noise)
+    n3 = int((hs.column("layer") == 3).sum())
+    assert (n3 <= 65_535) == (n_noise == 65_000)
     want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
     assert want["tracklets.h1"].size > 0
     b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, fuse_filter=True))
     b.close()
     bad = compare(got, want, FUSED_KEYS)
     assert not bad, "\n".join(bad)
-
-
-def test_grid_row_beyond_the_fine_walk_offsets_is_an_input_error(dev):
-    """A grid row of more than 65535 hits of one event and layer fails the run
-    as an input error (larger buffers would not help), with nothing walked."""
-    wl, cfg, hs = _one_big_grid_row(66_000)
-    b = Batch(dev, cfg, Caps.for_hits(hs.n_events, hs.n_hits, fuse_filter=True))
-    b.submit(hs)
-    with pytest.raises(TraxError, match="65535"):
-        b.wait()
-    b.close()

@@ -667,8 +667,6 @@ int trax_batch::finish() {
     }
     have_counts = true;
     const uint32_t err = h_counts->err;
-    if (err & kErrRowSize)
-        return fail(TRAX_ERR_INPUT, "a grid row holds more than 65535 hits of one event and layer (use more z rows)");
     if (err & kErrCapacity) return fail(TRAX_ERR_CAPACITY, "an output exceeded its capacity");
     if (err & kErrCellRange) return fail(TRAX_ERR_INPUT, "a hit lies outside the z-phi grid");
     if (err & kErrIndexRange) return fail(TRAX_ERR_INPUT, "an index read from device memory is out of range");
@@ -840,7 +838,7 @@ void trax_batch::layout(Arena& a) {
         cols(B.hin);
         cols(B.hout);
         B.hout.zphi = a.take<float2>((uint64_t)N + kZphiSlack);
-        B.fine.grid = a.take<uint16_t>(E * L * fine_geom(B.cfg.grid.geom(), fine_max(B.cfg.grid.geom())).cells);
+        B.fine.grid = a.take<uint32_t>(E * L * fine_geom(B.cfg.grid.geom(), fine_max(B.cfg.grid.geom())).cells);
         B.fine.hits = a.take<float4>((uint64_t)N + kFineSlack);
         if (caps.fuse_filter) {
             B.hout.xy = a.take<float2>(N);

@@ -16,7 +16,6 @@ constexpr int kWave = 64;
 constexpr uint32_t kErrCellRange = 1u;     // a hit maps outside the z-phi grid
 constexpr uint32_t kErrCapacity = 2u;      // an output exceeded its preallocated capacity
 constexpr uint32_t kErrIndexRange = 4u;    // an index read from an input is out of range
-constexpr uint32_t kErrRowSize = 8u;       // a grid row holds more hits than the fine walk's 16-bit offsets address
 
 // Device-resident sizes of one pipeline run.  Stage kernels read their work
 // size from here, so that one launch sequence (and one captured graph) serves
@@ -69,7 +68,7 @@ __device__ inline uint32_t bounded(uint32_t v, uint32_t cap, DevCounts* cnt) {
 // this at entry and does no work once a kernel before it has set one of them
 // (kernels on one stream run in order, so the bits an earlier kernel set are
 // visible).  The host reports the error and discards the run.
-constexpr uint32_t kErrVoid = kErrCellRange | kErrCapacity | kErrIndexRange | kErrRowSize;
+constexpr uint32_t kErrVoid = kErrCellRange | kErrCapacity | kErrIndexRange;
 __device__ inline uint32_t load_err(const DevCounts* cnt) {
     return __hip_atomic_load(&cnt->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

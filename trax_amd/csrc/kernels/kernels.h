@@ -53,18 +53,13 @@ struct LaunchCfg {
 // The fused predictor's own walk structure: every z row of every (event,
 // layer) grid split into F sub-rows by z within the row, the hits of a row
 // reordered (stable) by (sub-row, phi cell), each with its grid index.  Sub-row
-// s of row r holds the hits whose fine_bin(z) is s.  The reordering stays
-// inside a grid row, so the row's hits keep the grid's positions [row start,
-// row end) and the prefix entries are 16-bit offsets from the row's start
-// (the grid's entry of the row's first cell): per grid row, F sub-rows of
-// n_phi + 1 cell starts in (sub-row, cell) order, then the row's hit count,
-// padded to an even count (row_len).  A grid row holds at most 65535 hits
-// (more: kErrRowSize, an input error).
+// s of row r holds the hits whose fine_bin(z) is s; the prefix entries are laid
+// out like the grid's, (n_z * F) rows of n_phi + 1 cell starts and one final
+// entry, and use the grid's hit positions (the reordering stays inside a row).
 struct FineGeom {
     uint32_t F = 1, shift = 0;  // sub-rows per row, log2 F
     float inv_sub = 0.0f;       // F / sector_z
-    uint32_t row_len = 0;       // F * (n_phi + 1) + 1 entries per grid row, rounded up to even
-    uint32_t cells = 0;         // n_z * row_len entries per (event, layer)
+    uint32_t cells = 0;         // n_z * F * (n_phi + 1) + 1 prefix entries per (event, layer)
 };
 FineGeom fine_geom(const GridGeom& g, uint32_t F);
 // entries of room past the last hit of every (z, phi) column: the pair walks
@@ -77,7 +72,7 @@ constexpr uint32_t kFineMax = 16;
 // largest sub-row count the launch supports for this grid (power of two <= kFineMax)
 uint32_t fine_max(const GridGeom& g);
 struct FineBufs {
-    uint16_t* grid = nullptr;  // E * L * fine_geom(g, fine_max(g)).cells prefix entries
+    uint32_t* grid = nullptr;  // E * L * fine_geom(g, kFineMax).cells prefix entries
     float4* hits = nullptr;    // (z, phi, grid index as bits, 0) at fine positions; N + kFineSlack entries
     // the fused predictor + filter: per resident block, a pool of pool_cap
     // store-window matches beyond a pair's LDS list

@@ -713,13 +713,11 @@ __device__ inline void fine_range(const GridGeom& g, const FineGeom& fg, uint32_
 // Fine structure of one (event, layer, grid row): the row's hits counted per
 // (sub-row, phi cell) by the thread that owns the cell (cells hold their hits
 // in input order, so a sequential pass keeps the order stable), an LDS scan
-// in (sub-row, cell) order, the row's 16-bit offsets written two to a 32-bit
-// store, then the scatter to the row's grid positions.  One block per item;
-// items are (event, layer) * n_z + row.  A row of more than 65535 hits gets
-// empty offsets and kErrRowSize (the run is void).
+// in (sub-row, cell) order offset by the row's first hit, then the scatter.
+// One block per item; items are (event, layer) * n_z + row.
 __global__ __launch_bounds__(256) void k_fine_grid(GridGeom g, FineGeom fg, const uint32_t* __restrict__ grid,
                                                   const float2* __restrict__ hzp, uint32_t n_hit_cap,
-                                                  DevCounts* __restrict__ cnt, uint16_t* __restrict__ fgrid,
+                                                  const DevCounts* __restrict__ cnt, uint32_t* __restrict__ fgrid,
                                                   float4* __restrict__ fh) {
     extern __shared__ uint32_t s_cnt[];
     const uint32_t np1 = g.n_phi + 1, nb = fg.F * np1;
@@ -728,8 +726,7 @@ __global__ __launch_bounds__(256) void k_fine_grid(GridGeom g, FineGeom fg, cons
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t el = item / g.n_z, zs = item - el * g.n_z;
         const uint32_t* lg = grid + (size_t)el * g.cells + (size_t)zs * np1;
-        // row_len is even and so is cells: every row starts 4-byte aligned
-        uint32_t* fo = reinterpret_cast<uint32_t*>(fgrid + (size_t)el * fg.cells + (size_t)zs * fg.row_len);
+        uint32_t* fo = fgrid + (size_t)el * fg.cells + (size_t)zs * nb;
         const float row_lo = fine_row_lo(g, zs);
         for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) s_cnt[i] = 0;
         __syncthreads();
@@ -738,16 +735,11 @@ __global__ __launch_bounds__(256) void k_fine_grid(GridGeom g, FineGeom fg, cons
             for (uint32_t p = lg[c]; p < p1; ++p) s_cnt[fine_bin(fg, row_lo, hzp[p].x) * np1 + c] += 1u;
         }
         __syncthreads();
-        const uint32_t total = lds_excl_scan(s_cnt, nb, wsum);
-        const bool fits = total <= 0xffffu;
-        if (!fits && threadIdx.x == 0) atomicOr(&cnt->err, kErrRowSize);
-        for (uint32_t i = threadIdx.x; 2 * i < fg.row_len; i += blockDim.x) {
-            const uint32_t e0 = 2 * i, e1 = 2 * i + 1;
-            const uint32_t v0 = e0 < nb ? s_cnt[e0] : total, v1 = e1 < nb ? s_cnt[e1] : total;
-            fo[i] = fits ? (v0 | v1 << 16) : 0u;
-        }
-        __syncthreads();
+        lds_excl_scan(s_cnt, nb, wsum);
         const uint32_t base = lg[0];
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) fo[i] = base + s_cnt[i];
+        if (zs + 1 == g.n_z && threadIdx.x == 0) fo[nb] = lg[np1];  // the final entry: end of the last row
+        __syncthreads();
         for (uint32_t c = threadIdx.x; c < np1; c += blockDim.x) {
             const uint32_t p1 = lg[c + 1] < n_hit_cap ? lg[c + 1] : n_hit_cap;
             for (uint32_t p = lg[c]; p < p1; ++p) {
@@ -760,20 +752,6 @@ __global__ __launch_bounds__(256) void k_fine_grid(GridGeom g, FineGeom fg, cons
         }
         __syncthreads();
     }
-}
-
-// Sub-row ff (grid row ff >> shift) of a window's (event, layer): the grid
-// positions of its start and end and of the window's first and last phi cell
-// (the row's 16-bit offsets plus the grid's start of the row).
-struct RowPre {
-    uint32_t f, start, end, lo, hi;
-};
-__device__ inline RowPre fine_row(const PredWin2& w, const FineGeom& fg, const uint16_t* __restrict__ fl,
-                                  uint32_t np1, uint32_t ff) {
-    const uint32_t zs = ff >> fg.shift;
-    const uint16_t* r = fl + (size_t)zs * fg.row_len + (ff - (zs << fg.shift)) * np1;
-    const uint32_t base = w.lg3[zs * np1];
-    return RowPre{ff, base + r[0], base + r[np1], base + r[w.pLo], base + r[w.pHi]};
 }
 
 // The merged walk of both windows over the fine rows (as pred_walk_flat over
@@ -793,7 +771,7 @@ template <class M>
 __device__ inline M fine_bit(uint32_t k) { return (M)1 << k; }
 template <class M, class OV>
 __device__ inline void pred_walk_fine(const PredWin2& w, const GridGeom& g, const FineGeom& fg,
-                                      const uint16_t* __restrict__ fl, const float4* __restrict__ fh, uint32_t cap,
+                                      const uint32_t* __restrict__ fl, const float4* __restrict__ fh, uint32_t cap,
                                       uint32_t first, uint16_t* my, uint32_t& countA, uint32_t& nB, M& rmask,
                                       M& smask, OV&& over) {
     uint32_t aLo, aHi, bLo, bHi;
@@ -817,9 +795,18 @@ __device__ inline void pred_walk_fine(const PredWin2& w, const GridGeom& g, cons
             if ((ea && ff >= aLo && ff < aHi) || (eb && ff >= bLo && ff < bHi)) break;
         return ff;
     };
+    struct RowPre {
+        uint32_t f, start, end, lo, hi;
+    };
     auto load_row = [&](uint32_t ff) {
         RowPre rp{ff, 0u, 0u, 0u, 0u};
-        if (ff < f1) rp = fine_row(w, fg, fl, np1, ff);
+        if (ff < f1) {
+            const uint32_t* r = fl + (size_t)ff * np1;
+            rp.start = r[0];
+            rp.end = r[np1];
+            rp.lo = r[w.pLo];
+            rp.hi = r[w.pHi];
+        }
         return rp;
     };
     RowPre cur = load_row(active_from(f));
@@ -892,7 +879,7 @@ __device__ inline M fine_bits(uint32_t lo, uint32_t hi) {
 //    grid row.
 template <class M, class OV>
 __device__ inline void pred_walk_fine2(const PredWin2& w, const GridGeom& g, const FineGeom& fg,
-                                       const uint16_t* __restrict__ fl, const float4* __restrict__ fh, uint32_t cap,
+                                       const uint32_t* __restrict__ fl, const float4* __restrict__ fh, uint32_t cap,
                                        uint32_t first, uint16_t* my, uint32_t& countA, uint32_t& nB, M& rmask,
                                        M& smask, OV&& over) {
     uint32_t aLo, aHi, bLo, bHi;
@@ -925,9 +912,18 @@ __device__ inline void pred_walk_fine2(const PredWin2& w, const GridGeom& g, con
             if ((ea && ff >= aLo && ff < aHi) || (eb && ff >= bLo && ff < bHi)) break;
         return ff;
     };
+    struct RowPre {
+        uint32_t f, start, end, lo, hi;
+    };
     auto load_row = [&](uint32_t ff) {
         RowPre rp{ff, 0u, 0u, 0u, 0u};
-        if (ff < f1) rp = fine_row(w, fg, fl, np1, ff);
+        if (ff < f1) {
+            const uint32_t* r = fl + (size_t)ff * np1;
+            rp.start = r[0];
+            rp.end = r[np1];
+            rp.lo = r[w.pLo];
+            rp.hi = r[w.pHi];
+        }
         return rp;
     };
     RowPre cur = load_row(active_from(f));
@@ -1069,7 +1065,7 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(6)))
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cands, uint32_t* __restrict__ cand_offsets, uint32_t n_cand_cap,
     DevCounts* __restrict__ cnt, uint64_t* __restrict__ status, uint32_t* __restrict__ counter, FineGeom fg,
-    const uint16_t* __restrict__ fgrid, const float4* __restrict__ fh) {
+    const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh) {
     constexpr int kSortB = kSort ? kPredBlock : 1;
     using Mask = typename std::conditional<(kPredList <= 32), uint32_t, uint64_t>::type;
     static_assert(kPredList <= 64, "list masks hold 64 entries");
@@ -1280,7 +1276,7 @@ __global__ __launch_bounds__(kPredBlock) void k_predict_filter(
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cand_offsets, uint32_t* __restrict__ th1, uint32_t* __restrict__ th2,
     uint32_t* __restrict__ th3, uint32_t scratch_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
-    FineGeom fg, const uint16_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
+    FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
     uint2* __restrict__ pools,
     uint32_t pool_cap, uint32_t* __restrict__ chunk_cands, uint32_t* __restrict__ chunk_trk,
     uint32_t* __restrict__ chunk_at, uint32_t* __restrict__ group_chunk) {
@@ -1619,7 +1615,7 @@ __global__ __launch_bounds__(kPredBlock) __attribute__((amdgpu_waves_per_eu(4)))
     const uint32_t* __restrict__ hlayer, const uint32_t* __restrict__ hevt, const uint32_t* __restrict__ pairs,
     uint32_t* __restrict__ cand_offsets, uint32_t* __restrict__ th1, uint32_t* __restrict__ th2,
     uint32_t* __restrict__ th3, uint32_t scratch_cap, DevCounts* __restrict__ cnt, uint32_t* __restrict__ counter,
-    FineGeom fg, const uint16_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
+    FineGeom fg, const uint32_t* __restrict__ fgrid, const float4* __restrict__ fh, const float2* __restrict__ hxy,
     uint2* __restrict__ pools,
     uint32_t pool_cap, uint32_t* __restrict__ chunk_cands, uint32_t* __restrict__ chunk_trk,
     uint32_t* __restrict__ chunk_at, uint32_t* __restrict__ group_chunk, uint32_t per_claim) {
@@ -2212,8 +2208,7 @@ FineGeom fine_geom(const GridGeom& g, uint32_t F) {
     fg.shift = 0;
     while ((1u << fg.shift) < F) ++fg.shift;
     fg.inv_sub = (float)F / g.sector_z;
-    fg.row_len = (F * (g.n_phi + 1) + 2) & ~1u;
-    fg.cells = g.n_z * fg.row_len;
+    fg.cells = g.n_z * F * (g.n_phi + 1) + 1;
     return fg;
 }
 
