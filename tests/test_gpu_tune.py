@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import compare, hitset, small_workload
+from helpers import PIPELINE_KEYS, compare, hitset, small_workload
 from trax_amd.api import Batch, Caps, TraxError, autotune, run_batch_fit
 
 pytestmark = pytest.mark.gpu
@@ -97,6 +97,23 @@ def test_grid_build_variants_match_oracle(dev, stage_kib, order, block):
         keys = ["grid", "hits.x", "hits.y", "hits.z", "hits.layer", "hits.det_id", "hits.hit_id", "hits.event"]
         bad = compare(got, want, keys=keys)
         assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("stage_kib,block", [(64, 512), (80, 512), (24, 256)])
+def test_grid_records_feed_the_whole_pipeline(dev, stage_kib, block):
+    """Items the grid build does not stage in LDS leave 32-byte records whose
+    phi becomes the (z, phi) column the pair and predictor walks read: the
+    fused pipeline's results are the oracle's on a dense event (layers of up
+    to 3,600 hits: staged, recorded with LDS lists, and recorded with the
+    global scatter for the layers above the lists)."""
+    wl = small_workload(2, tracks=1500, noise=20000, events=1)
+    cfg, hs, _ = hitset(wl, seed=5)
+    want = oracle.run(hs.columns(), wl, cfg.min_radius_curvature)
+    b, got = run_batch_fit(dev, cfg, hs, Caps.for_hits(hs.n_events, hs.n_hits, fuse_filter=True),
+                           tuning=dict(grid_stage_kib=stage_kib, grid_count_block=block))
+    b.close()
+    bad = compare(got, want, [k for k in PIPELINE_KEYS if k != "cands"])
+    assert not bad, "\n".join(bad)
 
 
 def test_autotune_picks_supported_shapes_with_same_results(dev):

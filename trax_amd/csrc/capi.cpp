@@ -686,7 +686,7 @@ void trax_batch::enqueue_stage(int s) {
     auto ok = [](hipError_t e, const char* w) { hip_check(e, w); };
     switch (s) {
         case 0:  // grid_count: the whole grid build (count, scan, stable scatter) in one kernel
-            ok(launch_grid_build(g, E * L, eo, ln, lo, nullptr, hin, hout, grid, grid_perm, N, cnt, lc, st, true),
+            ok(launch_grid_build(g, E * L, eo, ln, lo, grid_rec, hin, hout, grid, grid_perm, N, cnt, lc, st, true),
                "grid build");  // the run's counts are cleared first, in the same launches
             break;
         case 1:  // grid_store: part of the grid build above
@@ -857,6 +857,7 @@ void trax_batch::layout(Arena& a) {
         B.sup = a.take<uint32_t>(E + 2 * E * L);
         B.grid = a.take<uint32_t>(E * L * C);
         B.grid_perm = a.take<uint32_t>(N + (uint64_t)E * L);  // scatter scratch, then the item order
+        B.grid_rec = a.take<float4>(2 * (uint64_t)N);
         B.seg_base = a.take<uint32_t>(E * T + 1);
         B.slot_counts = a.take<uint32_t>(slot_cap + 1);
         B.slot_seg = a.take<uint32_t>(slot_seg_entries(slot_cap));

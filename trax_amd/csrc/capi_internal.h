@@ -169,6 +169,7 @@ struct trax_batch {
     trax::CABufs cb;
     trax::FineBufs fine;  // the fused predictor's walk structure
     uint32_t* grid_perm = nullptr;  // the grid build's scatter scratch (N), then its item order (E * L)
+    float4* grid_rec = nullptr;     // the grid build's 32-byte hit records (2 N float4)
     hipGraph_t graphs[trax_capi::kNumStages] = {};
     hipGraphExec_t execs[trax_capi::kNumStages] = {};
     bool captured = false;

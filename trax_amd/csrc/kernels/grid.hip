@@ -150,12 +150,16 @@ __global__ __launch_bounds__(1024) void k_grid_store(GridGeom g, const uint32_t*
 //    LDS in output order.  Every input byte is read from HBM once, coalesced,
 //    and every output byte written once.  An item whose layer or event column
 //    is not uniform reads those two columns at the sources instead.
-//  - local (n <= lds_hits): cells and perm in LDS (2 B per hit each); the
-//    seven columns are gathered at the sources in output order.
-//  - global: perm in global memory at the output positions, x, y, z re-read
-//    for the scatter.
-// Reads 28 B per hit (staged) or 12 + 28 B per hit; writes 4 B per cell and
-// 44 B per hit of output (52 B with the (x, y) column).
+//  - local (n <= lds_hits): cells and perm in LDS (2 B per hit each); with a
+//    record buffer, the count pass also writes each hit's (x, y, z, phi) and
+//    (layer, detId, hitId, event) as one 32-byte record at the hit's source
+//    position, and the outputs gather one record per hit (one cache line, and
+//    no second atan2); without one, the seven columns are gathered.
+//  - global: perm in global memory at the output positions, x, y, z (or the
+//    record) re-read for the scatter.
+// Reads 28 B per hit (staged) or 28 + 32 B per hit (records; 12 + 28 B
+// without); writes 4 B per cell and 44 B per hit of output (52 B with the
+// (x, y) column), and 32 B per hit of records.
 
 // insertion sort of each cell's entries of perm; ends[c] is the end of cell c
 template <typename T>
@@ -239,7 +243,7 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                                                    HitColsOut out, uint32_t* __restrict__ grid,
                                                    uint32_t* __restrict__ perm,
                                                    const uint32_t* __restrict__ order, uint32_t n_hits_cap,
-                                                   uint32_t lds_hits, uint32_t stage_cap,
+                                                   uint32_t lds_hits, uint32_t stage_cap, float4* __restrict__ rec,
                                                    DevCounts* __restrict__ cnt) {
     extern __shared__ uint32_t lds[];
     const GridLds L = grid_lds(lds, g.cells, stage_cap, lds_hits);
@@ -264,6 +268,7 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
         const bool fits = (uint64_t)off + n <= n_hits_cap;
         const bool staged = fits && n <= stage_cap;  // block-uniform
         const bool local = n <= lds_hits;            // block-uniform
+        const bool use_rec = rec && fits && !staged;  // block-uniform: records at the sources
         // the item's layer and event: one value each for a uniform item
         const uint32_t la0 = (staged && n) ? in.layer[off] : 0u, ev0 = (staged && n) ? in.evt[off] : 0u;
         bool bad = false, mixed = false;
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                 const uint32_t i = i0 + k * bs;
                 if (i < n) {
                     x[k] = in.x[off + i]; y[k] = in.y[off + i]; z[k] = in.z[off + i];
-                    if (staged) {
+                    if (staged || use_rec) {
                         de[k] = in.det[off + i]; id[k] = in.id[off + i];
                         la[k] = in.layer[off + i]; ev[k] = in.evt[off + i];
                     }
@@ -290,7 +295,14 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                         L.sz[i] = __float_as_uint(z[k]); L.sdet[i] = de[k]; L.sid[i] = id[k];
                         mixed |= la[k] != la0 || ev[k] != ev0;
                     }
-                    const uint32_t c = hit_cell(g, x[k], y[k], z[k]);
+                    const float phi = tx_atan2f(y[k], x[k]);
+                    const uint32_t c = hit_cell_phi(g, z[k], phi);
+                    if (use_rec) {
+                        float4* r = rec + 2 * (size_t)(off + i);
+                        r[0] = make_float4(x[k], y[k], z[k], phi);
+                        r[1] = make_float4(__uint_as_float(la[k]), __uint_as_float(de[k]), __uint_as_float(id[k]),
+                                           __uint_as_float(ev[k]));
+                    }
                     if (c < g.cells) atomicAdd(&hist[c], 1u);
                     else bad = true;
                     if (local) L.lcell[i] = c < g.cells ? (uint16_t)c : kNoCell;
@@ -315,7 +327,13 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                 sort_cells(L.lperm, hist, g.cells);
             } else {
                 for (uint32_t i = threadIdx.x; i < n; i += bs) {
-                    const uint32_t c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
+                    uint32_t c;
+                    if (use_rec) {
+                        const float4 r = rec[2 * (size_t)(off + i)];
+                        c = hit_cell_phi(g, r.z, r.w);
+                    } else {
+                        c = hit_cell(g, in.x[off + i], in.y[off + i], in.z[off + i]);
+                    }
                     if (c < g.cells) perm[off + atomicAdd(&hist[c], 1u)] = i;
                 }
                 __syncthreads();
@@ -326,7 +344,7 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
             const bool from_lds = staged, cols_mixed = *L.mixed != 0u;
             for (uint32_t p0 = threadIdx.x; p0 < placed; p0 += kGridUnroll * bs) {
                 uint32_t src[kGridUnroll];
-                float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll];
+                float x[kGridUnroll], y[kGridUnroll], z[kGridUnroll], ph[kGridUnroll];
                 uint32_t la[kGridUnroll], de[kGridUnroll], id[kGridUnroll], ev[kGridUnroll];
 #pragma unroll
                 for (uint32_t k = 0; k < kGridUnroll; ++k) {
@@ -342,6 +360,12 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                             z[k] = __uint_as_float(L.sz[i]); de[k] = L.sdet[i]; id[k] = L.sid[i];
                             la[k] = cols_mixed ? in.layer[off + i] : la0;
                             ev[k] = cols_mixed ? in.evt[off + i] : ev0;
+                        } else if (use_rec) {  // one 32-byte record per hit
+                            const float4* r = rec + 2 * (size_t)(off + i);
+                            const float4 a = r[0], b = r[1];
+                            x[k] = a.x; y[k] = a.y; z[k] = a.z; ph[k] = a.w;
+                            la[k] = __float_as_uint(b.x); de[k] = __float_as_uint(b.y);
+                            id[k] = __float_as_uint(b.z); ev[k] = __float_as_uint(b.w);
                         } else {
                             const uint32_t j = off + i;
                             x[k] = in.x[j]; y[k] = in.y[j]; z[k] = in.z[j];
@@ -359,7 +383,7 @@ __global__ __launch_bounds__(1024) void k_grid_build(GridGeom g, const uint32_t*
                         out.det[pos] = de[k];
                         out.id[pos] = id[k];
                         out.evt[pos] = ev[k];
-                        if (out.zphi) out.zphi[pos] = make_float2(z[k], tx_atan2f(y[k], x[k]));
+                        if (out.zphi) out.zphi[pos] = make_float2(z[k], use_rec ? ph[k] : tx_atan2f(y[k], x[k]));
                         if (out.xy) out.xy[pos] = make_float2(x[k], y[k]);
                     }
                 }
@@ -388,11 +412,10 @@ void grid_lds_plan(const GridGeom& g, uint32_t block, size_t budget, uint32_t& l
 }  // namespace
 
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
-                             const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
+                             const uint32_t* lay_nhits, const uint32_t* lay_offset, float4* rec,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
                              uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
                              bool reset_counts) {
-    (void)grid_unused;
     // defaults: 1024 threads, 160 KiB of LDS (items staged whole), largest items first
     const uint32_t bs = lc.grid_count_block ? lc.grid_count_block : 1024;
     uint32_t lds_hits = 0, stage_cap = 0;
@@ -416,7 +439,7 @@ hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint
     HitCols ic{in.x, in.y, in.z, in.layer, in.det, in.id, in.evt};
     HitColsOut oc{out.x, out.y, out.z, out.layer, out.det, out.id, out.evt, out.zphi, out.xy};
     hipLaunchKernelGGL(k_grid_build, dim3(blocks), dim3(bs), lds, st, g, &cnt->n_events, evt_offset, lay_nhits,
-                       lay_offset, ic, oc, grid, perm, order, n_hits_cap, lds_hits, stage_cap, cnt);
+                       lay_offset, ic, oc, grid, perm, order, n_hits_cap, lds_hits, stage_cap, rec, cnt);
     return hipGetLastError();
 }
 

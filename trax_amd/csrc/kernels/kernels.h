@@ -160,7 +160,10 @@ hipError_t launch_chunk_copy3(const uint32_t* d_chunks, uint64_t max_chunks, con
 hipError_t launch_forward_fill(uint32_t* off, const DevCounts* cnt, uint32_t n_trip, hipStream_t st);
 
 // GridBuilder
-// count, scan and stable scatter in one kernel (batched path); perm: N u32 scratch
+// count, scan and stable scatter in one kernel (batched path); perm: N u32 scratch;
+// rec: 2 N float4 scratch (null: none) where the items that are not staged in
+// LDS leave each hit's (x, y, z, phi) and (layer, detId, hitId, event) in the
+// count pass, for the scatter to read as one 32-byte record per hit
 // hits of one (event, layer) that the one-kernel grid build keeps in LDS at a
 // block size (0 when the grid's cells do not fit a u16 or LDS); larger layers
 // take its global-memory path
@@ -168,7 +171,7 @@ constexpr uint32_t kGridLdsHitsMax = 8192;
 constexpr size_t kGridLdsBase = 64 * 1024;   // LDS per block of the unstaged build
 constexpr size_t kGridLdsMax = 160 * 1024;   // gfx950's LDS per workgroup
 hipError_t launch_grid_build(const GridGeom& g, uint32_t n_items_cap, const uint32_t* evt_offset,
-                             const uint32_t* lay_nhits, const uint32_t* lay_offset, const uint32_t* grid_unused,
+                             const uint32_t* lay_nhits, const uint32_t* lay_offset, float4* rec,
                              const HitColumns& in, const HitColumns& out, uint32_t* grid, uint32_t* perm,
                              uint32_t n_hits_cap, DevCounts* cnt, const LaunchCfg& lc, hipStream_t st,
                              bool reset_counts = false);  // also clear the run's counts (after the shape) first
