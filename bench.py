@@ -311,8 +311,9 @@ def main():
                     help="predictor and filter as separate passes (the candidates stored) instead of one")
     ap.add_argument("--tuning", type=str, default=None,
                     help="fixed launch shapes as JSON (e.g. the autotuner's choice), skipping the autotuner")
-    ap.add_argument("--e2e-reps", type=int, default=4,
-                    help="copies of the step's events in the end-to-end (first H2D to last D2H) run (0 = skip)")
+    ap.add_argument("--e2e-reps", type=int, default=8,
+                    help="copies of the step's events in the end-to-end (first H2D to last D2H) run (0 = skip); "
+                         "the first upload and the last download are not overlapped, so more copies weigh them less")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the float64 adjudication of the filter's decisions (a run for the profiler)")
     ap.add_argument("--copy-calibration", action="store_true",

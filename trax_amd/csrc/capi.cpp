@@ -574,9 +574,9 @@ void trax_batch::enqueue_all(bool phase_events) {
         enqueue_stage(k);
         if (k == kGridStage) {  // fork: the fine walk structure beside the pair stages
             hip_check(hipEventRecord(ev_grid, st), "event");
-            hip_check(hipStreamWaitEvent(st2, ev_grid, 0), "wait grid");
-            enqueue_fine(st2);
-            hip_check(hipEventRecord(ev_fine, st2), "event");
+            hip_check(hipStreamWaitEvent(side(), ev_grid, 0), "wait grid");
+            enqueue_fine(side());
+            hip_check(hipEventRecord(ev_fine, side()), "event");
         }
     }
     if (phase_events) hip_check(hipEventRecord(ev[kNumStages], st), "event");
@@ -600,8 +600,15 @@ bool trax_batch::capture_whole() {
     whole_exec = nullptr;
     (void)hipGetLastError();  // a failed capture leaves no error behind for later calls
     (void)hipStreamSynchronize(st);
-    (void)hipStreamSynchronize(st2);
+    (void)hipStreamSynchronize(side());
     return false;
+}
+
+void trax_batch::single_stream() {
+    if (!st2 || captured) return;
+    (void)hipStreamSynchronize(st2);
+    (void)hipStreamDestroy(st2);
+    st2 = nullptr;
 }
 
 void trax_batch::launch() {
@@ -614,9 +621,9 @@ void trax_batch::launch() {
                 hip_check(hipStreamEndCapture(st, &graphs[k]), "end capture");
                 hip_check(hipGraphInstantiate(&execs[k], graphs[k], nullptr, nullptr, 0), "instantiate");
             }
-            hip_check(hipStreamBeginCapture(st2, hipStreamCaptureModeThreadLocal), "begin capture");
-            enqueue_fine(st2);
-            hip_check(hipStreamEndCapture(st2, &fine_graph), "end capture");
+            hip_check(hipStreamBeginCapture(side(), hipStreamCaptureModeThreadLocal), "begin capture");
+            enqueue_fine(side());
+            hip_check(hipStreamEndCapture(side(), &fine_graph), "end capture");
             hip_check(hipGraphInstantiate(&fine_exec, fine_graph, nullptr, nullptr, 0), "instantiate");
         }
         captured = true;
@@ -634,14 +641,18 @@ void trax_batch::launch() {
             hip_check(hipGraphLaunch(execs[k], st), "graph launch");
             if (k == kGridStage) {  // fork: the fine walk structure beside the pair stages
                 hip_check(hipEventRecord(ev_grid, st), "event");
-                hip_check(hipStreamWaitEvent(st2, ev_grid, 0), "wait grid");
-                hip_check(hipGraphLaunch(fine_exec, st2), "graph launch");
-                hip_check(hipEventRecord(ev_fine, st2), "event");
+                hip_check(hipStreamWaitEvent(side(), ev_grid, 0), "wait grid");
+                hip_check(hipGraphLaunch(fine_exec, side()), "graph launch");
+                hip_check(hipEventRecord(ev_fine, side()), "event");
             }
         }
         hip_check(hipEventRecord(ev[kNumStages], st), "event");
     }
     hip_check(hipMemcpyAsync(h_counts, cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, st), "d2h");
+    // the CA's last iteration flags come back with the counts, queued behind
+    // this pipeline only: a read queued later on this stream could wait
+    // behind work that other streams queued in the meantime
+    hip_check(hipMemcpyAsync(h_flags, cb.flags, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "d2h");
     submitted = true;
     have_counts = false;
 }
@@ -649,14 +660,13 @@ void trax_batch::launch() {
 int trax_batch::finish() {
     // the CA forward phase ran a fixed number of iterations; continue on the
     // host if the last one still found living cells
-    uint32_t flags[3];
-    hip_check(hipMemcpyAsync(flags, cb.flags, sizeof(flags), hipMemcpyDeviceToHost, st), "d2h");
+    uint32_t* flags = h_flags;  // copied back with the counts at the end of launch()
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     uint32_t k = ca_iter;
     if (h_counts->n_conn && !(h_counts->err & kErrVoid) && flags[(k - 1) % 3]) {
         while (true) {
             hip_check(launch_ca_iterations(caps.max_conn, lb.basis, lb.foll, cb, cnt, k, 1, lc, st), "ca iteration");
-            hip_check(hipMemcpyAsync(flags, cb.flags, sizeof(flags), hipMemcpyDeviceToHost, st), "d2h");
+            hip_check(hipMemcpyAsync(flags, cb.flags, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st), "d2h");
             hip_check(hipStreamSynchronize(st), "sync");
             if (!flags[k % 3]) break;
             ++k;
@@ -977,6 +987,9 @@ extern "C" int trax_batch_create(trax_ctx* ctx, const trax_config* cfg, const tr
                   "hipHostMalloc");
         hip_check(hipHostMalloc(reinterpret_cast<void**>(&B.h_counts), sizeof(DevCounts), hipHostMallocDefault),
                   "hipHostMalloc");
+        // pinned, so that reading them back is an ordinary asynchronous copy on the batch's stream
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&B.h_flags), 4 * sizeof(uint32_t), hipHostMallocDefault),
+                  "hipHostMalloc");
         for (auto& e : B.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         hip_check(hipStreamCreateWithFlags(&B.st2, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&B.ev_grid, hipEventDisableTiming), "hipEventCreate");
@@ -1010,6 +1023,7 @@ extern "C" void trax_batch_destroy(trax_batch* b) {
     if (b->cnt) (void)hipFree(b->cnt);
     if (b->staging) (void)hipHostFree(b->staging);
     if (b->h_counts) (void)hipHostFree(b->h_counts);
+    if (b->h_flags) (void)hipHostFree(b->h_flags);
     delete b;
 }
 

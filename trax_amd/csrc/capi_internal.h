@@ -158,6 +158,7 @@ struct trax_batch {
     uint32_t* staging = nullptr;  // pinned: 7 hit columns, supplements, shape
     size_t staging_words = 0;
     trax::DevCounts* h_counts = nullptr;  // pinned copy after a run
+    uint32_t* h_flags = nullptr;          // pinned: the CA's last three iteration flags
     // device buffers
     trax::HitColumns hin, hout;
     uint32_t *sup = nullptr, *grid = nullptr, *seg_base = nullptr, *slot_seg = nullptr, *slot_counts = nullptr,
@@ -176,8 +177,13 @@ struct trax_batch {
     hipEvent_t ev[trax_capi::kNumStages + 1] = {};
     // the predictor's fine walk structure is built on a second stream, beside
     // the pair stages: it needs only the grid build (ev_grid), and the
-    // predict stage waits for it (ev_fine)
+    // predict stage waits for it (ev_fine).  Without one (single_stream())
+    // it is built on st after the grid build.
     hipStream_t st2 = nullptr;
+    hipStream_t side() const { return st2 ? st2 : st; }
+    // drop the second stream (before the first launch): every kernel of the
+    // batch then runs on st
+    void single_stream();
     hipEvent_t ev_grid = nullptr, ev_fine = nullptr;
     hipGraph_t fine_graph = nullptr;
     hipGraphExec_t fine_exec = nullptr;

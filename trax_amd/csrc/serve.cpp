@@ -18,7 +18,19 @@
 //     front, and the next batches take half as many events;
 //   - completion: the counts are read when the slot's stream has drained, the
 //     results of exactly that size are copied back, and the events complete
-//     when that copy has landed.
+//     when that copy has landed;
+//   - overlap: a slot's uploads go out as soon as it takes a batch, but its
+//     pipeline waits for the previously submitted one (an event wait on the
+//     device), so pipelines run one at a time and every slot's uploads and
+//     downloads overlap another slot's pipeline.  Two pipelines side by side
+//     take as long as one after the other, and would finish together, with
+//     both downloads left to follow.  Each slot's batch keeps all its work on
+//     the slot's one stream (no second stream for the fine walk structure):
+//     the device has few hardware queues (four by default), streams beyond
+//     them share one, and a slot whose stream shares a queue with another
+//     slot's pipeline sees its downloads' completion only after that
+//     pipeline.  The counts and the CA's flags are copied back at the end of
+//     the pipeline itself for the same reason.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -82,6 +94,10 @@ struct trax_server {
     uint32_t batches = 0, resubmits = 0, max_batch = 0;
     uint64_t batch_event_sum = 0;
     double sum_queue = 0, sum_pipe = 0, sum_copy = 0, sum_split = 0;
+    // the completion event of the most recently submitted batch's pipeline:
+    // each pipeline starts after the previous one, so the slots' pipelines
+    // run one at a time and a slot's copies overlap another slot's pipeline
+    hipEvent_t last_pipe = nullptr;
 
     ~trax_server() { release(); }
 
@@ -184,8 +200,11 @@ struct trax_server {
         hip_check(launch_event_ids(b->hin.evt, b->sup, b->cnt, Ecap, s.st), "event ids");
         b->n_events = E;
         b->n_hits = N;
+        // the uploads above go out at once; the pipeline waits for the one before it
+        if (last_pipe && last_pipe != s.ev_pipe) hip_check(hipStreamWaitEvent(s.st, last_pipe, 0), "wait");
         b->launch();
         hip_check(hipEventRecord(s.ev_pipe, s.st), "event");
+        last_pipe = s.ev_pipe;
         s.state = 1;
         ++batches;
         batch_event_sum += E;
@@ -345,6 +364,7 @@ extern "C" int trax_server_create(trax_ctx* ctx, const trax_config* cfg, const t
             ts.s = s.st;
             const int rc = trax_batch_create(ctx, cfg, &sv->caps, nullptr, &ts, &s.b);
             if (rc != TRAX_OK) return rc;
+            s.b->single_stream();  // one stream per slot (see the overlap note above)
             sv->device_bytes += trax_batch_footprint(s.b);
             const uint64_t R = sv->caps.max_tracklets;
             s.h1 = pinned<uint32_t>(R); s.h2 = pinned<uint32_t>(R); s.h3 = pinned<uint32_t>(R);
@@ -387,6 +407,7 @@ extern "C" int trax_server_replay(trax_server* sv, const trax_hits* events, cons
         sv->batches = sv->resubmits = sv->max_batch = 0;
         sv->batch_event_sum = 0;
         sv->sum_queue = sv->sum_pipe = sv->sum_copy = sv->sum_split = 0;
+        sv->last_pipe = nullptr;
 
         std::deque<uint32_t> waiting;
         uint32_t next = 0, finished = 0, limit = sv->opts.max_batch_events;
