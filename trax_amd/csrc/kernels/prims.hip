@@ -322,8 +322,15 @@ uint32_t stride_blocks(uint64_t work_cap, uint32_t block, const LaunchCfg& lc) {
 namespace {
 
 // each chunk's three output columns from a scratch to their place in chunk
-// order: one wave per chunk (chunk_n scanned: a chunk's count is the next
-// entry minus its own; chunk_at: where its entries are in the scratch)
+// order (chunk_n scanned: a chunk's count is the next entry minus its own;
+// chunk_at: where its entries are in the scratch).  Chunks hold a dozen or so
+// entries, so a wave takes kCopyGroup consecutive chunks at a time, whose
+// outputs are one contiguous range: its lanes write that range in order, each
+// finding its entry's chunk among the group's (one chunk's first output per
+// lane) by a binary search over lanes.  Writes are coalesced, and so are the
+// reads within a chunk; groups of 16 keep enough waves busy when there are
+// few chunks.
+constexpr uint32_t kCopyGroup = 16;
 __global__ void k_chunk_copy3(const uint32_t* __restrict__ d_chunks, const uint32_t* __restrict__ chunk_n,
                               const uint32_t* __restrict__ chunk_at, const uint32_t* __restrict__ scratch,
                               uint32_t scratch_cap, uint32_t* __restrict__ o1, uint32_t* __restrict__ o2,
@@ -331,19 +338,35 @@ __global__ void k_chunk_copy3(const uint32_t* __restrict__ d_chunks, const uint3
     if (run_void(cnt)) return;  // the chunks' ranges may not all be written
     const uint32_t n_chunks = *d_chunks;
     const uint32_t lane = lane_id();
+    const uint32_t n_waves = gridDim.x * blockDim.x / kWave;
     bool over = false;
-    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave; t < n_chunks;
-         t += gridDim.x * blockDim.x / kWave) {
-        const uint32_t dst = chunk_n[t], m = chunk_n[t + 1] - dst, src = chunk_at[t];
-        for (uint32_t k = lane; k < m; k += kWave) {
-            if ((uint64_t)dst + k >= out_cap) {
+    for (uint32_t g0 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave * kCopyGroup; g0 < n_chunks;
+         g0 += n_waves * kCopyGroup) {  // wave-uniform
+        const uint32_t nc = n_chunks - g0 < kCopyGroup ? n_chunks - g0 : kCopyGroup;
+        const uint32_t t = g0 + (lane < nc ? lane : 0u);
+        const uint32_t my_dst = chunk_n[t], my_src = chunk_at[t];
+        const uint32_t d0 = __shfl(my_dst, 0, kWave), d1 = chunk_n[g0 + nc];
+        for (uint32_t base = d0; base < d1; base += kWave) {  // wave-uniform
+            const uint32_t d = base + lane;
+            // the last of the group's chunks whose first output is at most d
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t step = kCopyGroup / 2; step; step >>= 1) {
+                const uint32_t cand = c + step;
+                const uint32_t v = __shfl(my_dst, cand < nc ? cand : 0u, kWave);
+                if (cand < nc && v <= d) c = cand;
+            }
+            const uint32_t c_dst = __shfl(my_dst, c, kWave), c_src = __shfl(my_src, c, kWave);
+            if (d >= d1) continue;
+            const uint32_t k = d - c_dst;
+            if ((uint64_t)d >= out_cap) {
                 over = true;
-            } else if ((uint64_t)src + k < scratch_cap) {
-                o1[dst + k] = scratch[src + k];
-                o2[dst + k] = scratch[(size_t)scratch_cap + src + k];
-                o3[dst + k] = scratch[2 * (size_t)scratch_cap + src + k];
+            } else if ((uint64_t)c_src + k < scratch_cap) {
+                o1[d] = scratch[c_src + k];
+                o2[d] = scratch[(size_t)scratch_cap + c_src + k];
+                o3[d] = scratch[2 * (size_t)scratch_cap + c_src + k];
             } else {  // never left unwritten below the output's capacity
-                o1[dst + k] = o2[dst + k] = o3[dst + k] = kSentinel;
+                o1[d] = o2[d] = o3[d] = kSentinel;
                 over = true;
             }
         }
@@ -357,7 +380,8 @@ hipError_t launch_chunk_copy3(const uint32_t* d_chunks, uint64_t max_chunks, con
                               const uint32_t* chunk_at, const uint32_t* scratch, uint32_t scratch_cap, uint32_t* o1,
                               uint32_t* o2, uint32_t* o3, uint32_t out_cap, DevCounts* cnt, const LaunchCfg& lc,
                               hipStream_t st) {
-    hipLaunchKernelGGL(k_chunk_copy3, dim3(stride_blocks(max_chunks * kWave, 256, lc)), dim3(256), 0, st, d_chunks,
+    hipLaunchKernelGGL(k_chunk_copy3, dim3(stride_blocks(max_chunks * (kWave / kCopyGroup), 256, lc)), dim3(256), 0,
+                       st, d_chunks,
                        chunk_n, chunk_at, scratch, scratch_cap, o1, o2, o3, out_cap, cnt);
     return hipGetLastError();
 }
