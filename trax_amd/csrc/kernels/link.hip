@@ -773,34 +773,36 @@ __global__ void k_ca_basis_store(const uint32_t* __restrict__ basis, const uint3
     if (over) atomicOr(&cnt->err, kErrCapacity);
 }
 
-// followerBestBasisStore (.h:224-263) over a basis list put back into
-// connection-pair (= ascending basis) order
+// followerBestBasisStore (.h:224-263) over a basis list in connection-pair
+// (= ascending basis) order: the first entry, replaced by each later entry of
+// a strictly smaller pT difference.  That is, with the list in any order: the
+// entry of the smallest basis index if its difference is NaN (nothing
+// compares below a NaN), else the smallest difference among the non-NaN ones,
+// the smallest basis index among equal ones (a NaN never replaces).  One pass
+// over the stored list, which stays as stored.
 __global__ void k_ca_best(CABufs b, DevCounts* __restrict__ cnt) {
     if (run_void(cnt)) return;
     const uint32_t R = bounded(cnt->n_tracklets, cnt->cap_tracklets, cnt);
     for (uint32_t t = gtid(); t < R; t += gstride()) {
         const uint32_t begin = b.fbc[t], end = b.fbc[t + 1];
         if (begin == end) continue;
-        for (uint32_t i = begin + 1; i < end; ++i) {
-            const uint32_t ki = b.bidx[i];
-            const float kd = b.bdiff[i];
-            uint32_t j = i;
-            while (j > begin && b.bidx[j - 1] > ki) {
-                b.bidx[j] = b.bidx[j - 1];
-                b.bdiff[j] = b.bdiff[j - 1];
-                --j;
-            }
-            b.bidx[j] = ki;
-            b.bdiff[j] = kd;
-        }
-        float minPt = b.bdiff[begin];
-        uint32_t minIdx = b.bidx[begin];
-        for (uint32_t i = begin + 1; i < end; ++i) {
+        uint32_t first_idx = kSentinel, best_idx = kSentinel;
+        float first_d = 0.0f, best_d = 0.0f;
+        bool have = false;
+        for (uint32_t i = begin; i < end; ++i) {
+            const uint32_t k = b.bidx[i];
             const float d = b.bdiff[i];
-            const uint32_t test = d < minPt;
-            minPt = minPt * (float)(!test) + d * (float)test;
-            minIdx = minIdx * (uint32_t)(!test) + b.bidx[i] * test;
+            if (k < first_idx || i == begin) {
+                first_idx = k;
+                first_d = d;
+            }
+            if (!tx_isnan(d) && (!have || d < best_d || (d == best_d && k < best_idx))) {
+                best_d = d;
+                best_idx = k;
+                have = true;
+            }
         }
+        const uint32_t minIdx = (tx_isnan(first_d) || !have) ? first_idx : best_idx;
         b.best[t] = minIdx;
         if (minIdx < R) b.isbest[minIdx] = 1u;
     }
