@@ -81,3 +81,40 @@ def test_stage_reports_a_too_small_output_and_stops_the_chain(dev):
         check(L.trax_stage_check(dev.h, None))
     finally:
         L.trax_device_config_destroy(dc)
+
+
+def test_ca_best_basis_is_the_ascending_scan_with_nan_and_ties(dev):
+    """followerBestBasisStore (CellularAutomaton.h:224-263) scans a follower's
+    bases in ascending basis order, keeps the first and takes each later one
+    of strictly smaller |pT difference|.  Random followers, each with one to
+    eight bases of state 1, pT drawn from four values (equal differences: the
+    lowest index stays) and 10 % NaN (a NaN difference never replaces, and
+    stays when it comes first): the kernel's best basis is that scan's."""
+    from trax_amd.stages import CellularAutomaton, DeviceVector
+    rng = np.random.default_rng(7)
+    n_b, n_f = 600, 200
+    R = n_b + n_f
+    conns = []
+    for f in range(n_f):
+        bases = sorted(int(x) for x in rng.choice(n_b, size=int(rng.integers(1, 9)), replace=False))
+        conns += [(b, n_b + f) for b in bases]
+    conns.sort()
+    pt = rng.choice(np.array([1.0, 2.0, 2.5, 3.0], np.float32), size=R)
+    pt[rng.random(R) < 0.1] = np.nan
+    basis, foll, ptv = DeviceVector(dev, len(conns)), DeviceVector(dev, len(conns)), DeviceVector(dev, R, np.float32)
+    basis.upload(np.array([c[0] for c in conns], np.uint32))
+    foll.upload(np.array([c[1] for c in conns], np.uint32))
+    ptv.upload(pt)
+    b, _ = CellularAutomaton(dev).run(basis, foll, ptv)
+    got = b["best_basis"].download()
+    n_nan_first = 0
+    for f in range(n_b, R):
+        bases = sorted(c[0] for c in conns if c[1] == f)
+        diffs = [np.abs(pt[x] - pt[f]) for x in bases]  # float32, as the kernel
+        best, m = bases[0], diffs[0]
+        n_nan_first += bool(np.isnan(m))
+        for x, d in zip(bases[1:], diffs[1:]):
+            if d < m:
+                best, m = x, d
+        assert got[f] == best, (f, bases, diffs, got[f])
+    assert n_nan_first > 5  # the NaN-first case is exercised
