@@ -714,8 +714,8 @@ __device__ inline void fine_range(const GridGeom& g, const FineGeom& fg, uint32_
 // (sub-row, phi cell) by the thread that owns the cell (cells hold their hits
 // in input order, so a sequential pass keeps the order stable), an LDS scan
 // in (sub-row, cell) order offset by the row's first hit, then the scatter.
-// One block per item; items are (event, layer) * n_z + row.
-__global__ __launch_bounds__(256) void k_fine_grid(GridGeom g, FineGeom fg, const uint32_t* __restrict__ grid,
+// A block takes items in turn; items are (event, layer) * n_z + row.
+__global__ __launch_bounds__(128) void k_fine_grid(GridGeom g, FineGeom fg, const uint32_t* __restrict__ grid,
                                                   const float2* __restrict__ hzp, uint32_t n_hit_cap,
                                                   const DevCounts* __restrict__ cnt, uint32_t* __restrict__ fgrid,
                                                   float4* __restrict__ fh) {
@@ -2241,11 +2241,18 @@ hipError_t launch_fine_grid(const GridGeom& g, uint32_t n_el_cap, uint32_t n_hit
     const uint32_t F = lc.predict_fine ? lc.predict_fine : 8;
     if (F > fine_max(g) || (F & (F - 1)) != 0 || !fb.grid || !fb.hits) return hipErrorInvalidValue;
     fg = fine_geom(g, F);
+    // four 128-thread blocks per CU, each taking items in turn: the build runs
+    // beside the pair phases, which keep most of the device (a block per item
+    // of 256 threads, half of them idle over the 129 cells, filled the device
+    // and slowed the pair count by more than the build itself takes)
     const uint64_t items = (uint64_t)n_el_cap * g.n_z;
-    const uint32_t blocks = (uint32_t)(items < 65536 ? (items ? items : 1) : 65536);
-    const size_t lds = ((size_t)F * (g.n_phi + 1) + 256 / kWave + 1) * sizeof(uint32_t);
-    hipLaunchKernelGGL(k_fine_grid, dim3(blocks), dim3(256), lds, st, g, fg, grid, hzp, n_hit_cap, cnt, fb.grid,
-                       fb.hits);
+    constexpr uint32_t kFineBlock = 128;
+    LaunchCfg per_cu = lc;
+    per_cu.blocks_per_cu = 4;
+    const uint32_t blocks = stride_blocks(items * kFineBlock, kFineBlock, per_cu);
+    const size_t lds = ((size_t)F * (g.n_phi + 1) + kFineBlock / kWave + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_fine_grid, dim3(blocks), dim3(kFineBlock), lds, st, g, fg, grid, hzp, n_hit_cap, cnt,
+                       fb.grid, fb.hits);
     return hipGetLastError();
 }
 
